@@ -1,0 +1,304 @@
+#!/usr/bin/env python3
+"""Benchmark of the relational operator path on one MI355X.
+
+Default workload (one step = one pass of the hot path over one batch):
+filtered hash join, R = 2^24 tuples (key = seeded permutation of [0, 2^24),
+payload uniform int32), S = 2^28 tuples (fk uniform over R keys, payload
+uniform int32, filter column uniform in [0, 100) with predicate < 50); a step
+builds the table on R and probes it with the filtered S, materialising
+(key, r_payload, s_payload).  Inputs are generated in HBM before timing.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config join|filter_agg|groupby|batched|served]
+
+Prints ONE JSON line.  `value` is S tuples per second of the whole join step
+(build + probe); the roofline block covers the probe kernel, timed with HIP
+events on the stream it runs on.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "probe/scan tuples/s + % HBM peak; batched-vs-plain queries/s; p99 ms"
+SEED_BASE = 0x51A7E5EED0000000
+INT_MIN = -(1 << 31)
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline (oracle, single thread) on a bounded sample of the workload.
+# ---------------------------------------------------------------------------
+def cpu_baseline_join(nr: int, ns: int, seed: int, sample: int) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+
+    rk = orc.gen_perm(nr, seed + 1).astype(np.int32)
+    rp = orc.gen_uniform_u32(nr, seed, 1, 0, 0).astype(np.int32)
+    u = orc.gen_uniform_u32(sample, seed, 2, 0, nr)
+    sk = rk[u.astype(np.int64)]  # fk = perm(u) and perm(i) is R's key column
+    sp = orc.gen_uniform_u32(sample, seed, 3, 0, 0).astype(np.int32)
+    sf = orc.gen_uniform_u32(sample, seed, 4, 0, 100).astype(np.int32)
+    lib = orc.load()
+    # Build alone (empty probe side), then build + probe of the sample.
+    e = np.zeros(1, np.int32)
+    t0 = time.perf_counter()
+    lib.or_hash_join(rk.ctypes.data, rp.ctypes.data, nr, e.ctypes.data, e.ctypes.data, None, 0, 0, 0,
+                     None, None, None, 0)
+    t1 = time.perf_counter()
+    m = lib.or_hash_join(rk.ctypes.data, rp.ctypes.data, nr, sk.ctypes.data, sp.ctypes.data, sf.ctypes.data,
+                         sample, INT_MIN, 49, None, None, None, 0)
+    t2 = time.perf_counter()
+    build_s = t1 - t0
+    probe_s = max((t2 - t1) - build_s, 1e-9)
+    full_step_s = build_s + probe_s * (ns / sample)
+    return {
+        "value": ns / full_step_s,
+        "unit": "S tuples/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle or_hash_join, 1 thread: build on full R ({nr} rows, {build_s:.2f} s) + filtered "
+                   f"probe of the first {sample} S rows ({probe_s:.2f} s, {m} matches); step time "
+                   f"extrapolated to |S|={ns} as build + probe*|S|/sample"),
+    }
+
+
+def cpu_baseline_filter_agg(n: int, seed: int, sample: int) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+
+    k = orc.gen_uniform_u32(sample, seed, 5, 0, 1 << 31).astype(np.int32)
+    v = orc.gen_uniform_u32(sample, seed, 6, 0, 1000).astype(np.int32)
+    t0 = time.perf_counter()
+    orc.filter_agg(k, v, 0, (1 << 30) - 1)
+    t1 = time.perf_counter()
+    return {"value": sample / (t1 - t0), "unit": "tuples/s", "cores": 1, "kind": "port",
+            "sample": f"oracle or_filter_agg, 1 thread, first {sample} of {n} tuples"}
+
+
+# ---------------------------------------------------------------------------
+def run_join(args, qo, lib) -> dict:
+    from query_on_opencl_fpga_amd import data
+    from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
+
+    nr, ns = args.nr, args.ns
+    seed = SEED_BASE + 2
+    log(f"[bench] join: generating R={nr} S={ns}")
+    t = data.JoinTables(nr, ns, seed)
+    tb = lib.qo_join_table_bytes(nr)
+    table = DeviceBuffer(tb)
+    cap = ns
+    outs = [DeviceBuffer(cap * 4) for _ in range(3)]
+    cnt = DeviceBuffer(8)
+    st = Stream()
+    evs = [[Event(), Event(), Event()] for _ in range(args.steps)]
+    e_start, e_end = Event(), Event()
+
+    def step(ev=None):
+        if ev:
+            ev[0].record(st)
+        qo._lib.call("qo_hash_build", table.ptr, tb, t.rkey.ptr, t.rpay.ptr, nr, st.handle)
+        if ev:
+            ev[1].record(st)
+        qo._lib.call("qo_hash_probe", table.ptr, t.skey.ptr, t.spay.ptr, t.sfilt.ptr, ns, INT_MIN, 49,
+                     outs[0].ptr, outs[1].ptr, outs[2].ptr, cap, cnt.ptr, st.handle)
+        if ev:
+            ev[2].record(st)
+
+    log("[bench] join: warmup")
+    for _ in range(args.warmup):
+        step()
+    st.sync()
+    qo._lib.call("qo_device_sync")
+    log("[bench] join: timed steps")
+    e_start.record(st)
+    for i in range(args.steps):
+        step(evs[i])
+    e_end.record(st)
+    st.sync()
+    total_ms = e_start.elapsed_ms(e_end)
+    build_ms = [evs[i][0].elapsed_ms(evs[i][1]) for i in range(args.steps)]
+    probe_ms = [evs[i][1].elapsed_ms(evs[i][2]) for i in range(args.steps)]
+    m = int(cnt.to_numpy(np.uint64, 1)[0])
+    # Size-independent check: every S fk is an R key, so matches = rows passing the filter.
+    fcnt = DeviceBuffer(24)
+    qo._lib.call("qo_filter_agg", t.sfilt.ptr, t.sfilt.ptr, ns, INT_MIN, 49, fcnt.ptr, None)
+    expect = int(fcnt.to_numpy(np.int64, 3)[0])
+    if m != expect:
+        raise SystemExit(f"join produced {m} matches, expected {expect}")
+    ms_step = total_ms / args.steps
+    probe_avg = float(np.mean(probe_ms))
+    build_avg = float(np.mean(build_ms))
+    probe_bytes = 12 * ns + 12 * m
+    achieved = probe_bytes / (probe_avg * 1e-3) / 1e9
+    res = {
+        "metric": METRIC,
+        "value": ns / (ms_step * 1e-3),
+        "unit": "S tuples/s",
+        "ms_per_step": ms_step,
+        "dtype": "int32",
+        "config": {"workload": "filter + hash join (build R, probe filtered S, materialise key/r_payload/s_payload)",
+                   "R_tuples": nr, "S_tuples": ns, "filter": "sfilt < 50 of U[0,100)", "matches": m,
+                   "table_bytes": tb},
+        "roofline": {"bound": "hbm", "kernel": "k_join_probe", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
+                     "alg_bytes_per_launch": probe_bytes, "avg_launch_ms": probe_avg},
+        "kernels": {"build_ms": build_avg, "probe_ms": probe_avg,
+                    "build_R_tuples_per_s": nr / (build_avg * 1e-3),
+                    "probe_S_tuples_per_s": ns / (probe_avg * 1e-3),
+                    "build_GBps_alg": 16 * nr / (build_avg * 1e-3) / 1e9},
+    }
+    for b in outs + [table, cnt, fcnt]:
+        b.free()
+    t.free()
+    if not args.no_cpu_baseline:
+        log("[bench] join: cpu baseline")
+        res["cpu_baseline"] = cpu_baseline_join(nr, ns, seed, args.cpu_sample)
+    return res
+
+
+def run_filter_agg(args, qo, lib) -> dict:
+    from query_on_opencl_fpga_amd import data
+    from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
+
+    n = args.n_filter
+    seed = SEED_BASE + 1
+    log(f"[bench] filter_agg: generating n={n}")
+    t = data.FilterAggTables(n, seed)
+    out = DeviceBuffer(24)
+    st = Stream()
+    evs = [(Event(), Event()) for _ in range(args.steps)]
+    for _ in range(args.warmup):
+        qo._lib.call("qo_filter_agg", t.key.ptr, t.val.ptr, n, 0, (1 << 30) - 1, out.ptr, st.handle)
+    st.sync()
+    e0, e1 = Event(), Event()
+    e0.record(st)
+    for i in range(args.steps):
+        evs[i][0].record(st)
+        qo._lib.call("qo_filter_agg", t.key.ptr, t.val.ptr, n, 0, (1 << 30) - 1, out.ptr, st.handle)
+        evs[i][1].record(st)
+    e1.record(st)
+    st.sync()
+    total_ms = e0.elapsed_ms(e1)
+    k_ms = float(np.mean([a.elapsed_ms(b) for a, b in evs]))
+    r = out.to_numpy(np.int64, 3)
+    achieved = 8 * n / (k_ms * 1e-3) / 1e9
+    res = {
+        "metric": METRIC, "value": n / (total_ms / args.steps * 1e-3), "unit": "tuples/s",
+        "ms_per_step": total_ms / args.steps, "dtype": "int32",
+        "config": {"workload": "fused filter + aggregation (COUNT, SUM int64) over key/value int32 columns",
+                   "tuples": n, "predicate": "0 <= key <= 2^30-1", "count": int(r[0]), "sum": int(r[1])},
+        "roofline": {"bound": "hbm", "kernel": "k_filter_agg", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
+                     "alg_bytes_per_launch": 8 * n, "avg_launch_ms": k_ms},
+    }
+    out.free()
+    t.free()
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_filter_agg(n, seed, min(n, args.cpu_sample * 4))
+    return res
+
+
+def run_groupby(args, qo, lib) -> dict:
+    from query_on_opencl_fpga_amd import data
+    from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
+
+    n = args.n_groupby
+    seed = SEED_BASE + 3
+    rows = []
+    st = Stream()
+    for G in (1 << 10, 1 << 20):
+        for z in (0.0, 0.5, 1.0):
+            log(f"[bench] groupby: n={n} G={G} z={z}")
+            t = data.GroupbyTables(n, G, z, seed)
+            outs = [DeviceBuffer(G * s) for s in (4, 8, 8, 4, 4)]
+            num = DeviceBuffer(8)
+
+            def step():
+                qo._lib.call("qo_groupby_agg", t.key.ptr, t.val.ptr, n, G, *[b.ptr for b in outs], num.ptr,
+                             st.handle)
+
+            for _ in range(args.warmup):
+                step()
+            st.sync()
+            e0, e1 = Event(), Event()
+            e0.record(st)
+            for _ in range(args.steps):
+                step()
+            e1.record(st)
+            st.sync()
+            ms = e0.elapsed_ms(e1) / args.steps
+            g = int(num.to_numpy(np.uint64, 1)[0])
+            alg = 8 * n + 28 * G
+            rows.append({"G": G, "z": z, "groups": g, "ms": ms, "tuples_per_s": n / (ms * 1e-3),
+                         "GBps_alg": alg / (ms * 1e-3) / 1e9, "frac_hbm": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+            for b in outs + [num]:
+                b.free()
+            t.free()
+    worst = min(rows, key=lambda r: r["tuples_per_s"])
+    return {"metric": METRIC, "value": worst["tuples_per_s"], "unit": "tuples/s (slowest of 6 cases)",
+            "ms_per_step": worst["ms"], "dtype": "int32",
+            "config": {"workload": "hash group-by (COUNT, SUM, MIN, MAX), key Zipf(z) over G groups", "tuples": n,
+                       "cases": rows},
+            "roofline": {"bound": "hbm", "kernel": "k_groupby_hash", "achieved": worst["GBps_alg"],
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": worst["frac_hbm"], "traffic": None}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="join", choices=["join", "filter_agg", "groupby", "batched", "served"])
+    ap.add_argument("--nr", type=int, default=1 << 24)
+    ap.add_argument("--ns", type=int, default=1 << 28)
+    ap.add_argument("--n-filter", type=int, default=1 << 28)
+    ap.add_argument("--n-groupby", type=int, default=1 << 28)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 25)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per launch of the roofline kernel from rocprofv3 --pmc, if measured")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the operators shard by "
+              "range/hash across GPUs by design, but this build measures one GPU)", file=sys.stderr)
+        return 2
+    import query_on_opencl_fpga_amd as qo
+
+    lib = qo._lib.load()
+    qo._lib.call("qo_init", 0)
+    info = qo.device.device_info()
+    log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}")
+    run = {"join": run_join, "filter_agg": run_filter_agg, "groupby": run_groupby}
+    if args.config not in run:
+        from query_on_opencl_fpga_amd import serving_bench  # batched / served modes
+        res = serving_bench.run(args, qo, lib)
+    else:
+        res = run[args.config](args, qo, lib)
+    line = {
+        "metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"),
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
+        "data": "synthetic (deterministic splitmix64 columns generated in HBM)",
+    }
+    line.update(res)
+    line["device"] = {"name": info["name"], "arch": info["arch"], "cu_count": info["cu_count"],
+                      "host_cores": os.cpu_count()}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,231 @@
+/*
+ * qo.h -- native C ABI of libqo, the MI355X (gfx950) relational operator library.
+ *
+ * Every entry point takes plain pointers and sizes.  Device pointers are HIP
+ * device memory; `qo_stream` is a hipStream_t passed as an opaque pointer
+ * (NULL selects the library's own per-thread stream).  Operators are
+ * asynchronous on the given stream unless stated otherwise; counts that the
+ * reference read back mid-pipeline are written to device memory instead.
+ * All functions return a qo_status; on failure qo_last_error() describes it.
+ * No function ever exits the process.
+ *
+ * Record layout is the reference's 8-byte AoS tuple (host/host/common.h:27-28):
+ * .x = value / row id, .y = key.  Column APIs take SoA int32 columns.
+ */
+#ifndef QO_H
+#define QO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qo_record { uint32_t x, y; } qo_record;
+typedef void *qo_stream;
+typedef void *qo_event;
+
+typedef enum qo_status {
+    QO_OK = 0,
+    QO_ERR_INVALID_ARG = 1,
+    QO_ERR_HIP = 2,
+    QO_ERR_CAPACITY = 3,   /* output buffer or hash table too small; count is still written */
+    QO_ERR_NO_MEMORY = 4,
+    QO_ERR_BUDGET = 5,     /* server admission: request can never fit the memory budget */
+    QO_ERR_INJECTED = 6,   /* fault injection flag set by qo_debug_inject_fault */
+    QO_ERR_NOT_FOUND = 7,
+    QO_ERR_CANCELLED = 8
+} qo_status;
+
+/* Aggregate operators, same codes as the reference (host/host/common.h:89-92). */
+enum { QO_REDUCE_SUM = 0, QO_REDUCE_MAX = 1, QO_REDUCE_MIN = 2, QO_REDUCE_AVERAGE = 3 };
+
+typedef struct qo_device_info {
+    char name[64];
+    char arch[32];
+    int cu_count;
+    int lds_bytes_per_block;
+    int wave_size;
+    size_t hbm_bytes;
+    int xcd_count;
+} qo_device_info;
+
+/* ---- runtime ---------------------------------------------------------- */
+const char *qo_version(void);
+const char *qo_last_error(void);
+void qo_clear_error(void);
+int qo_init(int device);
+int qo_get_device_info(qo_device_info *out);
+int qo_malloc(void **ptr, size_t bytes);
+int qo_free(void *ptr);
+int qo_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int qo_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int qo_memcpy_d2d(void *dst, const void *src, size_t bytes, qo_stream s);
+int qo_fill_u32(void *dst, uint32_t value, size_t count, qo_stream s);
+int qo_stream_create(qo_stream *s);
+int qo_stream_destroy(qo_stream s);
+int qo_stream_sync(qo_stream s);
+int qo_device_sync(void);
+int qo_event_create(qo_event *e);
+int qo_event_destroy(qo_event e);
+int qo_event_record(qo_event e, qo_stream s);
+int qo_event_sync(qo_event e);
+int qo_event_query(qo_event e);      /* 1 = complete, 0 = pending, <0 error */
+int qo_event_elapsed_ms(qo_event start, qo_event stop, float *ms);
+/* Release cached per-stream workspaces (they grow on demand). */
+int qo_release_workspaces(void);
+/* Fault injection for failure-handling tests: the next `count` operator calls
+ * return QO_ERR_INJECTED before launching anything. */
+int qo_debug_inject_fault(int count);
+
+/* ---- deterministic synthetic data (identical to the CPU generator spec) ---- */
+/* out[i] = lo + floor(hi32(r_i) * range / 2^32), r_i = splitmix64 stream (seed, stream_id)
+ * at position i; range == 0 means the full 32-bit draw. */
+int qo_gen_uniform_u32(uint32_t *out, size_t n, uint64_t seed, uint64_t stream_id,
+                       uint32_t lo, uint32_t range, qo_stream s);
+/* out[i] = i-th element of a seeded permutation of [0, n). */
+int qo_gen_perm_u32(uint32_t *out, size_t n, uint64_t perm_seed, qo_stream s);
+/* out[i] = bij_bits(i): keyed bijection on [0, 2^bits) (unique keys). */
+int qo_gen_bij_u32(uint32_t *out, size_t n, uint32_t bits, uint64_t seed, qo_stream s);
+/* Foreign keys drawn uniformly over the build keys: out[i] = keymap(u_i), u_i uniform in
+ * [0, nr); keymap = permutation of [0, nr) (bits == 0) or bij_bits (bits > 0). */
+int qo_gen_fk_u32(uint32_t *out, size_t n, uint64_t seed, uint64_t stream_id, uint64_t nr,
+                  uint32_t bits, uint64_t key_seed, qo_stream s);
+/* Host: fixed-point Zipf CDF table (G entries of uint64). */
+int qo_zipf_table(uint64_t *host_table, uint32_t G, double z);
+/* Zipf(z) ranks over G groups mapped through a seeded permutation of [0, G);
+ * d_table is the qo_zipf_table output copied to device memory. */
+int qo_gen_zipf_u32(uint32_t *out, size_t n, const uint64_t *d_table, uint32_t G,
+                    uint64_t seed, uint64_t stream_id, uint64_t perm_seed, qo_stream s);
+/* Interleave two columns into Records: out[i] = {x[i], y[i]}. */
+int qo_pack_records(const uint32_t *x, const uint32_t *y, size_t n, qo_record *out, qo_stream s);
+
+/* ---- filter ------------------------------------------------------------- */
+/* Fused filter + aggregate (COUNT, SUM of values where lo <= key <= hi).
+ * d_out[0] = COUNT, d_out[1] = SUM as int64, d_out[2] = SUM wrapped to int32
+ * (the reference's REDUCE_SUM), sign-extended. */
+int qo_filter_agg(const int32_t *key, const int32_t *val, size_t n, int32_t lo, int32_t hi,
+                  int64_t *d_out, qo_stream s);
+int qo_filter_agg_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi,
+                          int64_t *d_out, qo_stream s);
+/* Stable selection lo <= (int)R.y <= hi with compaction into rout (capacity n). */
+int qo_filter_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi,
+                      qo_record *rout, uint64_t *d_count, qo_stream s);
+/* Exclusive prefix sum of int32 (wraps modulo 2^32). in == out is allowed. */
+int qo_scan_exclusive_i32(const int32_t *in, int32_t *out, size_t n, qo_stream s);
+
+/* ---- sort ---------------------------------------------------------------- */
+/* Sort Records ascending by .y (uint32), stable, in place. */
+int qo_sort_records(qo_record *data, size_t n, qo_stream s);
+/* Sort join rows ascending by (key, s_payload) as signed int32, stable, in place. */
+int qo_sort_join_rows(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, qo_stream s);
+
+/* ---- sort-based group-by (reference pipeline) ------------------------------ */
+/* On Records sorted by .y: startPos[g] = first index of group g; *d_numGroups = G. */
+int qo_group_startpos(const qo_record *sorted, size_t n, int32_t *startPos,
+                      uint64_t *d_numGroups, qo_stream s);
+/* Per-group reduction of (int).x with the reference's operator semantics. */
+int qo_aggregate_groups(const qo_record *sorted, size_t n, const int32_t *startPos,
+                        size_t numGroups, int op, int32_t *out, qo_stream s);
+
+/* ---- hash group-by aggregation -------------------------------------------- */
+/* Per distinct key: (key, COUNT, SUM int64, MIN, MAX) ascending by key (signed).
+ * group_capacity bounds the number of distinct keys (output arrays hold it);
+ * more distinct keys than that gives QO_ERR_CAPACITY. */
+int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity,
+                   int32_t *okey, int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax,
+                   uint64_t *d_numGroups, qo_stream s);
+
+/* ---- hash join ------------------------------------------------------------ */
+/* Bytes of device memory a build table for `build_rows` rows needs. */
+size_t qo_join_table_bytes(size_t build_rows);
+int qo_hash_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay,
+                  size_t nr, qo_stream s);
+/* Probe with the S filter flo <= sfilt <= fhi fused (sfilt may be NULL: no filter).
+ * Matches are written in S order as (key, r_payload, s_payload); *d_count = matches. */
+int qo_hash_probe(const void *table, const int32_t *skey, const int32_t *spay, const int32_t *sfilt,
+                  size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay, int32_t *ospay,
+                  size_t out_capacity, uint64_t *d_count, qo_stream s);
+/* Filter S, build on R, probe, sort output by (key, s_payload).  Synchronous:
+ * *h_count receives the match count. */
+int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay,
+                      size_t nr, const int32_t *skey, const int32_t *spay, const int32_t *sfilt,
+                      size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
+                      int32_t *ospay, size_t out_capacity, uint64_t *h_count, qo_stream s);
+
+/* Reference-shaped bucket-chained table (hj_filter_sort_op.cl:25-109): B buckets
+ * (power of two) of bucketSize (key, val) slots plus per-bucket counts.  Input
+ * and output tables are interleaved uint pairs.  Overflowing a bucket gives
+ * QO_ERR_CAPACITY.  The probe writes compacted (s_val, r_val) pairs. */
+int qo_bucket_build(const uint32_t *rTable, uint32_t size, uint32_t *rHashTable, uint32_t B,
+                    uint32_t bucketSize, uint32_t *rHashCount, qo_stream s);
+int qo_bucket_probe(const uint32_t *rHashTable, const uint32_t *rHashCount, uint32_t B,
+                    uint32_t bucketSize, const uint32_t *sTable, uint32_t size,
+                    uint32_t *matchedTable, size_t capacity_pairs, uint64_t *d_count, qo_stream s);
+
+/* ---- batched hj_filter_sort (one hipGraph launch for many queries) ---------- */
+typedef struct qo_join_query {
+    const int32_t *rkey, *rpay;
+    const int32_t *skey, *spay, *sfilt;
+    uint64_t nr, ns;
+    int32_t flo, fhi;
+    int32_t *okey, *orpay, *ospay;
+    uint64_t out_capacity;
+} qo_join_query;
+typedef struct qo_batch qo_batch;
+/* Plan a batch: allocates the batch's tables, workspaces and the captured graph. */
+int qo_batch_create(const qo_join_query *queries, int nq, qo_batch **out);
+/* Launch the whole batch (one graph launch) on stream s. */
+int qo_batch_launch(qo_batch *b, qo_stream s);
+/* After completion: per-query match counts (host array of nq). */
+int qo_batch_counts(qo_batch *b, uint64_t *h_counts);
+int qo_batch_destroy(qo_batch *b);
+
+/* ---- autotuner -------------------------------------------------------------- */
+/* Operators: 0 filter_agg, 1 filter, 2 probe, 3 groupby, 4 sort. Size class = floor(log2 n). */
+int qo_tune_get(int op, int size_class, int *block, int *items, int *param);
+int qo_tune_set(int op, int size_class, int block, int items, int param);
+int qo_tune_load(const char *path);
+int qo_tune_save(const char *path);
+
+/* ---- serving loop ------------------------------------------------------------ */
+typedef struct qo_server qo_server;
+typedef struct qo_server_config {
+    uint64_t memory_budget_bytes;  /* device-memory admission budget */
+    int max_inflight_batches;      /* concurrent batches (streams) */
+    int max_batch_queries;         /* queries admitted into one batch */
+} qo_server_config;
+enum { QO_Q_FILTER = 0, QO_Q_GROUPBY = 1, QO_Q_JOIN_SORT = 2 };
+typedef struct qo_request {
+    int kind;
+    int table_a;        /* filter / group-by input, join build (R) */
+    int table_b;        /* join probe (S) */
+    int32_t lo, hi;     /* filter predicate (filter: key; join: S filter column) */
+    uint64_t group_capacity;
+} qo_request;
+typedef struct qo_result {
+    uint64_t id;
+    int status;
+    int kind;
+    uint64_t count;     /* filter: selected rows; group-by: groups; join: matches */
+    int64_t sum;        /* filter: SUM(value); group-by: SUM over all groups; join: checksum */
+    int64_t checksum;   /* order-sensitive checksum of the materialised output */
+    double submit_ms, start_ms, end_ms;  /* server clock */
+} qo_result;
+/* Tables: columns are (key, val[, filt]) int32 device arrays owned by the caller. */
+int qo_server_create(const qo_server_config *cfg, qo_server **out);
+int qo_server_add_table(qo_server *sv, const int32_t *key, const int32_t *val, const int32_t *filt,
+                        uint64_t n, int *table_id);
+int qo_server_submit(qo_server *sv, const qo_request *req, uint64_t *id);
+/* Wait for up to `max` results (timeout_ms < 0 waits for at least one). Returns count. */
+int qo_server_poll(qo_server *sv, qo_result *out, int max, int timeout_ms);
+int qo_server_drain(qo_server *sv);
+int qo_server_destroy(qo_server *sv);
+/* Execute one request synchronously on the plain per-query path (same result). */
+int qo_server_run_plain(qo_server *sv, const qo_request *req, qo_result *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QO_H */

@@ -1,0 +1,118 @@
+/*
+ * qo_compat.h -- drop-in replacements for the reference's host operator entry
+ * points (Xtra-Computing/Query_on_OpenCL_FPGA, host/host/*.cpp), backed by
+ * libqo's gfx950 kernels.
+ *
+ * Names, parameter order and meanings follow the reference.  `cl_mem` becomes
+ * the opaque `qo_mem` (device pointer + byte size); the reference's vestigial
+ * scheduling parameters (index, eventList, Kernel, Flag_CPU_GPU, burden,
+ * _CPU_GPU) are accepted and ignored; numThread/numBlock are hints only.
+ * Every call is synchronous, like the reference's clFinish after each launch.
+ * On error a call returns -1 (int results) or leaves its outputs untouched and
+ * sets qo_last_error(); with QO_COMPAT_EXIT_ON_ERROR=1 in the environment it
+ * prints the error and exits like the reference's checkError.
+ */
+#ifndef QO_COMPAT_H
+#define QO_COMPAT_H
+
+#include "qo.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef qo_record Record;                 /* host/host/common.h:27-28 (uint2 {x, y}) */
+typedef struct qo_mem_s *qo_mem;          /* stands where the reference passes cl_mem */
+
+#ifndef REDUCE_SUM
+#define REDUCE_SUM (0)
+#define REDUCE_MAX (1)
+#define REDUCE_MIN (2)
+#define REDUCE_AVERAGE (3)
+#endif
+#ifndef TEST_MAX
+#define TEST_MAX (1 << 30)                /* host/host/common.h:29 */
+#endif
+
+/* Device buffer handles (the role of clCreateBuffer / clReleaseMemObject). */
+qo_mem qo_mem_create(size_t bytes);
+qo_mem qo_mem_wrap(void *device_ptr, size_t bytes);   /* caller keeps ownership of the memory */
+void *qo_mem_ptr(qo_mem m);
+size_t qo_mem_size(qo_mem m);
+void qo_mem_free(qo_mem m);
+int qo_mem_write(qo_mem m, const void *host, size_t bytes);   /* cl_writebuffer, common.cpp:93 */
+int qo_mem_read(void *host, qo_mem m, size_t bytes);          /* cl_readbuffer, common.cpp:64 */
+
+/* Scan plan (host/host/testScan.cpp:26-75, 229-270). */
+typedef struct ScanPara {
+    int gLength;
+    int blockSize;
+    int pass;
+    void *ws;
+} ScanPara;
+void initScan(int rLen, ScanPara *SP);
+void closeScan(ScanPara *SP);
+/* Exclusive int32 prefix sum of d_Src into d_Dst (testScan.cpp:278). */
+void scanImpl(qo_mem d_Src, int rLen, qo_mem d_Dst, int *index, void *eventList, void *kernel,
+              int *Flag_CPU_GPU, double *burden, ScanPara *SP, int _CPU_GPU);
+
+/* Selection (host/host/testFilter.cpp:92, host/host/Selection.cpp:6,19).
+ * *d_Rout is allocated by the callee (rLen Records) and freed by the caller. */
+void filterImpl(qo_mem d_Rin, int beginPos, int rLen, qo_mem *d_Rout, int *outSize, int numThread,
+                int numBlock, int smallKey, int largeKey, int *index, void *eventList, void *Kernel,
+                int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+int range_selection(qo_mem d_Rin, int rLen, int rangeSmallKey, int rangeLargeKey, qo_mem *d_Rout,
+                    int numThreadPB, int numBlock, int *index, void *eventList, void *Kernel,
+                    int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+/* *h_Rout is malloc'ed by the callee with outSize Records; the caller frees it. */
+int CL_RangeSelection(Record *h_Rin, int rLen, int rangeSmallKey, int rangeLargeKey, Record **h_Rout,
+                      int numThreadPB, int numBlock, int _CPU_GPU);
+
+/* Sort ascending by .y (host/host/testSort.cpp:24, host/host/Sort.cpp:34).  Stable. */
+void radixSortImpl_sm(qo_mem d_R, int rLen, int meaningless, int numThreadPB, int numBlock, int *index,
+                      void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+void CL_RadixSort(Record *h_Rin, int rLen, Record *h_Rout, int numThread, int numBlock, int _CPU_GPU);
+
+/* Group-by (host/host/testGroupBy.cpp:81, host/host/GroupBy.cpp:7).  groupByImpl
+ * sorts d_Rout in place (after copying d_Rin into it when they differ) and
+ * allocates *d_startPos with numGroups ints; returns numGroups. */
+int groupByImpl(qo_mem d_Rin, int rLen, qo_mem d_Rout, qo_mem *d_startPos, int numThread, int numBlock,
+                int *index, void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+int CL_GroupBy(Record *h_Rin, int rLen, Record *h_Rout, int **h_startPos, int numThread, int numBlock,
+               int _CPU_GPU);
+
+/* Aggregation after group-by (host/host/testAggAfterGB.cpp:69,92; AggAfterGroupBy.cpp:30-170).
+ * With d_Ragg / h_Ragg non-NULL the rows are first gathered S[i] = Ragg[Rin[i].x]
+ * (gatherBeforeAgg, testAggAfterGB.cpp:58-67); with NULL the groups reduce
+ * Rin[i].x directly, as testAggAfterGroupByImpl does.  Results are int32 with
+ * the reference's semantics (SUM wraps, MAX from 0, MIN from TEST_MAX, AVG
+ * truncates). */
+void parallelAggregate_init(qo_mem d_S, qo_mem d_startPos, qo_mem d_aggResults, int OPERATOR, int blockOffset,
+                            int numGroups, int numBlock, int numThread, int sharedMemSize, int rLen, int *index,
+                            void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+void aggAfterGroupByImpl(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                         qo_mem d_aggResults, int OPERATOR, int numThread, int *index, void *eventList,
+                         void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+void CL_agg_max_afterGroupBy(Record *h_Rin, int rLen, int *h_startPos, int numGroups, Record *h_Ragg,
+                             int *h_aggResults, int numThread, int _CPU_GPU);
+void CL_agg_min_afterGroupBy(Record *h_Rin, int rLen, int *h_startPos, int numGroups, Record *h_Ragg,
+                             int *h_aggResults, int numThread, int _CPU_GPU);
+void CL_agg_sum_afterGroupBy(Record *h_Rin, int rLen, int *h_startPos, int numGroups, Record *h_Ragg,
+                             int *h_aggResults, int numThread, int _CPU_GPU);
+void CL_agg_avg_afterGroupBy(Record *h_Rin, int rLen, int *h_startPos, int numGroups, Record *h_Ragg,
+                             int *h_aggResults, int numThread, int _CPU_GPU);
+
+/* Hash join with the kernel-shaped parameters of kernel/hj_filter_sort/1/
+ * hj_filter_sort_op.cl:27 and :64 (device pointers; interleaved (key, val) uint
+ * pairs; B a power of two).  buildHashTable zeroes rHashCount itself.
+ * probeHashTable writes COMPACTED (s_val, r_val) pairs and returns their count
+ * (matchedTable must hold 2*size uints, as the reference sizes it). */
+int buildHashTable(uint32_t *rTableOnDevice, uint32_t *rHashTable, uint32_t size, uint32_t rHashTableBucketNum,
+                   uint32_t hashBucketSize, uint32_t *rHashCount);
+int probeHashTable(uint32_t *rHashTable, uint32_t *sTableOnDevice, uint32_t *matchedTable, uint32_t size,
+                   uint32_t rHashTableBucketNum, uint32_t hashBucketSize, uint32_t *rHashCount);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QO_COMPAT_H */

@@ -1,0 +1,434 @@
+// Selection operators: fused filter+aggregate, stable filter compaction and the
+// exclusive scan.
+//
+// Reference path replaced (kernel/filter/filter_op.cl): filterImpl_map_kernel
+// (:6-31) marks rows, ScanLargeArrays/prefixSum/blockAddition (:51-207) scan the
+// marks in 2*pass+1 launches, filterImpl_outSize_kernel (:211-215) is read back
+// by the host before filterImpl_write_kernel (:218-238) scatters.  Here one
+// kernel does all of it: predicate, wave ballot ranks, a decoupled look-back
+// across tiles, and a stable scatter; the count stays on the device.
+#include <map>
+#include <mutex>
+
+#include "qo_device.hpp"
+#include "qo_internal.hpp"
+
+namespace qo {
+
+// ---------------------------------------------------------------------------
+// Per-stream self-resetting counters for single-launch reductions.
+// ---------------------------------------------------------------------------
+static uint32_t *sync_counters(hipStream_t st, int *status);
+
+// ---------------------------------------------------------------------------
+// Fused filter + aggregate.  Each thread streams VW-tuple vectors (UNROLL loads
+// of each column in flight), keeps COUNT and an exact int64 SUM, the block
+// reduces through LDS, and the last block to finish (ticket on a self-resetting
+// counter) folds the per-block partials and writes the result.
+// ---------------------------------------------------------------------------
+template <bool AOS>
+struct FaTraits;
+
+template <>
+struct FaTraits<false> {  // SoA key / value columns, int4 = 4 tuples
+    static constexpr int VW = 4;
+    static QO_DEV void vec(const int32_t *k, const int32_t *v, size_t i, int32_t lo, int32_t hi, uint64_t &c,
+                           int64_t &s) {
+        int4 kk = ld_stream(reinterpret_cast<const int4 *>(k) + i);
+        int4 vv = ld_stream(reinterpret_cast<const int4 *>(v) + i);
+        bool p0 = (kk.x >= lo) & (kk.x <= hi), p1 = (kk.y >= lo) & (kk.y <= hi);
+        bool p2 = (kk.z >= lo) & (kk.z <= hi), p3 = (kk.w >= lo) & (kk.w <= hi);
+        c += (unsigned)p0 + (unsigned)p1 + (unsigned)p2 + (unsigned)p3;
+        s += (int64_t)(p0 ? vv.x : 0) + (int64_t)(p1 ? vv.y : 0) + (int64_t)(p2 ? vv.z : 0) +
+             (int64_t)(p3 ? vv.w : 0);
+    }
+    static QO_DEV void one(const int32_t *k, const int32_t *v, size_t e, int32_t lo, int32_t hi, uint64_t &c,
+                           int64_t &s) {
+        int32_t kk = k[e];
+        bool p = (kk >= lo) & (kk <= hi);
+        c += p;
+        s += p ? (int64_t)v[e] : 0;
+    }
+};
+
+template <>
+struct FaTraits<true> {  // AoS Records {x = value, y = key}, int4 = 2 records
+    static constexpr int VW = 2;
+    static QO_DEV void vec(const int32_t *r, const int32_t *, size_t i, int32_t lo, int32_t hi, uint64_t &c,
+                           int64_t &s) {
+        int4 q = ld_stream(reinterpret_cast<const int4 *>(r) + i);
+        bool p0 = (q.y >= lo) & (q.y <= hi), p1 = (q.w >= lo) & (q.w <= hi);
+        c += (unsigned)p0 + (unsigned)p1;
+        s += (int64_t)(p0 ? q.x : 0) + (int64_t)(p1 ? q.z : 0);
+    }
+    static QO_DEV void one(const int32_t *r, const int32_t *, size_t e, int32_t lo, int32_t hi, uint64_t &c,
+                           int64_t &s) {
+        int32_t kk = r[2 * e + 1];
+        bool p = (kk >= lo) & (kk <= hi);
+        c += p;
+        s += p ? (int64_t)r[2 * e] : 0;
+    }
+};
+
+template <bool AOS, int UNROLL, bool VEC>
+__global__ __launch_bounds__(256) void k_filter_agg(const int32_t *__restrict__ a, const int32_t *__restrict__ b,
+                                                    size_t n, int32_t lo, int32_t hi, uint64_t *partials,
+                                                    uint32_t *done, int64_t *out) {
+    using Tr = FaTraits<AOS>;
+    uint64_t cnt = 0;
+    int64_t sum = 0;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t scalar_from = 0;
+    if (VEC) {
+        const size_t nv = n / Tr::VW;
+        size_t i = tid;
+        for (; i + (UNROLL - 1) * stride < nv; i += UNROLL * stride) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) Tr::vec(a, b, i + u * stride, lo, hi, cnt, sum);
+        }
+        for (; i < nv; i += stride) Tr::vec(a, b, i, lo, hi, cnt, sum);
+        scalar_from = nv * Tr::VW;
+    }
+    for (size_t e = scalar_from + tid; e < n; e += stride) Tr::one(a, b, e, lo, hi, cnt, sum);
+
+    __shared__ uint64_t s_c[4], s_s[4];
+    __shared__ bool s_last;
+    cnt = wave_sum(cnt);
+    uint64_t usum = wave_sum((uint64_t)sum);
+    if (lane_id() == 0) {
+        s_c[wave_id()] = cnt;
+        s_s[wave_id()] = usum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t c = 0, s = 0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
+            c += s_c[w];
+            s += s_s[w];
+        }
+        partials[2 * blockIdx.x] = c;
+        partials[2 * blockIdx.x + 1] = s;
+        __threadfence();
+        s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    uint64_t c = 0, s = 0;
+    for (unsigned bidx = threadIdx.x; bidx < gridDim.x; bidx += blockDim.x) {
+        c += __hip_atomic_load(&partials[2 * bidx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s += __hip_atomic_load(&partials[2 * bidx + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    c = wave_sum(c);
+    s = wave_sum(s);
+    __syncthreads();
+    if (lane_id() == 0) {
+        s_c[wave_id()] = c;
+        s_s[wave_id()] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t tc = 0, ts = 0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
+            tc += s_c[w];
+            ts += s_s[w];
+        }
+        out[0] = (int64_t)tc;
+        out[1] = (int64_t)ts;
+        out[2] = (int64_t)(int32_t)(uint32_t)ts;
+        *done = 0;  // ready for the next launch on this stream
+    }
+}
+
+static int launch_filter_agg(bool aos, const int32_t *a, const int32_t *b, size_t n, int32_t lo, int32_t hi,
+                             int64_t *d_out, hipStream_t st) {
+    TuneParams tp = tune_get(TUNE_FILTER_AGG, n);
+    const int vw = aos ? 2 : 4;
+    bool vec = ((uintptr_t)a % 16 == 0) && (aos || (uintptr_t)b % 16 == 0);
+    unsigned cap = (unsigned)props().cu_count * (unsigned)(tp.param > 0 ? tp.param : 8);
+    unsigned grid = grid_for(n / vw + 1, 256u * (unsigned)tp.items, cap);
+    int status = QO_OK;
+    uint32_t *cnt = sync_counters(st, &status);
+    if (!cnt) return status;
+    void *ws = workspace(st, (size_t)grid * 16, &status);
+    if (!ws) return status;
+    uint64_t *partials = static_cast<uint64_t *>(ws);
+    uint32_t *done = cnt + 0;
+#define QO_FA(AOSV, U, V) k_filter_agg<AOSV, U, V><<<grid, 256, 0, st>>>(a, b, n, lo, hi, partials, done, d_out)
+    if (aos) {
+        if (!vec) QO_FA(true, 1, false);
+        else if (tp.items >= 8) QO_FA(true, 8, true);
+        else if (tp.items >= 4) QO_FA(true, 4, true);
+        else QO_FA(true, 2, true);
+    } else {
+        if (!vec) QO_FA(false, 1, false);
+        else if (tp.items >= 8) QO_FA(false, 8, true);
+        else if (tp.items >= 4) QO_FA(false, 4, true);
+        else QO_FA(false, 2, true);
+    }
+#undef QO_FA
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stable filter compaction of Records (single pass).  A tile is 256 threads x
+// J loads of VEC records, striped so every load instruction is coalesced.
+// ---------------------------------------------------------------------------
+template <int VEC>
+struct RecVec;
+template <>
+struct RecVec<2> {
+    using T = uint4;
+};
+template <>
+struct RecVec<1> {
+    using T = uint2;
+};
+
+template <int J, int VEC>
+__global__ __launch_bounds__(256) void k_filter_records(const qo_record *__restrict__ rin, size_t n, int32_t lo,
+                                                        int32_t hi, qo_record *__restrict__ rout,
+                                                        uint64_t *status, uint32_t *ticket, uint64_t *d_count,
+                                                        uint32_t num_tiles) {
+    constexpr int NW = 4;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_tot[J * NW];
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t vbase = (size_t)tile * (J * 256);
+    qo_record r[J][VEC];
+    uint32_t c[J], ex[J];
+    uint32_t fl[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
+        const size_t e0 = v * VEC;
+        fl[j] = 0;
+        if (e0 + VEC <= n) {
+            if constexpr (VEC == 2) {
+                uint4 q = ld_stream(reinterpret_cast<const uint4 *>(rin) + v);
+                r[j][0] = {q.x, q.y};
+                r[j][1] = {q.z, q.w};
+            } else {
+                uint2 q = ld_stream(reinterpret_cast<const uint2 *>(rin) + v);
+                r[j][0] = {q.x, q.y};
+            }
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                int32_t key = (int32_t)r[j][k].y;
+                fl[j] |= ((key >= lo) & (key <= hi)) ? (1u << k) : 0u;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                if (e0 + k < n) {
+                    r[j][k] = rin[e0 + k];
+                    int32_t key = (int32_t)r[j][k].y;
+                    fl[j] |= ((key >= lo) & (key <= hi)) ? (1u << k) : 0u;
+                }
+            }
+        }
+        c[j] = __popc(fl[j]);
+    }
+    const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
+    if (threadIdx.x < kWave) {
+        uint64_t e = lookback(status, tile, total);
+        if (threadIdx.x == 0) s_excl = e;
+    }
+    __syncthreads();
+    const uint64_t base = s_excl;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        uint64_t pos = base + ex[j];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k)
+            if (fl[j] & (1u << k)) rout[pos++] = r[j][k];
+    }
+    if (tile == num_tiles - 1 && threadIdx.x == 0) *d_count = base + total;
+}
+
+static size_t filter_ws_bytes(size_t tiles) { return tiles * 8 + 256; }
+
+static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi, qo_record *rout,
+                                 uint64_t *d_count, hipStream_t st) {
+    constexpr int J = 4;
+    const bool vec = ((uintptr_t)rin % 16) == 0;
+    const int VEC = vec ? 2 : 1;
+    const size_t per_tile = (size_t)J * 256 * VEC;
+    const size_t tiles = (n + per_tile - 1) / per_tile;
+    QO_REQUIRE(tiles < 0xFFFFFFFFull, "input too large");
+    int status = QO_OK;
+    void *ws = workspace(st, filter_ws_bytes(tiles), &status);
+    if (!ws) return status;
+    Carver cv(ws);
+    uint64_t *stat = cv.take<uint64_t>(tiles);
+    uint32_t *ticket = cv.take<uint32_t>(1);
+    QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
+    if (vec)
+        k_filter_records<J, 2><<<(unsigned)tiles, 256, 0, st>>>(rin, n, lo, hi, rout, stat, ticket, d_count,
+                                                                (uint32_t)tiles);
+    else
+        k_filter_records<J, 1><<<(unsigned)tiles, 256, 0, st>>>(rin, n, lo, hi, rout, stat, ticket, d_count,
+                                                                (uint32_t)tiles);
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of 32-bit integers (modulo 2^32), single pass.
+// ---------------------------------------------------------------------------
+template <int J, bool VEC>
+__global__ __launch_bounds__(256) void k_scan_excl(const uint32_t *in, uint32_t *out, size_t n, uint64_t *status,
+                                                   uint32_t *ticket) {
+    constexpr int NW = 4;
+    constexpr int VW = VEC ? 4 : 1;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_tot[J * NW];
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t vbase = (size_t)tile * (J * 256);
+    uint32_t x[J][VW];
+    uint32_t c[J], ex[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
+        const size_t e0 = v * VW;
+        if (VEC && e0 + VW <= n) {
+            uint4 q = reinterpret_cast<const uint4 *>(in)[v];
+            x[j][0] = q.x;
+            if constexpr (VW == 4) {
+                x[j][1] = q.y;
+                x[j][2] = q.z;
+                x[j][3] = q.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VW; ++k) x[j][k] = (e0 + k < n) ? in[e0 + k] : 0u;
+        }
+        uint32_t sacc = 0;
+#pragma unroll
+        for (int k = 0; k < VW; ++k) sacc += x[j][k];
+        c[j] = sacc;
+    }
+    const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
+    if (threadIdx.x < kWave) {
+        uint64_t e = lookback(status, tile, total);
+        if (threadIdx.x == 0) s_excl = e;
+    }
+    __syncthreads();
+    const uint32_t base = (uint32_t)s_excl;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
+        const size_t e0 = v * VW;
+        uint32_t run = base + ex[j];
+        uint32_t o[VW];
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            o[k] = run;
+            run += x[j][k];
+        }
+        if (VEC && e0 + VW <= n) {
+            if constexpr (VW == 4) reinterpret_cast<uint4 *>(out)[v] = uint4{o[0], o[1], o[2], o[3]};
+            else out[e0] = o[0];
+        } else {
+#pragma unroll
+            for (int k = 0; k < VW; ++k)
+                if (e0 + k < n) out[e0 + k] = o[k];
+        }
+    }
+}
+
+size_t scan_workspace_bytes(size_t n) {
+    const size_t per_tile = 4 * 256 * 4;
+    const size_t tiles = (n + per_tile - 1) / per_tile + 1;
+    return tiles * 8 + 512;
+}
+
+int launch_scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, void *ws, hipStream_t st) {
+    if (n == 0) return QO_OK;
+    constexpr int J = 4;
+    const bool vec = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
+    const size_t per_tile = (size_t)J * 256 * (vec ? 4 : 1);
+    const size_t tiles = (n + per_tile - 1) / per_tile;
+    QO_REQUIRE(tiles < 0xFFFFFFFFull, "scan input too large");
+    Carver cv(ws);
+    uint64_t *stat = cv.take<uint64_t>(tiles);
+    uint32_t *ticket = cv.take<uint32_t>(1);
+    QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
+    if (vec) k_scan_excl<J, true><<<(unsigned)tiles, 256, 0, st>>>(in, out, n, stat, ticket);
+    else k_scan_excl<J, false><<<(unsigned)tiles, 256, 0, st>>>(in, out, n, stat, ticket);
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_cnt_mu;
+std::map<hipStream_t, uint32_t *> g_cnt;
+}  // namespace
+
+static uint32_t *sync_counters(hipStream_t st, int *status) {
+    *status = QO_OK;
+    std::lock_guard<std::mutex> lk(g_cnt_mu);
+    auto it = g_cnt.find(st);
+    if (it != g_cnt.end()) return it->second;
+    uint32_t *p = nullptr;
+    hipError_t e = hipMalloc(&p, 4096);
+    if (e == hipSuccess) e = hipMemset(p, 0, 4096);
+    if (e != hipSuccess) {
+        *status = set_error(QO_ERR_NO_MEMORY, "counter block: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    g_cnt[st] = p;
+    return p;
+}
+
+}  // namespace qo
+
+using namespace qo;
+
+extern "C" {
+
+int qo_filter_agg(const int32_t *key, const int32_t *val, size_t n, int32_t lo, int32_t hi, int64_t *d_out,
+                  qo_stream s) {
+    QO_ENTER();
+    QO_REQUIRE(d_out && (n == 0 || (key && val)), "null pointer");
+    return launch_filter_agg(false, key, val, n, lo, hi, d_out, stream_of(s));
+}
+
+int qo_filter_agg_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi, int64_t *d_out, qo_stream s) {
+    QO_ENTER();
+    QO_REQUIRE(d_out && (n == 0 || rin), "null pointer");
+    return launch_filter_agg(true, reinterpret_cast<const int32_t *>(rin), nullptr, n, lo, hi, d_out, stream_of(s));
+}
+
+int qo_filter_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi, qo_record *rout, uint64_t *d_count,
+                      qo_stream s) {
+    QO_ENTER();
+    QO_REQUIRE(d_count, "null count pointer");
+    hipStream_t st = stream_of(s);
+    if (n == 0) return launch_fill_u64(d_count, 0, 1, st);
+    QO_REQUIRE(rin && rout, "null pointer");
+    return launch_filter_records(rin, n, lo, hi, rout, d_count, st);
+}
+
+int qo_scan_exclusive_i32(const int32_t *in, int32_t *out, size_t n, qo_stream s) {
+    QO_ENTER();
+    if (n == 0) return QO_OK;
+    QO_REQUIRE(in && out, "null pointer");
+    hipStream_t st = stream_of(s);
+    int status = QO_OK;
+    void *ws = workspace(st, scan_workspace_bytes(n), &status);
+    if (!ws) return status;
+    return launch_scan_excl_u32(reinterpret_cast<const uint32_t *>(in), reinterpret_cast<uint32_t *>(out), n, ws,
+                                st);
+}
+
+}  // extern "C"

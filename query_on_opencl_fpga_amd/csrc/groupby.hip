@@ -1,0 +1,445 @@
+// Group-by aggregation.
+//
+// 1. Hash aggregation (qo_groupby_agg): every block keeps an LDS-resident
+//    partial table (key, COUNT, SUM, MIN, MAX) with linear probing; tuples whose
+//    key does not find an LDS slot within a few probes update a global
+//    open-addressing table directly.  At the end each block merges its LDS table
+//    into the global one; the global entries are compacted, sorted by key and
+//    gathered into the SoA outputs.
+// 2. The reference's sort-based pipeline for drop-in callers: group start
+//    positions over Records sorted by .y (scanGroupLabel_kernel +
+//    groupByImpl_write_kernel, kernel/aggregation/1/group_by_cu_op_dp.cl:293-327,
+//    505-528) and the per-group reduction with the reference's operator
+//    semantics (parallelAggregate_kernel, :547-716).
+#include <algorithm>
+#include <climits>
+
+#include "qo_device.hpp"
+#include "qo_internal.hpp"
+
+namespace qo {
+
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+
+struct alignas(32) GEntry {
+    uint32_t key;
+    int32_t mn;
+    int32_t mx;
+    uint32_t pad;
+    unsigned long long cnt;
+    unsigned long long sum;
+};
+static_assert(sizeof(GEntry) == 32, "entry size");
+
+__global__ void k_gtable_init(GEntry *t, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        GEntry e;
+        e.key = kEmptyKey;
+        e.mn = INT_MAX;
+        e.mx = INT_MIN;
+        e.pad = 0;
+        e.cnt = 0;
+        e.sum = 0;
+        t[i] = e;
+    }
+}
+
+// Entry for `key` in the global table (index mask+1 is reserved for key -1,
+// whose bit pattern is the empty marker).  nullptr when the table is full.
+QO_DEV GEntry *g_entry(GEntry *t, uint64_t mask, uint32_t key) {
+    if (key == kEmptyKey) return &t[mask + 1];
+    uint64_t h = hash32(key) & mask;
+    for (uint64_t p = 0; p <= mask; ++p) {
+        uint32_t k = __hip_atomic_load(&t[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == key) return &t[h];
+        if (k == kEmptyKey) {
+            uint32_t prev = atomicCAS(&t[h].key, kEmptyKey, key);
+            if (prev == kEmptyKey || prev == key) return &t[h];
+        }
+        h = (h + 1) & mask;
+    }
+    return nullptr;
+}
+
+QO_DEV void g_update(GEntry *t, uint64_t mask, uint32_t key, uint64_t cnt, uint64_t sum, int32_t mn, int32_t mx,
+                     uint32_t *overflow) {
+    GEntry *e = g_entry(t, mask, key);
+    if (!e) {
+        atomicExch(overflow, 1u);
+        return;
+    }
+    atomicAdd(&e->cnt, (unsigned long long)cnt);
+    atomicAdd(&e->sum, (unsigned long long)sum);
+    atomicMin(&e->mn, mn);
+    atomicMax(&e->mx, mx);
+}
+
+// LDS slot for `key` within `probes` probes, or -1.
+template <int C>
+QO_DEV int l_slot(uint32_t *lkey, uint32_t key, int probes) {
+    uint32_t h = hash32(key) & (C - 1);
+    for (int p = 0; p < probes; ++p) {
+        uint32_t k = lkey[h];
+        if (k == key) return (int)h;
+        if (k == kEmptyKey) {
+            uint32_t prev = atomicCAS(&lkey[h], kEmptyKey, key);
+            if (prev == kEmptyKey || prev == key) return (int)h;
+        }
+        h = (h + 1) & (C - 1);
+    }
+    return -1;
+}
+
+template <int C>
+struct LTable {
+    uint32_t key[C];
+    uint32_t cnt[C];
+    int32_t mn[C];
+    int32_t mx[C];
+    unsigned long long sum[C];
+};
+
+template <int C, bool VEC>
+__global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict__ key, const int32_t *__restrict__ val,
+                                                      size_t n, GEntry *gt, uint64_t gmask, int probes,
+                                                      uint32_t *overflow) {
+    extern __shared__ unsigned char s_raw[];
+    LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
+    for (int i = threadIdx.x; i < C; i += blockDim.x) {
+        L.key[i] = kEmptyKey;
+        L.cnt[i] = 0;
+        L.mn[i] = INT_MAX;
+        L.mx[i] = INT_MIN;
+        L.sum[i] = 0;
+    }
+    __syncthreads();
+    auto add = [&](int32_t k, int32_t v) {
+        int s = (uint32_t)k == kEmptyKey ? -1 : l_slot<C>(L.key, (uint32_t)k, probes);
+        if (s >= 0) {
+            atomicAdd(&L.cnt[s], 1u);
+            atomicAdd(&L.sum[s], (unsigned long long)(long long)v);
+            atomicMin(&L.mn[s], v);
+            atomicMax(&L.mx[s], v);
+        } else {
+            g_update(gt, gmask, (uint32_t)k, 1, (uint64_t)(int64_t)v, v, v, overflow);
+        }
+    };
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t from = 0;
+    if (VEC) {
+        const size_t nv = n / 4;
+        for (size_t i = tid; i < nv; i += stride) {
+            int4 kk = ld_stream(reinterpret_cast<const int4 *>(key) + i);
+            int4 vv = ld_stream(reinterpret_cast<const int4 *>(val) + i);
+            add(kk.x, vv.x);
+            add(kk.y, vv.y);
+            add(kk.z, vv.z);
+            add(kk.w, vv.w);
+        }
+        from = nv * 4;
+    }
+    for (size_t e = from + tid; e < n; e += stride) add(key[e], val[e]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < C; i += blockDim.x)
+        if (L.cnt[i]) g_update(gt, gmask, L.key[i], L.cnt[i], L.sum[i], L.mn[i], L.mx[i], overflow);
+}
+
+// Occupied global entries -> (key ^ signbit) << 32 | index, appended by atomics
+// (the order is fixed by the sort that follows).
+__global__ void k_gtable_compact(const GEntry *t, size_t entries, uint64_t *list, uint64_t *d_count, size_t cap) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < entries; i += stride) {
+        const GEntry &e = t[i];
+        if (e.cnt == 0) continue;
+        unsigned long long p = atomicAdd(reinterpret_cast<unsigned long long *>(d_count), 1ull);
+        uint32_t key = (i == entries - 1) ? kEmptyKey : e.key;
+        if (p < cap) list[p] = ((uint64_t)(key ^ 0x80000000u) << 32) | (uint32_t)i;
+    }
+}
+
+__global__ void k_gtable_gather(const GEntry *t, const uint64_t *list, const uint64_t *d_count, size_t cap,
+                                const uint32_t *overflow, int32_t *okey, int64_t *ocount, int64_t *osum,
+                                int32_t *omin, int32_t *omax, uint64_t *d_num) {
+    size_t g = d_count[0];
+    size_t lim = g < cap ? g : cap;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < lim; i += stride) {
+        uint64_t v = list[i];
+        const GEntry &e = t[(uint32_t)v];
+        okey[i] = (int32_t)((uint32_t)(v >> 32) ^ 0x80000000u);
+        ocount[i] = (int64_t)e.cnt;
+        osum[i] = (int64_t)e.sum;
+        omin[i] = e.mn;
+        omax[i] = e.mx;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_num = *overflow ? ~uint64_t(0) : g;
+}
+
+struct GroupbyPlan {
+    uint64_t gmask;
+    size_t entries;
+};
+
+static GroupbyPlan groupby_plan(size_t group_capacity) {
+    size_t cap = 64;
+    while (cap < 2 * group_capacity + 1) cap <<= 1;
+    return {cap - 1, cap + 1};
+}
+
+size_t groupby_ws_bytes(size_t group_capacity) {
+    GroupbyPlan p = groupby_plan(group_capacity);
+    return p.entries * sizeof(GEntry) + 1024 + group_capacity * 8 + 256 +
+           radix_workspace_bytes(group_capacity, false) + 256;
+}
+
+int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity, int32_t *okey,
+                   int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax, uint64_t *d_num, void *ws,
+                   hipStream_t st) {
+    GroupbyPlan p = groupby_plan(group_capacity);
+    Carver cv(ws);
+    GEntry *gt = cv.take<GEntry>(p.entries);
+    uint64_t *counters = cv.take<uint64_t>(4);
+    uint64_t *list = cv.take<uint64_t>(group_capacity ? group_capacity : 1);
+    void *rws = cv.take<char>(radix_workspace_bytes(group_capacity, false));
+    uint32_t *overflow = reinterpret_cast<uint32_t *>(counters + 1);
+    QO_TRY(launch_fill_u64(counters, 0, 4, st));
+    const unsigned ig = grid_for(p.entries, 256, (unsigned)props().cu_count * 16);
+    k_gtable_init<<<ig, 256, 0, st>>>(gt, p.entries);
+    QO_LAUNCHED();
+    if (n) {
+        TuneParams tp = tune_get(TUNE_GROUPBY, n);
+        const bool vec = ((uintptr_t)key % 16 == 0) && ((uintptr_t)val % 16 == 0);
+        int C = tp.items >= 4096 ? 4096 : 2048;
+        const size_t lds4096 = sizeof(LTable<4096>);
+        if (C == 4096 && (size_t)props().lds_per_block < lds4096) C = 2048;
+        const int block = 512;
+        const unsigned grid = grid_for(n, block * 4, (unsigned)props().cu_count * (unsigned)std::max(1, tp.param));
+        // Low cardinality: the LDS table holds every group.  High cardinality:
+        // it caches the keys seen first and the rest go straight to global.
+        const int probes = group_capacity * 2 <= (size_t)C ? C : 8;
+        if (C == 4096) {
+            const size_t lds = sizeof(LTable<4096>);
+            if (vec)
+                k_groupby_hash<4096, true><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+            else
+                k_groupby_hash<4096, false><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+        } else {
+            const size_t lds = sizeof(LTable<2048>);
+            if (vec)
+                k_groupby_hash<2048, true><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+            else
+                k_groupby_hash<2048, false><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+        }
+        QO_LAUNCHED();
+    }
+    k_gtable_compact<<<ig, 256, 0, st>>>(gt, p.entries, list, counters, group_capacity);
+    QO_LAUNCHED();
+    if (group_capacity > 1) QO_TRY(launch_radix_sort_u64_dev(list, nullptr, counters, group_capacity, 32, 64, rws, st));
+    const unsigned gg = grid_for(group_capacity, 256, (unsigned)props().cu_count * 8);
+    k_gtable_gather<<<gg, 256, 0, st>>>(gt, list, counters, group_capacity, overflow, okey, ocount, osum, omin, omax,
+                                        d_num);
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sort-based pipeline (reference semantics).
+// ---------------------------------------------------------------------------
+template <int J>
+__global__ __launch_bounds__(256) void k_group_startpos(const qo_record *__restrict__ s, size_t n, int32_t *startPos,
+                                                        uint64_t *status, uint32_t *ticket, uint64_t *d_num,
+                                                        uint32_t num_tiles) {
+    constexpr int NW = 4;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_tot[J * NW];
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t base_i = (size_t)tile * (J * 256);
+    uint32_t c[J], ex[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        size_t i = base_i + (size_t)j * 256 + threadIdx.x;
+        c[j] = 0;
+        if (i < n) c[j] = (i == 0) || (s[i].y != s[i - 1].y);
+    }
+    const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
+    if (threadIdx.x < kWave) {
+        uint64_t e = lookback(status, tile, total);
+        if (threadIdx.x == 0) s_excl = e;
+    }
+    __syncthreads();
+    const uint64_t base = s_excl;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+        if (c[j]) startPos[base + ex[j]] = (int32_t)(base_i + (size_t)j * 256 + threadIdx.x);
+    if (tile == num_tiles - 1 && threadIdx.x == 0) *d_num = base + total;
+}
+
+// Work units of one wave: group g, chunk q covers startPos[g] + q*kAggChunk.
+constexpr int kAggChunk = 512;
+
+// Range of group g, clamped to the array (start positions come from the caller).
+QO_DEV void group_range(const int32_t *startPos, uint32_t g, uint32_t G, size_t n, int64_t &s, int64_t &e) {
+    s = startPos[g];
+    e = (g == G - 1) ? (int64_t)n : startPos[g + 1];
+    if (s < 0) s = 0;
+    if (e > (int64_t)n) e = (int64_t)n;
+}
+
+__global__ void k_agg_chunks(const int32_t *startPos, uint32_t G, size_t n, uint32_t *chunks) {
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    int64_t s, e;
+    group_range(startPos, (uint32_t)g, G, n, s, e);
+    int64_t len = e - s;
+    chunks[g] = len > 0 ? (uint32_t)((len + kAggChunk - 1) / kAggChunk) : 0u;
+}
+
+QO_DEV int32_t agg_identity(int op) { return op == QO_REDUCE_MIN ? (1 << 30) : 0; }
+
+__global__ __launch_bounds__(256) void k_agg_groups(const qo_record *__restrict__ s, size_t n,
+                                                    const int32_t *__restrict__ startPos, uint32_t G,
+                                                    const uint32_t *__restrict__ cofs, const uint32_t *__restrict__ chunks,
+                                                    int op, int32_t *acc) {
+    const uint32_t total = cofs[G - 1] + chunks[G - 1];
+    const unsigned lane = lane_id();
+    const size_t waves = (size_t)gridDim.x * (blockDim.x / kWave);
+    for (size_t w = (size_t)blockIdx.x * (blockDim.x / kWave) + wave_id(); w < total; w += waves) {
+        // Group of work unit w: the last g with cofs[g] <= w.  An empty group
+        // shares its cofs with the following group, so the last such g is the
+        // non-empty group that owns unit w.
+        uint32_t lo = 0, hi = G - 1;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (cofs[mid] <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t g = lo;
+        int64_t gs, ge;
+        group_range(startPos, g, G, n, gs, ge);
+        const int64_t a = gs + (int64_t)(w - cofs[g]) * kAggChunk;
+        const int64_t b = a + kAggChunk < ge ? a + kAggChunk : ge;
+        int32_t local = agg_identity(op);
+        uint32_t usum = 0;
+        for (int64_t i = a + lane; i < b; i += kWave) {
+            int32_t v = (int32_t)s[i].x;
+            if (op == QO_REDUCE_MAX) local = v > local ? v : local;
+            else if (op == QO_REDUCE_MIN) local = v < local ? v : local;
+            else usum += (uint32_t)v;
+        }
+        if (op == QO_REDUCE_MAX) {
+            for (int o = 32; o > 0; o >>= 1) local = max(local, __shfl_xor(local, o, kWave));
+            if (lane == 0) atomicMax(&acc[g], local);
+        } else if (op == QO_REDUCE_MIN) {
+            for (int o = 32; o > 0; o >>= 1) local = min(local, __shfl_xor(local, o, kWave));
+            if (lane == 0) atomicMin(&acc[g], local);
+        } else {
+            usum = wave_sum(usum);
+            if (lane == 0) atomicAdd(reinterpret_cast<uint32_t *>(&acc[g]), usum);
+        }
+    }
+}
+
+__global__ void k_agg_average(const int32_t *startPos, uint32_t G, size_t n, const int32_t *sum, int32_t *out) {
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    // Count as the reference divides by it: end - start, unclamped.
+    int64_t s = startPos[g];
+    int64_t e = (g == G - 1) ? (int64_t)n : startPos[g + 1];
+    int32_t cnt = (int32_t)(e - s);
+    out[g] = cnt > 0 ? sum[g] / cnt : 0;
+}
+
+size_t aggregate_ws_bytes(size_t G) { return G * 12 + scan_workspace_bytes(G) + 1024; }
+
+int launch_aggregate_groups(const qo_record *s, size_t n, const int32_t *startPos, size_t G, int op, int32_t *out,
+                            void *ws, hipStream_t st) {
+    if (G == 0) return QO_OK;
+    Carver cv(ws);
+    uint32_t *chunks = cv.take<uint32_t>(G);
+    uint32_t *cofs = cv.take<uint32_t>(G);
+    int32_t *sum = cv.take<int32_t>(G);
+    void *sws = cv.take<char>(scan_workspace_bytes(G));
+    int32_t *acc = op == QO_REDUCE_AVERAGE ? sum : out;
+    QO_TRY(launch_fill_u32(acc, op == QO_REDUCE_MIN ? (1u << 30) : 0u, G, st));
+    const unsigned gb = grid_for(G, 256, 1u << 30);
+    k_agg_chunks<<<gb, 256, 0, st>>>(startPos, (uint32_t)G, n, chunks);
+    QO_LAUNCHED();
+    QO_TRY(launch_scan_excl_u32(chunks, cofs, G, sws, st));
+    const unsigned grid = grid_for((size_t)n / kAggChunk + G, 4, (unsigned)props().cu_count * 16);
+    k_agg_groups<<<grid, 256, 0, st>>>(s, n, startPos, (uint32_t)G, cofs, chunks, op, acc);
+    QO_LAUNCHED();
+    if (op == QO_REDUCE_AVERAGE) {
+        k_agg_average<<<gb, 256, 0, st>>>(startPos, (uint32_t)G, n, sum, out);
+        QO_LAUNCHED();
+    }
+    return QO_OK;
+}
+
+int launch_group_startpos(const qo_record *sorted, size_t n, int32_t *startPos, uint64_t *d_num, void *ws,
+                          hipStream_t st) {
+    if (n == 0) return launch_fill_u64(d_num, 0, 1, st);
+    constexpr int J = 8;
+    const size_t tiles = (n + J * 256 - 1) / (J * 256);
+    Carver cv(ws);
+    uint64_t *stat = cv.take<uint64_t>(tiles);
+    uint32_t *ticket = cv.take<uint32_t>(1);
+    QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
+    k_group_startpos<J><<<(unsigned)tiles, 256, 0, st>>>(sorted, n, startPos, stat, ticket, d_num, (uint32_t)tiles);
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
+size_t group_startpos_ws_bytes(size_t n) { return (n / (8 * 256) + 2) * 8 + 512; }
+
+}  // namespace qo
+
+using namespace qo;
+
+extern "C" {
+
+int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity, int32_t *okey,
+                   int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax, uint64_t *d_numGroups,
+                   qo_stream s) {
+    QO_ENTER();
+    QO_REQUIRE(d_numGroups && (n == 0 || (key && val)), "null pointer");
+    QO_REQUIRE(group_capacity == 0 || (okey && ocount && osum && omin && omax), "null output column");
+    QO_REQUIRE(group_capacity < (size_t(1) << 31), "group capacity too large");
+    hipStream_t st = stream_of(s);
+    int status = QO_OK;
+    void *ws = workspace(st, groupby_ws_bytes(group_capacity), &status);
+    if (!ws) return status;
+    return launch_groupby(key, val, n, group_capacity, okey, ocount, osum, omin, omax, d_numGroups, ws, st);
+}
+
+int qo_group_startpos(const qo_record *sorted, size_t n, int32_t *startPos, uint64_t *d_numGroups, qo_stream s) {
+    QO_ENTER();
+    QO_REQUIRE(d_numGroups && (n == 0 || (sorted && startPos)), "null pointer");
+    hipStream_t st = stream_of(s);
+    int status = QO_OK;
+    void *ws = workspace(st, group_startpos_ws_bytes(n), &status);
+    if (!ws) return status;
+    return launch_group_startpos(sorted, n, startPos, d_numGroups, ws, st);
+}
+
+int qo_aggregate_groups(const qo_record *sorted, size_t n, const int32_t *startPos, size_t numGroups, int op,
+                        int32_t *out, qo_stream s) {
+    QO_ENTER();
+    QO_REQUIRE(op >= QO_REDUCE_SUM && op <= QO_REDUCE_AVERAGE, "unknown operator %d", op);
+    if (numGroups == 0) return QO_OK;
+    QO_REQUIRE(sorted && startPos && out && numGroups < 0xFFFFFFFFull, "bad aggregate arguments");
+    hipStream_t st = stream_of(s);
+    int status = QO_OK;
+    void *ws = workspace(st, aggregate_ws_bytes(numGroups), &status);
+    if (!ws) return status;
+    return launch_aggregate_groups(sorted, n, startPos, numGroups, op, out, ws, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+// Device-side primitives for gfx950 (wave64): lane masks, wave scans and
+// reductions, block-level striped offsets and a decoupled look-back across
+// tiles handed out by a ticket counter.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define QO_DEV __device__ __forceinline__
+
+namespace qo {
+
+constexpr int kWave = 64;
+
+QO_DEV unsigned lane_id() { return __lane_id(); }
+QO_DEV unsigned wave_id() { return threadIdx.x / kWave; }
+QO_DEV uint64_t lanemask_lt() { return (uint64_t(1) << lane_id()) - 1; }
+QO_DEV uint64_t ballot(bool p) { return __ballot(p); }
+QO_DEV unsigned popc(uint64_t m) { return __popcll(m); }
+
+template <class T>
+QO_DEV T wave_incl_scan(T v) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        T t = __shfl_up(v, o, kWave);
+        if (lane_id() >= (unsigned)o) v += t;
+    }
+    return v;
+}
+
+template <class T>
+QO_DEV T wave_sum(T v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Streaming loads that should not displace cache-resident data.  The HIP vector
+// structs are loaded through the equivalent native clang vector types.
+typedef int32_t nv4i __attribute__((ext_vector_type(4)));
+typedef uint32_t nv4u __attribute__((ext_vector_type(4)));
+typedef uint32_t nv2u __attribute__((ext_vector_type(2)));
+
+template <class T>
+QO_DEV T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
+QO_DEV int4 ld_stream(const int4 *p) {
+    nv4i v = __builtin_nontemporal_load(reinterpret_cast<const nv4i *>(p));
+    return int4{v.x, v.y, v.z, v.w};
+}
+QO_DEV uint4 ld_stream(const uint4 *p) {
+    nv4u v = __builtin_nontemporal_load(reinterpret_cast<const nv4u *>(p));
+    return uint4{v.x, v.y, v.z, v.w};
+}
+QO_DEV uint2 ld_stream(const uint2 *p) {
+    nv2u v = __builtin_nontemporal_load(reinterpret_cast<const nv2u *>(p));
+    return uint2{v.x, v.y};
+}
+template <class T>
+QO_DEV void st_stream(T *p, T v) { __builtin_nontemporal_store(v, p); }
+
+QO_DEV uint64_t ld_agent(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+QO_DEV void st_agent(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Striped tile offsets.  A block of NW waves holds J groups of per-lane counts
+// c[j]; group j of thread t precedes group j of thread t+1 and all of group j+1
+// (order (j, wave, lane)).  On return excl[j] is the exclusive offset of the
+// lane's items inside the tile and the tile total is returned.  s_tot must hold
+// J*NW entries; the caller must __syncthreads() before reusing it.
+template <int J, int NW, class T>
+QO_DEV T tile_offsets(const T (&c)[J], T (&excl)[J], T *s_tot) {
+    const unsigned w = wave_id();
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        T inc = wave_incl_scan(c[j]);
+        excl[j] = inc - c[j];
+        if (lane_id() == kWave - 1) s_tot[j * NW + w] = inc;
+    }
+    __syncthreads();
+    T run = 0;
+    T mine[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) {
+            T v = s_tot[j * NW + ww];
+            if (ww == (int)w) mine[j] = run;
+            run += v;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) excl[j] += mine[j];
+    return run;
+}
+
+// Decoupled look-back status word: [63:62] flag (0 empty, 1 aggregate,
+// 2 inclusive prefix), [61:0] value.
+constexpr uint64_t kFlagAgg = uint64_t(1) << 62;
+constexpr uint64_t kFlagInc = uint64_t(2) << 62;
+constexpr uint64_t kValMask = (uint64_t(1) << 62) - 1;
+
+// Called by ONE whole wave of the block that owns `tile`.  Publishes the tile
+// aggregate, accumulates predecessors 64 at a time and publishes the inclusive
+// prefix.  Returns the exclusive prefix (sum of all earlier tiles, modulo 2^62).
+// Tiles must be numbered by a ticket counter, so every predecessor is already
+// resident and the spin always ends.
+QO_DEV uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t aggregate) {
+    const unsigned lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0) st_agent(&status[0], kFlagInc | (aggregate & kValMask));
+        return 0;
+    }
+    if (lane == 0) st_agent(&status[tile], kFlagAgg | (aggregate & kValMask));
+    uint64_t excl = 0;
+    int64_t base = (int64_t)tile - 1;
+    while (true) {
+        int64_t idx = base - (int64_t)lane;
+        uint64_t st = (idx >= 0) ? ld_agent(&status[idx]) : kFlagInc;
+        uint64_t flag = st >> 62;
+        uint64_t inc_mask = ballot(flag == 2);
+        uint64_t empty_mask = ballot(flag == 0);
+        unsigned first_inc = inc_mask ? (unsigned)__builtin_ctzll(inc_mask) : 64u;
+        uint64_t relevant = first_inc >= 63 ? ~uint64_t(0) : ((uint64_t(2) << first_inc) - 1);
+        if (empty_mask & relevant) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t v = (lane <= first_inc) ? (st & kValMask) : 0;
+        excl += wave_sum(v);
+        if (first_inc < 64) break;
+        base -= kWave;
+    }
+    excl &= kValMask;
+    if (lane == 0) st_agent(&status[tile], kFlagInc | ((excl + aggregate) & kValMask));
+    return excl;
+}
+
+// 32-bit finaliser of MurmurHash3 (Appleby): spreads sequential keys.
+QO_DEV uint32_t hash32(uint32_t k) {
+    k ^= k >> 16;
+    k *= 0x85EBCA6Bu;
+    k ^= k >> 13;
+    k *= 0xC2B2AE35u;
+    k ^= k >> 16;
+    return k;
+}
+
+}  // namespace qo

@@ -1,0 +1,128 @@
+// Internal host-side runtime shared by every translation unit of libqo.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/qo.h"
+
+namespace qo {
+
+// Error state (thread-local message, never exits).
+int set_error(int code, const char *fmt, ...);
+bool consume_injected_fault();
+
+struct DeviceProps {
+    int device = 0;
+    int cu_count = 256;
+    int lds_per_block = 65536;
+    size_t hbm_bytes = 0;
+    char name[64] = {0};
+    char arch[32] = {0};
+};
+const DeviceProps &props();
+int ensure_init();
+
+// The stream to launch on: a caller stream, or the library's per-thread stream.
+hipStream_t stream_of(qo_stream s);
+
+// Per-stream grow-only workspace.  Operations on one stream are ordered by the
+// stream, so one workspace per stream is never used by two operations at once.
+void *workspace(hipStream_t s, size_t bytes, int *status);
+
+// Carves aligned sub-buffers out of a workspace; a null base only measures.
+struct Carver {
+    char *base;
+    size_t off = 0;
+    explicit Carver(void *b) : base(static_cast<char *>(b)) {}
+    template <class T>
+    T *take(size_t count, size_t align = 256) {
+        off = (off + align - 1) / align * align;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+// Autotuned launch parameters.
+struct TuneParams {
+    int block;
+    int items;
+    int param;
+};
+enum TuneOp { TUNE_FILTER_AGG = 0, TUNE_FILTER = 1, TUNE_PROBE = 2, TUNE_GROUPBY = 3, TUNE_SORT = 4, TUNE_NUM_OPS = 5 };
+TuneParams tune_get(int op, size_t n);
+
+inline int size_class(size_t n) {
+    int c = 0;
+    while (c < 63 && (size_t(1) << (c + 1)) <= n) ++c;
+    return c;
+}
+
+inline unsigned grid_for(size_t work, unsigned per_block, unsigned cap) {
+    size_t g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<unsigned>(g);
+}
+
+// Internal launchers used by several operators (all asynchronous on `st`).
+int launch_fill_u32(void *dst, uint32_t value, size_t count, hipStream_t st);
+int launch_fill_u64(void *dst, uint64_t value, size_t count, hipStream_t st);
+
+// Exclusive scan with caller-provided workspace (see scan_workspace_bytes).
+size_t scan_workspace_bytes(size_t n);
+int launch_scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, void *ws, hipStream_t st);
+
+// LSD radix sort of 64-bit keys with optional 32-bit values over bits [lo_bit, hi_bit).
+size_t radix_workspace_bytes(size_t n, bool with_vals);
+int launch_radix_sort_u64(uint64_t *keys, uint32_t *vals, size_t n, int lo_bit, int hi_bit, void *ws,
+                          hipStream_t st);
+// Same, reading the element count from device memory (grid sized for `capacity`).
+int launch_radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, const uint64_t *d_n, size_t capacity,
+                              int lo_bit, int hi_bit, void *ws, hipStream_t st);
+
+// Sort of join rows by (key, s_payload) with caller workspace.
+size_t sort_join_rows_ws_bytes(size_t m);
+int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void *ws, hipStream_t st);
+
+// Join table helpers (join.hip).
+size_t join_table_capacity_log2(size_t table_bytes);
+int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
+                      hipStream_t st);
+size_t join_probe_ws_bytes(size_t ns);
+int launch_join_probe(const void *table, const int32_t *skey, const int32_t *spay, const int32_t *sfilt,
+                      size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay, int32_t *ospay,
+                      size_t cap, uint64_t *d_count, void *ws, hipStream_t st);
+
+}  // namespace qo
+
+#define QO_HIP(call)                                                                                \
+    do {                                                                                            \
+        hipError_t qo_e_ = (call);                                                                  \
+        if (qo_e_ != hipSuccess)                                                                    \
+            return ::qo::set_error(QO_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,           \
+                                   hipGetErrorString(qo_e_));                                       \
+    } while (0)
+
+#define QO_LAUNCHED() QO_HIP(hipGetLastError())
+
+#define QO_REQUIRE(cond, ...)                                                                       \
+    do {                                                                                            \
+        if (!(cond)) return ::qo::set_error(QO_ERR_INVALID_ARG, __VA_ARGS__);                      \
+    } while (0)
+
+#define QO_TRY(expr)                                                                                \
+    do {                                                                                            \
+        int qo_rc_ = (expr);                                                                        \
+        if (qo_rc_ != QO_OK) return qo_rc_;                                                         \
+    } while (0)
+
+#define QO_ENTER()                                                                                  \
+    do {                                                                                            \
+        QO_TRY(::qo::ensure_init());                                                                \
+        if (::qo::consume_injected_fault())                                                         \
+            return ::qo::set_error(QO_ERR_INJECTED, "injected fault (qo_debug_inject_fault)");     \
+    } while (0)

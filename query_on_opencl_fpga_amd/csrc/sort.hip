@@ -1,0 +1,283 @@
+// Sorting: a stable LSD radix sort over 64-bit keys (8-bit digits) with an
+// optional 32-bit value, three launches per digit: per-tile digit histograms,
+// one exclusive scan over the digit-major histogram, and a scatter that ranks
+// keys with wave-level match masks (8 ballots per key), stages the tile in LDS
+// in digit order and writes contiguous runs.
+//
+// Reference path replaced: radixSortImpl_sm (host/host/testSort.cpp:24-136)
+// driving bitonicSortShared / bitonicMergeGlobal (kernel/aggregation/1/
+// group_by_cu_op_dp.cl:89-291): O(log^2 N) in-place passes over the array.
+// Here a Record sort is 4 passes over .y; ties keep input order (stable).
+#include "qo_device.hpp"
+#include "qo_internal.hpp"
+
+namespace qo {
+
+constexpr int kSortItems = 16;
+constexpr int kSortTile = 256 * kSortItems;
+
+// Lanes of this wave that are active and hold the same 8-bit digit.
+QO_DEV uint64_t match_digit(uint32_t d, bool valid) {
+    uint64_t m = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        uint64_t bb = ballot((d >> b) & 1u);
+        m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    return m;
+}
+
+QO_DEV uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w /* 4 */) {
+    uint32_t inc = wave_incl_scan(v);
+    if (lane_id() == kWave - 1) s_w[wave_id()] = inc;
+    __syncthreads();
+    uint32_t add = 0;
+    for (unsigned w = 0; w < wave_id(); ++w) add += s_w[w];
+    return inc - v + add;
+}
+
+// Element count of a device-count sort, clamped to the buffer capacity.
+QO_DEV size_t tile_limit(const uint64_t *d_n, size_t cap) {
+    size_t v = d_n ? (size_t)*d_n : cap;
+    return v < cap ? v : cap;
+}
+
+template <bool DEVN>
+__global__ __launch_bounds__(256) void k_radix_hist(const uint64_t *__restrict__ kin, size_t n_host,
+                                                    const uint64_t *d_n, int shift, uint32_t *hist,
+                                                    uint32_t num_tiles) {
+    __shared__ uint32_t s_h[4][256];
+    const size_t n = DEVN ? tile_limit(d_n, n_host) : n_host;
+    for (int i = threadIdx.x; i < 1024; i += 256) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = blockIdx.x;
+    const unsigned w = wave_id(), lane = lane_id();
+    const size_t wbase = (size_t)tile * kSortTile + (size_t)w * (kSortTile / 4);
+    if ((size_t)tile * kSortTile < n) {
+#pragma unroll 4
+        for (int j = 0; j < kSortItems; ++j) {
+            size_t idx = wbase + (size_t)j * kWave + lane;
+            bool valid = idx < n;
+            uint32_t d = valid ? (uint32_t)(kin[idx] >> shift) & 255u : 0u;
+            uint64_t m = match_digit(d, valid);
+            if (valid && popc(m & lanemask_lt()) == 0) s_h[w][d] += popc(m);
+        }
+    }
+    __syncthreads();
+    const unsigned d = threadIdx.x;
+    hist[(size_t)d * num_tiles + tile] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+}
+
+template <bool HAS_VAL, bool DEVN>
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restrict__ kin,
+                                                       const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+                                                       uint32_t *__restrict__ vout, size_t n_host, const uint64_t *d_n,
+                                                       int shift, const uint32_t *__restrict__ offs,
+                                                       uint32_t num_tiles) {
+    __shared__ uint32_t s_cnt[4][256];
+    __shared__ uint32_t s_dpre[256];
+    __shared__ uint32_t s_gofs[256];
+    __shared__ uint32_t s_w[4];
+    __shared__ uint64_t s_k[kSortTile];
+    __shared__ uint32_t s_v[HAS_VAL ? kSortTile : 1];
+    const size_t n = DEVN ? tile_limit(d_n, n_host) : n_host;
+    const uint32_t tile = blockIdx.x;
+    if ((size_t)tile * kSortTile >= n) return;
+    for (int i = threadIdx.x; i < 1024; i += 256) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+    const unsigned w = wave_id(), lane = lane_id();
+    const size_t wbase = (size_t)tile * kSortTile + (size_t)w * (kSortTile / 4);
+    uint64_t k[kSortItems];
+    uint32_t v[kSortItems];
+    uint32_t rank[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        size_t idx = wbase + (size_t)j * kWave + lane;
+        bool valid = idx < n;
+        k[j] = valid ? kin[idx] : ~uint64_t(0);
+        if (HAS_VAL) v[j] = valid ? vin[idx] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        size_t idx = wbase + (size_t)j * kWave + lane;
+        bool valid = idx < n;
+        uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+        uint64_t m = match_digit(d, valid);
+        uint32_t pre = s_cnt[w][d];
+        uint32_t r = popc(m & lanemask_lt());
+        if (valid && r == 0) s_cnt[w][d] = pre + popc(m);
+        rank[j] = pre + r;
+    }
+    __syncthreads();
+    {
+        const unsigned d = threadIdx.x;
+        uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
+        uint32_t tot = c0 + c1 + c2 + c3;
+        s_gofs[d] = offs[(size_t)d * num_tiles + tile];
+        uint32_t ex = block_excl_scan256(tot, s_w);
+        s_dpre[d] = ex;
+        s_cnt[0][d] = 0;
+        s_cnt[1][d] = c0;
+        s_cnt[2][d] = c0 + c1;
+        s_cnt[3][d] = c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        size_t idx = wbase + (size_t)j * kWave + lane;
+        if (idx < n) {
+            uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+            uint32_t p = s_dpre[d] + s_cnt[w][d] + rank[j];
+            s_k[p] = k[j];
+            if (HAS_VAL) s_v[p] = v[j];
+        }
+    }
+    __syncthreads();
+    const size_t tstart = (size_t)tile * kSortTile;
+    const uint32_t tcount = (uint32_t)((n - tstart) < (size_t)kSortTile ? (n - tstart) : (size_t)kSortTile);
+    for (uint32_t i = threadIdx.x; i < tcount; i += 256) {
+        uint64_t kk = s_k[i];
+        uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        size_t gp = (size_t)s_gofs[d] + (i - s_dpre[d]);
+        kout[gp] = kk;
+        if (HAS_VAL) vout[gp] = s_v[i];
+    }
+}
+
+static size_t radix_tiles(size_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+size_t radix_workspace_bytes(size_t n, bool with_vals) {
+    size_t tiles = radix_tiles(n) + 1;
+    size_t b = 0;
+    b += n * 8 + 256;                    // alternate keys
+    if (with_vals) b += n * 4 + 256;     // alternate values
+    b += tiles * 256 * 4 + 256;          // histogram / offsets
+    b += scan_workspace_bytes(tiles * 256) + 256;
+    return b;
+}
+
+static int radix_impl(uint64_t *keys, uint32_t *vals, size_t n, const uint64_t *d_n, int lo_bit, int hi_bit,
+                      void *ws, hipStream_t st) {
+    if (n == 0) return QO_OK;
+    QO_REQUIRE(lo_bit >= 0 && hi_bit <= 64 && lo_bit < hi_bit, "bad bit range");
+    const size_t tiles = radix_tiles(n);
+    QO_REQUIRE(tiles * 256 < 0xFFFFFFFFull && n < 0xFFFFFFFFull, "sort input too large");
+    Carver cv(ws);
+    uint64_t *kalt = cv.take<uint64_t>(n);
+    uint32_t *valt = vals ? cv.take<uint32_t>(n) : nullptr;
+    uint32_t *hist = cv.take<uint32_t>(tiles * 256);
+    void *scan_ws = cv.take<char>(scan_workspace_bytes(tiles * 256));
+    uint64_t *ksrc = keys, *kdst = kalt;
+    uint32_t *vsrc = vals, *vdst = valt;
+    int passes = 0;
+    for (int sh = lo_bit; sh < hi_bit; sh += 8, ++passes) {
+        if (d_n) k_radix_hist<true><<<(unsigned)tiles, 256, 0, st>>>(ksrc, n, d_n, sh, hist, (uint32_t)tiles);
+        else k_radix_hist<false><<<(unsigned)tiles, 256, 0, st>>>(ksrc, n, d_n, sh, hist, (uint32_t)tiles);
+        QO_LAUNCHED();
+        QO_TRY(launch_scan_excl_u32(hist, hist, tiles * 256, scan_ws, st));
+        if (vals) {
+            if (d_n)
+                k_radix_scatter<true, true><<<(unsigned)tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, n, d_n, sh, hist,
+                                                                           (uint32_t)tiles);
+            else
+                k_radix_scatter<true, false><<<(unsigned)tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, n, d_n, sh,
+                                                                            hist, (uint32_t)tiles);
+        } else {
+            if (d_n)
+                k_radix_scatter<false, true><<<(unsigned)tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, n, d_n,
+                                                                            sh, hist, (uint32_t)tiles);
+            else
+                k_radix_scatter<false, false><<<(unsigned)tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, n, d_n,
+                                                                             sh, hist, (uint32_t)tiles);
+        }
+        QO_LAUNCHED();
+        std::swap(ksrc, kdst);
+        std::swap(vsrc, vdst);
+    }
+    if (passes & 1) {
+        QO_HIP(hipMemcpyAsync(keys, ksrc, n * 8, hipMemcpyDeviceToDevice, st));
+        if (vals) QO_HIP(hipMemcpyAsync(vals, vsrc, n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    return QO_OK;
+}
+
+int launch_radix_sort_u64(uint64_t *keys, uint32_t *vals, size_t n, int lo_bit, int hi_bit, void *ws,
+                          hipStream_t st) {
+    return radix_impl(keys, vals, n, nullptr, lo_bit, hi_bit, ws, st);
+}
+
+int launch_radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, const uint64_t *d_n, size_t capacity, int lo_bit,
+                              int hi_bit, void *ws, hipStream_t st) {
+    return radix_impl(keys, vals, capacity, d_n, lo_bit, hi_bit, ws, st);
+}
+
+// (key, s_payload) as signed int32 -> order-preserving unsigned 64-bit key.
+__global__ void k_pack_join(const int32_t *key, const int32_t *spay, const int32_t *rpay, size_t m, uint64_t *k64,
+                            uint32_t *v32) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < m; i += stride) {
+        k64[i] = ((uint64_t)((uint32_t)key[i] ^ 0x80000000u) << 32) | ((uint32_t)spay[i] ^ 0x80000000u);
+        v32[i] = (uint32_t)rpay[i];
+    }
+}
+
+__global__ void k_unpack_join(const uint64_t *k64, const uint32_t *v32, size_t m, int32_t *key, int32_t *spay,
+                              int32_t *rpay) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < m; i += stride) {
+        uint64_t k = k64[i];
+        key[i] = (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
+        spay[i] = (int32_t)((uint32_t)k ^ 0x80000000u);
+        rpay[i] = (int32_t)v32[i];
+    }
+}
+
+int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void *ws, hipStream_t st) {
+    if (m < 2) return QO_OK;
+    Carver cv(ws);
+    uint64_t *k64 = cv.take<uint64_t>(m);
+    uint32_t *v32 = cv.take<uint32_t>(m);
+    void *rws = cv.take<char>(radix_workspace_bytes(m, true));
+    unsigned g = grid_for(m, 256, (unsigned)props().cu_count * 8);
+    k_pack_join<<<g, 256, 0, st>>>(key, spay, rpay, m, k64, v32);
+    QO_LAUNCHED();
+    QO_TRY(launch_radix_sort_u64(k64, v32, m, 0, 64, rws, st));
+    k_unpack_join<<<g, 256, 0, st>>>(k64, v32, m, key, spay, rpay);
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
+size_t sort_join_rows_ws_bytes(size_t m) { return m * 12 + 1024 + radix_workspace_bytes(m, true); }
+
+}  // namespace qo
+
+using namespace qo;
+
+extern "C" {
+
+int qo_sort_records(qo_record *data, size_t n, qo_stream s) {
+    QO_ENTER();
+    if (n < 2) return QO_OK;
+    QO_REQUIRE(data && ((uintptr_t)data % 8) == 0, "records must be 8-byte aligned device memory");
+    hipStream_t st = stream_of(s);
+    int status = QO_OK;
+    void *ws = workspace(st, radix_workspace_bytes(n, false), &status);
+    if (!ws) return status;
+    // A Record {x, y} read as little-endian uint64 has y in bits 32..63.
+    return launch_radix_sort_u64(reinterpret_cast<uint64_t *>(data), nullptr, n, 32, 64, ws, st);
+}
+
+int qo_sort_join_rows(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, qo_stream s) {
+    QO_ENTER();
+    if (m < 2) return QO_OK;
+    QO_REQUIRE(key && rpay && spay, "null pointer");
+    hipStream_t st = stream_of(s);
+    int status = QO_OK;
+    void *ws = workspace(st, sort_join_rows_ws_bytes(m), &status);
+    if (!ws) return status;
+    return sort_join_rows_ws(key, rpay, spay, m, ws, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+"""Host-side operator interface mirroring the reference's entry points.
+
+Records are numpy ``uint32`` arrays of shape ``(n, 2)``: column 0 is ``.x``
+(value / row id), column 1 is ``.y`` (key), the reference's ``Record = uint2``
+(host/host/common.h:27-28).  Functions named like the reference
+(``CL_RangeSelection``, ``CL_RadixSort``, ``CL_GroupBy``,
+``CL_agg_*_afterGroupBy``, ``buildHashTable`` / ``probeHashTable``) call the
+reference-named C entry points of libqo; the lower-case functions call the
+native column API.  Every call runs on the GPU through libqo; nothing here
+computes a result on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceBuffer
+
+REDUCE_SUM, REDUCE_MAX, REDUCE_MIN, REDUCE_AVERAGE = 0, 1, 2, 3
+TEST_MAX = 1 << 30
+
+
+def _records(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    if a.ndim != 2 or a.shape[1] != 2:
+        raise ValueError("records must have shape (n, 2) = (x, y)")
+    return a
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _free_c(ptr) -> None:
+    C.CDLL(None).free(C.c_void_p(ptr))
+
+
+# ---------------------------------------------------------------------------
+# Reference-named host-pointer entry points (host/host/Selection.cpp:19,
+# Sort.cpp:34, GroupBy.cpp:7, AggAfterGroupBy.cpp:30-170).
+# ---------------------------------------------------------------------------
+def CL_RangeSelection(h_Rin, rangeSmallKey: int, rangeLargeKey: int, numThreadPB: int = 256,
+                      numBlock: int = 128) -> np.ndarray:
+    """Records with rangeSmallKey <= (int)y <= rangeLargeKey, in input order."""
+    rin = _records(h_Rin)
+    lib = _lib.load()
+    out = C.c_void_p()
+    n = lib.CL_RangeSelection(rin.ctypes.data, rin.shape[0], int(rangeSmallKey), int(rangeLargeKey),
+                              C.byref(out), numThreadPB, numBlock, 0)
+    if n < 0:
+        raise _lib.QoError(n, "CL_RangeSelection", _lib.last_error())
+    res = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint32)), shape=(max(n, 1) * 2,))[: 2 * n]
+    res = res.reshape(n, 2).copy()
+    _free_c(out.value)
+    return res
+
+
+def CL_RadixSort(h_Rin, numThread: int = 256, numBlock: int = 128) -> np.ndarray:
+    """Records sorted ascending by y (uint32); equal keys keep input order."""
+    rin = _records(h_Rin)
+    out = np.empty_like(rin)
+    lib = _lib.load()
+    _clear_error()
+    lib.CL_RadixSort(rin.ctypes.data, rin.shape[0], out.ctypes.data, numThread, numBlock, 0)
+    _raise_if_error("CL_RadixSort")
+    return out
+
+
+def CL_GroupBy(h_Rin, numThread: int = 256, numBlock: int = 128):
+    """(sorted records, startPos) as groupByImpl computes them."""
+    rin = _records(h_Rin)
+    out = np.empty_like(rin)
+    sp = C.c_void_p()
+    lib = _lib.load()
+    g = lib.CL_GroupBy(rin.ctypes.data, rin.shape[0], out.ctypes.data, C.byref(sp), numThread, numBlock, 0)
+    if g < 0:
+        raise _lib.QoError(g, "CL_GroupBy", _lib.last_error())
+    start = np.ctypeslib.as_array(C.cast(sp, C.POINTER(C.c_int32)), shape=(max(g, 1),))[:g].copy()
+    _free_c(sp.value)
+    return out, start
+
+
+def _cl_agg(name: str, h_Rin, startPos, h_Ragg=None, numThread: int = 256) -> np.ndarray:
+    rin = _records(h_Rin)
+    sp = _i32(startPos)
+    res = np.zeros(sp.shape[0], dtype=np.int32)
+    ragg = None if h_Ragg is None else _records(h_Ragg)
+    lib = _lib.load()
+    _clear_error()
+    getattr(lib, name)(rin.ctypes.data, rin.shape[0], sp.ctypes.data, sp.shape[0],
+                       None if ragg is None else ragg.ctypes.data, res.ctypes.data, numThread, 0)
+    _raise_if_error(name)
+    return res
+
+
+def CL_agg_max_afterGroupBy(h_Rin, startPos, h_Ragg=None, numThread: int = 256):
+    return _cl_agg("CL_agg_max_afterGroupBy", h_Rin, startPos, h_Ragg, numThread)
+
+
+def CL_agg_min_afterGroupBy(h_Rin, startPos, h_Ragg=None, numThread: int = 256):
+    return _cl_agg("CL_agg_min_afterGroupBy", h_Rin, startPos, h_Ragg, numThread)
+
+
+def CL_agg_sum_afterGroupBy(h_Rin, startPos, h_Ragg=None, numThread: int = 256):
+    return _cl_agg("CL_agg_sum_afterGroupBy", h_Rin, startPos, h_Ragg, numThread)
+
+
+def CL_agg_avg_afterGroupBy(h_Rin, startPos, h_Ragg=None, numThread: int = 256):
+    return _cl_agg("CL_agg_avg_afterGroupBy", h_Rin, startPos, h_Ragg, numThread)
+
+
+def scanImpl(values) -> np.ndarray:
+    """Exclusive int32 prefix sum (host/host/testScan.cpp:278)."""
+    v = _i32(values)
+    n = v.shape[0]
+    if n == 0:
+        return v.copy()
+    lib = _lib.load()
+    with DeviceBuffer.from_numpy(v) as src, DeviceBuffer(v.nbytes) as dst:
+        msrc, mdst = lib.qo_mem_wrap(src.ptr, v.nbytes), lib.qo_mem_wrap(dst.ptr, v.nbytes)
+        sp = _lib.ScanPara()
+        lib.initScan(n, C.byref(sp))
+        _clear_error()
+        lib.scanImpl(msrc, n, mdst, None, None, None, None, None, C.byref(sp), 0)
+        lib.closeScan(C.byref(sp))
+        lib.qo_mem_free(msrc)
+        lib.qo_mem_free(mdst)
+        _raise_if_error("scanImpl")
+        return dst.to_numpy(np.int32, n)
+
+
+def buildHashTable(rTable, B: int, bucketSize: int):
+    """Bucket-chained build (hj_filter_sort_op.cl:27) from interleaved (key, val) pairs.
+
+    Returns device buffers (table, counts) for probeHashTable."""
+    r = np.ascontiguousarray(rTable, dtype=np.uint32).reshape(-1)
+    size = r.shape[0] // 2
+    lib = _lib.load()
+    rbuf = DeviceBuffer.from_numpy(r)
+    table = DeviceBuffer(B * bucketSize * 8)
+    counts = DeviceBuffer(B * 4)
+    rc = lib.buildHashTable(rbuf.ptr, table.ptr, size, B, bucketSize, counts.ptr)
+    rbuf.free()
+    if rc != 0:
+        table.free()
+        counts.free()
+        raise _lib.QoError(rc, "buildHashTable", _lib.last_error())
+    return table, counts
+
+
+def probeHashTable(table: DeviceBuffer, counts: DeviceBuffer, sTable, B: int, bucketSize: int) -> np.ndarray:
+    """Compacted (s_val, r_val) pairs of every match (hj_filter_sort_op.cl:64)."""
+    s = np.ascontiguousarray(sTable, dtype=np.uint32).reshape(-1)
+    size = s.shape[0] // 2
+    lib = _lib.load()
+    with DeviceBuffer.from_numpy(s) as sbuf, DeviceBuffer(max(size, 1) * 8) as mbuf:
+        m = lib.probeHashTable(table.ptr, sbuf.ptr, mbuf.ptr, size, B, bucketSize, counts.ptr)
+        if m < 0:
+            raise _lib.QoError(m, "probeHashTable", _lib.last_error())
+        return mbuf.to_numpy(np.uint32, 2 * m).reshape(m, 2)
+
+
+# ---------------------------------------------------------------------------
+# Native column API (include/qo.h), host arrays in and out.
+# ---------------------------------------------------------------------------
+def filter_agg(key, val, lo: int, hi: int):
+    """(COUNT, SUM int64, SUM wrapped to int32) of val where lo <= key <= hi."""
+    k, v = _i32(key), _i32(val)
+    with DeviceBuffer.from_numpy(k) as dk, DeviceBuffer.from_numpy(v) as dv, DeviceBuffer(24) as out:
+        _lib.call("qo_filter_agg", dk.ptr, dv.ptr, k.shape[0], int(lo), int(hi), out.ptr, None)
+        r = out.to_numpy(np.int64, 3)
+    return int(r[0]), int(r[1]), int(r[2])
+
+
+def filter_records(h_Rin, lo: int, hi: int) -> np.ndarray:
+    rin = _records(h_Rin)
+    n = rin.shape[0]
+    with DeviceBuffer.from_numpy(rin) as d_in, DeviceBuffer(max(n, 1) * 8) as d_out, DeviceBuffer(8) as cnt:
+        _lib.call("qo_filter_records", d_in.ptr, n, int(lo), int(hi), d_out.ptr, cnt.ptr, None)
+        m = int(cnt.to_numpy(np.uint64, 1)[0])
+        return d_out.to_numpy(np.uint32, 2 * m).reshape(m, 2)
+
+
+def groupby_agg(key, val, group_capacity: int):
+    """Per key: (keys, COUNT, SUM, MIN, MAX) ascending by key."""
+    k, v = _i32(key), _i32(val)
+    g = int(group_capacity)
+    gc = max(g, 1)
+    bufs = [DeviceBuffer(gc * s) for s in (4, 8, 8, 4, 4)]
+    with DeviceBuffer.from_numpy(k) as dk, DeviceBuffer.from_numpy(v) as dv, DeviceBuffer(8) as num:
+        _lib.call("qo_groupby_agg", dk.ptr, dv.ptr, k.shape[0], g, *[b.ptr for b in bufs], num.ptr, None)
+        ng = int(num.to_numpy(np.uint64, 1)[0])
+    if ng > g:
+        for b in bufs:
+            b.free()
+        raise _lib.QoError(3, "groupby_agg", f"{ng} groups exceed capacity {g}")
+    out = tuple(b.to_numpy(dt, ng) for b, dt in zip(bufs, (np.int32, np.int64, np.int64, np.int32, np.int32)))
+    for b in bufs:
+        b.free()
+    return out
+
+
+def hash_join(rkey, rpay, skey, spay, sfilt=None, flo: int = -(1 << 31), fhi: int = (1 << 31) - 1,
+              capacity: int | None = None):
+    """(key, r_payload, s_payload) of every match, in S order."""
+    rk, rp, sk, sp = _i32(rkey), _i32(rpay), _i32(skey), _i32(spay)
+    nr, ns = rk.shape[0], sk.shape[0]
+    cap = ns if capacity is None else int(capacity)
+    lib = _lib.load()
+    tb = lib.qo_join_table_bytes(nr)
+    bufs = [DeviceBuffer.from_numpy(a) for a in (rk, rp, sk, sp)]
+    fbuf = DeviceBuffer.from_numpy(_i32(sfilt)) if sfilt is not None else None
+    outs = [DeviceBuffer(max(cap, 1) * 4) for _ in range(3)]
+    with DeviceBuffer(tb) as table, DeviceBuffer(8) as cnt:
+        _lib.call("qo_hash_build", table.ptr, tb, bufs[0].ptr, bufs[1].ptr, nr, None)
+        _lib.call("qo_hash_probe", table.ptr, bufs[2].ptr, bufs[3].ptr, fbuf.ptr if fbuf else None, ns,
+                  int(flo), int(fhi), outs[0].ptr, outs[1].ptr, outs[2].ptr, cap, cnt.ptr, None)
+        m = int(cnt.to_numpy(np.uint64, 1)[0])
+    res = tuple(o.to_numpy(np.int32, min(m, cap)) for o in outs)
+    for b in bufs + outs + ([fbuf] if fbuf else []):
+        b.free()
+    return res + (m,)
+
+
+def hj_filter_sort(rkey, rpay, skey, spay, sfilt, flo: int, fhi: int, capacity: int | None = None):
+    """Filter S, join with R, sort by (key, s_payload): returns (key, r_payload, s_payload)."""
+    rk, rp, sk, sp, sf = _i32(rkey), _i32(rpay), _i32(skey), _i32(spay), _i32(sfilt)
+    nr, ns = rk.shape[0], sk.shape[0]
+    cap = ns if capacity is None else int(capacity)
+    lib = _lib.load()
+    tb = lib.qo_join_table_bytes(nr)
+    bufs = [DeviceBuffer.from_numpy(a) for a in (rk, rp, sk, sp, sf)]
+    outs = [DeviceBuffer(max(cap, 1) * 4) for _ in range(3)]
+    m = C.c_uint64()
+    with DeviceBuffer(tb) as table:
+        _lib.call("qo_hj_filter_sort", table.ptr, tb, bufs[0].ptr, bufs[1].ptr, nr, bufs[2].ptr, bufs[3].ptr,
+                  bufs[4].ptr, ns, int(flo), int(fhi), outs[0].ptr, outs[1].ptr, outs[2].ptr, cap,
+                  C.byref(m), None)
+    res = tuple(o.to_numpy(np.int32, m.value) for o in outs)
+    for b in bufs + outs:
+        b.free()
+    return res
+
+
+def sort_records(h_Rin) -> np.ndarray:
+    rin = _records(h_Rin)
+    with DeviceBuffer.from_numpy(rin) as d:
+        _lib.call("qo_sort_records", d.ptr, rin.shape[0], None)
+        return d.to_numpy(np.uint32, rin.size).reshape(-1, 2)
+
+
+def sort_join_rows(key, rpay, spay):
+    k, r, s = _i32(key), _i32(rpay), _i32(spay)
+    with DeviceBuffer.from_numpy(k) as dk, DeviceBuffer.from_numpy(r) as dr, DeviceBuffer.from_numpy(s) as ds:
+        _lib.call("qo_sort_join_rows", dk.ptr, dr.ptr, ds.ptr, k.shape[0], None)
+        n = k.shape[0]
+        return dk.to_numpy(np.int32, n), dr.to_numpy(np.int32, n), ds.to_numpy(np.int32, n)
+
+
+# ---------------------------------------------------------------------------
+def _clear_error() -> None:
+    # void entry points report failures only through qo_last_error().
+    _lib.load().qo_clear_error()
+
+
+def _raise_if_error(where: str) -> None:
+    msg = _lib.last_error()
+    if msg:
+        raise _lib.QoError(-1, where, msg)

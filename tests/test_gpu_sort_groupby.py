@@ -1,0 +1,114 @@
+"""GPU parity: Record sort, join-row sort, hash group-by aggregation and the
+reference's sort-based group-by pipeline, against the oracle (bit-exact)."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+KA = json.loads((GOLD / "known_answers.json").read_text())
+BF = json.loads((GOLD / "brute_force.json").read_text())
+INT_MIN, INT_MAX = -(1 << 31), (1 << 31) - 1
+
+
+def test_sort_known_answers(qo):
+    for c in KA["sort"] + BF["sort"]:
+        assert qo.operators.sort_records(np.array(c["in"], np.uint32)).tolist() == c["out"]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4095, 4096, 4097, 100_000, 1 << 20])
+@pytest.mark.parametrize("kmax", [1, 50, 1 << 32])
+def test_sort_records_matches_oracle(qo, orc, n, kmax):
+    rng = np.random.default_rng(n + kmax)
+    r = np.stack([np.arange(n), rng.integers(0, kmax, n, dtype=np.uint64)], 1).astype(np.uint32)
+    assert np.array_equal(qo.operators.sort_records(r), orc.sort_records(r))
+
+
+def test_sort_join_rows_matches_oracle(qo, orc):
+    c = KA["join_sort"]
+    a = np.array(c["in"], np.int32)
+    got = np.stack(qo.operators.sort_join_rows(a[:, 0], a[:, 1], a[:, 2]), 1).tolist()
+    assert got == c["out"]
+    rng = np.random.default_rng(3)
+    for m, kr in ((2, 5), (10_000, 100), (300_000, 1 << 31), (1 << 20, 1000)):
+        k = rng.integers(-kr, kr, m, dtype=np.int64).astype(np.int32)
+        r = rng.integers(INT_MIN, INT_MAX, m, dtype=np.int64).astype(np.int32)
+        s = rng.integers(-50, 50, m, dtype=np.int64).astype(np.int32)
+        got = qo.operators.sort_join_rows(k, r, s)
+        exp = orc.sort_join_rows(k, r, s)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e)
+
+
+def _gb_rows(res):
+    return np.stack([a.astype(np.int64) for a in res], 1).tolist() if len(res[0]) else []
+
+
+def test_groupby_known_answers(qo):
+    c = KA["groupby"]
+    assert _gb_rows(qo.operators.groupby_agg(c["key"], c["val"], 8)) == c["out"]
+    for c in BF["groupby_hash"]:
+        assert _gb_rows(qo.operators.groupby_agg(c["key"], c["val"], max(1, len(c["key"])))) == c["out"]
+
+
+@pytest.mark.parametrize("G,z", [(1, 0.0), (1024, 0.0), (1024, 1.0), (50_000, 0.5), (1 << 20, 0.0), (1 << 20, 1.0)])
+def test_groupby_matches_oracle(qo, orc, G, z):
+    n = 2_000_003
+    t = orc.zipf_table(G, z)
+    keys = orc.gen_zipf_keys(n, t, 9, 1, 17).astype(np.int32)
+    vals = orc.gen_uniform_u32(n, 9, 2, 0, 1 << 16).astype(np.int32)
+    got = qo.operators.groupby_agg(keys, vals, G)
+    exp = orc.groupby_agg(keys, vals)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_groupby_signed_keys_and_sentinel(qo, orc):
+    rng = np.random.default_rng(4)
+    n = 500_000
+    keys = rng.integers(-3000, 3000, n, dtype=np.int64).astype(np.int32)
+    keys[::7] = -1  # the empty-slot bit pattern is a valid key
+    vals = rng.integers(INT_MIN, INT_MAX, n, dtype=np.int64).astype(np.int32)
+    got = qo.operators.groupby_agg(keys, vals, 6001)
+    exp = orc.groupby_agg(keys, vals)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_groupby_capacity_overflow_reports(qo):
+    keys = np.arange(10_000, dtype=np.int32)
+    with pytest.raises(qo._lib.QoError):
+        qo.operators.groupby_agg(keys, keys, 100)
+
+
+def test_startpos_and_aggregate_known_answers(qo):
+    from query_on_opencl_fpga_amd.device import DeviceBuffer
+    c = KA["parallel_aggregate"]
+    r = np.array(c["in"], np.uint32)
+    sp = np.array(c["startPos"], np.int32)
+    for op, name in ((0, "sum"), (1, "max"), (2, "min"), (3, "avg")):
+        with DeviceBuffer.from_numpy(r) as dr, DeviceBuffer.from_numpy(sp) as dsp, DeviceBuffer(12) as out:
+            qo._lib.call("qo_aggregate_groups", dr.ptr, r.shape[0], dsp.ptr, 3, op, out.ptr, None)
+            assert out.to_numpy(np.int32, 3).tolist() == c[name], name
+
+
+@pytest.mark.parametrize("n,kmax", [(1, 1), (1000, 3), (300_000, 50), (1 << 20, 1 << 20)])
+def test_sort_based_groupby_matches_oracle(qo, orc, n, kmax):
+    from query_on_opencl_fpga_amd.device import DeviceBuffer
+    rng = np.random.default_rng(kmax)
+    r = np.stack([rng.integers(INT_MIN, INT_MAX, n, dtype=np.int64), rng.integers(0, kmax, n)], 1).astype(np.int64)
+    r = r.astype(np.uint32)
+    s = orc.sort_records(r)
+    exp_sp = orc.group_startpos(s)
+    with DeviceBuffer.from_numpy(s) as ds, DeviceBuffer(n * 4) as dsp, DeviceBuffer(8) as num:
+        qo._lib.call("qo_group_startpos", ds.ptr, n, dsp.ptr, num.ptr, None)
+        g = int(num.to_numpy(np.uint64, 1)[0])
+        sp = dsp.to_numpy(np.int32, g)
+        assert np.array_equal(sp, exp_sp)
+        for op in range(4):
+            with DeviceBuffer(max(g, 1) * 4) as out:
+                qo._lib.call("qo_aggregate_groups", ds.ptr, n, dsp.ptr, g, op, out.ptr, None)
+                assert np.array_equal(out.to_numpy(np.int32, g), orc.parallel_aggregate(s, sp, op)), op
