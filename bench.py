@@ -98,6 +98,8 @@ def run_join(args, qo, lib) -> dict:
     log(f"[bench] join: generating R={nr} S={ns}")
     t = data.JoinTables(nr, ns, seed)
     tb = lib.qo_join_table_bytes(nr)
+    if args.load:
+        tb = 256 + 8 * (int(nr * 100 / args.load) + 1)
     table = DeviceBuffer(tb)
     cap = ns
     outs = [DeviceBuffer(cap * 4) for _ in range(3)]
@@ -211,6 +213,139 @@ def run_filter_agg(args, qo, lib) -> dict:
     return res
 
 
+def run_copy(args, qo, lib) -> dict:
+    """Plain device copy of a known size: the achievable-HBM reference and the
+    calibration kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
+
+    nbytes = args.copy_bytes
+    src, dst = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+    qo._lib.call("qo_fill_u32", src.ptr, 7, nbytes // 4, None)
+    st = Stream()
+    fn = "qo_copy_nt" if args.copy_nt else "qo_copy"
+    for _ in range(args.warmup):
+        qo._lib.call(fn, dst.ptr, src.ptr, nbytes, st.handle)
+    st.sync()
+    evs = [(Event(), Event()) for _ in range(args.steps)]
+    for a, b in evs:
+        a.record(st)
+        qo._lib.call(fn, dst.ptr, src.ptr, nbytes, st.handle)
+        b.record(st)
+    st.sync()
+    ms = float(np.mean([a.elapsed_ms(b) for a, b in evs]))
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    src.free()
+    dst.free()
+    return {"metric": METRIC, "value": gbs, "unit": "GB/s (read+write)", "ms_per_step": ms, "dtype": "u8",
+            "config": {"workload": "device copy (k_copy16)", "bytes": nbytes},
+            "roofline": {"bound": "hbm", "kernel": "k_copy16", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": 2 * nbytes,
+                         "avg_launch_ms": ms}}
+
+
+def _mk_batch_queries(args, qo, seed):
+    """Config-4 style queries: |R| log-uniform in [4K, 4M] with unique 31-bit keys,
+    |S| = |R| * U[4, 16], fk uniform over R, filter U[0,1000) < 1000*sigma,
+    sigma ~ U[0.05, 0.5].  All columns generated in HBM."""
+    from query_on_opencl_fpga_amd import data
+    from query_on_opencl_fpga_amd.device import DeviceBuffer
+
+    rng = np.random.default_rng(seed & 0xFFFFFFFF)
+    qs = []
+    for i in range(args.queries):
+        u = rng.random()
+        nr = int(2 ** (12 + 10 * u * args.batch_scale))
+        ns = int(nr * (4 + 12 * rng.random()))
+        sigma = 0.05 + 0.45 * rng.random()
+        fhi = int(1000 * sigma) - 1
+        qseed = seed + 1000 + i
+        q = {"nr": nr, "ns": ns, "flo": INT_MIN, "fhi": fhi,
+             "rkey": data.gen_bij(nr, 31, qseed), "rpay": data.gen_uniform(nr, qseed, 1, 0, 0),
+             "skey": data.gen_fk(ns, qseed, 2, nr, 31, qseed), "spay": data.gen_uniform(ns, qseed, 3, 0, 0),
+             "sfilt": data.gen_uniform(ns, qseed, 4, 0, 1000)}
+        q["cap"] = int(ns * (fhi + 1) / 1000 * 1.1) + 4096
+        for c in ("okey", "orpay", "ospay"):
+            q[c] = DeviceBuffer(q["cap"] * 4)
+        qs.append(q)
+    qo._lib.call("qo_device_sync")
+    return qs
+
+
+def run_batched(args, qo, lib) -> dict:
+    """Batched hj_filter_sort: all queries in one graph launch vs the plain path
+    (qo_hj_filter_sort per query, synchronous like the reference's host loop)."""
+    import ctypes as C
+
+    from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
+    from query_on_opencl_fpga_amd.operators import JoinBatch
+
+    seed = SEED_BASE + 4
+    log(f"[bench] batched: generating {args.queries} queries")
+    qs = _mk_batch_queries(args, qo, seed)
+    tot_r = sum(q["nr"] for q in qs)
+    tot_s = sum(q["ns"] for q in qs)
+    log(f"[bench] batched: |R| total {tot_r}, |S| total {tot_s}; creating batch")
+    b = JoinBatch(qs)
+    st = Stream()
+    for _ in range(args.warmup):
+        b.launch(st)
+    st.sync()
+    e0, e1 = Event(), Event()
+    e0.record(st)
+    for _ in range(args.steps):
+        b.launch(st)
+    e1.record(st)
+    st.sync()
+    t_batch = e0.elapsed_ms(e1) / args.steps
+    counts_b = b.counts()
+    sums_b = [int(qs[i]["okey"].to_numpy(np.int32, int(counts_b[i])).astype(np.int64).sum()) for i in (0, len(qs) - 1)]
+    b.destroy()
+    log(f"[bench] batched: {t_batch:.2f} ms per batch; plain path")
+    maxr = max(q["nr"] for q in qs)
+    tb = lib.qo_join_table_bytes(maxr)
+    table = DeviceBuffer(tb)
+    m = C.c_uint64()
+    lat = []
+
+    def plain_all(record):
+        t0 = time.perf_counter()
+        for q in qs:
+            qo._lib.call("qo_hj_filter_sort", table.ptr, lib.qo_join_table_bytes(q["nr"]), q["rkey"].ptr,
+                         q["rpay"].ptr, q["nr"], q["skey"].ptr, q["spay"].ptr, q["sfilt"].ptr, q["ns"], q["flo"],
+                         q["fhi"], q["okey"].ptr, q["orpay"].ptr, q["ospay"].ptr, q["cap"], C.byref(m), st.handle)
+            if record:
+                lat.append((time.perf_counter() - t0) * 1e3)
+                counts_p.append(m.value)
+        return (time.perf_counter() - t0) * 1e3
+
+    counts_p = []
+    plain_all(False)  # warm-up (workspaces grow to the largest query)
+    counts_p.clear()
+    t_plain = plain_all(True)
+    sums_p = [int(qs[i]["okey"].to_numpy(np.int32, int(counts_p[i])).astype(np.int64).sum()) for i in (0, len(qs) - 1)]
+    if list(map(int, counts_b)) != counts_p or sums_b != sums_p:
+        raise SystemExit("batched and plain results differ")
+    table.free()
+    for q in qs:
+        for k, v in q.items():
+            if hasattr(v, "free"):
+                v.free()
+    nq = len(qs)
+    lat = np.array(lat)
+    res = {
+        "metric": METRIC, "value": nq / (t_batch * 1e-3), "unit": "queries/s (batched)",
+        "ms_per_step": t_batch, "dtype": "int32",
+        "config": {"workload": "batched hj_filter_sort: filter S, build R, probe, sort by (key, s_payload)",
+                   "queries": nq, "R_total": tot_r, "S_total": tot_s, "matches_total": int(sum(counts_p))},
+        "batched": {"queries_per_s": nq / (t_batch * 1e-3), "p50_ms": t_batch, "p99_ms": t_batch,
+                    "launches": 1},
+        "plain": {"queries_per_s": nq / (t_plain * 1e-3), "p50_ms": float(np.percentile(lat, 50)),
+                  "p99_ms": float(np.percentile(lat, 99)), "total_ms": t_plain},
+        "speedup_batched_vs_plain": t_plain / t_batch,
+    }
+    return res
+
+
 def run_groupby(args, qo, lib) -> dict:
     from query_on_opencl_fpga_amd import data
     from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
@@ -261,7 +396,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="join", choices=["join", "filter_agg", "groupby", "batched", "served"])
+    ap.add_argument("--config", default="join",
+                    choices=["join", "filter_agg", "groupby", "batched", "served", "copy"])
+    ap.add_argument("--load", type=int, default=0, help="join table load factor in percent (default: tuned)")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--copy-nt", action="store_true", help="copy with non-temporal loads")
+    ap.add_argument("--queries", type=int, default=512)
+    ap.add_argument("--batch-scale", type=float, default=1.0,
+                    help="fraction of the log2 size range for batched queries (1.0 = [4K, 4M])")
     ap.add_argument("--nr", type=int, default=1 << 24)
     ap.add_argument("--ns", type=int, default=1 << 28)
     ap.add_argument("--n-filter", type=int, default=1 << 28)
@@ -280,13 +422,13 @@ def main() -> int:
     lib = qo._lib.load()
     qo._lib.call("qo_init", 0)
     info = qo.device.device_info()
-    log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}")
-    run = {"join": run_join, "filter_agg": run_filter_agg, "groupby": run_groupby}
+    log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']} LDS/block={info['lds_bytes_per_block']}")
+    run = {"join": run_join, "filter_agg": run_filter_agg, "groupby": run_groupby, "copy": run_copy,
+           "batched": run_batched}
     if args.config not in run:
-        from query_on_opencl_fpga_amd import serving_bench  # batched / served modes
-        res = serving_bench.run(args, qo, lib)
-    else:
-        res = run[args.config](args, qo, lib)
+        print(f"config {args.config} is not available", file=sys.stderr)
+        return 2
+    res = run[args.config](args, qo, lib)
     line = {
         "metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"),
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),

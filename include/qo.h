@@ -63,6 +63,11 @@ int qo_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int qo_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int qo_memcpy_d2d(void *dst, const void *src, size_t bytes, qo_stream s);
 int qo_fill_u32(void *dst, uint32_t value, size_t count, qo_stream s);
+/* Device-to-device streaming copy (a plain read+write kernel; also the HBM
+ * bandwidth calibration kernel of the benchmark). */
+int qo_copy(void *dst, const void *src, size_t bytes, qo_stream s);
+/* Same with non-temporal loads (the load type the streaming operators use). */
+int qo_copy_nt(void *dst, const void *src, size_t bytes, qo_stream s);
 int qo_stream_create(qo_stream *s);
 int qo_stream_destroy(qo_stream s);
 int qo_stream_sync(qo_stream s);
@@ -143,7 +148,9 @@ size_t qo_join_table_bytes(size_t build_rows);
 int qo_hash_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay,
                   size_t nr, qo_stream s);
 /* Probe with the S filter flo <= sfilt <= fhi fused (sfilt may be NULL: no filter).
- * Matches are written in S order as (key, r_payload, s_payload); *d_count = matches. */
+ * Matches are written as (key, r_payload, s_payload): each tile of 4096 S tuples
+ * keeps S order, tiles follow in completion order; *d_count = matches (when it
+ * exceeds out_capacity only out_capacity rows are written). */
 int qo_hash_probe(const void *table, const int32_t *skey, const int32_t *spay, const int32_t *sfilt,
                   size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay, int32_t *ospay,
                   size_t out_capacity, uint64_t *d_count, qo_stream s);

@@ -98,6 +98,8 @@ PROTOS = {
     "qo_memcpy_d2h": (C.c_int, [vp, vp, c_size]),
     "qo_memcpy_d2d": (C.c_int, [vp, vp, c_size, vp]),
     "qo_fill_u32": (C.c_int, [vp, c_u32, c_size, vp]),
+    "qo_copy": (C.c_int, [vp, vp, c_size, vp]),
+    "qo_copy_nt": (C.c_int, [vp, vp, c_size, vp]),
     "qo_stream_create": (C.c_int, [P(vp)]),
     "qo_stream_destroy": (C.c_int, [vp]),
     "qo_stream_sync": (C.c_int, [vp]),

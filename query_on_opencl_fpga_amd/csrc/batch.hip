@@ -1,0 +1,324 @@
+// Batched hj_filter_sort: many independent queries of uneven size executed by
+// one hipGraph launch.
+//
+// Every stage is one kernel over the tiles of ALL queries.  Build and probe are
+// persistent kernels pulling tiles from a work queue (an atomic ticket); a tile
+// maps to its query through the per-query first-tile table, so a 4K-row query
+// and a 4M-row query share the same launch.  The probe writes sort-ready
+// (key, s_payload) 64-bit keys with the r_payload beside them; a segmented LSD
+// radix sort (one segment per query, element counts read from device memory)
+// orders every query's matches, and an unpack kernel writes the SoA outputs.
+// The stage sequence (fills included, no memset nodes) is captured once into a
+// graph at creation and replayed by qo_batch_launch.
+//
+// Reference path: buildHashTable / probeHashTable + the sort kernels of the
+// hj_filter_sort image (kernel/hj_filter_sort/1/hj_filter_sort_op.cl:25-109,
+// 362-564), which the reference never composes on the host; here one query of
+// the batch is exactly qo_hj_filter_sort.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "qo_internal.hpp"
+#include "qo_join.hpp"
+#include "qo_sort.hpp"
+
+namespace qo {
+
+constexpr uint32_t kBuildTile = 4096;
+
+struct BQuery {
+    const int32_t *rkey, *rpay, *skey, *spay, *sfilt;
+    uint64_t nr, ns;
+    int32_t flo, fhi;
+    int32_t *okey, *orpay, *ospay;
+    uint64_t out_cap;
+    JoinHeader *hdr;        // table header; slots follow it
+    uint64_t table_cap;     // slots
+    uint64_t m_base;        // first element of this query in the match arrays
+    unsigned long long *count;
+    uint32_t rtile0, nrtile;
+    uint32_t stile0, nstile;
+    uint32_t mtile0, nmtile;
+    uint32_t vec;           // S columns are 16-byte aligned
+    uint32_t pad;
+};
+
+// Last query whose first tile (selected by `which`) is <= t.
+template <int WHICH>
+QO_DEV int query_of(const BQuery *q, int nq, uint32_t t) {
+    int lo = 0, hi = nq - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        uint32_t t0 = WHICH == 0 ? q[mid].rtile0 : WHICH == 1 ? q[mid].stile0 : q[mid].mtile0;
+        if (t0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_b_init(const BQuery *q, int nq) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    JoinHeader *h = q[i].hdr;
+    h->cap = q[i].table_cap;
+    h->has_dups = 0;
+    h->sentinel_count = 0;
+    h->overflow = 0;
+    h->nr = q[i].nr;
+    *q[i].count = 0;
+}
+
+// Persistent loop: thread 0 draws tickets; the whole block works on each tile.
+#define QO_TILE_LOOP(TICKET, TOTAL, T)                                                                              \
+    __shared__ uint32_t s_ticket;                                                                                   \
+    for (;;) {                                                                                                      \
+        if (threadIdx.x == 0) s_ticket = atomicAdd((TICKET), 1u);                                                  \
+        __syncthreads();                                                                                            \
+        const uint32_t T = s_ticket;                                                                                \
+        __syncthreads();                                                                                            \
+        if (T >= (TOTAL)) break;
+
+#define QO_TILE_LOOP_END }
+
+__global__ __launch_bounds__(256) void k_b_build(const BQuery *q, int nq, uint32_t total, uint32_t *ticket) {
+    QO_TILE_LOOP(ticket, total, t)
+    const int qi = query_of<0>(q, nq, t);
+    const BQuery &Q = q[qi];
+    const uint64_t r0 = (uint64_t)(t - Q.rtile0) * kBuildTile;
+    const uint64_t r1 = r0 + kBuildTile < Q.nr ? r0 + kBuildTile : Q.nr;
+    uint64_t *slots = reinterpret_cast<uint64_t *>(Q.hdr + 1);
+    for (uint64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) join_insert(slots, Q.table_cap, Q.hdr, Q.rkey[i], Q.rpay[i]);
+    QO_TILE_LOOP_END
+}
+
+struct PackedEmit {
+    uint64_t *keys;
+    uint32_t *vals;
+    uint64_t cap;
+    QO_DEV void operator()(uint64_t pos, int32_t k, int32_t rp, int32_t sp) const {
+        if (pos < cap) {
+            keys[pos] = join_sort_key(k, sp);
+            vals[pos] = (uint32_t)rp;
+        }
+    }
+};
+
+__global__ __launch_bounds__(256) void k_b_probe(const BQuery *q, int nq, uint32_t total, uint32_t *ticket,
+                                                 uint64_t *mkeys, uint32_t *mvals) {
+    __shared__ uint32_t s_tot[kProbeJ * 4];
+    __shared__ uint64_t s_base;
+    QO_TILE_LOOP(ticket, total, t)
+    const int qi = query_of<1>(q, nq, t);
+    const BQuery &Q = q[qi];
+    const uint64_t *slots = reinterpret_cast<const uint64_t *>(Q.hdr + 1);
+    PackedEmit emit{mkeys + Q.m_base, mvals + Q.m_base, Q.out_cap};
+    if (Q.sfilt)
+        probe_tile<true>(Q.hdr, slots, Q.skey, Q.spay, Q.sfilt, Q.ns, Q.flo, Q.fhi, Q.vec != 0, t - Q.stile0, Q.count,
+                         emit, s_tot, &s_base);
+    else
+        probe_tile<false>(Q.hdr, slots, Q.skey, Q.spay, Q.sfilt, Q.ns, Q.flo, Q.fhi, Q.vec != 0, t - Q.stile0, Q.count,
+                          emit, s_tot, &s_base);
+    QO_TILE_LOOP_END
+}
+
+__global__ __launch_bounds__(256) void k_b_unpack(const BQuery *q, int nq, const uint64_t *mkeys, const uint32_t *mvals) {
+    const uint32_t t = blockIdx.x;
+    const int qi = query_of<2>(q, nq, t);
+    const BQuery &Q = q[qi];
+    uint64_t n = *Q.count;
+    if (n > Q.out_cap) n = Q.out_cap;
+    const uint64_t e0 = (uint64_t)(t - Q.mtile0) * kSortTile;
+    const uint64_t e1 = e0 + kSortTile < n ? e0 + kSortTile : n;
+    for (uint64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
+        const uint64_t k = mkeys[Q.m_base + i];
+        Q.okey[i] = join_sort_key_key(k);
+        Q.ospay[i] = join_sort_key_spay(k);
+        Q.orpay[i] = (int32_t)mvals[Q.m_base + i];
+    }
+}
+
+}  // namespace qo
+
+struct qo_batch {
+    int nq = 0;
+    void *arena = nullptr;
+    qo::BQuery *d_q = nullptr;
+    std::vector<qo::BQuery> h_q;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    unsigned long long *d_counts = nullptr;
+};
+
+using namespace qo;
+
+static int batch_record(qo_batch *b, uint64_t *slots_all, uint64_t total_slots, uint32_t *small, size_t small_words,
+                        uint32_t rt, uint32_t stt, uint32_t mt, uint64_t *mkeys, uint32_t *mvals, uint64_t *kalt,
+                        uint32_t *valt, const RadixSeg &seg_first, RadixSeg *d_segs, uint32_t *hist,
+                        void *scan_ws, hipStream_t st) {
+    const unsigned persist = (unsigned)props().cu_count * 8;
+    QO_TRY(launch_fill_u64(slots_all, kEmptySlot, total_slots, st));
+    QO_TRY(launch_fill_u32(small, 0, small_words, st));
+    k_b_init<<<(b->nq + 255) / 256, 256, 0, st>>>(b->d_q, b->nq);
+    QO_LAUNCHED();
+    uint32_t *tickets = small;
+    if (rt) {
+        k_b_build<<<std::min<unsigned>(rt, persist), 256, 0, st>>>(b->d_q, b->nq, rt, tickets + 0);
+        QO_LAUNCHED();
+    }
+    if (stt) {
+        k_b_probe<<<std::min<unsigned>(stt, persist), 256, 0, st>>>(b->d_q, b->nq, stt, tickets + 1, mkeys, mvals);
+        QO_LAUNCHED();
+    }
+    if (mt) {
+        bool in_alt = false;
+        // With one query the sort takes its segment by value (seg_first).
+        QO_TRY(launch_radix_segments(mkeys, mvals, kalt, valt, seg_first, d_segs, b->nq, mt, 0, 64, hist, scan_ws,
+                                     st, &in_alt));
+        k_b_unpack<<<mt, 256, 0, st>>>(b->d_q, b->nq, in_alt ? kalt : mkeys, in_alt ? valt : mvals);
+        QO_LAUNCHED();
+    }
+    return QO_OK;
+}
+
+extern "C" {
+
+int qo_batch_create(const qo_join_query *queries, int nq, qo_batch **out) {
+    QO_ENTER();
+    QO_REQUIRE(out && nq >= 1 && queries, "bad batch arguments");
+    *out = nullptr;
+    std::vector<BQuery> hq(nq);
+    uint64_t total_slots = 0, total_match = 0;
+    uint64_t rt = 0, stt = 0, mt = 0;
+    const int load = std::max(10, std::min(90, tune_get(TUNE_PROBE, 1 << 20).param));
+    for (int i = 0; i < nq; ++i) {
+        const qo_join_query &u = queries[i];
+        QO_REQUIRE((u.nr == 0 || (u.rkey && u.rpay)) && (u.ns == 0 || (u.skey && u.spay)),
+                   "query %d: null input column", i);
+        QO_REQUIRE(u.out_capacity == 0 || (u.okey && u.orpay && u.ospay), "query %d: null output column", i);
+        BQuery &q = hq[i];
+        memset(&q, 0, sizeof(q));
+        q.rkey = u.rkey; q.rpay = u.rpay; q.skey = u.skey; q.spay = u.spay; q.sfilt = u.sfilt;
+        q.nr = u.nr; q.ns = u.ns; q.flo = u.flo; q.fhi = u.fhi;
+        q.okey = u.okey; q.orpay = u.orpay; q.ospay = u.ospay; q.out_cap = u.out_capacity;
+        // Rounded to 32 slots so every query's 256-byte header stays 256-byte aligned.
+        q.table_cap = (std::max<uint64_t>(64, (uint64_t)((double)u.nr * 100.0 / load) + 1) + 31) / 32 * 32;
+        QO_REQUIRE(q.table_cap <= 0xFFFFFFFFull, "query %d: build side too large", i);
+        q.vec = ((uintptr_t)u.skey % 16 == 0 && (uintptr_t)u.spay % 16 == 0 &&
+                 (!u.sfilt || (uintptr_t)u.sfilt % 16 == 0)) ? 1u : 0u;
+        q.rtile0 = (uint32_t)rt;
+        q.nrtile = (uint32_t)((u.nr + kBuildTile - 1) / kBuildTile);
+        q.stile0 = (uint32_t)stt;
+        q.nstile = (uint32_t)((u.ns + kProbeTile - 1) / kProbeTile);
+        q.mtile0 = (uint32_t)mt;
+        q.nmtile = (uint32_t)((u.out_capacity + kSortTile - 1) / kSortTile);
+        q.m_base = total_match;
+        rt += q.nrtile;
+        stt += q.nstile;
+        mt += q.nmtile;
+        total_slots += 32 + q.table_cap;  // 256-byte header = 32 slots
+        total_match += (u.out_capacity + 31) / 32 * 32;
+    }
+    QO_REQUIRE(rt < 0xFFFFFFFFull && stt < 0xFFFFFFFFull && mt * 256 < 0xFFFFFFFFull, "batch too large");
+    // Arena layout.
+    const size_t small_words = 64 + (size_t)nq * 2;
+    Carver cv(nullptr);
+    cv.take<uint64_t>(total_slots);
+    cv.take<uint32_t>(small_words);
+    cv.take<BQuery>(nq);
+    cv.take<RadixSeg>(nq);
+    cv.take<uint64_t>(total_match);
+    cv.take<uint32_t>(total_match);
+    cv.take<uint64_t>(total_match);
+    cv.take<uint32_t>(total_match);
+    cv.take<uint32_t>(mt * 256 + 1);
+    cv.take<char>(scan_workspace_bytes(mt * 256 + 1));
+    qo_batch *b = new qo_batch;
+    b->nq = nq;
+    hipError_t e = hipMalloc(&b->arena, cv.off + 256);
+    if (e != hipSuccess) {
+        delete b;
+        return set_error(QO_ERR_NO_MEMORY, "batch arena of %zu bytes: %s", cv.off, hipGetErrorString(e));
+    }
+    Carver c2(b->arena);
+    uint64_t *slots_all = c2.take<uint64_t>(total_slots);
+    uint32_t *small = c2.take<uint32_t>(small_words);
+    b->d_q = c2.take<BQuery>(nq);
+    RadixSeg *d_segs = c2.take<RadixSeg>(nq);
+    uint64_t *mkeys = c2.take<uint64_t>(total_match);
+    uint32_t *mvals = c2.take<uint32_t>(total_match);
+    uint64_t *kalt = c2.take<uint64_t>(total_match);
+    uint32_t *valt = c2.take<uint32_t>(total_match);
+    uint32_t *hist = c2.take<uint32_t>(mt * 256 + 1);
+    void *scan_ws = c2.take<char>(scan_workspace_bytes(mt * 256 + 1));
+    // Per-query pointers into the arena.
+    std::vector<RadixSeg> hs(nq);
+    uint64_t slot_off = 0;
+    unsigned long long *counts = reinterpret_cast<unsigned long long *>(small + 64);
+    for (int i = 0; i < nq; ++i) {
+        BQuery &q = hq[i];
+        q.hdr = reinterpret_cast<JoinHeader *>(slots_all + slot_off);
+        slot_off += 32 + q.table_cap;
+        q.count = counts + i;
+        hs[i] = RadixSeg{q.m_base, q.out_cap, reinterpret_cast<const uint64_t *>(q.count), q.mtile0, q.nmtile};
+    }
+    b->h_q = hq;
+    b->d_counts = counts;
+    int rc = QO_OK;
+    if (hipMemcpy(b->d_q, hq.data(), sizeof(BQuery) * nq, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_segs, hs.data(), sizeof(RadixSeg) * nq, hipMemcpyHostToDevice) != hipSuccess)
+        rc = set_error(QO_ERR_HIP, "batch descriptor upload failed");
+    if (rc == QO_OK && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess)
+        rc = set_error(QO_ERR_HIP, "batch stream");
+    if (rc == QO_OK) {
+        e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) rc = set_error(QO_ERR_HIP, "begin capture: %s", hipGetErrorString(e));
+    }
+    if (rc == QO_OK) {
+        int rec = batch_record(b, slots_all, total_slots, small, small_words, (uint32_t)rt, (uint32_t)stt,
+                               (uint32_t)mt, mkeys, mvals, kalt, valt, hs[0], d_segs, hist, scan_ws, b->cap_stream);
+        e = hipStreamEndCapture(b->cap_stream, &b->graph);
+        if (rec != QO_OK) rc = rec;
+        else if (e != hipSuccess) rc = set_error(QO_ERR_HIP, "end capture: %s", hipGetErrorString(e));
+    }
+    if (rc == QO_OK) {
+        e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = set_error(QO_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    if (rc != QO_OK) {
+        qo_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return QO_OK;
+}
+
+int qo_batch_launch(qo_batch *b, qo_stream s) {
+    QO_ENTER();
+    QO_REQUIRE(b && b->exec, "batch not created");
+    QO_HIP(hipGraphLaunch(b->exec, stream_of(s)));
+    return QO_OK;
+}
+
+int qo_batch_counts(qo_batch *b, uint64_t *h_counts) {
+    QO_TRY(ensure_init());
+    QO_REQUIRE(b && h_counts, "null argument");
+    QO_HIP(hipDeviceSynchronize());
+    QO_HIP(hipMemcpy(h_counts, b->d_counts, 8 * (size_t)b->nq, hipMemcpyDeviceToHost));
+    return QO_OK;
+}
+
+int qo_batch_destroy(qo_batch *b) {
+    if (!b) return QO_OK;
+    QO_TRY(ensure_init());
+    (void)hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    if (b->arena) (void)hipFree(b->arena);
+    delete b;
+    return QO_OK;
+}
+
+}  // extern "C"

@@ -7,8 +7,9 @@
 // spin-locks into fixed-capacity buckets; probeHashTable (:62-109) scans a
 // bucket per S tuple and writes (s_val, r_val) pairs into per-work-item slots
 // that are never compacted.  Here the build is a grid-wide CAS insert into a
-// bandwidth-sized power-of-two table and the probe streams S once, filters,
-// looks up, and appends matches in S order through a decoupled look-back.
+// bandwidth-sized open-addressing table and the probe streams S once, filters,
+// looks up, and appends each tile's matches (in S order) at a range reserved
+// with one atomic add.
 #include <algorithm>
 
 #include "qo_internal.hpp"
@@ -16,17 +17,14 @@
 
 namespace qo {
 
-size_t join_table_capacity_log2(size_t table_bytes) {
+size_t join_table_slots(size_t table_bytes) {
     if (table_bytes < sizeof(JoinHeader) + 8) return 0;
-    size_t slots = (table_bytes - sizeof(JoinHeader)) / 8;
-    size_t lg = 0;
-    while ((size_t(1) << (lg + 1)) <= slots) ++lg;
-    return lg;
+    return (table_bytes - sizeof(JoinHeader)) / 8;
 }
 
-__global__ void k_join_init(JoinHeader *hdr, uint32_t log2cap, uint64_t nr) {
+__global__ void k_join_init(JoinHeader *hdr, uint64_t cap, uint64_t nr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        hdr->log2cap = log2cap;
+        hdr->cap = cap;
         hdr->has_dups = 0;
         hdr->sentinel_count = 0;
         hdr->overflow = 0;
@@ -35,210 +33,63 @@ __global__ void k_join_init(JoinHeader *hdr, uint32_t log2cap, uint64_t nr) {
 }
 
 __global__ __launch_bounds__(256) void k_join_build(const int32_t *__restrict__ rkey, const int32_t *__restrict__ rpay,
-                                                    size_t nr, uint64_t *slots, JoinHeader *hdr, uint64_t mask) {
+                                                    size_t nr, uint64_t *slots, JoinHeader *hdr, uint64_t cap) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < nr; i += stride) join_insert(slots, mask, hdr, rkey[i], rpay[i]);
+    for (; i < nr; i += stride) join_insert(slots, cap, hdr, rkey[i], rpay[i]);
 }
 
 int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                       hipStream_t st) {
-    const size_t lg = join_table_capacity_log2(table_bytes);
-    QO_REQUIRE(lg >= 1 && lg <= 40, "join table of %zu bytes is too small", table_bytes);
-    const size_t cap = size_t(1) << lg;
+    const size_t cap = join_table_slots(table_bytes);
+    QO_REQUIRE(cap >= 2 && cap <= 0xFFFFFFFFull, "join table of %zu bytes has an unsupported size", table_bytes);
     QO_REQUIRE(nr < cap, "join table with %zu slots cannot hold %zu rows", cap, nr);
     JoinHeader *hdr = static_cast<JoinHeader *>(table);
     uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
-    k_join_init<<<1, 64, 0, st>>>(hdr, (uint32_t)lg, nr);
+    k_join_init<<<1, 64, 0, st>>>(hdr, cap, nr);
     QO_LAUNCHED();
     QO_TRY(launch_fill_u64(slots, kEmptySlot, cap, st));
     if (nr) {
         unsigned g = grid_for(nr, 256, (unsigned)props().cu_count * 16);
-        k_join_build<<<g, 256, 0, st>>>(rkey, rpay, nr, slots, hdr, cap - 1);
+        k_join_build<<<g, 256, 0, st>>>(rkey, rpay, nr, slots, hdr, cap);
         QO_LAUNCHED();
     }
     return QO_OK;
 }
 
-// Probe.  Tile = 256 threads x J int4 groups (4 S tuples each), striped.  The
-// common case (unique build keys) resolves each tuple with one chain walk and
-// keeps the payload in registers; tables with duplicate keys take the general
-// path that counts, then re-walks to emit every match.
-template <int J, bool HAS_FILTER, bool VEC>
+// Probe.  One block per tile of kProbeTile S tuples; *d_count accumulates the
+// matches (it is zeroed first, in stream order).
+template <bool HAS_FILTER>
 __global__ __launch_bounds__(256) void k_join_probe(const JoinHeader *__restrict__ hdr,
                                                     const uint64_t *__restrict__ slots,
                                                     const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
                                                     const int32_t *__restrict__ sfilt, size_t ns, int32_t flo,
-                                                    int32_t fhi, int32_t *__restrict__ okey,
-                                                    int32_t *__restrict__ orpay, int32_t *__restrict__ ospay,
-                                                    size_t cap, uint64_t *status, uint32_t *ticket,
-                                                    uint64_t *d_count, uint32_t num_tiles) {
-    constexpr int NW = 4;
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_tot[J * NW];
-    __shared__ uint64_t s_excl;
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    const uint64_t mask = (uint64_t(1) << hdr->log2cap) - 1;
-    const bool general = (hdr->has_dups | hdr->sentinel_count) != 0;
-    const uint32_t sentinel = hdr->sentinel_count;
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const size_t vbase = (size_t)tile * (J * 256);
-
-    int32_t key[J][4], sp[J][4], rp[J][4];
-    uint32_t pass[J];
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
-        const size_t e0 = v * 4;
-        pass[j] = 0;
-        if (VEC && e0 + 4 <= ns) {
-            int4 kk = ld_stream(reinterpret_cast<const int4 *>(skey) + v);
-            int4 pp = ld_stream(reinterpret_cast<const int4 *>(spay) + v);
-            key[j][0] = kk.x; key[j][1] = kk.y; key[j][2] = kk.z; key[j][3] = kk.w;
-            sp[j][0] = pp.x; sp[j][1] = pp.y; sp[j][2] = pp.z; sp[j][3] = pp.w;
-            if (HAS_FILTER) {
-                int4 ff = ld_stream(reinterpret_cast<const int4 *>(sfilt) + v);
-                pass[j] = ((ff.x >= flo) & (ff.x <= fhi)) | (((ff.y >= flo) & (ff.y <= fhi)) << 1) |
-                          (((ff.z >= flo) & (ff.z <= fhi)) << 2) | (((ff.w >= flo) & (ff.w <= fhi)) << 3);
-            } else {
-                pass[j] = 15u;
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const size_t e = e0 + k;
-                if (e < ns) {
-                    key[j][k] = skey[e];
-                    sp[j][k] = spay[e];
-                    bool ok = true;
-                    if (HAS_FILTER) {
-                        int32_t f = sfilt[e];
-                        ok = (f >= flo) & (f <= fhi);
-                    }
-                    pass[j] |= ok ? (1u << k) : 0u;
-                }
-            }
-        }
-    }
-
-    uint32_t c[J], ex[J];
-    uint32_t hit[J];
-    if (!general) {
-        // Issue every first-slot load before resolving any chain.
-        uint64_t hs[J][4], sv[J][4];
-#pragma unroll
-        for (int j = 0; j < J; ++j)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                hs[j][k] = hash32((uint32_t)key[j][k]) & mask;
-                sv[j][k] = (pass[j] >> k) & 1u ? slots[hs[j][k]] : kEmptySlot;
-            }
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            hit[j] = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if ((pass[j] >> k) & 1u)
-                    if (join_find(slots, mask, key[j][k], hs[j][k], sv[j][k], rp[j][k])) hit[j] |= 1u << k;
-            c[j] = __popc(hit[j]);
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            c[j] = 0;
-            hit[j] = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if ((pass[j] >> k) & 1u) {
-                    uint32_t m = join_count(slots, mask, sentinel, key[j][k]);
-                    c[j] += m;
-                    if (m) hit[j] |= 1u << k;
-                }
-        }
-    }
-
-    const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
-    if (threadIdx.x < kWave) {
-        uint64_t e = lookback(status, tile, total);
-        if (threadIdx.x == 0) s_excl = e;
-    }
-    __syncthreads();
-    const uint64_t base = s_excl;
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        uint64_t pos = base + ex[j];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (!((hit[j] >> k) & 1u)) continue;
-            if (!general) {
-                if (pos < cap) {
-                    okey[pos] = key[j][k];
-                    orpay[pos] = rp[j][k];
-                    ospay[pos] = sp[j][k];
-                }
-                ++pos;
-            } else {
-                const int32_t kk = key[j][k];
-                uint64_t h = hash32((uint32_t)kk) & mask;
-                for (uint64_t probe = 0; probe <= mask; ++probe) {
-                    uint64_t s = slots[h];
-                    if (s == kEmptySlot) break;
-                    if ((uint32_t)s == (uint32_t)kk) {
-                        if (pos < cap) {
-                            okey[pos] = kk;
-                            orpay[pos] = (int32_t)(s >> 32);
-                            ospay[pos] = sp[j][k];
-                        }
-                        ++pos;
-                    }
-                    h = (h + 1) & mask;
-                }
-                if (kk == -1)
-                    for (uint32_t q = 0; q < sentinel; ++q) {
-                        if (pos < cap) {
-                            okey[pos] = kk;
-                            orpay[pos] = -1;
-                            ospay[pos] = sp[j][k];
-                        }
-                        ++pos;
-                    }
-            }
-        }
-    }
-    if (tile == num_tiles - 1 && threadIdx.x == 0) *d_count = base + total;
+                                                    int32_t fhi, bool vec, SoaEmit emit, uint64_t *d_count) {
+    __shared__ uint32_t s_tot[kProbeJ * 4];
+    __shared__ uint64_t s_base;
+    probe_tile<HAS_FILTER>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, blockIdx.x,
+                           reinterpret_cast<unsigned long long *>(d_count), emit, s_tot, &s_base);
 }
 
-constexpr int kProbeJ = 4;
-constexpr size_t kProbeTile = (size_t)kProbeJ * 256 * 4;
-
-size_t join_probe_ws_bytes(size_t ns) { return ((ns + kProbeTile - 1) / kProbeTile + 1) * 8 + 512; }
+size_t join_probe_ws_bytes(size_t) { return 256; }
 
 int launch_join_probe(const void *table, const int32_t *skey, const int32_t *spay, const int32_t *sfilt, size_t ns,
                       int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay, int32_t *ospay, size_t cap,
-                      uint64_t *d_count, void *ws, hipStream_t st) {
-    if (ns == 0) return launch_fill_u64(d_count, 0, 1, st);
+                      uint64_t *d_count, void *, hipStream_t st) {
+    QO_TRY(launch_fill_u64(d_count, 0, 1, st));
+    if (ns == 0) return QO_OK;
     const size_t tiles = (ns + kProbeTile - 1) / kProbeTile;
     QO_REQUIRE(tiles < 0xFFFFFFFFull, "probe input too large");
     const JoinHeader *hdr = static_cast<const JoinHeader *>(table);
     const uint64_t *slots = reinterpret_cast<const uint64_t *>(hdr + 1);
-    Carver cv(ws);
-    uint64_t *stat = cv.take<uint64_t>(tiles);
-    uint32_t *ticket = cv.take<uint32_t>(1);
-    QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
     const bool vec = ((uintptr_t)skey % 16 == 0) && ((uintptr_t)spay % 16 == 0) && (!sfilt || (uintptr_t)sfilt % 16 == 0);
-#define QO_PROBE(F, V)                                                                                              \
-    k_join_probe<kProbeJ, F, V><<<(unsigned)tiles, 256, 0, st>>>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, okey, \
-                                                                 orpay, ospay, cap, stat, ticket, d_count,          \
-                                                                 (uint32_t)tiles)
-    if (sfilt) {
-        if (vec) QO_PROBE(true, true);
-        else QO_PROBE(true, false);
-    } else {
-        if (vec) QO_PROBE(false, true);
-        else QO_PROBE(false, false);
-    }
-#undef QO_PROBE
+    SoaEmit emit{okey, orpay, ospay, cap};
+    if (sfilt)
+        k_join_probe<true><<<(unsigned)tiles, 256, 0, st>>>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, emit,
+                                                            d_count);
+    else
+        k_join_probe<false><<<(unsigned)tiles, 256, 0, st>>>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, emit,
+                                                             d_count);
     QO_LAUNCHED();
     return QO_OK;
 }
@@ -326,8 +177,11 @@ using namespace qo;
 extern "C" {
 
 size_t qo_join_table_bytes(size_t build_rows) {
-    size_t cap = 64;
-    while (cap < 2 * build_rows + 1) cap <<= 1;
+    // Load factor (percent) is a tuned parameter of the probe; 50% by default.
+    int load = tune_get(TUNE_PROBE, build_rows).param;
+    if (load < 10 || load > 90) load = 50;
+    size_t cap = (size_t)((double)build_rows * 100.0 / load) + 1;
+    if (cap < 64) cap = 64;
     return sizeof(JoinHeader) + cap * 8;
 }
 

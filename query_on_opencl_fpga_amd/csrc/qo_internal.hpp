@@ -89,7 +89,7 @@ size_t sort_join_rows_ws_bytes(size_t m);
 int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void *ws, hipStream_t st);
 
 // Join table helpers (join.hip).
-size_t join_table_capacity_log2(size_t table_bytes);
+size_t join_table_slots(size_t table_bytes);
 int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                       hipStream_t st);
 size_t join_probe_ws_bytes(size_t ns);

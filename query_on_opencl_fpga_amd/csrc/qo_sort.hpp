@@ -1,0 +1,44 @@
+// Segmented radix sort interface and the join-row sort key shared by the plain
+// and batched paths.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qo {
+
+constexpr int kSortItems = 16;
+constexpr int kSortTile = 256 * kSortItems;
+
+// One independently sorted array: elements [base, base + n) of the shared key /
+// value arrays, n = *d_n (clamped to cap) or cap when d_n is null.  The
+// segment owns global tiles [tile0, tile0 + ntile), ntile = ceil(cap / tile).
+struct RadixSeg {
+    uint64_t base;
+    uint64_t cap;
+    const uint64_t *d_n;
+    uint32_t tile0;
+    uint32_t ntile;
+};
+
+size_t radix_hist_bytes(uint32_t total_tiles);
+// Sorts every segment by bits [lo_bit, hi_bit) of the keys.  With nseg == 1 the
+// segment is passed by value (seg0) and d_segs may be null.  On return *in_alt
+// tells whether the sorted data ended in kalt / valt (odd pass count).
+int launch_radix_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &seg0,
+                          const RadixSeg *d_segs, int nseg, uint32_t total_tiles, int lo_bit, int hi_bit,
+                          uint32_t *hist, void *scan_ws, hipStream_t st, bool *in_alt);
+
+// Join rows sort ascending by (key, s_payload) as signed int32: flip the sign
+// bits so unsigned 64-bit order is the signed lexicographic order.
+__host__ __device__ __forceinline__ uint64_t join_sort_key(int32_t key, int32_t spay) {
+    return ((uint64_t)((uint32_t)key ^ 0x80000000u) << 32) | ((uint32_t)spay ^ 0x80000000u);
+}
+__host__ __device__ __forceinline__ int32_t join_sort_key_key(uint64_t k) {
+    return (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
+}
+__host__ __device__ __forceinline__ int32_t join_sort_key_spay(uint64_t k) {
+    return (int32_t)((uint32_t)k ^ 0x80000000u);
+}
+
+}  // namespace qo

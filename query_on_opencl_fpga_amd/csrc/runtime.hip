@@ -125,6 +125,34 @@ int launch_fill_u32(void *dst, uint32_t value, size_t count, hipStream_t st) {
     return QO_OK;
 }
 
+// Plain streaming copy, 16 bytes per lane per iteration (HBM calibration kernel).
+// NT selects the non-temporal loads the streaming operators use.
+template <bool NT>
+__device__ __forceinline__ uint4 copy_ld(const uint4 *p) {
+    if constexpr (NT) {
+        typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+        v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(p));
+        return uint4{v.x, v.y, v.z, v.w};
+    } else {
+        return *p;
+    }
+}
+
+template <bool NT>
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        uint4 a = copy_ld<NT>(src + i), b = copy_ld<NT>(src + i + stride), c = copy_ld<NT>(src + i + 2 * stride),
+              d = copy_ld<NT>(src + i + 3 * stride);
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 int launch_fill_u64(void *dst, uint64_t value, size_t count, hipStream_t st) {
     if (count == 0) return QO_OK;
     unsigned g = grid_for(count, 256, (unsigned)props().cu_count * 8);
@@ -146,7 +174,7 @@ TuneParams tune_default(int op, int cls) {
     switch (op) {
         case TUNE_FILTER_AGG: return {256, 4, 8};      // block, unroll, blocks per CU
         case TUNE_FILTER: return {256, 4, 0};          // block, uint4 loads per thread
-        case TUNE_PROBE: return {256, 4, 0};           // block, int4 groups per thread
+        case TUNE_PROBE: return {256, 4, 50};          // block, int4 groups per thread, load %
         case TUNE_GROUPBY: return {512, 2048, 3};      // block, LDS slots, blocks per CU
         case TUNE_SORT: return {256, 16, 0};           // block, keys per thread
         default: return {256, 4, 0};
@@ -233,6 +261,25 @@ int qo_memcpy_d2d(void *dst, const void *src, size_t bytes, qo_stream s) {
     if (bytes) QO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_of(s)));
     return QO_OK;
 }
+
+static int copy_impl(void *dst, const void *src, size_t bytes, bool nt, qo_stream s) {
+    QO_TRY(ensure_init());
+    if (bytes == 0) return QO_OK;
+    hipStream_t st = stream_of(s);
+    if ((uintptr_t)dst % 16 || (uintptr_t)src % 16 || bytes % 16) {
+        QO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+        return QO_OK;
+    }
+    unsigned g = grid_for(bytes / 16, 256 * 4, (unsigned)g_props.cu_count * 8);
+    if (nt) k_copy16<true><<<g, 256, 0, st>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), bytes / 16);
+    else k_copy16<false><<<g, 256, 0, st>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), bytes / 16);
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
+int qo_copy(void *dst, const void *src, size_t bytes, qo_stream s) { return copy_impl(dst, src, bytes, false, s); }
+
+int qo_copy_nt(void *dst, const void *src, size_t bytes, qo_stream s) { return copy_impl(dst, src, bytes, true, s); }
 
 int qo_fill_u32(void *dst, uint32_t value, size_t count, qo_stream s) {
     QO_TRY(ensure_init());

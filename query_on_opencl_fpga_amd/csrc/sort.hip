@@ -4,17 +4,20 @@
 // keys with wave-level match masks (8 ballots per key), stages the tile in LDS
 // in digit order and writes contiguous runs.
 //
+// The sort is segmented: many independent arrays (one per query of a batch)
+// are sorted by the same launches.  A segment owns a contiguous tile range; its
+// element count may live in device memory (written by an earlier kernel), so a
+// whole batch can be captured in one graph without host round trips.
+//
 // Reference path replaced: radixSortImpl_sm (host/host/testSort.cpp:24-136)
 // driving bitonicSortShared / bitonicMergeGlobal (kernel/aggregation/1/
 // group_by_cu_op_dp.cl:89-291): O(log^2 N) in-place passes over the array.
 // Here a Record sort is 4 passes over .y; ties keep input order (stable).
 #include "qo_device.hpp"
 #include "qo_internal.hpp"
+#include "qo_sort.hpp"
 
 namespace qo {
-
-constexpr int kSortItems = 16;
-constexpr int kSortTile = 256 * kSortItems;
 
 // Lanes of this wave that are active and hold the same 8-bit digit.
 QO_DEV uint64_t match_digit(uint32_t d, bool valid) {
@@ -36,57 +39,70 @@ QO_DEV uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w /* 4 */) {
     return inc - v + add;
 }
 
-// Element count of a device-count sort, clamped to the buffer capacity.
-QO_DEV size_t tile_limit(const uint64_t *d_n, size_t cap) {
-    size_t v = d_n ? (size_t)*d_n : cap;
-    return v < cap ? v : cap;
+// Segment that owns global tile `tile` (segments are ordered by tile0).
+QO_DEV RadixSeg seg_of(const RadixSeg &seg0, const RadixSeg *segs, int nseg, uint32_t tile) {
+    if (nseg <= 1) return seg0;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return segs[lo];
 }
 
-template <bool DEVN>
-__global__ __launch_bounds__(256) void k_radix_hist(const uint64_t *__restrict__ kin, size_t n_host,
-                                                    const uint64_t *d_n, int shift, uint32_t *hist,
-                                                    uint32_t num_tiles) {
+QO_DEV size_t seg_count(const RadixSeg &s) {
+    size_t v = s.d_n ? (size_t)*s.d_n : s.cap;
+    return v < s.cap ? v : s.cap;
+}
+
+__global__ __launch_bounds__(256) void k_radix_hist(const uint64_t *__restrict__ kin, RadixSeg seg0,
+                                                    const RadixSeg *segs, int nseg, int shift, uint32_t *hist) {
     __shared__ uint32_t s_h[4][256];
-    const size_t n = DEVN ? tile_limit(d_n, n_host) : n_host;
+    const uint32_t tile = blockIdx.x;
+    const RadixSeg sg = seg_of(seg0, segs, nseg, tile);
+    const uint32_t lt = tile - sg.tile0;
+    const size_t n = seg_count(sg);
     for (int i = threadIdx.x; i < 1024; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
-    const uint32_t tile = blockIdx.x;
     const unsigned w = wave_id(), lane = lane_id();
-    const size_t wbase = (size_t)tile * kSortTile + (size_t)w * (kSortTile / 4);
-    if ((size_t)tile * kSortTile < n) {
+    const size_t wbase = (size_t)lt * kSortTile + (size_t)w * (kSortTile / 4);
+    if ((size_t)lt * kSortTile < n) {
 #pragma unroll 4
         for (int j = 0; j < kSortItems; ++j) {
             size_t idx = wbase + (size_t)j * kWave + lane;
             bool valid = idx < n;
-            uint32_t d = valid ? (uint32_t)(kin[idx] >> shift) & 255u : 0u;
+            uint32_t d = valid ? (uint32_t)(kin[sg.base + idx] >> shift) & 255u : 0u;
             uint64_t m = match_digit(d, valid);
             if (valid && popc(m & lanemask_lt()) == 0) s_h[w][d] += popc(m);
         }
     }
     __syncthreads();
     const unsigned d = threadIdx.x;
-    hist[(size_t)d * num_tiles + tile] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+    hist[(size_t)sg.tile0 * 256 + (size_t)d * sg.ntile + lt] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
 }
 
-template <bool HAS_VAL, bool DEVN>
+template <bool HAS_VAL>
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restrict__ kin,
                                                        const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
-                                                       uint32_t *__restrict__ vout, size_t n_host, const uint64_t *d_n,
-                                                       int shift, const uint32_t *__restrict__ offs,
-                                                       uint32_t num_tiles) {
+                                                       uint32_t *__restrict__ vout, RadixSeg seg0,
+                                                       const RadixSeg *segs, int nseg, int shift,
+                                                       const uint32_t *__restrict__ offs) {
     __shared__ uint32_t s_cnt[4][256];
     __shared__ uint32_t s_dpre[256];
     __shared__ uint32_t s_gofs[256];
     __shared__ uint32_t s_w[4];
     __shared__ uint64_t s_k[kSortTile];
     __shared__ uint32_t s_v[HAS_VAL ? kSortTile : 1];
-    const size_t n = DEVN ? tile_limit(d_n, n_host) : n_host;
     const uint32_t tile = blockIdx.x;
-    if ((size_t)tile * kSortTile >= n) return;
+    const RadixSeg sg = seg_of(seg0, segs, nseg, tile);
+    const uint32_t lt = tile - sg.tile0;
+    const size_t n = seg_count(sg);
+    if ((size_t)lt * kSortTile >= n) return;
     for (int i = threadIdx.x; i < 1024; i += 256) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
     const unsigned w = wave_id(), lane = lane_id();
-    const size_t wbase = (size_t)tile * kSortTile + (size_t)w * (kSortTile / 4);
+    const size_t wbase = (size_t)lt * kSortTile + (size_t)w * (kSortTile / 4);
     uint64_t k[kSortItems];
     uint32_t v[kSortItems];
     uint32_t rank[kSortItems];
@@ -94,8 +110,8 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restric
     for (int j = 0; j < kSortItems; ++j) {
         size_t idx = wbase + (size_t)j * kWave + lane;
         bool valid = idx < n;
-        k[j] = valid ? kin[idx] : ~uint64_t(0);
-        if (HAS_VAL) v[j] = valid ? vin[idx] : 0u;
+        k[j] = valid ? kin[sg.base + idx] : ~uint64_t(0);
+        if (HAS_VAL) v[j] = valid ? vin[sg.base + idx] : 0u;
     }
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
@@ -113,7 +129,8 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restric
         const unsigned d = threadIdx.x;
         uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
         uint32_t tot = c0 + c1 + c2 + c3;
-        s_gofs[d] = offs[(size_t)d * num_tiles + tile];
+        const size_t h0 = (size_t)sg.tile0 * 256;
+        s_gofs[d] = offs[h0 + (size_t)d * sg.ntile + lt] - offs[h0];
         uint32_t ex = block_excl_scan256(tot, s_w);
         s_dpre[d] = ex;
         s_cnt[0][d] = 0;
@@ -133,25 +150,53 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restric
         }
     }
     __syncthreads();
-    const size_t tstart = (size_t)tile * kSortTile;
+    const size_t tstart = (size_t)lt * kSortTile;
     const uint32_t tcount = (uint32_t)((n - tstart) < (size_t)kSortTile ? (n - tstart) : (size_t)kSortTile);
     for (uint32_t i = threadIdx.x; i < tcount; i += 256) {
         uint64_t kk = s_k[i];
         uint32_t d = (uint32_t)(kk >> shift) & 255u;
-        size_t gp = (size_t)s_gofs[d] + (i - s_dpre[d]);
+        size_t gp = sg.base + (size_t)s_gofs[d] + (i - s_dpre[d]);
         kout[gp] = kk;
         if (HAS_VAL) vout[gp] = s_v[i];
     }
 }
 
-static size_t radix_tiles(size_t n) { return (n + kSortTile - 1) / kSortTile; }
+size_t radix_hist_bytes(uint32_t total_tiles) { return (size_t)total_tiles * 256 * 4 + 256; }
+
+int launch_radix_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &seg0,
+                          const RadixSeg *d_segs, int nseg, uint32_t total_tiles, int lo_bit, int hi_bit,
+                          uint32_t *hist, void *scan_ws, hipStream_t st, bool *in_alt) {
+    *in_alt = false;
+    if (total_tiles == 0) return QO_OK;
+    QO_REQUIRE(lo_bit >= 0 && hi_bit <= 64 && lo_bit < hi_bit, "bad bit range");
+    QO_REQUIRE((size_t)total_tiles * 256 < 0xFFFFFFFFull, "sort input too large");
+    uint64_t *ksrc = keys, *kdst = kalt;
+    uint32_t *vsrc = vals, *vdst = valt;
+    for (int sh = lo_bit; sh < hi_bit; sh += 8) {
+        k_radix_hist<<<total_tiles, 256, 0, st>>>(ksrc, seg0, d_segs, nseg, sh, hist);
+        QO_LAUNCHED();
+        QO_TRY(launch_scan_excl_u32(hist, hist, (size_t)total_tiles * 256, scan_ws, st));
+        if (vals)
+            k_radix_scatter<true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, seg0, d_segs, nseg, sh, hist);
+        else
+            k_radix_scatter<false><<<total_tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, seg0, d_segs, nseg, sh,
+                                                               hist);
+        QO_LAUNCHED();
+        std::swap(ksrc, kdst);
+        std::swap(vsrc, vdst);
+        *in_alt = !*in_alt;
+    }
+    return QO_OK;
+}
+
+static uint32_t radix_tiles(size_t n) { return (uint32_t)((n + kSortTile - 1) / kSortTile); }
 
 size_t radix_workspace_bytes(size_t n, bool with_vals) {
     size_t tiles = radix_tiles(n) + 1;
     size_t b = 0;
     b += n * 8 + 256;                    // alternate keys
     if (with_vals) b += n * 4 + 256;     // alternate values
-    b += tiles * 256 * 4 + 256;          // histogram / offsets
+    b += radix_hist_bytes((uint32_t)tiles) + 256;
     b += scan_workspace_bytes(tiles * 256) + 256;
     return b;
 }
@@ -159,44 +204,22 @@ size_t radix_workspace_bytes(size_t n, bool with_vals) {
 static int radix_impl(uint64_t *keys, uint32_t *vals, size_t n, const uint64_t *d_n, int lo_bit, int hi_bit,
                       void *ws, hipStream_t st) {
     if (n == 0) return QO_OK;
-    QO_REQUIRE(lo_bit >= 0 && hi_bit <= 64 && lo_bit < hi_bit, "bad bit range");
-    const size_t tiles = radix_tiles(n);
-    QO_REQUIRE(tiles * 256 < 0xFFFFFFFFull && n < 0xFFFFFFFFull, "sort input too large");
+    QO_REQUIRE(n < 0xFFFFFFFFull, "sort input too large");
+    const uint32_t tiles = radix_tiles(n);
     Carver cv(ws);
     uint64_t *kalt = cv.take<uint64_t>(n);
     uint32_t *valt = vals ? cv.take<uint32_t>(n) : nullptr;
-    uint32_t *hist = cv.take<uint32_t>(tiles * 256);
-    void *scan_ws = cv.take<char>(scan_workspace_bytes(tiles * 256));
-    uint64_t *ksrc = keys, *kdst = kalt;
-    uint32_t *vsrc = vals, *vdst = valt;
-    int passes = 0;
-    for (int sh = lo_bit; sh < hi_bit; sh += 8, ++passes) {
-        if (d_n) k_radix_hist<true><<<(unsigned)tiles, 256, 0, st>>>(ksrc, n, d_n, sh, hist, (uint32_t)tiles);
-        else k_radix_hist<false><<<(unsigned)tiles, 256, 0, st>>>(ksrc, n, d_n, sh, hist, (uint32_t)tiles);
-        QO_LAUNCHED();
-        QO_TRY(launch_scan_excl_u32(hist, hist, tiles * 256, scan_ws, st));
-        if (vals) {
-            if (d_n)
-                k_radix_scatter<true, true><<<(unsigned)tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, n, d_n, sh, hist,
-                                                                           (uint32_t)tiles);
-            else
-                k_radix_scatter<true, false><<<(unsigned)tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, n, d_n, sh,
-                                                                            hist, (uint32_t)tiles);
-        } else {
-            if (d_n)
-                k_radix_scatter<false, true><<<(unsigned)tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, n, d_n,
-                                                                            sh, hist, (uint32_t)tiles);
-            else
-                k_radix_scatter<false, false><<<(unsigned)tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, n, d_n,
-                                                                             sh, hist, (uint32_t)tiles);
-        }
-        QO_LAUNCHED();
-        std::swap(ksrc, kdst);
-        std::swap(vsrc, vdst);
-    }
-    if (passes & 1) {
-        QO_HIP(hipMemcpyAsync(keys, ksrc, n * 8, hipMemcpyDeviceToDevice, st));
-        if (vals) QO_HIP(hipMemcpyAsync(vals, vsrc, n * 4, hipMemcpyDeviceToDevice, st));
+    uint32_t *hist = cv.take<uint32_t>((size_t)tiles * 256);
+    void *scan_ws = cv.take<char>(scan_workspace_bytes((size_t)tiles * 256));
+    RadixSeg seg{0, n, d_n, 0, tiles};
+    bool in_alt = false;
+    QO_TRY(launch_radix_segments(keys, vals, kalt, valt, seg, nullptr, 1, tiles, lo_bit, hi_bit, hist, scan_ws, st,
+                                 &in_alt));
+    if (in_alt) {
+        // Odd number of passes: the result is in the alternate buffers.  The
+        // count may only be known on the device, so copy the full capacity.
+        QO_HIP(hipMemcpyAsync(keys, kalt, n * 8, hipMemcpyDeviceToDevice, st));
+        if (vals) QO_HIP(hipMemcpyAsync(vals, valt, n * 4, hipMemcpyDeviceToDevice, st));
     }
     return QO_OK;
 }
@@ -217,7 +240,7 @@ __global__ void k_pack_join(const int32_t *key, const int32_t *spay, const int32
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < m; i += stride) {
-        k64[i] = ((uint64_t)((uint32_t)key[i] ^ 0x80000000u) << 32) | ((uint32_t)spay[i] ^ 0x80000000u);
+        k64[i] = join_sort_key(key[i], spay[i]);
         v32[i] = (uint32_t)rpay[i];
     }
 }
@@ -228,8 +251,8 @@ __global__ void k_unpack_join(const uint64_t *k64, const uint32_t *v32, size_t m
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < m; i += stride) {
         uint64_t k = k64[i];
-        key[i] = (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
-        spay[i] = (int32_t)((uint32_t)k ^ 0x80000000u);
+        key[i] = join_sort_key_key(k);
+        spay[i] = join_sort_key_spay(k);
         rpay[i] = (int32_t)v32[i];
     }
 }

@@ -244,6 +244,73 @@ def hj_filter_sort(rkey, rpay, skey, spay, sfilt, flo: int, fhi: int, capacity: 
     return res
 
 
+class JoinBatch:
+    """A batch of filtered join+sort queries over device-resident columns,
+    captured once and replayed with one graph launch (qo_batch_*).
+
+    Each query is a dict of DeviceBuffers / ints: rkey, rpay, skey, spay, sfilt
+    (optional), nr, ns, flo, fhi, okey, orpay, ospay, cap."""
+
+    def __init__(self, queries):
+        arr = (_lib.QoJoinQuery * len(queries))()
+        for i, q in enumerate(queries):
+            a = arr[i]
+            a.rkey, a.rpay = q["rkey"].ptr, q["rpay"].ptr
+            a.skey, a.spay = q["skey"].ptr, q["spay"].ptr
+            a.sfilt = q["sfilt"].ptr if q.get("sfilt") is not None else None
+            a.nr, a.ns = int(q["nr"]), int(q["ns"])
+            a.flo, a.fhi = int(q["flo"]), int(q["fhi"])
+            a.okey, a.orpay, a.ospay = q["okey"].ptr, q["orpay"].ptr, q["ospay"].ptr
+            a.out_capacity = int(q["cap"])
+        h = C.c_void_p()
+        _lib.call("qo_batch_create", arr, len(queries), C.byref(h))
+        self.handle = h.value
+        self.nq = len(queries)
+
+    def launch(self, stream=None) -> None:
+        _lib.call("qo_batch_launch", self.handle, stream.handle if stream is not None else None)
+
+    def counts(self) -> np.ndarray:
+        out = np.zeros(self.nq, np.uint64)
+        _lib.call("qo_batch_counts", self.handle, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return out
+
+    def destroy(self) -> None:
+        if self.handle:
+            _lib.call("qo_batch_destroy", self.handle)
+            self.handle = None
+
+
+def batch_hj_filter_sort(queries):
+    """Host-array convenience wrapper: queries are tuples (rkey, rpay, skey, spay,
+    sfilt, flo, fhi); returns a list of (key, r_payload, s_payload)."""
+    dev, bufs = [], []
+    for rk, rp, sk, sp, sf, flo, fhi in queries:
+        rk, rp, sk, sp, sf = (_i32(a) for a in (rk, rp, sk, sp, sf))
+        d = {"rkey": DeviceBuffer.from_numpy(rk), "rpay": DeviceBuffer.from_numpy(rp),
+             "skey": DeviceBuffer.from_numpy(sk), "spay": DeviceBuffer.from_numpy(sp),
+             "sfilt": DeviceBuffer.from_numpy(sf), "nr": rk.shape[0], "ns": sk.shape[0],
+             "flo": flo, "fhi": fhi, "cap": sk.shape[0]}
+        for c in ("okey", "orpay", "ospay"):
+            d[c] = DeviceBuffer(max(sk.shape[0], 1) * 4)
+        dev.append(d)
+        bufs += [v for v in d.values() if isinstance(v, DeviceBuffer)]
+    b = JoinBatch(dev)
+    try:
+        b.launch()
+        counts = b.counts()
+        for d, m in zip(dev, counts):
+            if int(m) > d["cap"]:
+                raise _lib.QoError(3, "batch_hj_filter_sort", f"{int(m)} matches exceed capacity {d['cap']}")
+        out = [tuple(d[c].to_numpy(np.int32, int(m)) for c in ("okey", "orpay", "ospay"))
+               for d, m in zip(dev, counts)]
+    finally:
+        b.destroy()
+        for x in bufs:
+            x.free()
+    return out
+
+
 def sort_records(h_Rin) -> np.ndarray:
     rin = _records(h_Rin)
     with DeviceBuffer.from_numpy(rin) as d:
