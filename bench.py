@@ -346,6 +346,98 @@ def run_batched(args, qo, lib) -> dict:
     return res
 
 
+def _serve(qo, tables, schedule, plain: bool, args) -> dict:
+    """Submits `schedule` (list of (t_ms, request tuple)) at its arrival times and
+    returns per-request latencies measured on the server clock."""
+    from query_on_opencl_fpga_amd.serving import Server
+
+    sv = Server(budget_bytes=args.budget_gb * 1e9, inflight=args.inflight, batch=args.batch_max, plain=plain)
+    ids = [sv.add_table(t["n"], key=t["key"], fk=t["fk"], grp=t["grp"], val=t["val"], filt=t["filt"],
+                        group_capacity=t["G"]) for t in tables]
+    # Warm-up: one of each kind (workspaces and pools reach steady state).
+    warm = [(k, ids[i % len(ids)], ids[-1], INT_MIN, 499) for i, k in enumerate((0, 1, 2))]
+    for r in warm:
+        sv.submit(*r)
+    sv.drain()
+    sv.poll(timeout_ms=0)
+    base = sv.now_ms()
+    t_start = time.perf_counter()
+    for t_ms, (kind, a, b, lo, hi) in schedule:
+        delay = t_ms / 1e3 - (time.perf_counter() - t_start)
+        if delay > 0:
+            time.sleep(delay)
+        sv.submit(kind, ids[a], ids[b] if b >= 0 else -1, lo, hi)
+    sv.drain()
+    res = sv.poll(max_results=len(schedule) + 16, timeout_ms=0)
+    sv.destroy()
+    bad = [r for r in res if r["status"] != 0]
+    if bad:
+        raise SystemExit(f"{len(bad)} served requests failed, first: {bad[0]}")
+    lat = np.array([r["end_ms"] - r["submit_ms"] for r in res])
+    span = max(r["end_ms"] for r in res) - base
+    return {"queries_per_s": len(res) / (span * 1e-3), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "span_ms": span,
+            "checks": {r["id"]: (r["count"], r["sum"], r["checksum"]) for r in res}}
+
+
+def run_served(args, qo, lib) -> dict:
+    """Served mixed stream: Poisson arrivals of filter (40%), group-by (30%) and
+    join+sort (30%) over a pool of HBM-resident tables (sizes log-uniform in
+    [2^16, 2^26], group cardinality log-uniform in [16, 2^20], Zipf z=0.8),
+    continuous batching vs the plain per-query path on the same arrivals."""
+    from query_on_opencl_fpga_amd import data
+
+    seed = SEED_BASE + 5
+    rng = np.random.default_rng(seed & 0xFFFFFFFF)
+    tables = []
+    log(f"[bench] served: generating {args.tables} tables")
+    for i in range(args.tables):
+        n = int(2 ** rng.uniform(16, 16 + 10 * args.batch_scale))
+        G = int(2 ** rng.uniform(4, 20))
+        s = seed + 100 + i
+        tables.append({"n": n, "G": G, "key": data.gen_bij(n, 31, seed), "fk": data.gen_fk(n, s, 1, n, 31, seed),
+                       "grp": data.gen_zipf(n, G, 0.8, s, 2, s + 7), "val": data.gen_uniform(n, s, 3, 0, 1 << 16),
+                       "filt": data.gen_uniform(n, s, 4, 0, 1000)})
+    qo._lib.call("qo_device_sync")
+    sched, t = [], 0.0
+    for _ in range(args.requests):
+        t += rng.exponential(1000.0 / args.qps)
+        u = rng.random()
+        a = int(rng.integers(len(tables)))
+        hi = int(rng.integers(50, 500))
+        if u < 0.4:
+            req = (0, a, -1, INT_MIN, hi)
+        elif u < 0.7:
+            req = (1, a, -1, INT_MIN, hi)
+        else:
+            b = int(rng.integers(len(tables)))
+            lo_t, hi_t = sorted((a, b), key=lambda k: tables[k]["n"])
+            req = (2, lo_t, hi_t, INT_MIN, hi)
+        sched.append((t, req))
+    log(f"[bench] served: {args.requests} requests at {args.qps} qps (served mode)")
+    served = _serve(qo, tables, sched, False, args)
+    log("[bench] served: plain mode")
+    plain = _serve(qo, tables, sched, True, args)
+    same = sorted(served.pop("checks").values()) == sorted(plain.pop("checks").values())
+    if not same:
+        raise SystemExit("served and plain results differ")
+    for tb in tables:
+        for k in ("key", "fk", "grp", "val", "filt"):
+            tb[k].free()
+    return {
+        "metric": METRIC, "value": served["queries_per_s"], "unit": "queries/s (served)",
+        "ms_per_step": served["span_ms"], "dtype": "int32",
+        "config": {"workload": "served mixed stream (filter 40%, group-by 30%, join+sort 30%), Poisson arrivals",
+                   "offered_qps": args.qps, "requests": args.requests, "tables": args.tables,
+                   "table_rows_total": int(sum(tb["n"] for tb in tables)), "budget_gb": args.budget_gb,
+                   "inflight_batches": args.inflight, "max_batch": args.batch_max},
+        "served": served, "plain": plain,
+        "throughput_ratio_served_vs_plain": served["queries_per_s"] / plain["queries_per_s"],
+        "p99_ratio_plain_vs_served": plain["p99_ms"] / served["p99_ms"],
+        "results_identical": same,
+    }
+
+
 def run_groupby(args, qo, lib) -> dict:
     from query_on_opencl_fpga_amd import data
     from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
@@ -402,6 +494,12 @@ def main() -> int:
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--copy-nt", action="store_true", help="copy with non-temporal loads")
     ap.add_argument("--queries", type=int, default=512)
+    ap.add_argument("--qps", type=float, default=1000.0, help="served mode: offered load")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--tables", type=int, default=16)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--batch-max", type=int, default=64)
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="fraction of the log2 size range for batched queries (1.0 = [4K, 4M])")
     ap.add_argument("--nr", type=int, default=1 << 24)
@@ -424,7 +522,7 @@ def main() -> int:
     info = qo.device.device_info()
     log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']} LDS/block={info['lds_bytes_per_block']}")
     run = {"join": run_join, "filter_agg": run_filter_agg, "groupby": run_groupby, "copy": run_copy,
-           "batched": run_batched}
+           "batched": run_batched, "served": run_served}
     if args.config not in run:
         print(f"config {args.config} is not available", file=sys.stderr)
         return 2

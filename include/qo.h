@@ -136,8 +136,11 @@ int qo_aggregate_groups(const qo_record *sorted, size_t n, const int32_t *startP
 
 /* ---- hash group-by aggregation -------------------------------------------- */
 /* Per distinct key: (key, COUNT, SUM int64, MIN, MAX) ascending by key (signed).
- * group_capacity bounds the number of distinct keys (output arrays hold it);
- * more distinct keys than that gives QO_ERR_CAPACITY. */
+ * group_capacity bounds the number of distinct keys (output arrays hold it).
+ * The call is asynchronous: *d_numGroups (device memory) receives the number of
+ * groups, a value above group_capacity when there were more distinct keys
+ * (only the first group_capacity rows are then written; ~0 if the internal
+ * table overflowed).  The host checks it after synchronizing. */
 int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity,
                    int32_t *okey, int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax,
                    uint64_t *d_numGroups, qo_stream s);
@@ -197,40 +200,60 @@ int qo_tune_load(const char *path);
 int qo_tune_save(const char *path);
 
 /* ---- serving loop ------------------------------------------------------------ */
+/* A request loop over tables resident in HBM.  Submitted requests are admitted
+ * against a device-memory budget; whenever a stream slot is free, every admitted
+ * request not yet started is launched as one batch on that slot's stream
+ * (continuous batching), so batches overlap on the device.  With `plain` set the
+ * loop runs each request alone, in arrival order, through the per-query
+ * operator API (the baseline).  Results are summaries computed on the device. */
 typedef struct qo_server qo_server;
 typedef struct qo_server_config {
     uint64_t memory_budget_bytes;  /* device-memory admission budget */
-    int max_inflight_batches;      /* concurrent batches (streams) */
-    int max_batch_queries;         /* queries admitted into one batch */
+    int max_inflight_batches;      /* concurrent batches, one stream each */
+    int max_batch_queries;         /* requests launched together in one batch */
+    int plain;                     /* 1: no batching or overlap (per-query path) */
 } qo_server_config;
 enum { QO_Q_FILTER = 0, QO_Q_GROUPBY = 1, QO_Q_JOIN_SORT = 2 };
+/* Columns of a resident table (int32 device arrays owned by the caller; a
+ * column an operator does not use may be NULL). */
+typedef struct qo_table_desc {
+    const int32_t *key;      /* unique keys: join build side */
+    const int32_t *fk;       /* foreign keys: join probe side */
+    const int32_t *grp;      /* group-by key */
+    const int32_t *val;      /* value / payload */
+    const int32_t *filt;     /* filter column */
+    uint64_t n;              /* rows */
+    uint64_t group_capacity; /* bound on the distinct grp values */
+} qo_table_desc;
 typedef struct qo_request {
-    int kind;
-    int table_a;        /* filter / group-by input, join build (R) */
-    int table_b;        /* join probe (S) */
-    int32_t lo, hi;     /* filter predicate (filter: key; join: S filter column) */
-    uint64_t group_capacity;
+    int kind;           /* QO_Q_* */
+    int table_a;        /* filter / group-by input; join build side (key, val) */
+    int table_b;        /* join probe side (fk, val, filt) */
+    int32_t lo, hi;     /* predicate lo <= filt <= hi (filter and join) */
 } qo_request;
 typedef struct qo_result {
     uint64_t id;
-    int status;
+    int status;         /* QO_OK or the error of this request alone */
     int kind;
     uint64_t count;     /* filter: selected rows; group-by: groups; join: matches */
-    int64_t sum;        /* filter: SUM(value); group-by: SUM over all groups; join: checksum */
-    int64_t checksum;   /* order-sensitive checksum of the materialised output */
-    double submit_ms, start_ms, end_ms;  /* server clock */
+    int64_t sum;        /* filter: SUM(val); group-by: sum of the group SUMs; join: SUM(key) */
+    int64_t checksum;   /* order-sensitive checksum of the materialised, sorted output */
+    double submit_ms, start_ms, end_ms;  /* server clock (ms since creation) */
 } qo_result;
-/* Tables: columns are (key, val[, filt]) int32 device arrays owned by the caller. */
 int qo_server_create(const qo_server_config *cfg, qo_server **out);
-int qo_server_add_table(qo_server *sv, const int32_t *key, const int32_t *val, const int32_t *filt,
-                        uint64_t n, int *table_id);
+int qo_server_add_table(qo_server *sv, const qo_table_desc *table, int *table_id);
 int qo_server_submit(qo_server *sv, const qo_request *req, uint64_t *id);
-/* Wait for up to `max` results (timeout_ms < 0 waits for at least one). Returns count. */
+/* Copies up to `max` finished results; waits up to timeout_ms for at least one
+ * (timeout_ms < 0: wait until one is available or nothing is pending).
+ * Returns the number copied (>= 0) or a negative qo_status. */
 int qo_server_poll(qo_server *sv, qo_result *out, int max, int timeout_ms);
+/* Blocks until every submitted request has finished. */
 int qo_server_drain(qo_server *sv);
 int qo_server_destroy(qo_server *sv);
-/* Execute one request synchronously on the plain per-query path (same result). */
+/* Runs one request synchronously on the plain per-query path (same result). */
 int qo_server_run_plain(qo_server *sv, const qo_request *req, qo_result *out);
+/* Server clock now (ms since creation). */
+double qo_server_now_ms(qo_server *sv);
 
 #ifdef __cplusplus
 }

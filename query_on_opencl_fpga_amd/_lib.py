@@ -47,39 +47,6 @@ class QoJoinQuery(C.Structure):
     ]
 
 
-class QoServerConfig(C.Structure):
-    _fields_ = [
-        ("memory_budget_bytes", c_u64),
-        ("max_inflight_batches", C.c_int),
-        ("max_batch_queries", C.c_int),
-    ]
-
-
-class QoRequest(C.Structure):
-    _fields_ = [
-        ("kind", C.c_int),
-        ("table_a", C.c_int),
-        ("table_b", C.c_int),
-        ("lo", c_i32),
-        ("hi", c_i32),
-        ("group_capacity", c_u64),
-    ]
-
-
-class QoResult(C.Structure):
-    _fields_ = [
-        ("id", c_u64),
-        ("status", C.c_int),
-        ("kind", C.c_int),
-        ("count", c_u64),
-        ("sum", c_i64),
-        ("checksum", c_i64),
-        ("submit_ms", C.c_double),
-        ("start_ms", C.c_double),
-        ("end_ms", C.c_double),
-    ]
-
-
 class ScanPara(C.Structure):
     _fields_ = [("gLength", C.c_int), ("blockSize", C.c_int), ("pass_", C.c_int), ("ws", vp)]
 
@@ -151,14 +118,7 @@ PROTOS = {
     "qo_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "qo_tune_load": (C.c_int, [C.c_char_p]),
     "qo_tune_save": (C.c_int, [C.c_char_p]),
-    # server
-    "qo_server_create": (C.c_int, [P(QoServerConfig), P(vp)]),
-    "qo_server_add_table": (C.c_int, [vp, vp, vp, vp, c_u64, P(C.c_int)]),
-    "qo_server_submit": (C.c_int, [vp, P(QoRequest), P(c_u64)]),
-    "qo_server_poll": (C.c_int, [vp, P(QoResult), C.c_int, C.c_int]),
-    "qo_server_drain": (C.c_int, [vp]),
-    "qo_server_destroy": (C.c_int, [vp]),
-    "qo_server_run_plain": (C.c_int, [vp, P(QoRequest), P(QoResult)]),
+    # server prototypes: serving.py
     # compat (reference-named)
     "qo_mem_create": (vp, [c_size]),
     "qo_mem_wrap": (vp, [vp, c_size]),

@@ -22,27 +22,9 @@
 #include "qo_internal.hpp"
 #include "qo_join.hpp"
 #include "qo_sort.hpp"
+#include "qo_batch.hpp"
 
 namespace qo {
-
-constexpr uint32_t kBuildTile = 4096;
-
-struct BQuery {
-    const int32_t *rkey, *rpay, *skey, *spay, *sfilt;
-    uint64_t nr, ns;
-    int32_t flo, fhi;
-    int32_t *okey, *orpay, *ospay;
-    uint64_t out_cap;
-    JoinHeader *hdr;        // table header; slots follow it
-    uint64_t table_cap;     // slots
-    uint64_t m_base;        // first element of this query in the match arrays
-    unsigned long long *count;
-    uint32_t rtile0, nrtile;
-    uint32_t stile0, nstile;
-    uint32_t mtile0, nmtile;
-    uint32_t vec;           // S columns are 16-byte aligned
-    uint32_t pad;
-};
 
 // Last query whose first tile (selected by `which`) is <= t.
 template <int WHICH>
@@ -138,58 +120,15 @@ __global__ __launch_bounds__(256) void k_b_unpack(const BQuery *q, int nq, const
     }
 }
 
-}  // namespace qo
-
-struct qo_batch {
-    int nq = 0;
-    void *arena = nullptr;
-    qo::BQuery *d_q = nullptr;
-    std::vector<qo::BQuery> h_q;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipStream_t cap_stream = nullptr;
-    unsigned long long *d_counts = nullptr;
-};
-
-using namespace qo;
-
-static int batch_record(qo_batch *b, uint64_t *slots_all, uint64_t total_slots, uint32_t *small, size_t small_words,
-                        uint32_t rt, uint32_t stt, uint32_t mt, uint64_t *mkeys, uint32_t *mvals, uint64_t *kalt,
-                        uint32_t *valt, const RadixSeg &seg_first, RadixSeg *d_segs, uint32_t *hist,
-                        void *scan_ws, hipStream_t st) {
-    const unsigned persist = (unsigned)props().cu_count * 8;
-    QO_TRY(launch_fill_u64(slots_all, kEmptySlot, total_slots, st));
-    QO_TRY(launch_fill_u32(small, 0, small_words, st));
-    k_b_init<<<(b->nq + 255) / 256, 256, 0, st>>>(b->d_q, b->nq);
-    QO_LAUNCHED();
-    uint32_t *tickets = small;
-    if (rt) {
-        k_b_build<<<std::min<unsigned>(rt, persist), 256, 0, st>>>(b->d_q, b->nq, rt, tickets + 0);
-        QO_LAUNCHED();
-    }
-    if (stt) {
-        k_b_probe<<<std::min<unsigned>(stt, persist), 256, 0, st>>>(b->d_q, b->nq, stt, tickets + 1, mkeys, mvals);
-        QO_LAUNCHED();
-    }
-    if (mt) {
-        bool in_alt = false;
-        // With one query the sort takes its segment by value (seg_first).
-        QO_TRY(launch_radix_segments(mkeys, mvals, kalt, valt, seg_first, d_segs, b->nq, mt, 0, 64, hist, scan_ws,
-                                     st, &in_alt));
-        k_b_unpack<<<mt, 256, 0, st>>>(b->d_q, b->nq, in_alt ? kalt : mkeys, in_alt ? valt : mvals);
-        QO_LAUNCHED();
-    }
-    return QO_OK;
-}
-
-extern "C" {
-
-int qo_batch_create(const qo_join_query *queries, int nq, qo_batch **out) {
-    QO_ENTER();
-    QO_REQUIRE(out && nq >= 1 && queries, "bad batch arguments");
-    *out = nullptr;
-    std::vector<BQuery> hq(nq);
-    uint64_t total_slots = 0, total_match = 0;
+// ---------------------------------------------------------------------------
+// Plan of one join batch: sizes, then (after binding an arena) device pointers.
+// ---------------------------------------------------------------------------
+int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
+    QO_REQUIRE(nq >= 1 && queries, "bad batch arguments");
+    p.nq = nq;
+    p.hq.assign(nq, BQuery{});
+    p.hs.assign(nq, RadixSeg{});
+    p.total_slots = p.total_match = 0;
     uint64_t rt = 0, stt = 0, mt = 0;
     const int load = std::max(10, std::min(90, tune_get(TUNE_PROBE, 1 << 20).param));
     for (int i = 0; i < nq; ++i) {
@@ -197,7 +136,7 @@ int qo_batch_create(const qo_join_query *queries, int nq, qo_batch **out) {
         QO_REQUIRE((u.nr == 0 || (u.rkey && u.rpay)) && (u.ns == 0 || (u.skey && u.spay)),
                    "query %d: null input column", i);
         QO_REQUIRE(u.out_capacity == 0 || (u.okey && u.orpay && u.ospay), "query %d: null output column", i);
-        BQuery &q = hq[i];
+        BQuery &q = p.hq[i];
         memset(&q, 0, sizeof(q));
         q.rkey = u.rkey; q.rpay = u.rpay; q.skey = u.skey; q.spay = u.spay; q.sfilt = u.sfilt;
         q.nr = u.nr; q.ns = u.ns; q.flo = u.flo; q.fhi = u.fhi;
@@ -213,71 +152,129 @@ int qo_batch_create(const qo_join_query *queries, int nq, qo_batch **out) {
         q.nstile = (uint32_t)((u.ns + kProbeTile - 1) / kProbeTile);
         q.mtile0 = (uint32_t)mt;
         q.nmtile = (uint32_t)((u.out_capacity + kSortTile - 1) / kSortTile);
-        q.m_base = total_match;
+        q.m_base = p.total_match;
         rt += q.nrtile;
         stt += q.nstile;
         mt += q.nmtile;
-        total_slots += 32 + q.table_cap;  // 256-byte header = 32 slots
-        total_match += (u.out_capacity + 31) / 32 * 32;
+        p.total_slots += 32 + q.table_cap;  // 256-byte header = 32 slots
+        p.total_match += (u.out_capacity + 31) / 32 * 32;
     }
     QO_REQUIRE(rt < 0xFFFFFFFFull && stt < 0xFFFFFFFFull && mt * 256 < 0xFFFFFFFFull, "batch too large");
-    // Arena layout.
-    const size_t small_words = 64 + (size_t)nq * 2;
+    p.rt = (uint32_t)rt;
+    p.stt = (uint32_t)stt;
+    p.mt = (uint32_t)mt;
+    p.small_words = 64 + (size_t)nq * 2;
+    return QO_OK;
+}
+
+static void plan_carve(JoinBatchPlan &p, Carver &cv) {
+    p.slots_all = cv.take<uint64_t>(p.total_slots);
+    p.small = cv.take<uint32_t>(p.small_words);
+    p.d_q = cv.take<BQuery>(p.nq);
+    p.d_segs = cv.take<RadixSeg>(p.nq);
+    p.mkeys = cv.take<uint64_t>(p.total_match);
+    p.mvals = cv.take<uint32_t>(p.total_match);
+    p.kalt = cv.take<uint64_t>(p.total_match);
+    p.valt = cv.take<uint32_t>(p.total_match);
+    p.hist = cv.take<uint32_t>((size_t)p.mt * 256 + 1);
+    p.scan_ws = cv.take<char>(scan_workspace_bytes((size_t)p.mt * 256 + 1));
+}
+
+size_t join_plan_bytes(JoinBatchPlan &p) {
     Carver cv(nullptr);
-    cv.take<uint64_t>(total_slots);
-    cv.take<uint32_t>(small_words);
-    cv.take<BQuery>(nq);
-    cv.take<RadixSeg>(nq);
-    cv.take<uint64_t>(total_match);
-    cv.take<uint32_t>(total_match);
-    cv.take<uint64_t>(total_match);
-    cv.take<uint32_t>(total_match);
-    cv.take<uint32_t>(mt * 256 + 1);
-    cv.take<char>(scan_workspace_bytes(mt * 256 + 1));
+    plan_carve(p, cv);
+    return cv.off + 256;
+}
+
+void join_plan_bind(JoinBatchPlan &p, void *arena) {
+    Carver cv(arena);
+    plan_carve(p, cv);
+    uint64_t slot_off = 0;
+    p.counts = reinterpret_cast<unsigned long long *>(p.small + 64);
+    for (int i = 0; i < p.nq; ++i) {
+        BQuery &q = p.hq[i];
+        q.hdr = reinterpret_cast<JoinHeader *>(p.slots_all + slot_off);
+        slot_off += 32 + q.table_cap;
+        q.count = p.counts + i;
+        p.hs[i] = RadixSeg{q.m_base, q.out_cap, reinterpret_cast<const uint64_t *>(q.count), q.mtile0, q.nmtile};
+    }
+}
+
+int join_plan_upload(const JoinBatchPlan &p, hipStream_t st) {
+    QO_HIP(hipMemcpyAsync(p.d_q, p.hq.data(), sizeof(BQuery) * p.nq, hipMemcpyHostToDevice, st));
+    QO_HIP(hipMemcpyAsync(p.d_segs, p.hs.data(), sizeof(RadixSeg) * p.nq, hipMemcpyHostToDevice, st));
+    return QO_OK;
+}
+
+int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
+    const unsigned persist = (unsigned)props().cu_count * 8;
+    QO_TRY(launch_fill_u64(p.slots_all, kEmptySlot, p.total_slots, st));
+    QO_TRY(launch_fill_u32(p.small, 0, p.small_words, st));
+    k_b_init<<<(p.nq + 255) / 256, 256, 0, st>>>(p.d_q, p.nq);
+    QO_LAUNCHED();
+    uint32_t *tickets = p.small;
+    if (p.rt) {
+        k_b_build<<<std::min<unsigned>(p.rt, persist), 256, 0, st>>>(p.d_q, p.nq, p.rt, tickets + 0);
+        QO_LAUNCHED();
+    }
+    if (p.stt) {
+        k_b_probe<<<std::min<unsigned>(p.stt, persist), 256, 0, st>>>(p.d_q, p.nq, p.stt, tickets + 1, p.mkeys,
+                                                                      p.mvals);
+        QO_LAUNCHED();
+    }
+    if (p.mt) {
+        bool in_alt = false;
+        // With one query the sort takes its segment by value (hs[0]).
+        QO_TRY(launch_radix_segments(p.mkeys, p.mvals, p.kalt, p.valt, p.hs[0], p.d_segs, p.nq, p.mt, 0, 64, p.hist,
+                                     p.scan_ws, st, &in_alt));
+        k_b_unpack<<<p.mt, 256, 0, st>>>(p.d_q, p.nq, in_alt ? p.kalt : p.mkeys, in_alt ? p.valt : p.mvals);
+        QO_LAUNCHED();
+    }
+    return QO_OK;
+}
+
+}  // namespace qo
+
+struct qo_batch {
+    void *arena = nullptr;
+    qo::JoinBatchPlan plan;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+};
+
+using namespace qo;
+
+extern "C" {
+
+int qo_batch_create(const qo_join_query *queries, int nq, qo_batch **out) {
+    QO_ENTER();
+    QO_REQUIRE(out, "null output");
+    *out = nullptr;
     qo_batch *b = new qo_batch;
-    b->nq = nq;
-    hipError_t e = hipMalloc(&b->arena, cv.off + 256);
+    int rc = join_plan_init(queries, nq, b->plan);
+    if (rc != QO_OK) {
+        delete b;
+        return rc;
+    }
+    const size_t bytes = join_plan_bytes(b->plan);
+    hipError_t e = hipMalloc(&b->arena, bytes);
     if (e != hipSuccess) {
         delete b;
-        return set_error(QO_ERR_NO_MEMORY, "batch arena of %zu bytes: %s", cv.off, hipGetErrorString(e));
+        return set_error(QO_ERR_NO_MEMORY, "batch arena of %zu bytes: %s", bytes, hipGetErrorString(e));
     }
-    Carver c2(b->arena);
-    uint64_t *slots_all = c2.take<uint64_t>(total_slots);
-    uint32_t *small = c2.take<uint32_t>(small_words);
-    b->d_q = c2.take<BQuery>(nq);
-    RadixSeg *d_segs = c2.take<RadixSeg>(nq);
-    uint64_t *mkeys = c2.take<uint64_t>(total_match);
-    uint32_t *mvals = c2.take<uint32_t>(total_match);
-    uint64_t *kalt = c2.take<uint64_t>(total_match);
-    uint32_t *valt = c2.take<uint32_t>(total_match);
-    uint32_t *hist = c2.take<uint32_t>(mt * 256 + 1);
-    void *scan_ws = c2.take<char>(scan_workspace_bytes(mt * 256 + 1));
-    // Per-query pointers into the arena.
-    std::vector<RadixSeg> hs(nq);
-    uint64_t slot_off = 0;
-    unsigned long long *counts = reinterpret_cast<unsigned long long *>(small + 64);
-    for (int i = 0; i < nq; ++i) {
-        BQuery &q = hq[i];
-        q.hdr = reinterpret_cast<JoinHeader *>(slots_all + slot_off);
-        slot_off += 32 + q.table_cap;
-        q.count = counts + i;
-        hs[i] = RadixSeg{q.m_base, q.out_cap, reinterpret_cast<const uint64_t *>(q.count), q.mtile0, q.nmtile};
-    }
-    b->h_q = hq;
-    b->d_counts = counts;
-    int rc = QO_OK;
-    if (hipMemcpy(b->d_q, hq.data(), sizeof(BQuery) * nq, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_segs, hs.data(), sizeof(RadixSeg) * nq, hipMemcpyHostToDevice) != hipSuccess)
-        rc = set_error(QO_ERR_HIP, "batch descriptor upload failed");
-    if (rc == QO_OK && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess)
+    join_plan_bind(b->plan, b->arena);
+    if (hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess)
         rc = set_error(QO_ERR_HIP, "batch stream");
+    if (rc == QO_OK) rc = join_plan_upload(b->plan, b->cap_stream);
+    if (rc == QO_OK && hipStreamSynchronize(b->cap_stream) != hipSuccess)
+        rc = set_error(QO_ERR_HIP, "batch descriptor upload failed");
     if (rc == QO_OK) {
         e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) rc = set_error(QO_ERR_HIP, "begin capture: %s", hipGetErrorString(e));
     }
     if (rc == QO_OK) {
-        int rec = batch_record(b, slots_all, total_slots, small, small_words, (uint32_t)rt, (uint32_t)stt,
-                               (uint32_t)mt, mkeys, mvals, kalt, valt, hs[0], d_segs, hist, scan_ws, b->cap_stream);
+        int rec = join_plan_launch(b->plan, b->cap_stream);
         e = hipStreamEndCapture(b->cap_stream, &b->graph);
         if (rec != QO_OK) rc = rec;
         else if (e != hipSuccess) rc = set_error(QO_ERR_HIP, "end capture: %s", hipGetErrorString(e));
@@ -305,7 +302,7 @@ int qo_batch_counts(qo_batch *b, uint64_t *h_counts) {
     QO_TRY(ensure_init());
     QO_REQUIRE(b && h_counts, "null argument");
     QO_HIP(hipDeviceSynchronize());
-    QO_HIP(hipMemcpy(h_counts, b->d_counts, 8 * (size_t)b->nq, hipMemcpyDeviceToHost));
+    QO_HIP(hipMemcpy(h_counts, b->plan.counts, 8 * (size_t)b->plan.nq, hipMemcpyDeviceToHost));
     return QO_OK;
 }
 

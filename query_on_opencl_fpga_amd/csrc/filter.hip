@@ -346,7 +346,8 @@ __global__ __launch_bounds__(256) void k_scan_excl(const uint32_t *in, uint32_t 
 }
 
 size_t scan_workspace_bytes(size_t n) {
-    const size_t per_tile = 4 * 256 * 4;
+    // Sized for the unaligned path (one element per load), which has the most tiles.
+    const size_t per_tile = 4 * 256;
     const size_t tiles = (n + per_tile - 1) / per_tile + 1;
     return tiles * 8 + 512;
 }

@@ -88,6 +88,12 @@ int launch_radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, const uint64_t *d_
 size_t sort_join_rows_ws_bytes(size_t m);
 int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void *ws, hipStream_t st);
 
+// Hash group-by with caller workspace (groupby.hip).
+size_t groupby_ws_bytes(size_t group_capacity);
+int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity, int32_t *okey,
+                   int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax, uint64_t *d_num, void *ws,
+                   hipStream_t st);
+
 // Join table helpers (join.hip).
 size_t join_table_slots(size_t table_bytes);
 int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,

@@ -1,0 +1,705 @@
+// Serving loop: a scheduler thread keeps requests flowing over HBM-resident
+// tables.  Requests are admitted against a device-memory budget; whenever one
+// of `max_inflight_batches` stream slots is free, every admitted request not
+// yet started is launched as ONE batch on that slot's stream (continuous
+// batching): filters of the batch share one multi-query kernel, joins share
+// the batched join pipeline (qo_batch.hpp), group-bys run their kernels
+// back to back on the same stream.  Batches on different slots overlap on the
+// device.  Completion is observed through an event per batch; each request's
+// summary (count, sum, checksum of its materialised output) is computed on the
+// device and copied back with the batch.
+//
+// The reference has no serving layer (host/host/main.cpp runs one operator per
+// process); this is the serving mode of the operator library.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "qo_batch.hpp"
+#include "qo_internal.hpp"
+#include "qo_rng.hpp"
+
+namespace qo {
+
+struct ResSlot {
+    unsigned long long count;
+    unsigned long long sum;
+    unsigned long long checksum;
+    unsigned long long pad;
+};
+
+// Order-sensitive row checksum: sum over rows of mix(row contents, row index).
+QO_DEV uint64_t row_term(uint64_t a, uint64_t b, uint64_t i) { return mix64(mix64(a + kGamma * (i + 1)) ^ b); }
+
+__global__ void k_s_checksum_join(const int32_t *okey, const int32_t *orpay, const int32_t *ospay,
+                                  const unsigned long long *count, uint64_t cap, ResSlot *res) {
+    uint64_t n = *count;
+    if (n > cap) n = cap;
+    uint64_t cs = 0, sum = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t a = ((uint64_t)(uint32_t)okey[i] << 32) | (uint32_t)orpay[i];
+        cs += row_term(a, (uint32_t)ospay[i], i);
+        sum += (uint64_t)(int64_t)okey[i];
+    }
+    cs = wave_sum(cs);
+    sum = wave_sum(sum);
+    if (lane_id() == 0) {
+        atomicAdd(&res->checksum, (unsigned long long)cs);
+        atomicAdd(&res->sum, (unsigned long long)sum);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) res->count = *count;
+}
+
+__global__ void k_s_checksum_groupby(const int32_t *okey, const int64_t *ocount, const int64_t *osum,
+                                     const int32_t *omin, const int32_t *omax, const uint64_t *d_num, uint64_t cap,
+                                     ResSlot *res) {
+    uint64_t n = *d_num;
+    if (n > cap) n = cap;
+    uint64_t cs = 0, sum = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t a = ((uint64_t)(uint32_t)okey[i] << 32) ^ (uint64_t)ocount[i];
+        uint64_t b = (uint64_t)osum[i] ^ (((uint64_t)(uint32_t)omin[i] << 32) | (uint32_t)omax[i]);
+        cs += row_term(a, b, i);
+        sum += (uint64_t)osum[i];
+    }
+    cs = wave_sum(cs);
+    sum = wave_sum(sum);
+    if (lane_id() == 0) {
+        atomicAdd(&res->checksum, (unsigned long long)cs);
+        atomicAdd(&res->sum, (unsigned long long)sum);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) res->count = *d_num;
+}
+
+// Filter + aggregate of many requests in one persistent launch: each tile is
+// kFilterTile rows of one request; per-tile COUNT / SUM go to the request's slot.
+struct FQuery {
+    const int32_t *filt, *val;
+    uint64_t n;
+    int32_t lo, hi;
+    ResSlot *res;
+    uint32_t tile0, ntile;
+};
+constexpr uint32_t kFilterTile = 16384;
+
+__global__ __launch_bounds__(256) void k_s_filter(const FQuery *q, int nq, uint32_t total, uint32_t *ticket) {
+    __shared__ uint32_t s_t;
+    __shared__ uint64_t s_c[4], s_s[4];
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t t = s_t;
+        __syncthreads();
+        if (t >= total) break;
+        int lo = 0, hi = nq - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (q[mid].tile0 <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const FQuery &Q = q[lo];
+        const uint64_t r0 = (uint64_t)(t - Q.tile0) * kFilterTile;
+        const uint64_t r1 = r0 + kFilterTile < Q.n ? r0 + kFilterTile : Q.n;
+        uint64_t c = 0, s = 0;
+        for (uint64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+            int32_t f = Q.filt[i];
+            bool p = (f >= Q.lo) & (f <= Q.hi);
+            c += p;
+            s += p ? (uint64_t)(int64_t)Q.val[i] : 0;
+        }
+        c = wave_sum(c);
+        s = wave_sum(s);
+        if (lane_id() == 0) {
+            s_c[wave_id()] = c;
+            s_s[wave_id()] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(&Q.res->count, (unsigned long long)(s_c[0] + s_c[1] + s_c[2] + s_c[3]));
+            atomicAdd(&Q.res->sum, (unsigned long long)(s_s[0] + s_s[1] + s_s[2] + s_s[3]));
+        }
+    }
+}
+
+struct Table {
+    qo_table_desc d;
+};
+
+struct Req {
+    uint64_t id;
+    qo_request r;
+    size_t est;
+    double submit_ms = 0, start_ms = 0, end_ms = 0;
+    int status = QO_OK;
+    ResSlot res{};
+};
+
+// Caching device allocator: blocks are kept after use and reused for requests
+// of up to twice their size, so steady-state serving makes no hipMalloc calls.
+class DevPool {
+  public:
+    void *get(size_t bytes, int *status) {
+        bytes = (bytes + (1 << 21) - 1) & ~size_t((1 << 21) - 1);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.lower_bound(bytes);
+            if (it != free_.end() && it->first <= 2 * bytes) {
+                void *p = it->second;
+                size_[p] = it->first;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            // Give cached blocks back and retry once.
+            trim();
+            e = hipMalloc(&p, bytes);
+        }
+        if (e != hipSuccess) {
+            *status = set_error(QO_ERR_NO_MEMORY, "server pool: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+            return nullptr;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        size_[p] = bytes;
+        return p;
+    }
+    void put(void *p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(size_[p], p);
+    }
+    void trim() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto &kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+    }
+    ~DevPool() { trim(); }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> size_;
+};
+
+struct Batch {
+    std::vector<std::shared_ptr<Req>> reqs;
+    int slot = -1;
+    void *arena = nullptr;
+    ResSlot *d_res = nullptr;
+    ResSlot *h_res = nullptr;  // pinned
+    hipEvent_t done = nullptr;
+    JoinBatchPlan jplan;
+    std::vector<qo_join_query> jq;
+    std::vector<FQuery> fq;
+    FQuery *d_fq = nullptr;
+    size_t est = 0;
+};
+
+}  // namespace qo
+
+using namespace qo;
+
+struct qo_server {
+    qo_server_config cfg{};
+    std::chrono::steady_clock::time_point t0;
+    std::vector<Table> tables;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Req>> pending, admitted;
+    std::vector<std::unique_ptr<Batch>> inflight;
+    std::vector<qo_result> results;
+    uint64_t next_id = 1;
+    uint64_t outstanding = 0;  // submitted, not finished
+    size_t in_use = 0;
+    bool stop = false;
+    std::vector<hipStream_t> streams;
+    std::vector<bool> slot_busy;
+    hipStream_t plain_stream = nullptr;
+    DevPool pool;
+    std::thread worker;
+
+    double now() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace qo {
+
+// Device bytes one request needs (its outputs, workspaces and join tables).
+static size_t request_bytes(qo_server *sv, const qo_request &r) {
+    const Table &A = sv->tables[r.table_a];
+    if (r.kind == QO_Q_FILTER) return 4096;
+    if (r.kind == QO_Q_GROUPBY) {
+        size_t G = A.d.group_capacity ? A.d.group_capacity : 1;
+        return G * 28 + groupby_ws_bytes(G) + 8192;
+    }
+    const Table &B = sv->tables[r.table_b];
+    qo_join_query q{};
+    q.rkey = A.d.key; q.rpay = A.d.val; q.skey = B.d.fk; q.spay = B.d.val; q.sfilt = B.d.filt;
+    q.nr = A.d.n; q.ns = B.d.n; q.flo = r.lo; q.fhi = r.hi;
+    q.okey = q.orpay = q.ospay = reinterpret_cast<int32_t *>(16);
+    q.out_capacity = B.d.n;
+    JoinBatchPlan p;
+    if (join_plan_init(&q, 1, p) != QO_OK) return ~size_t(0);
+    return join_plan_bytes(p) + B.d.n * 12 + 4096;
+}
+
+static int validate(qo_server *sv, const qo_request &r) {
+    const int nt = (int)sv->tables.size();
+    QO_REQUIRE(r.kind >= QO_Q_FILTER && r.kind <= QO_Q_JOIN_SORT, "unknown request kind %d", r.kind);
+    QO_REQUIRE(r.table_a >= 0 && r.table_a < nt, "bad table_a %d", r.table_a);
+    const qo_table_desc &A = sv->tables[r.table_a].d;
+    if (r.kind == QO_Q_FILTER) QO_REQUIRE(A.filt && A.val, "filter needs filt and val columns");
+    if (r.kind == QO_Q_GROUPBY) QO_REQUIRE(A.grp && A.val && A.group_capacity, "group-by needs grp and val columns");
+    if (r.kind == QO_Q_JOIN_SORT) {
+        QO_REQUIRE(r.table_b >= 0 && r.table_b < nt, "bad table_b %d", r.table_b);
+        const qo_table_desc &B = sv->tables[r.table_b].d;
+        QO_REQUIRE(A.key && A.val && B.fk && B.val && B.filt, "join needs key/val on A and fk/val/filt on B");
+    }
+    return QO_OK;
+}
+
+static const unsigned kGrid = 512;
+
+// Enqueues one batch on its stream; returns a status for the whole launch.
+static int launch_batch(qo_server *sv, Batch &b, hipStream_t st) {
+    const int n = (int)b.reqs.size();
+    // Per-request failure isolation: an injected fault fails that request only.
+    std::vector<std::shared_ptr<Req>> live;
+    for (auto &r : b.reqs) {
+        if (consume_injected_fault()) {
+            r->status = QO_ERR_INJECTED;
+            continue;
+        }
+        live.push_back(r);
+    }
+    // Layout: result slots, filter descriptors, group-by outputs/workspaces, join plan + outputs.
+    std::vector<int> fi, gi, ji;
+    for (int i = 0; i < (int)live.size(); ++i) {
+        int k = live[i]->r.kind;
+        (k == QO_Q_FILTER ? fi : k == QO_Q_GROUPBY ? gi : ji).push_back(i);
+    }
+    b.jq.clear();
+    for (int i : ji) {
+        const qo_request &r = live[i]->r;
+        const qo_table_desc &A = sv->tables[r.table_a].d, &B = sv->tables[r.table_b].d;
+        qo_join_query q{};
+        q.rkey = A.key; q.rpay = A.val; q.skey = B.fk; q.spay = B.val; q.sfilt = B.filt;
+        q.nr = A.n; q.ns = B.n; q.flo = r.lo; q.fhi = r.hi;
+        q.out_capacity = B.n;
+        b.jq.push_back(q);
+    }
+    Carver cv(nullptr);
+    cv.take<ResSlot>(n);
+    cv.take<FQuery>(fi.size() + 1);
+    cv.take<uint32_t>(64);
+    std::vector<size_t> gsz;
+    for (int i : gi) {
+        size_t G = sv->tables[live[i]->r.table_a].d.group_capacity;
+        cv.take<int32_t>(G); cv.take<int64_t>(G); cv.take<int64_t>(G); cv.take<int32_t>(G); cv.take<int32_t>(G);
+        cv.take<uint64_t>(1);
+        cv.take<char>(groupby_ws_bytes(G));
+        gsz.push_back(G);
+    }
+    for (auto &q : b.jq) {
+        cv.take<int32_t>(q.out_capacity); cv.take<int32_t>(q.out_capacity); cv.take<int32_t>(q.out_capacity);
+    }
+    size_t jbytes = 0;
+    if (!b.jq.empty()) {
+        // Output pointers are bound below; sizing does not read them.
+        for (auto &q : b.jq) q.okey = q.orpay = q.ospay = reinterpret_cast<int32_t *>(16);
+        QO_TRY(join_plan_init(b.jq.data(), (int)b.jq.size(), b.jplan));
+        jbytes = join_plan_bytes(b.jplan);
+    }
+    cv.take<char>(jbytes);
+    int status = QO_OK;
+    b.arena = sv->pool.get(cv.off + 256, &status);
+    if (!b.arena) return status;
+    // Bind.
+    Carver c2(b.arena);
+    b.d_res = c2.take<ResSlot>(n);
+    b.d_fq = c2.take<FQuery>(fi.size() + 1);
+    uint32_t *tickets = c2.take<uint32_t>(64);
+    QO_TRY(launch_fill_u32(b.d_res, 0, (size_t)n * sizeof(ResSlot) / 4, st));
+    QO_TRY(launch_fill_u32(tickets, 0, 64, st));
+    // Slot index of each live request in d_res = its position in b.reqs.
+    auto slot_of = [&](const std::shared_ptr<Req> &r) {
+        for (int i = 0; i < n; ++i)
+            if (b.reqs[i] == r) return i;
+        return 0;
+    };
+    // Filters: one multi-request kernel.
+    if (!fi.empty()) {
+        b.fq.clear();
+        uint32_t t0 = 0;
+        for (int i : fi) {
+            const qo_request &r = live[i]->r;
+            const qo_table_desc &A = sv->tables[r.table_a].d;
+            FQuery f{A.filt, A.val, A.n, r.lo, r.hi, b.d_res + slot_of(live[i]), t0,
+                     (uint32_t)((A.n + kFilterTile - 1) / kFilterTile)};
+            t0 += f.ntile;
+            b.fq.push_back(f);
+        }
+        QO_HIP(hipMemcpyAsync(b.d_fq, b.fq.data(), sizeof(FQuery) * b.fq.size(), hipMemcpyHostToDevice, st));
+        if (t0) {
+            k_s_filter<<<std::min<unsigned>(t0, (unsigned)props().cu_count * 8), 256, 0, st>>>(b.d_fq, (int)b.fq.size(),
+                                                                                              t0, tickets);
+            QO_LAUNCHED();
+        }
+    }
+    // Group-bys: back to back on this stream.
+    for (size_t k = 0; k < gi.size(); ++k) {
+        const qo_request &r = live[gi[k]]->r;
+        const qo_table_desc &A = sv->tables[r.table_a].d;
+        size_t G = gsz[k];
+        int32_t *okey = c2.take<int32_t>(G);
+        int64_t *ocnt = c2.take<int64_t>(G);
+        int64_t *osum = c2.take<int64_t>(G);
+        int32_t *omin = c2.take<int32_t>(G);
+        int32_t *omax = c2.take<int32_t>(G);
+        uint64_t *num = c2.take<uint64_t>(1);
+        void *ws = c2.take<char>(groupby_ws_bytes(G));
+        QO_TRY(launch_groupby(A.grp, A.val, A.n, G, okey, ocnt, osum, omin, omax, num, ws, st));
+        k_s_checksum_groupby<<<std::min<unsigned>(kGrid, (unsigned)((G + 255) / 256) + 1), 256, 0, st>>>(
+            okey, ocnt, osum, omin, omax, num, G, b.d_res + slot_of(live[gi[k]]));
+        QO_LAUNCHED();
+    }
+    // Joins: the batched pipeline.
+    if (!b.jq.empty()) {
+        for (auto &q : b.jq) {
+            q.okey = c2.take<int32_t>(q.out_capacity);
+            q.orpay = c2.take<int32_t>(q.out_capacity);
+            q.ospay = c2.take<int32_t>(q.out_capacity);
+        }
+        QO_TRY(join_plan_init(b.jq.data(), (int)b.jq.size(), b.jplan));
+        void *jarena = c2.take<char>(jbytes);
+        join_plan_bind(b.jplan, jarena);
+        QO_TRY(join_plan_upload(b.jplan, st));
+        QO_TRY(join_plan_launch(b.jplan, st));
+        for (size_t k = 0; k < ji.size(); ++k) {
+            const qo_join_query &q = b.jq[k];
+            k_s_checksum_join<<<std::min<unsigned>(kGrid, (unsigned)((q.out_capacity + 255) / 256) + 1), 256, 0, st>>>(
+                q.okey, q.orpay, q.ospay, b.jplan.counts + k, q.out_capacity, b.d_res + slot_of(live[ji[k]]));
+            QO_LAUNCHED();
+        }
+    }
+    QO_HIP(hipMemcpyAsync(b.h_res, b.d_res, sizeof(ResSlot) * n, hipMemcpyDeviceToHost, st));
+    QO_HIP(hipEventRecord(b.done, st));
+    return QO_OK;
+}
+
+static void finish_batch(qo_server *sv, Batch &b, int launch_status) {
+    const double t = sv->now();
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (size_t i = 0; i < b.reqs.size(); ++i) {
+        Req &r = *b.reqs[i];
+        qo_result out{};
+        out.id = r.id;
+        out.kind = r.r.kind;
+        out.status = r.status != QO_OK ? r.status : launch_status;
+        if (out.status == QO_OK) {
+            out.count = b.h_res[i].count;
+            out.sum = (int64_t)b.h_res[i].sum;
+            out.checksum = (int64_t)b.h_res[i].checksum;
+            if (r.r.kind == QO_Q_GROUPBY && out.count > sv->tables[r.r.table_a].d.group_capacity)
+                out.status = QO_ERR_CAPACITY;
+            if (r.r.kind == QO_Q_JOIN_SORT && out.count > sv->tables[r.r.table_b].d.n) out.status = QO_ERR_CAPACITY;
+        }
+        out.submit_ms = r.submit_ms;
+        out.start_ms = r.start_ms;
+        out.end_ms = t;
+        sv->results.push_back(out);
+        sv->outstanding--;
+    }
+    sv->in_use -= b.est;
+    if (b.slot >= 0) sv->slot_busy[b.slot] = false;
+    sv->cv_done.notify_all();
+    sv->cv_work.notify_all();
+}
+
+// Runs one request alone through the per-query operator API, synchronously.
+static int run_plain(qo_server *sv, const qo_request &r, qo_result &out, hipStream_t st) {
+    out = qo_result{};
+    out.kind = r.kind;
+    QO_TRY(validate(sv, r));
+    if (consume_injected_fault()) return set_error(QO_ERR_INJECTED, "injected fault");
+    const qo_table_desc &A = sv->tables[r.table_a].d;
+    int status = QO_OK;
+    size_t bytes = sizeof(ResSlot) + 4096;
+    if (r.kind == QO_Q_GROUPBY) bytes += A.group_capacity * 28 + 1024;
+    if (r.kind == QO_Q_JOIN_SORT) bytes += sv->tables[r.table_b].d.n * 12 + qo_join_table_bytes(A.n) + 2048;
+    void *mem = sv->pool.get(bytes, &status);
+    if (!mem) return status;
+    Carver cv(mem);
+    ResSlot *d_res = cv.take<ResSlot>(1);
+    uint64_t m = 0;  // join match count; read by an async copy before the final sync
+    int rc = launch_fill_u32(d_res, 0, sizeof(ResSlot) / 4, st);
+    if (rc == QO_OK && r.kind == QO_Q_FILTER) {
+        int64_t *o = cv.take<int64_t>(3);
+        rc = qo_filter_agg(A.filt, A.val, A.n, r.lo, r.hi, o, st);
+        // COUNT and SUM land in the first two words of the slot.
+        if (rc == QO_OK && hipMemcpyAsync(d_res, o, 16, hipMemcpyDeviceToDevice, st) != hipSuccess)
+            rc = set_error(QO_ERR_HIP, "copy");
+    } else if (rc == QO_OK && r.kind == QO_Q_GROUPBY) {
+        size_t G = A.group_capacity;
+        int32_t *okey = cv.take<int32_t>(G);
+        int64_t *ocnt = cv.take<int64_t>(G);
+        int64_t *osum = cv.take<int64_t>(G);
+        int32_t *omin = cv.take<int32_t>(G);
+        int32_t *omax = cv.take<int32_t>(G);
+        uint64_t *num = cv.take<uint64_t>(1);
+        rc = qo_groupby_agg(A.grp, A.val, A.n, G, okey, ocnt, osum, omin, omax, num, st);
+        if (rc == QO_OK) {
+            k_s_checksum_groupby<<<std::min<unsigned>(kGrid, (unsigned)((G + 255) / 256) + 1), 256, 0, st>>>(
+                okey, ocnt, osum, omin, omax, num, G, d_res);
+            if (hipGetLastError() != hipSuccess) rc = set_error(QO_ERR_HIP, "checksum launch");
+        }
+    } else if (rc == QO_OK) {
+        const qo_table_desc &B = sv->tables[r.table_b].d;
+        size_t cap = B.n;
+        int32_t *okey = cv.take<int32_t>(cap);
+        int32_t *orp = cv.take<int32_t>(cap);
+        int32_t *osp = cv.take<int32_t>(cap);
+        unsigned long long *cnt = cv.take<unsigned long long>(1);
+        size_t tb = qo_join_table_bytes(A.n);
+        void *table = cv.take<char>(tb);
+        rc = qo_hj_filter_sort(table, tb, A.key, A.val, A.n, B.fk, B.val, B.filt, B.n, r.lo, r.hi, okey, orp, osp,
+                               cap, &m, st);
+        if (rc == QO_OK && hipMemcpyAsync(cnt, &m, 8, hipMemcpyHostToDevice, st) != hipSuccess)
+            rc = set_error(QO_ERR_HIP, "copy");
+        if (rc == QO_OK) {
+            k_s_checksum_join<<<std::min<unsigned>(kGrid, (unsigned)((cap + 255) / 256) + 1), 256, 0, st>>>(
+                okey, orp, osp, cnt, cap, d_res);
+            if (hipGetLastError() != hipSuccess) rc = set_error(QO_ERR_HIP, "checksum launch");
+        }
+    }
+    ResSlot h{};
+    if (rc == QO_OK && hipMemcpyAsync(&h, d_res, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = set_error(QO_ERR_HIP, "copy back");
+    if (hipStreamSynchronize(st) != hipSuccess && rc == QO_OK) rc = set_error(QO_ERR_HIP, "sync");
+    sv->pool.put(mem);
+    if (rc != QO_OK) return rc;
+    out.count = h.count;
+    out.sum = (int64_t)h.sum;
+    out.checksum = (int64_t)h.checksum;
+    return QO_OK;
+}
+
+static void scheduler(qo_server *sv) {
+    (void)ensure_init();
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (true) {
+        // 1. Completions.
+        bool progressed = false;
+        for (size_t i = 0; i < sv->inflight.size();) {
+            Batch &b = *sv->inflight[i];
+            hipError_t q = hipEventQuery(b.done);
+            if (q == hipErrorNotReady) {
+                ++i;
+                continue;
+            }
+            std::unique_ptr<Batch> done = std::move(sv->inflight[i]);
+            sv->inflight.erase(sv->inflight.begin() + i);
+            lk.unlock();
+            finish_batch(sv, *done, q == hipSuccess ? QO_OK : QO_ERR_HIP);
+            sv->pool.put(done->arena);
+            (void)hipEventDestroy(done->done);
+            (void)hipHostFree(done->h_res);
+            lk.lock();
+            progressed = true;
+        }
+        if (sv->stop && sv->pending.empty() && sv->admitted.empty() && sv->inflight.empty()) break;
+        // 2. Admission (FIFO) against the budget.
+        while (!sv->pending.empty()) {
+            auto &r = sv->pending.front();
+            if (r->est > sv->cfg.memory_budget_bytes) {
+                auto rej = r;
+                sv->pending.pop_front();
+                qo_result out{};
+                out.id = rej->id;
+                out.kind = rej->r.kind;
+                out.status = QO_ERR_BUDGET;
+                out.submit_ms = rej->submit_ms;
+                out.start_ms = out.end_ms = sv->now();
+                sv->results.push_back(out);
+                sv->outstanding--;
+                sv->cv_done.notify_all();
+                progressed = true;
+                continue;
+            }
+            if (sv->in_use + r->est > sv->cfg.memory_budget_bytes) break;
+            sv->in_use += r->est;
+            sv->admitted.push_back(r);
+            sv->pending.pop_front();
+            progressed = true;
+        }
+        // 3. Launch: one batch per free slot, from everything admitted so far.
+        for (int s = 0; s < (int)sv->streams.size() && !sv->admitted.empty(); ++s) {
+            if (sv->slot_busy[s]) continue;
+            auto b = std::make_unique<Batch>();
+            const int lim = sv->cfg.plain ? 1 : std::max(1, sv->cfg.max_batch_queries);
+            while (!sv->admitted.empty() && (int)b->reqs.size() < lim) {
+                b->est += sv->admitted.front()->est;
+                b->reqs.push_back(sv->admitted.front());
+                sv->admitted.pop_front();
+            }
+            b->slot = s;
+            sv->slot_busy[s] = true;
+            const double t = sv->now();
+            for (auto &r : b->reqs) r->start_ms = t;
+            lk.unlock();
+            int rc = QO_OK;
+            if (sv->cfg.plain) {
+                // Baseline: the request alone through the per-query API.
+                qo_result out{};
+                rc = run_plain(sv, b->reqs[0]->r, out, sv->streams[s]);
+                b->reqs[0]->status = rc;
+                ResSlot h{out.count, (unsigned long long)out.sum, (unsigned long long)out.checksum, 0};
+                b->h_res = &h;
+                finish_batch(sv, *b, QO_OK);
+                b->h_res = nullptr;
+                lk.lock();
+                progressed = true;
+                continue;
+            }
+            if (hipHostMalloc(&b->h_res, sizeof(ResSlot) * b->reqs.size(), hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
+                rc = set_error(QO_ERR_HIP, "batch bookkeeping");
+            if (rc == QO_OK) rc = launch_batch(sv, *b, sv->streams[s]);
+            lk.lock();
+            if (rc != QO_OK) {
+                // Launch failed before the event: make sure queued work is done, then fail the batch.
+                lk.unlock();
+                (void)hipStreamSynchronize(sv->streams[s]);
+                if (b->h_res) memset(b->h_res, 0, sizeof(ResSlot) * b->reqs.size());
+                finish_batch(sv, *b, rc);
+                sv->pool.put(b->arena);
+                if (b->done) (void)hipEventDestroy(b->done);
+                if (b->h_res) (void)hipHostFree(b->h_res);
+                lk.lock();
+            } else {
+                sv->inflight.push_back(std::move(b));
+            }
+            progressed = true;
+        }
+        if (!progressed) sv->cv_work.wait_for(lk, std::chrono::microseconds(sv->inflight.empty() ? 2000 : 50));
+    }
+}
+
+}  // namespace qo
+
+extern "C" {
+
+int qo_server_create(const qo_server_config *cfg, qo_server **out) {
+    QO_TRY(ensure_init());
+    QO_REQUIRE(cfg && out, "null argument");
+    // Switched off: a served mix of requests (three batches in flight) hung
+    // without a fault and no cause has been found in this code; the loop stays
+    // off until one is shown.
+    if (cfg) return set_error(QO_ERR_NOT_FOUND, "the serving loop is disabled (a served mix hung; cause not found)");
+    QO_REQUIRE(cfg->max_inflight_batches >= 1 && cfg->max_inflight_batches <= 16, "max_inflight_batches in [1, 16]");
+    auto sv = new qo_server;
+    sv->cfg = *cfg;
+    if (sv->cfg.plain) sv->cfg.max_inflight_batches = 1;
+    sv->t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < sv->cfg.max_inflight_batches; ++i) {
+        hipStream_t st;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            delete sv;
+            return set_error(QO_ERR_HIP, "server stream");
+        }
+        sv->streams.push_back(st);
+        sv->slot_busy.push_back(false);
+    }
+    if (hipStreamCreateWithFlags(&sv->plain_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete sv;
+        return set_error(QO_ERR_HIP, "server stream");
+    }
+    sv->worker = std::thread(scheduler, sv);
+    *out = sv;
+    return QO_OK;
+}
+
+int qo_server_add_table(qo_server *sv, const qo_table_desc *t, int *table_id) {
+    QO_REQUIRE(sv && t && table_id, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->tables.push_back(Table{*t});
+    *table_id = (int)sv->tables.size() - 1;
+    return QO_OK;
+}
+
+int qo_server_submit(qo_server *sv, const qo_request *req, uint64_t *id) {
+    QO_REQUIRE(sv && req && id, "null argument");
+    auto r = std::make_shared<Req>();
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        QO_TRY(validate(sv, *req));
+        r->r = *req;
+        r->est = request_bytes(sv, *req);
+        r->id = sv->next_id++;
+        r->submit_ms = sv->now();
+        *id = r->id;
+        sv->pending.push_back(r);
+        sv->outstanding++;
+    }
+    sv->cv_work.notify_all();
+    return QO_OK;
+}
+
+int qo_server_poll(qo_server *sv, qo_result *out, int max, int timeout_ms) {
+    QO_REQUIRE(sv && (max == 0 || out), "null argument");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto ready = [&] { return !sv->results.empty() || sv->outstanding == 0; };
+    if (timeout_ms < 0) sv->cv_done.wait(lk, ready);
+    else sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    int n = std::min<int>(max, (int)sv->results.size());
+    for (int i = 0; i < n; ++i) out[i] = sv->results[i];
+    sv->results.erase(sv->results.begin(), sv->results.begin() + n);
+    return n;
+}
+
+int qo_server_drain(qo_server *sv) {
+    QO_REQUIRE(sv, "null server");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv_done.wait(lk, [&] { return sv->outstanding == 0; });
+    return QO_OK;
+}
+
+int qo_server_destroy(qo_server *sv) {
+    if (!sv) return QO_OK;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    for (auto st : sv->streams) (void)hipStreamDestroy(st);
+    if (sv->plain_stream) (void)hipStreamDestroy(sv->plain_stream);
+    delete sv;
+    return QO_OK;
+}
+
+int qo_server_run_plain(qo_server *sv, const qo_request *req, qo_result *out) {
+    QO_TRY(ensure_init());
+    QO_REQUIRE(sv && req && out, "null argument");
+    qo_result r{};
+    r.submit_ms = r.start_ms = sv->now();
+    int rc = run_plain(sv, *req, r, sv->plain_stream);
+    r.status = rc;
+    r.end_ms = sv->now();
+    *out = r;
+    return rc;
+}
+
+double qo_server_now_ms(qo_server *sv) { return sv ? sv->now() : 0.0; }
+
+}  // extern "C"
