@@ -114,7 +114,7 @@ def run_join(args, qo, lib) -> dict:
         qo._lib.call("qo_hash_build", table.ptr, tb, t.rkey.ptr, t.rpay.ptr, nr, st.handle)
         if ev:
             ev[1].record(st)
-        qo._lib.call("qo_hash_probe", table.ptr, t.skey.ptr, t.spay.ptr, t.sfilt.ptr, ns, INT_MIN, 49,
+        qo._lib.call("qo_hash_probe", table.ptr, tb, t.skey.ptr, t.spay.ptr, t.sfilt.ptr, ns, INT_MIN, 49,
                      outs[0].ptr, outs[1].ptr, outs[2].ptr, cap, cnt.ptr, st.handle)
         if ev:
             ev[2].record(st)
@@ -491,6 +491,9 @@ def main() -> int:
     ap.add_argument("--config", default="join",
                     choices=["join", "filter_agg", "groupby", "batched", "served", "copy"])
     ap.add_argument("--load", type=int, default=0, help="join table load factor in percent (default: tuned)")
+    ap.add_argument("--join-mode", choices=["auto", "partitioned", "direct"], default="auto",
+                    help="join build/probe strategy (default: partitioned once the table outgrows one L2)")
+    ap.add_argument("--slice-kib", type=int, default=0, help="partitioned join: table slice per partition")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--copy-nt", action="store_true", help="copy with non-temporal loads")
     ap.add_argument("--queries", type=int, default=512)
@@ -521,6 +524,13 @@ def main() -> int:
     qo._lib.call("qo_init", 0)
     info = qo.device.device_info()
     log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']} LDS/block={info['lds_bytes_per_block']}")
+    if args.join_mode != "auto" or args.slice_kib:
+        import ctypes as C
+        mode = {"auto": 0, "partitioned": 1, "direct": 2}[args.join_mode]
+        for cls in range(40):
+            b, i, p = C.c_int(), C.c_int(), C.c_int()
+            qo._lib.call("qo_tune_get", 5, cls, C.byref(b), C.byref(i), C.byref(p))
+            qo._lib.call("qo_tune_set", 5, cls, b.value, args.slice_kib or i.value, mode)
     run = {"join": run_join, "filter_agg": run_filter_agg, "groupby": run_groupby, "copy": run_copy,
            "batched": run_batched, "served": run_served}
     if args.config not in run:

@@ -148,15 +148,19 @@ int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t grou
 /* ---- hash join ------------------------------------------------------------ */
 /* Bytes of device memory a build table for `build_rows` rows needs. */
 size_t qo_join_table_bytes(size_t build_rows);
+/* Builds the table in `table` (256-byte aligned device memory of table_bytes,
+ * e.g. qo_join_table_bytes(nr)).  Tables larger than one L2 are built
+ * partition by partition through a per-stream workspace. */
 int qo_hash_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay,
                   size_t nr, qo_stream s);
 /* Probe with the S filter flo <= sfilt <= fhi fused (sfilt may be NULL: no filter).
- * Matches are written as (key, r_payload, s_payload): each tile of 4096 S tuples
- * keeps S order, tiles follow in completion order; *d_count = matches (when it
- * exceeds out_capacity only out_capacity rows are written). */
-int qo_hash_probe(const void *table, const int32_t *skey, const int32_t *spay, const int32_t *sfilt,
-                  size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay, int32_t *ospay,
-                  size_t out_capacity, uint64_t *d_count, qo_stream s);
+ * table_bytes is the size the table was built with (it selects the direct or the
+ * partitioned probe).  Matches are written as (key, r_payload, s_payload) in an
+ * unspecified row order; *d_count = matches (when it exceeds out_capacity only
+ * out_capacity rows are written). */
+int qo_hash_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
+                  const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey,
+                  int32_t *orpay, int32_t *ospay, size_t out_capacity, uint64_t *d_count, qo_stream s);
 /* Filter S, build on R, probe, sort output by (key, s_payload).  Synchronous:
  * *h_count receives the match count. */
 int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay,
@@ -193,7 +197,9 @@ int qo_batch_counts(qo_batch *b, uint64_t *h_counts);
 int qo_batch_destroy(qo_batch *b);
 
 /* ---- autotuner -------------------------------------------------------------- */
-/* Operators: 0 filter_agg, 1 filter, 2 probe, 3 groupby, 4 sort. Size class = floor(log2 n). */
+/* Operators: 0 filter_agg, 1 filter, 2 probe (param = table load %), 3 groupby, 4 sort,
+ * 5 partitioned join (items = table slice KiB per partition, param = 0 when the
+ * table outgrows one L2 / 1 always / 2 never).  Size class = floor(log2 n). */
 int qo_tune_get(int op, int size_class, int *block, int *items, int *param);
 int qo_tune_set(int op, int size_class, int block, int items, int param);
 int qo_tune_load(const char *path);

@@ -103,7 +103,7 @@ PROTOS = {
     # join
     "qo_join_table_bytes": (c_size, [c_size]),
     "qo_hash_build": (C.c_int, [vp, c_size, vp, vp, c_size, vp]),
-    "qo_hash_probe": (C.c_int, [vp, vp, vp, vp, c_size, c_i32, c_i32, vp, vp, vp, c_size, vp, vp]),
+    "qo_hash_probe": (C.c_int, [vp, c_size, vp, vp, vp, c_size, c_i32, c_i32, vp, vp, vp, c_size, vp, vp]),
     "qo_hj_filter_sort": (C.c_int, [vp, c_size, vp, vp, c_size, vp, vp, vp, c_size, c_i32, c_i32,
                                     vp, vp, vp, c_size, P(c_u64), vp]),
     "qo_bucket_build": (C.c_int, [vp, c_u32, vp, c_u32, c_u32, vp, vp]),

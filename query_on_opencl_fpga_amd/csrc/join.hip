@@ -1,15 +1,20 @@
-// Hash join: open-addressing build, probe with the S filter fused and a stable
-// (S-order) compaction of matches, plus the reference-shaped bucket-chained
-// build/probe kept for drop-in callers.
+// Hash join: open-addressing build, probe with the S filter fused, plus the
+// reference-shaped bucket-chained build/probe kept for drop-in callers.
 //
 // Reference path replaced (kernel/hj_filter_sort/1/hj_filter_sort_op.cl):
 // buildHashTable (:25-60) is one 128-item work-group inserting under local
 // spin-locks into fixed-capacity buckets; probeHashTable (:62-109) scans a
 // bucket per S tuple and writes (s_val, r_val) pairs into per-work-item slots
-// that are never compacted.  Here the build is a grid-wide CAS insert into a
-// bandwidth-sized open-addressing table and the probe streams S once, filters,
-// looks up, and appends each tile's matches (in S order) at a range reserved
-// with one atomic add.
+// that are never compacted.
+//
+// Two strategies, picked from the table size:
+//  * direct (tables up to one L2, or small inputs), here: a grid-wide CAS
+//    insert, and a probe that streams S once, filters, looks up and appends
+//    each tile's matches at a range reserved with one atomic add;
+//  * partitioned (larger tables), join_part.hip: the inputs are split by key
+//    hash so that every lookup and insert happens in an LDS copy of one table
+//    slice.
+// Output order is unspecified: rows appear in per-tile reservation order.
 #include <algorithm>
 
 #include "qo_internal.hpp"
@@ -39,8 +44,10 @@ __global__ __launch_bounds__(256) void k_join_build(const int32_t *__restrict__ 
     for (; i < nr; i += stride) join_insert(slots, cap, hdr, rkey[i], rpay[i]);
 }
 
+size_t join_build_ws_bytes(size_t table_bytes, size_t nr) { return join_part_ws_bytes(table_bytes, nr, true); }
+
 int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
-                      hipStream_t st) {
+                      void *ws, hipStream_t st) {
     const size_t cap = join_table_slots(table_bytes);
     QO_REQUIRE(cap >= 2 && cap <= 0xFFFFFFFFull, "join table of %zu bytes has an unsupported size", table_bytes);
     QO_REQUIRE(nr < cap, "join table with %zu slots cannot hold %zu rows", cap, nr);
@@ -48,12 +55,15 @@ int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, cons
     uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
     k_join_init<<<1, 64, 0, st>>>(hdr, cap, nr);
     QO_LAUNCHED();
+    // The partitioned build writes every slot itself.
+    bool used = false;
+    QO_TRY(launch_join_part_build(table, table_bytes, rkey, rpay, nr, ws, st, &used));
+    if (used) return QO_OK;
     QO_TRY(launch_fill_u64(slots, kEmptySlot, cap, st));
-    if (nr) {
-        unsigned g = grid_for(nr, 256, (unsigned)props().cu_count * 16);
-        k_join_build<<<g, 256, 0, st>>>(rkey, rpay, nr, slots, hdr, cap);
-        QO_LAUNCHED();
-    }
+    if (!nr) return QO_OK;
+    unsigned g = grid_for(nr, 256, (unsigned)props().cu_count * 16);
+    k_join_build<<<g, 256, 0, st>>>(rkey, rpay, nr, slots, hdr, cap);
+    QO_LAUNCHED();
     return QO_OK;
 }
 
@@ -71,11 +81,13 @@ __global__ __launch_bounds__(256) void k_join_probe(const JoinHeader *__restrict
                            reinterpret_cast<unsigned long long *>(d_count), emit, s_tot, &s_base);
 }
 
-size_t join_probe_ws_bytes(size_t) { return 256; }
+size_t join_probe_ws_bytes(size_t table_bytes, size_t ns) {
+    return std::max<size_t>(256, join_part_ws_bytes(table_bytes, ns, false));
+}
 
-int launch_join_probe(const void *table, const int32_t *skey, const int32_t *spay, const int32_t *sfilt, size_t ns,
-                      int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay, int32_t *ospay, size_t cap,
-                      uint64_t *d_count, void *, hipStream_t st) {
+int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
+                      const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
+                      int32_t *ospay, size_t cap, uint64_t *d_count, void *ws, hipStream_t st) {
     QO_TRY(launch_fill_u64(d_count, 0, 1, st));
     if (ns == 0) return QO_OK;
     const size_t tiles = (ns + kProbeTile - 1) / kProbeTile;
@@ -84,6 +96,10 @@ int launch_join_probe(const void *table, const int32_t *skey, const int32_t *spa
     const uint64_t *slots = reinterpret_cast<const uint64_t *>(hdr + 1);
     const bool vec = ((uintptr_t)skey % 16 == 0) && ((uintptr_t)spay % 16 == 0) && (!sfilt || (uintptr_t)sfilt % 16 == 0);
     SoaEmit emit{okey, orpay, ospay, cap};
+    bool used = false;
+    QO_TRY(launch_join_part_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, cap,
+                                  d_count, ws, st, &used));
+    if (used) return QO_OK;
     if (sfilt)
         k_join_probe<true><<<(unsigned)tiles, 256, 0, st>>>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, emit,
                                                             d_count);
@@ -190,21 +206,29 @@ int qo_hash_build(void *table, size_t table_bytes, const int32_t *rkey, const in
     QO_ENTER();
     QO_REQUIRE(table && (nr == 0 || (rkey && rpay)), "null pointer");
     QO_REQUIRE((uintptr_t)table % 256 == 0, "join table must be 256-byte aligned");
-    return launch_join_build(table, table_bytes, rkey, rpay, nr, stream_of(s));
+    hipStream_t st = stream_of(s);
+    void *ws = nullptr;
+    if (const size_t wb = join_build_ws_bytes(table_bytes, nr)) {
+        int status = QO_OK;
+        ws = workspace(st, wb, &status);
+        if (!ws) return status;
+    }
+    return launch_join_build(table, table_bytes, rkey, rpay, nr, ws, st);
 }
 
-int qo_hash_probe(const void *table, const int32_t *skey, const int32_t *spay, const int32_t *sfilt, size_t ns,
-                  int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay, int32_t *ospay, size_t out_capacity,
-                  uint64_t *d_count, qo_stream s) {
+int qo_hash_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
+                  const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
+                  int32_t *ospay, size_t out_capacity, uint64_t *d_count, qo_stream s) {
     QO_ENTER();
     QO_REQUIRE(table && d_count && (ns == 0 || (skey && spay)), "null pointer");
     QO_REQUIRE(out_capacity == 0 || (okey && orpay && ospay), "null output column");
+    QO_REQUIRE(join_table_slots(table_bytes) >= 2, "join table of %zu bytes is too small", table_bytes);
     hipStream_t st = stream_of(s);
     int status = QO_OK;
-    void *ws = workspace(st, join_probe_ws_bytes(ns), &status);
+    void *ws = workspace(st, join_probe_ws_bytes(table_bytes, ns), &status);
     if (!ws) return status;
-    return launch_join_probe(table, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, out_capacity, d_count, ws,
-                             st);
+    return launch_join_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, out_capacity,
+                             d_count, ws, st);
 }
 
 int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
@@ -214,24 +238,26 @@ int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, cons
     QO_ENTER();
     QO_REQUIRE(table && h_count, "null pointer");
     hipStream_t st = stream_of(s);
-    QO_TRY(launch_join_build(table, table_bytes, rkey, rpay, nr, st));
+    // One workspace serves the three stages in stream order: build buckets,
+    // probe buckets, then the sort (after the count has been read back).
+    const size_t stage_bytes = std::max({join_build_ws_bytes(table_bytes, nr), join_probe_ws_bytes(table_bytes, ns),
+                                         sort_join_rows_ws_bytes(out_capacity)});
     int status = QO_OK;
-    size_t ws_bytes = std::max(join_probe_ws_bytes(ns) + 256, sort_join_rows_ws_bytes(out_capacity) + 256);
-    void *ws = workspace(st, ws_bytes, &status);
+    void *ws = workspace(st, stage_bytes + 512, &status);
     if (!ws) return status;
     Carver cv(ws);
     uint64_t *d_count = cv.take<uint64_t>(1);
-    void *pws = cv.take<char>(join_probe_ws_bytes(ns));
-    QO_TRY(launch_join_probe(table, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, out_capacity, d_count, pws,
-                             st));
+    void *sws = cv.take<char>(stage_bytes);
+    QO_TRY(launch_join_build(table, table_bytes, rkey, rpay, nr, sws, st));
+    QO_TRY(launch_join_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, out_capacity,
+                             d_count, sws, st));
     uint64_t m = 0;
     QO_HIP(hipMemcpyAsync(&m, d_count, 8, hipMemcpyDeviceToHost, st));
     QO_HIP(hipStreamSynchronize(st));
     *h_count = m;
     if (m > out_capacity) return set_error(QO_ERR_CAPACITY, "join produced %llu rows, capacity %zu",
                                            (unsigned long long)m, out_capacity);
-    // The sort workspace reuses the whole buffer: the probe is finished.
-    QO_TRY(sort_join_rows_ws(okey, orpay, ospay, m, ws, st));
+    QO_TRY(sort_join_rows_ws(okey, orpay, ospay, m, sws, st));
     QO_HIP(hipStreamSynchronize(st));
     return QO_OK;
 }

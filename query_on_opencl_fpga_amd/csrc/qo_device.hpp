@@ -138,6 +138,17 @@ QO_DEV uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t aggregate) {
     return excl;
 }
 
+// Exclusive scan of one value per thread over a 256-thread block.  s_w holds 4
+// words; the caller must __syncthreads() before reusing it.
+QO_DEV uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w) {
+    uint32_t inc = wave_incl_scan(v);
+    if (lane_id() == kWave - 1) s_w[wave_id()] = inc;
+    __syncthreads();
+    uint32_t add = 0;
+    for (unsigned w = 0; w < wave_id(); ++w) add += s_w[w];
+    return inc - v + add;
+}
+
 // 32-bit finaliser of MurmurHash3 (Appleby): spreads sequential keys.
 QO_DEV uint32_t hash32(uint32_t k) {
     k ^= k >> 16;

@@ -52,7 +52,17 @@ struct TuneParams {
     int items;
     int param;
 };
-enum TuneOp { TUNE_FILTER_AGG = 0, TUNE_FILTER = 1, TUNE_PROBE = 2, TUNE_GROUPBY = 3, TUNE_SORT = 4, TUNE_NUM_OPS = 5 };
+enum TuneOp {
+    TUNE_FILTER_AGG = 0,
+    TUNE_FILTER = 1,
+    TUNE_PROBE = 2,
+    TUNE_GROUPBY = 3,
+    TUNE_SORT = 4,
+    TUNE_JOIN_PART = 5,  // partitioned build/probe: items = table slice KiB per partition, param = mode
+    TUNE_NUM_OPS = 6
+};
+// TUNE_JOIN_PART modes: partition when the table outgrows one L2, always, never.
+enum JoinPartMode { JOIN_PART_AUTO = 0, JOIN_PART_ALWAYS = 1, JOIN_PART_NEVER = 2 };
 TuneParams tune_get(int op, size_t n);
 
 inline int size_class(size_t n) {
@@ -96,12 +106,23 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
 
 // Join table helpers (join.hip).
 size_t join_table_slots(size_t table_bytes);
+// Build and probe take the table size so they can pick the partitioned path
+// (workspace sized by the matching *_ws_bytes; may be nullptr when that is 0).
+size_t join_build_ws_bytes(size_t table_bytes, size_t nr);
 int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
-                      hipStream_t st);
-size_t join_probe_ws_bytes(size_t ns);
-int launch_join_probe(const void *table, const int32_t *skey, const int32_t *spay, const int32_t *sfilt,
-                      size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay, int32_t *ospay,
-                      size_t cap, uint64_t *d_count, void *ws, hipStream_t st);
+                      void *ws, hipStream_t st);
+size_t join_probe_ws_bytes(size_t table_bytes, size_t ns);
+// Partitioned (per-slice LDS) build and probe, join_part.hip.  *used = false
+// when the table is small enough for the direct path (nothing launched).
+size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build);
+int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
+                           void *ws, hipStream_t st, bool *used);
+int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
+                           const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
+                           int32_t *ospay, size_t out_cap, uint64_t *d_count, void *ws, hipStream_t st, bool *used);
+int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
+                      const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
+                      int32_t *ospay, size_t cap, uint64_t *d_count, void *ws, hipStream_t st);
 
 }  // namespace qo
 

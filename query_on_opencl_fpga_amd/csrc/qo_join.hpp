@@ -7,6 +7,11 @@
 // (r_payload << 32) | key; kEmptySlot marks a free slot.  The one tuple whose
 // packed form equals kEmptySlot (key = -1, payload = -1) is counted in the
 // header instead of being stored.
+//
+// Because the home slot is monotone in hash32(key), the keys whose hash has
+// top bits p (of L) all have home slots in [p*cap/2^L, (p+1)*cap/2^L]: a
+// contiguous 1/2^L of the table.  The partitioned build and probe use this to
+// work on one table slice at a time in LDS.
 #pragma once
 #include "qo_device.hpp"
 
@@ -80,28 +85,133 @@ QO_DEV bool join_find(const uint64_t *slots, uint64_t cap, int32_t key, uint64_t
 constexpr int kProbeJ = 4;                                   // int4 groups per thread
 constexpr size_t kProbeTile = (size_t)kProbeJ * 256 * 4;     // S tuples per tile
 
-// Probe one tile of S: tuples [tile*kProbeTile, ...) of (skey, spay, sfilt).
-// Filters, looks up, reserves the tile's output range with one atomic add on
-// `counter` (the running match count) and emits the tile's matches there in S
-// order (for one S tuple, in table-chain order).  Tiles land in completion
-// order: a look-back across tiles would make every tile wait for the slowest
-// lookup of its predecessors.  Emit(pos, key, r_payload, s_payload).
-// Must be called by all 256 threads of the block; s_tot holds kProbeJ*4 words.
-template <bool HAS_FILTER, class Emit>
-QO_DEV void probe_tile(const JoinHeader *__restrict__ hdr, const uint64_t *__restrict__ slots,
-                       const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
-                       const int32_t *__restrict__ sfilt, size_t ns, int32_t flo, int32_t fhi, bool vec,
-                       uint32_t tile, unsigned long long *counter, const Emit &emit, uint32_t *s_tot,
-                       uint64_t *s_excl) {
-    constexpr int J = kProbeJ;
-    constexpr int NW = 4;
-    const uint64_t cap_slots = hdr->cap;
-    const bool general = (hdr->has_dups | hdr->sentinel_count) != 0;
-    const uint32_t sentinel = hdr->sentinel_count;
-    const size_t vbase = (size_t)tile * (J * 256);
+// Emits every match of `key` from position `pos` on, walking the whole chain
+// (table-chain order, then the (-1, -1) build tuples counted in the header).
+template <class Emit>
+QO_DEV uint64_t emit_chain(const uint64_t *slots, uint64_t cap, uint32_t sentinel, int32_t key, int32_t sp,
+                           uint64_t pos, const Emit &emit) {
+    uint64_t h = join_home(key, cap);
+    for (uint64_t probe = 0; probe < cap; ++probe) {
+        uint64_t s = slots[h];
+        if (s == kEmptySlot) break;
+        if ((uint32_t)s == (uint32_t)key) emit(pos++, key, (int32_t)(s >> 32), sp);
+        h = join_next(h, cap);
+    }
+    if (key == -1)
+        for (uint32_t q = 0; q < sentinel; ++q) emit(pos++, key, -1, sp);
+    return pos;
+}
 
-    int32_t key[J][4], sp[J][4], rp[J][4];
-    uint32_t pass[J];
+// One S tuple on its own (the rare overflow path of the partitioned probe):
+// counts its matches, reserves them with one atomic add and emits them.
+template <class Emit>
+QO_DEV void probe_one(const uint64_t *slots, uint64_t cap, uint32_t sentinel, int32_t key, int32_t sp,
+                      unsigned long long *counter, const Emit &emit) {
+    const uint32_t m = join_count(slots, cap, sentinel, key);
+    if (!m) return;
+    const uint64_t pos = atomicAdd(counter, (unsigned long long)m);
+    emit_chain(slots, cap, sentinel, key, sp, pos, emit);
+}
+
+// Looks up the block's tuples (J groups of 4 per thread; bit k of pass[j]
+// marks element k of group j live), reserves the block's output range with one
+// atomic add on `counter` and emits the matches there in (group, wave, lane,
+// element) order; for one tuple, in table-chain order.  Emit(pos, key,
+// r_payload, s_payload).  Must be called by all 256 threads of the block;
+// s_tot holds J*4 words.
+template <int J, class Emit>
+QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, bool general, uint32_t sentinel,
+                        const int32_t (&key)[J][4], const int32_t (&sp)[J][4], const uint32_t (&pass)[J],
+                        unsigned long long *counter, const Emit &emit, uint32_t *s_tot, uint64_t *s_excl) {
+    uint32_t c[J], ex[J];
+    uint32_t hit[J];
+    // On a hit of the unique-key path sv holds the matching slot (its payload
+    // is emitted from there).  Slot indices fit 32 bits (cap < 2^32).
+    uint64_t sv[J][4];
+    if (!general) {
+        // Chains are resolved in rounds: every unresolved tuple of the thread
+        // issues its next slot load in the same round, so a wave waits one
+        // memory round trip per chain step, not one per tuple and step.
+        const uint32_t cap32 = (uint32_t)cap_slots;
+        uint32_t hs[J][4];
+        uint32_t todo[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            todo[j] = pass[j];
+            hit[j] = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                hs[j][k] = (uint32_t)join_home(key[j][k], cap_slots);
+                sv[j][k] = (pass[j] >> k) & 1u ? slots[hs[j][k]] : kEmptySlot;
+            }
+        }
+        for (uint32_t round = 1;; ++round) {
+            uint32_t left = 0;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (!((todo[j] >> k) & 1u)) continue;
+                    if (sv[j][k] == kEmptySlot) {
+                        todo[j] &= ~(1u << k);
+                    } else if ((uint32_t)sv[j][k] == (uint32_t)key[j][k]) {
+                        hit[j] |= 1u << k;
+                        todo[j] &= ~(1u << k);
+                    } else {
+                        hs[j][k] = hs[j][k] + 1 == cap32 ? 0u : hs[j][k] + 1;
+                    }
+                }
+                left |= todo[j];
+            }
+            if (!left || round >= cap32) break;  // a chain never exceeds the table
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((todo[j] >> k) & 1u) sv[j][k] = slots[hs[j][k]];
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) c[j] = __popc(hit[j]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            c[j] = 0;
+            hit[j] = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                sv[j][k] = 0;
+                if ((pass[j] >> k) & 1u) {
+                    uint32_t m = join_count(slots, cap_slots, sentinel, key[j][k]);
+                    c[j] += m;
+                    if (m) hit[j] |= 1u << k;
+                }
+            }
+        }
+    }
+
+    const uint32_t total = tile_offsets<J, 4>(c, ex, s_tot);
+    if (threadIdx.x == 0) *s_excl = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+    __syncthreads();
+    const uint64_t base = *s_excl;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        uint64_t pos = base + ex[j];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!((hit[j] >> k) & 1u)) continue;
+            if (!general) emit(pos++, key[j][k], (int32_t)(sv[j][k] >> 32), sp[j][k]);
+            else pos = emit_chain(slots, cap_slots, sentinel, key[j][k], sp[j][k], pos, emit);
+        }
+    }
+}
+
+// Loads J int4 groups per thread of S tile `tile` (tuples [tile*J*1024, ...))
+// and evaluates the filter: bit k of pass[j] marks element k of group j.
+template <int J, bool HAS_FILTER>
+QO_DEV void load_s_tile(const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
+                        const int32_t *__restrict__ sfilt, size_t ns, int32_t flo, int32_t fhi, bool vec,
+                        uint32_t tile, int32_t (&key)[J][4], int32_t (&sp)[J][4], uint32_t (&pass)[J]) {
+    const size_t vbase = (size_t)tile * (J * 256);
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
@@ -138,69 +248,28 @@ QO_DEV void probe_tile(const JoinHeader *__restrict__ hdr, const uint64_t *__res
             }
         }
     }
+}
 
-    uint32_t c[J], ex[J];
-    uint32_t hit[J];
-    if (!general) {
-        // Issue every first-slot load before resolving any chain.
-        uint64_t hs[J][4], sv[J][4];
-#pragma unroll
-        for (int j = 0; j < J; ++j)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                hs[j][k] = join_home(key[j][k], cap_slots);
-                sv[j][k] = (pass[j] >> k) & 1u ? slots[hs[j][k]] : kEmptySlot;
-            }
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            hit[j] = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if ((pass[j] >> k) & 1u)
-                    if (join_find(slots, cap_slots, key[j][k], hs[j][k], sv[j][k], rp[j][k])) hit[j] |= 1u << k;
-            c[j] = __popc(hit[j]);
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            c[j] = 0;
-            hit[j] = 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if ((pass[j] >> k) & 1u) {
-                    uint32_t m = join_count(slots, cap_slots, sentinel, key[j][k]);
-                    c[j] += m;
-                    if (m) hit[j] |= 1u << k;
-                }
-        }
-    }
-
-    const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
-    if (threadIdx.x == 0) *s_excl = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
-    __syncthreads();
-    const uint64_t base = *s_excl;
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        uint64_t pos = base + ex[j];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (!((hit[j] >> k) & 1u)) continue;
-            if (!general) {
-                emit(pos++, key[j][k], rp[j][k], sp[j][k]);
-            } else {
-                const int32_t kk = key[j][k];
-                uint64_t h = join_home(kk, cap_slots);
-                for (uint64_t probe = 0; probe < cap_slots; ++probe) {
-                    uint64_t s = slots[h];
-                    if (s == kEmptySlot) break;
-                    if ((uint32_t)s == (uint32_t)kk) emit(pos++, kk, (int32_t)(s >> 32), sp[j][k]);
-                    h = join_next(h, cap_slots);
-                }
-                if (kk == -1)
-                    for (uint32_t q = 0; q < sentinel; ++q) emit(pos++, kk, -1, sp[j][k]);
-            }
-        }
-    }
+// Probe one tile of S: tuples [tile*kProbeTile, ...) of (skey, spay, sfilt).
+// Filters, looks up, reserves the tile's output range with one atomic add on
+// `counter` (the running match count) and emits the tile's matches there in S
+// order (for one S tuple, in table-chain order).  Tiles land in completion
+// order: a look-back across tiles would make every tile wait for the slowest
+// lookup of its predecessors.  Must be called by all 256 threads of the block;
+// s_tot holds kProbeJ*4 words.
+template <bool HAS_FILTER, class Emit>
+QO_DEV void probe_tile(const JoinHeader *__restrict__ hdr, const uint64_t *__restrict__ slots,
+                       const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
+                       const int32_t *__restrict__ sfilt, size_t ns, int32_t flo, int32_t fhi, bool vec,
+                       uint32_t tile, unsigned long long *counter, const Emit &emit, uint32_t *s_tot,
+                       uint64_t *s_excl) {
+    const uint64_t cap_slots = hdr->cap;
+    const bool general = (hdr->has_dups | hdr->sentinel_count) != 0;
+    const uint32_t sentinel = hdr->sentinel_count;
+    int32_t key[kProbeJ][4], sp[kProbeJ][4];
+    uint32_t pass[kProbeJ];
+    load_s_tile<kProbeJ, HAS_FILTER>(skey, spay, sfilt, ns, flo, fhi, vec, tile, key, sp, pass);
+    lookup_emit<kProbeJ>(slots, cap_slots, general, sentinel, key, sp, pass, counter, emit, s_tot, s_excl);
 }
 
 // Emits match rows into three SoA int32 columns (capacity-checked).

@@ -30,15 +30,6 @@ QO_DEV uint64_t match_digit(uint32_t d, bool valid) {
     return m;
 }
 
-QO_DEV uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w /* 4 */) {
-    uint32_t inc = wave_incl_scan(v);
-    if (lane_id() == kWave - 1) s_w[wave_id()] = inc;
-    __syncthreads();
-    uint32_t add = 0;
-    for (unsigned w = 0; w < wave_id(); ++w) add += s_w[w];
-    return inc - v + add;
-}
-
 // Segment that owns global tile `tile` (segments are ordered by tile0).
 QO_DEV RadixSeg seg_of(const RadixSeg &seg0, const RadixSeg *segs, int nseg, uint32_t tile) {
     if (nseg <= 1) return seg0;

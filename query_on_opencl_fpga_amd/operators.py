@@ -204,7 +204,7 @@ def groupby_agg(key, val, group_capacity: int):
 
 def hash_join(rkey, rpay, skey, spay, sfilt=None, flo: int = -(1 << 31), fhi: int = (1 << 31) - 1,
               capacity: int | None = None):
-    """(key, r_payload, s_payload) of every match, in S order."""
+    """(key, r_payload, s_payload) of every match (row order unspecified) and the match count."""
     rk, rp, sk, sp = _i32(rkey), _i32(rpay), _i32(skey), _i32(spay)
     nr, ns = rk.shape[0], sk.shape[0]
     cap = ns if capacity is None else int(capacity)
@@ -215,7 +215,7 @@ def hash_join(rkey, rpay, skey, spay, sfilt=None, flo: int = -(1 << 31), fhi: in
     outs = [DeviceBuffer(max(cap, 1) * 4) for _ in range(3)]
     with DeviceBuffer(tb) as table, DeviceBuffer(8) as cnt:
         _lib.call("qo_hash_build", table.ptr, tb, bufs[0].ptr, bufs[1].ptr, nr, None)
-        _lib.call("qo_hash_probe", table.ptr, bufs[2].ptr, bufs[3].ptr, fbuf.ptr if fbuf else None, ns,
+        _lib.call("qo_hash_probe", table.ptr, tb, bufs[2].ptr, bufs[3].ptr, fbuf.ptr if fbuf else None, ns,
                   int(flo), int(fhi), outs[0].ptr, outs[1].ptr, outs[2].ptr, cap, cnt.ptr, None)
         m = int(cnt.to_numpy(np.uint64, 1)[0])
     res = tuple(o.to_numpy(np.int32, min(m, cap)) for o in outs)

@@ -129,3 +129,16 @@ def test_scan_known_answers_and_in_place(qo, orc):
     with DeviceBuffer.from_numpy(x) as d:
         qo._lib.call("qo_scan_exclusive_i32", d.ptr, d.ptr, x.shape[0], None)
         assert np.array_equal(d.to_numpy(np.int32, x.shape[0]), np.arange(x.shape[0], dtype=np.int32))
+
+
+def test_scan_unaligned_pointers(qo, orc):
+    """Columns 4 bytes off a 16-byte boundary take the one-element-per-load path
+    (four times as many tiles, and look-back status words, as the aligned path)."""
+    from query_on_opencl_fpga_amd.device import DeviceBuffer
+    n = 3_000_001
+    rng = np.random.default_rng(11)
+    x = rng.integers(-1000, 1000, n + 1, dtype=np.int64).astype(np.int32)
+    with DeviceBuffer.from_numpy(x) as d, DeviceBuffer((n + 1) * 4) as o:
+        qo._lib.call("qo_scan_exclusive_i32", d.ptr + 4, o.ptr + 4, n, None)
+        got = o.to_numpy(np.int32, n + 1)[1:]
+    assert np.array_equal(got, orc.scan_excl_i32(x[1:]))

@@ -111,6 +111,105 @@ def test_hj_filter_sort_matches_oracle(qo, orc, nr, mult):
         assert np.array_equal(g, e)
 
 
+# ---------------------------------------------------------------------------
+# Direct vs partitioned build/probe (the partitioned path is forced on small
+# inputs with a 64 KiB table slice per partition, i.e. many partitions, and
+# small buckets so skewed keys exercise the overflow path).
+# ---------------------------------------------------------------------------
+TUNE_JOIN_PART = 5
+
+
+@pytest.fixture(params=["direct", "partitioned"])
+def join_mode(request, qo):
+    import ctypes as C
+    lib = qo._lib.load()
+    mode, slice_kib = (2, 2048) if request.param == "direct" else (1, 64)
+    saved = []
+    for cls in range(40):
+        b, i, p = C.c_int(), C.c_int(), C.c_int()
+        qo._lib.call("qo_tune_get", TUNE_JOIN_PART, cls, C.byref(b), C.byref(i), C.byref(p))
+        saved.append((cls, b.value, i.value, p.value))
+        qo._lib.call("qo_tune_set", TUNE_JOIN_PART, cls, 256, slice_kib, mode)
+    yield request.param
+    for cls, b, i, p in saved:
+        lib.qo_tune_set(TUNE_JOIN_PART, cls, b, i, p)
+
+
+@pytest.mark.parametrize("nr,ns", [(1000, 3000), (50_000, 1_000_003), (1 << 20, 1 << 21)])
+def test_join_modes_match_oracle(qo, orc, join_mode, nr, ns):
+    rk = orc.gen_perm(nr, 15).astype(np.int32)
+    rp = orc.gen_uniform_u32(nr, 16, 0, 0, 0).astype(np.int32)
+    sk = orc.gen_perm(nr, 15)[orc.gen_uniform_u32(ns, 16, 1, 0, nr)].astype(np.int32)
+    sk[::7] = -5  # misses
+    sp = orc.gen_uniform_u32(ns, 16, 2, 0, 0).astype(np.int32)
+    sf = orc.gen_uniform_u32(ns, 16, 3, 0, 100).astype(np.int32)
+    got = qo.operators.hash_join(rk, rp, sk, sp, sf, 10, 69)
+    exp = orc.hash_join(rk, rp, sk, sp, sf, 10, 69)
+    assert got[3] == len(exp[0])
+    assert np.array_equal(canon(rows(got)), canon(rows(exp)))
+    got = qo.operators.hash_join(rk, rp, sk, sp)  # no filter
+    exp = orc.hash_join(rk, rp, sk, sp)
+    assert np.array_equal(canon(rows(got)), canon(rows(exp)))
+
+
+def test_join_modes_skewed_probe_side(qo, orc, join_mode):
+    """Most S tuples carry one key: its partition bucket overflows and the rest
+    of its tuples are probed straight from the split pass."""
+    nr, ns = 20_000, 400_000
+    rk = orc.gen_perm(nr, 21).astype(np.int32)
+    rp = np.arange(nr, dtype=np.int32) * 3
+    sk = np.full(ns, rk[17], np.int32)
+    sk[::5] = rk[orc.gen_uniform_u32(ns // 5, 22, 0, 0, nr)]
+    sp = np.arange(ns, dtype=np.int32)
+    got = qo.operators.hash_join(rk, rp, sk, sp)
+    exp = orc.hash_join(rk, rp, sk, sp)
+    assert got[3] == ns
+    assert np.array_equal(canon(rows(got)), canon(rows(exp)))
+
+
+def test_join_modes_skewed_build_side(qo, orc, join_mode):
+    """Thousands of build tuples share one key (and some are the (-1, -1)
+    bit pattern): the build bucket overflows into direct inserts and the probe
+    takes the duplicate-key path."""
+    rng = np.random.default_rng(23)
+    nr = 30_000
+    rk = rng.permutation(np.arange(10, 10 + nr)).astype(np.int32)
+    rk[:6000] = 77
+    rk[6000:6003] = -1
+    rp = rng.integers(-1000, 1000, nr, dtype=np.int64).astype(np.int32)
+    rp[6000:6002] = -1
+    sk = rng.integers(0, 12 + nr, 50_000, dtype=np.int64).astype(np.int32)
+    sk[:40] = 77
+    sk[40:50] = -1
+    sp = np.arange(50_000, dtype=np.int32)
+    sf = rng.integers(0, 4, 50_000, dtype=np.int64).astype(np.int32)
+    got = qo.operators.hash_join(rk, rp, sk, sp, sf, 0, 2, capacity=1_000_000)
+    exp = orc.hash_join(rk, rp, sk, sp, sf, 0, 2)
+    assert got[3] == len(exp[0])
+    assert np.array_equal(canon(rows(got)), canon(rows(exp)))
+
+
+def test_join_modes_capacity_overflow(qo, join_mode):
+    rk = np.arange(5000, dtype=np.int32)
+    sk = np.tile(rk, 20)
+    got = qo.operators.hash_join(rk, rk, sk, sk, capacity=777)
+    assert got[3] == 100_000 and len(got[0]) == 777
+    assert np.array_equal(got[0], got[1]) and np.array_equal(got[0], got[2])
+
+
+def test_hj_filter_sort_modes(qo, orc, join_mode):
+    nr, ns = 60_000, 700_000
+    rk = orc.gen_perm(nr, 31).astype(np.int32) - 30_000
+    rp = orc.gen_uniform_u32(nr, 32, 0, 0, 0).astype(np.int32)
+    sk = rk[orc.gen_uniform_u32(ns, 32, 1, 0, nr)]
+    sp = orc.gen_uniform_u32(ns, 32, 2, 0, 1000).astype(np.int32)  # repeated (key, spay) ties
+    sf = orc.gen_uniform_u32(ns, 32, 3, 0, 1000).astype(np.int32)
+    got = qo.operators.hj_filter_sort(rk, rp, sk, sp, sf, 100, 599)
+    exp = orc.sort_join_rows(*orc.hash_join(rk, rp, sk, sp, sf, 100, 599))
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
 def test_bucket_build_probe_matches_oracle(qo, orc):
     rng = np.random.default_rng(3)
     nr, ns, B, bs = 20_000, 100_000, 1 << 15, 8
