@@ -38,6 +38,20 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def profile_traffic(name: str, match: dict):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)
+    for this workload, from profiles/<name>_traffic.json when it was measured on
+    the same configuration; None otherwise."""
+    path = ROOT / "profiles" / f"{name}_traffic.json"
+    try:
+        rec = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+    if any(rec.get(k) != v for k, v in match.items()):
+        return None
+    return rec.get("traffic_bytes")
+
+
 # ---------------------------------------------------------------------------
 # CPU baseline (oracle, single thread) on a bounded sample of the workload.
 # ---------------------------------------------------------------------------
@@ -145,6 +159,18 @@ def run_join(args, qo, lib) -> dict:
     build_avg = float(np.mean(build_ms))
     probe_bytes = 12 * ns + 12 * m
     achieved = probe_bytes / (probe_avg * 1e-3) / 1e9
+    import ctypes as C
+    part, bits, splits = C.c_int(), C.c_int(), C.c_int()
+    qo._lib.call("qo_join_strategy", tb, ns, C.byref(part), C.byref(bits), C.byref(splits))
+    if part.value:
+        kernel = ("probe stage (one qo_hash_probe call): k_split_cols"
+                  + (" + k_split_pairs" if splits.value == 2 else "") + " + k_slice_probe")
+        strategy = f"partitioned, {1 << bits.value} LDS slices, {splits.value} split pass(es)"
+    else:
+        kernel, strategy = "k_join_probe", "direct"
+    traffic = args.traffic
+    if traffic is None:
+        traffic = profile_traffic("join_probe", {"R_tuples": nr, "S_tuples": ns, "strategy": strategy})
     res = {
         "metric": METRIC,
         "value": ns / (ms_step * 1e-3),
@@ -153,10 +179,12 @@ def run_join(args, qo, lib) -> dict:
         "dtype": "int32",
         "config": {"workload": "filter + hash join (build R, probe filtered S, materialise key/r_payload/s_payload)",
                    "R_tuples": nr, "S_tuples": ns, "filter": "sfilt < 50 of U[0,100)", "matches": m,
-                   "table_bytes": tb},
-        "roofline": {"bound": "hbm", "kernel": "k_join_probe", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
-                     "alg_bytes_per_launch": probe_bytes, "avg_launch_ms": probe_avg},
+                   "table_bytes": tb, "strategy": strategy},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": probe_bytes, "alg_bytes_formula": "12*|S| + 12*matches",
+                     "avg_launch_ms": probe_avg,
+                     "timed_with": "HIP events on the probe stream around each qo_hash_probe call"},
         "kernels": {"build_ms": build_avg, "probe_ms": probe_avg,
                     "build_R_tuples_per_s": nr / (build_avg * 1e-3),
                     "probe_S_tuples_per_s": ns / (probe_avg * 1e-3),
@@ -203,7 +231,8 @@ def run_filter_agg(args, qo, lib) -> dict:
         "config": {"workload": "fused filter + aggregation (COUNT, SUM int64) over key/value int32 columns",
                    "tuples": n, "predicate": "0 <= key <= 2^30-1", "count": int(r[0]), "sum": int(r[1])},
         "roofline": {"bound": "hbm", "kernel": "k_filter_agg", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": args.traffic if args.traffic is not None else profile_traffic("filter_agg", {"tuples": n}),
                      "alg_bytes_per_launch": 8 * n, "avg_launch_ms": k_ms},
     }
     out.free()

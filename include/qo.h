@@ -148,6 +148,10 @@ int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t grou
 /* ---- hash join ------------------------------------------------------------ */
 /* Bytes of device memory a build table for `build_rows` rows needs. */
 size_t qo_join_table_bytes(size_t build_rows);
+/* Strategy the build/probe of n rows against a table of table_bytes takes:
+ * *partitioned = 1 for the per-slice LDS path (then *bits = log2 of the slice
+ * count, *splits = 1 or 2 split passes), 0 for the direct path.  Host-only. */
+int qo_join_strategy(size_t table_bytes, size_t n, int *partitioned, int *bits, int *splits);
 /* Builds the table in `table` (256-byte aligned device memory of table_bytes,
  * e.g. qo_join_table_bytes(nr)).  Tables larger than one L2 are built
  * partition by partition through a per-stream workspace. */

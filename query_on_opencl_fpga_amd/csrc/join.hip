@@ -192,6 +192,12 @@ using namespace qo;
 
 extern "C" {
 
+int qo_join_strategy(size_t table_bytes, size_t n, int *partitioned, int *bits, int *splits) {
+    QO_REQUIRE(partitioned && bits && splits, "null argument");
+    join_part_strategy(table_bytes, n, partitioned, bits, splits);
+    return QO_OK;
+}
+
 size_t qo_join_table_bytes(size_t build_rows) {
     // Load factor (percent) is a tuned parameter of the probe; 50% by default.
     int load = tune_get(TUNE_PROBE, build_rows).param;

@@ -98,6 +98,13 @@ static PartBufs part_carve(const PartPlan &p, size_t n, bool build, Carver &cv) 
     return b;
 }
 
+void join_part_strategy(size_t table_bytes, size_t n, int *partitioned, int *bits, int *splits) {
+    const PartPlan p = part_plan(join_table_slots(table_bytes), n);
+    *partitioned = p.on ? 1 : 0;
+    *bits = p.on ? p.L : 0;
+    *splits = p.on ? (p.L2 > 0 ? 2 : 1) : 0;
+}
+
 size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build) {
     const PartPlan p = part_plan(join_table_slots(table_bytes), n);
     if (!p.on) return 0;

@@ -115,6 +115,7 @@ size_t join_probe_ws_bytes(size_t table_bytes, size_t ns);
 // Partitioned (per-slice LDS) build and probe, join_part.hip.  *used = false
 // when the table is small enough for the direct path (nothing launched).
 size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build);
+void join_part_strategy(size_t table_bytes, size_t n, int *partitioned, int *bits, int *splits);
 int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                            void *ws, hipStream_t st, bool *used);
 int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
