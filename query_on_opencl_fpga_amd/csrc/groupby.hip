@@ -16,6 +16,7 @@
 
 #include "qo_device.hpp"
 #include "qo_internal.hpp"
+#include "qo_split.hpp"
 
 namespace qo {
 
@@ -101,12 +102,8 @@ struct LTable {
     unsigned long long sum[C];
 };
 
-template <int C, bool VEC>
-__global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict__ key, const int32_t *__restrict__ val,
-                                                      size_t n, GEntry *gt, uint64_t gmask, int probes,
-                                                      uint32_t *overflow) {
-    extern __shared__ unsigned char s_raw[];
-    LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
+template <int C>
+QO_DEV void ltable_init(LTable<C> &L) {
     for (int i = threadIdx.x; i < C; i += blockDim.x) {
         L.key[i] = kEmptyKey;
         L.cnt[i] = 0;
@@ -114,18 +111,40 @@ __global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict_
         L.mx[i] = INT_MIN;
         L.sum[i] = 0;
     }
+}
+
+// One tuple into the block's LDS table, or into the global table when its key
+// finds no LDS slot within `probes` probes (or is the empty-marker pattern).
+template <int C>
+QO_DEV void ltable_add(LTable<C> &L, int32_t k, int32_t v, int probes, GEntry *gt, uint64_t gmask,
+                       uint32_t *overflow) {
+    int s = (uint32_t)k == kEmptyKey ? -1 : l_slot<C>(L.key, (uint32_t)k, probes);
+    if (s >= 0) {
+        atomicAdd(&L.cnt[s], 1u);
+        atomicAdd(&L.sum[s], (unsigned long long)(long long)v);
+        atomicMin(&L.mn[s], v);
+        atomicMax(&L.mx[s], v);
+    } else {
+        g_update(gt, gmask, (uint32_t)k, 1, (uint64_t)(int64_t)v, v, v, overflow);
+    }
+}
+
+// Merges the block's LDS partials into the global table (after a barrier).
+template <int C>
+QO_DEV void ltable_flush(const LTable<C> &L, GEntry *gt, uint64_t gmask, uint32_t *overflow) {
+    for (int i = threadIdx.x; i < C; i += blockDim.x)
+        if (L.cnt[i]) g_update(gt, gmask, L.key[i], L.cnt[i], L.sum[i], L.mn[i], L.mx[i], overflow);
+}
+
+template <int C, bool VEC>
+__global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict__ key, const int32_t *__restrict__ val,
+                                                      size_t n, GEntry *gt, uint64_t gmask, int probes,
+                                                      uint32_t *overflow) {
+    extern __shared__ unsigned char s_raw[];
+    LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
+    ltable_init<C>(L);
     __syncthreads();
-    auto add = [&](int32_t k, int32_t v) {
-        int s = (uint32_t)k == kEmptyKey ? -1 : l_slot<C>(L.key, (uint32_t)k, probes);
-        if (s >= 0) {
-            atomicAdd(&L.cnt[s], 1u);
-            atomicAdd(&L.sum[s], (unsigned long long)(long long)v);
-            atomicMin(&L.mn[s], v);
-            atomicMax(&L.mx[s], v);
-        } else {
-            g_update(gt, gmask, (uint32_t)k, 1, (uint64_t)(int64_t)v, v, v, overflow);
-        }
-    };
+    auto add = [&](int32_t k, int32_t v) { ltable_add<C>(L, k, v, probes, gt, gmask, overflow); };
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     size_t from = 0;
@@ -143,8 +162,53 @@ __global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict_
     }
     for (size_t e = from + tid; e < n; e += stride) add(key[e], val[e]);
     __syncthreads();
-    for (int i = threadIdx.x; i < C; i += blockDim.x)
-        if (L.cnt[i]) g_update(gt, gmask, L.key[i], L.cnt[i], L.sum[i], L.mn[i], L.mx[i], overflow);
+    ltable_flush<C>(L, gt, gmask, overflow);
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned hash group-by (more groups than one LDS table holds).  The
+// (key, value) tuples are split exactly by key hash (qo_split.hpp: a histogram
+// pass sizes every bucket, so skewed keys cannot overflow) into 2^L fine
+// buckets of about `target` groups each.  One work-group per chunk of a fine
+// bucket aggregates in its LDS table and merges its partials into the global
+// table, so global atomics fall from four per tuple to four per (chunk, group);
+// a heavy key's bucket is simply cut into more chunks.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kGbChunk = 1u << 18;  // tuples per aggregation work item at most
+
+// Chunks per fine bucket: cch[f] = ceil(hist[f] / kGbChunk), cch[nb] = 0.
+__global__ void k_chunk_counts(const uint32_t *hist, uint32_t nb, uint32_t *cch) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f <= nb) cch[f] = f < nb ? (hist[f] + kGbChunk - 1) / kGbChunk : 0u;
+}
+
+// Work item w = chunk (w - cpre[f]) of fine bucket f, cpre[f] <= w < cpre[f + 1].
+template <int C>
+__global__ __launch_bounds__(512) void k_groupby_chunks(const uint64_t *__restrict__ fine, const uint32_t *off,
+                                                        const uint32_t *cpre, uint32_t nb, GEntry *gt, uint64_t gmask,
+                                                        int probes, uint32_t *overflow) {
+    extern __shared__ unsigned char s_raw[];
+    LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
+    const uint32_t w = blockIdx.x;
+    if (w >= cpre[nb]) return;  // whole block: the grid is an upper bound
+    int lo = 0, hi = (int)nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cpre[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint64_t e0 = off[lo] + (uint64_t)(w - cpre[lo]) * kGbChunk;
+    const uint64_t e1 = off[lo + 1];
+    const uint32_t m = (uint32_t)(e1 - e0 < kGbChunk ? e1 - e0 : kGbChunk);
+    const uint64_t *src = fine + e0;
+    ltable_init<C>(L);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint64_t v = ld_stream(src + i);
+        ltable_add<C>(L, (int32_t)(uint32_t)v, (int32_t)(v >> 32), probes, gt, gmask, overflow);
+    }
+    __syncthreads();
+    ltable_flush<C>(L, gt, gmask, overflow);
 }
 
 // Occupied global entries -> (key ^ signbit) << 32 | index, appended by atomics
@@ -191,27 +255,88 @@ static GroupbyPlan groupby_plan(size_t group_capacity) {
     return {cap - 1, cap + 1};
 }
 
-size_t groupby_ws_bytes(size_t group_capacity) {
-    GroupbyPlan p = groupby_plan(group_capacity);
-    return p.entries * sizeof(GEntry) + 1024 + group_capacity * 8 + 256 +
-           radix_workspace_bytes(group_capacity, false) + 256;
+struct GbPartPlan {
+    bool on = false;
+    SplitPlan s;
+};
+
+// Partition when the groups outgrow one LDS table: 2^L fine buckets of about
+// `target` groups each (tune items), L <= 14.
+static GbPartPlan groupby_part_plan(size_t group_capacity, size_t n) {
+    GbPartPlan p;
+    const TuneParams tp = tune_get(TUNE_GROUPBY_PART, n);
+    if (tp.param == PART_NEVER || n == 0 || n >= 0xFFFFFFFFull || group_capacity < 2) return p;
+    if (tp.param != PART_ALWAYS && (group_capacity <= 1024 || n < (size_t(1) << 20))) return p;
+    const size_t target = (size_t)std::max(16, tp.items);
+    int L = 1;
+    while (L < 2 * kMaxSplitBits && (group_capacity >> L) > target) ++L;
+    p.on = true;
+    p.s = split_plan(L, n);
+    return p;
+}
+
+struct GroupbyBufs {
+    GEntry *gt;
+    uint64_t *counters;  // [0] groups found, [1] table-full flag (low word)
+    uint64_t *list;
+    void *rws;
+    ExactSplitBufs sb;   // partitioned path only
+    uint32_t *cch = nullptr, *cpre = nullptr;
+    void *scan_ws = nullptr;
+};
+
+static GroupbyBufs groupby_carve(const GroupbyPlan &p, const GbPartPlan &pp, size_t group_capacity, size_t n,
+                                 Carver &cv) {
+    GroupbyBufs b;
+    b.gt = cv.take<GEntry>(p.entries);
+    b.counters = cv.take<uint64_t>(4);
+    b.list = cv.take<uint64_t>(group_capacity ? group_capacity : 1);
+    b.rws = cv.take<char>(radix_workspace_bytes(group_capacity, false));
+    if (pp.on) {
+        const size_t nb = size_t(1) << pp.s.L;
+        b.sb = exact_split_carve(pp.s, n, cv);
+        b.cch = cv.take<uint32_t>(nb + 1);
+        b.cpre = cv.take<uint32_t>(nb + 1);
+        b.scan_ws = cv.take<char>(scan_workspace_bytes(nb + 1));
+    }
+    return b;
+}
+
+size_t groupby_ws_bytes(size_t group_capacity, size_t n) {
+    Carver cv(nullptr);
+    groupby_carve(groupby_plan(group_capacity), groupby_part_plan(group_capacity, n), group_capacity, n, cv);
+    return cv.off + 256;
 }
 
 int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity, int32_t *okey,
                    int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax, uint64_t *d_num, void *ws,
                    hipStream_t st) {
     GroupbyPlan p = groupby_plan(group_capacity);
+    const GbPartPlan pp = groupby_part_plan(group_capacity, n);
     Carver cv(ws);
-    GEntry *gt = cv.take<GEntry>(p.entries);
-    uint64_t *counters = cv.take<uint64_t>(4);
-    uint64_t *list = cv.take<uint64_t>(group_capacity ? group_capacity : 1);
-    void *rws = cv.take<char>(radix_workspace_bytes(group_capacity, false));
+    const GroupbyBufs b = groupby_carve(p, pp, group_capacity, n, cv);
+    GEntry *gt = b.gt;
+    uint64_t *counters = b.counters;
+    uint64_t *list = b.list;
+    void *rws = b.rws;
     uint32_t *overflow = reinterpret_cast<uint32_t *>(counters + 1);
     QO_TRY(launch_fill_u64(counters, 0, 4, st));
     const unsigned ig = grid_for(p.entries, 256, (unsigned)props().cu_count * 16);
     k_gtable_init<<<ig, 256, 0, st>>>(gt, p.entries);
     QO_LAUNCHED();
-    if (n) {
+    if (n && pp.on) {
+        const uint32_t nb = 1u << pp.s.L;
+        QO_TRY(launch_exact_splits(pp.s, b.sb, key, val, n, st));
+        k_chunk_counts<<<(nb + 1 + 255) / 256, 256, 0, st>>>(b.sb.hist, nb, b.cch);
+        QO_LAUNCHED();
+        QO_TRY(launch_scan_excl_u32(b.cch, b.cpre, nb + 1, b.scan_ws, st));
+        // Upper bound on the chunk count: every bucket one partial chunk.
+        const uint64_t items = (uint64_t)nb + (n + kGbChunk - 1) / kGbChunk;
+        QO_REQUIRE(items < 0x7FFFFFFFull, "group-by input too large");
+        k_groupby_chunks<2048><<<(unsigned)items, 512, sizeof(LTable<2048>), st>>>(b.sb.fine, b.sb.off, b.cpre, nb, gt,
+                                                                                    p.gmask, 64, overflow);
+        QO_LAUNCHED();
+    } else if (n) {
         TuneParams tp = tune_get(TUNE_GROUPBY, n);
         const bool vec = ((uintptr_t)key % 16 == 0) && ((uintptr_t)val % 16 == 0);
         int C = tp.items >= 4096 ? 4096 : 2048;
@@ -414,7 +539,7 @@ int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t grou
     QO_REQUIRE(group_capacity < (size_t(1) << 31), "group capacity too large");
     hipStream_t st = stream_of(s);
     int status = QO_OK;
-    void *ws = workspace(st, groupby_ws_bytes(group_capacity), &status);
+    void *ws = workspace(st, groupby_ws_bytes(group_capacity, n), &status);
     if (!ws) return status;
     return launch_groupby(key, val, n, group_capacity, okey, ocount, osum, omin, omax, d_numGroups, ws, st);
 }

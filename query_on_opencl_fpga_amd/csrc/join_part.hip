@@ -29,14 +29,10 @@
 
 #include "qo_internal.hpp"
 #include "qo_join.hpp"
+#include "qo_split.hpp"
 
 namespace qo {
 
-constexpr int kSplitJ = 8;                          // int4 groups (4 tuples each) per thread
-constexpr uint32_t kSplitTile = kSplitJ * 1024;     // tuples per split tile
-constexpr int kMaxSplitBits = 7;                    // buckets per split pass <= 128
-constexpr uint32_t kMaxSplit = 1u << kMaxSplitBits;
-constexpr uint32_t kCtrStride = 16;                 // bucket counters 64 bytes apart
 constexpr uint32_t kSliceMargin = 256;              // slots past a slice end also kept in LDS by the probe
 constexpr uint32_t kMaxSliceSlots = 8192;           // LDS slots of one work-group (64 KiB)
 constexpr int kPairJ = 4;                           // pair quads per thread per probe step
@@ -44,16 +40,16 @@ constexpr uint32_t kPairStep = kPairJ * 1024;       // pairs per probe step
 
 struct PartPlan {
     bool on = false;
-    int L = 0, L1 = 0, L2 = 0;      // fine bucket bits = L1 (split A) + L2 (split B)
-    uint64_t bcapA = 0, bcapB = 0;  // pairs per bucket of split A / split B
-    uint32_t slice_max = 0;         // slots of the largest slice
+    int L = 0, L2 = 0;       // fine bucket (table slice) bits; second split bits
+    SplitPlan s;
+    uint32_t slice_max = 0;  // slots of the largest slice
 };
 
 static PartPlan part_plan(uint64_t cap, size_t n) {
     PartPlan p;
     const TuneParams tp = tune_get(TUNE_JOIN_PART, n);
-    if (tp.param == JOIN_PART_NEVER || n == 0 || n >= 0xFFFFFFFFull || cap < 2) return p;
-    if (tp.param != JOIN_PART_ALWAYS && (cap * 8 <= (uint64_t(4) << 20) || n < (size_t(1) << 20))) return p;
+    if (tp.param == PART_NEVER || n == 0 || n >= 0xFFFFFFFFull || cap < 2) return p;
+    if (tp.param != PART_ALWAYS && (cap * 8 <= (uint64_t(4) << 20) || n < (size_t(1) << 20))) return p;
     // Slice target in slots (tune items = KiB per slice).
     const uint64_t target = std::min<uint64_t>(kMaxSliceSlots - kSliceMargin - 2, (uint64_t)std::max(1, tp.items) * 128);
     int L = 1;
@@ -61,36 +57,22 @@ static PartPlan part_plan(uint64_t cap, size_t n) {
     if ((cap >> L) + 2 + kSliceMargin > kMaxSliceSlots) return p;  // needs more than two splits
     p.on = true;
     p.L = L;
-    // One split when it suffices; else the first (filtered) split takes the
-    // smaller fan-out so both write runs of similar length.
-    p.L1 = L <= kMaxSplitBits ? L : L / 2;
-    p.L2 = L - p.L1;
-    // Hash buckets of uniform keys stay within a few standard deviations of
-    // their mean; the slack leaves only skewed inputs to the overflow path.
-    const uint64_t perA = (n + (uint64_t(1) << p.L1) - 1) >> p.L1;
-    p.bcapA = (perA + perA / 16 + 1024 + 31) / 32 * 32;
-    const uint64_t perB = (n + (uint64_t(1) << L) - 1) >> L;
-    p.bcapB = (perB + perB / 8 + 256 + 31) / 32 * 32;
+    p.s = split_plan(L, n);
+    p.L2 = p.s.L2;
     p.slice_max = (uint32_t)((cap >> L) + 2);
     return p;
 }
 
 // Workspace carve-up (same order for sizing and binding).
 struct PartBufs {
-    uint32_t *ctr = nullptr;        // (2^L1 + 2^L) counters, kCtrStride apart: A buckets then fine buckets
-    uint64_t *bucketA = nullptr;
-    uint64_t *bucketB = nullptr;    // only when L2 > 0
+    SplitBufs sb;
     unsigned long long *spill_n = nullptr;
-    uint64_t *spill = nullptr;      // build only: nr entries
-    size_t ctr_words = 0;
+    uint64_t *spill = nullptr;  // build only: nr entries
 };
 
 static PartBufs part_carve(const PartPlan &p, size_t n, bool build, Carver &cv) {
     PartBufs b;
-    b.ctr_words = ((size_t(1) << p.L1) + (size_t(1) << p.L)) * kCtrStride;
-    b.ctr = cv.take<uint32_t>(b.ctr_words);
-    b.bucketA = cv.take<uint64_t>((size_t(1) << p.L1) * p.bcapA);
-    if (p.L2 > 0) b.bucketB = cv.take<uint64_t>((size_t(1) << p.L) * p.bcapB);
+    b.sb = split_carve(p.s, cv);
     if (build) {
         b.spill_n = cv.take<unsigned long long>(1);
         b.spill = cv.take<uint64_t>(n);
@@ -113,15 +95,6 @@ size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build) {
     return cv.off + 256;
 }
 
-// ---------------------------------------------------------------------------
-// Splits.
-// ---------------------------------------------------------------------------
-struct SplitLds {
-    uint64_t stage[kSplitTile];
-    uint32_t cnt[kMaxSplit], loff[kMaxSplit], gbase[kMaxSplit];
-    uint32_t w[4];
-};
-
 // Tuples that find their bucket full.  Probe: looked up and emitted at once.
 struct ProbeOverflow {
     const JoinHeader *hdr;
@@ -132,126 +105,6 @@ struct ProbeOverflow {
         probe_one(slots, hdr->cap, hdr->sentinel_count, (int32_t)(uint32_t)v, (int32_t)(v >> 32), counter, emit);
     }
 };
-// Build: set aside for the global insert after the slices are written.
-struct BuildOverflow {
-    uint64_t *spill;
-    unsigned long long *spill_n;
-    QO_DEV void operator()(uint64_t v) const { spill[atomicAdd(spill_n, 1ull)] = v; }
-};
-
-// Splits the block's live tuples (J groups of 4 per thread, bit k of pass[j]
-// marks element k of group j) into `parts` buckets by
-// part = (hash32(key) >> shift) & (parts - 1); bucket index = bucket0 + part.
-template <int J, class Overflow>
-QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], const uint32_t (&pass)[J],
-                        int shift, uint32_t parts, uint32_t bucket0, uint64_t *__restrict__ bucket, uint64_t bcap,
-                        uint32_t *ctr, SplitLds &sh, const Overflow &overflow) {
-    const uint32_t mask = parts - 1;
-    for (uint32_t i = threadIdx.x; i < parts; i += 256) sh.cnt[i] = 0;
-    __syncthreads();
-    // Rank inside the tile: (part << 16) | arrival order within the part.
-    uint32_t rank[J][4];
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            rank[j][k] = 0;
-            if ((pass[j] >> k) & 1u) {
-                const uint32_t p = (hash32((uint32_t)key[j][k]) >> shift) & mask;
-                rank[j][k] = (p << 16) | atomicAdd(&sh.cnt[p], 1u);
-            }
-        }
-    __syncthreads();
-    // Part offsets inside the tile and one bucket reservation per non-empty part.
-    const uint32_t c0 = threadIdx.x < parts ? sh.cnt[threadIdx.x] : 0u;
-    const uint32_t ex = block_excl_scan256(c0, sh.w);
-    if (threadIdx.x < parts) {
-        sh.loff[threadIdx.x] = ex;
-        sh.gbase[threadIdx.x] = c0 ? atomicAdd(&ctr[(bucket0 + threadIdx.x) * kCtrStride], c0) : 0u;
-    }
-    __syncthreads();
-    const uint32_t total = sh.loff[parts - 1] + sh.cnt[parts - 1];
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if ((pass[j] >> k) & 1u)
-                sh.stage[sh.loff[rank[j][k] >> 16] + (rank[j][k] & 0xFFFFu)] = join_pack(key[j][k], pay[j][k]);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < total; i += 256) {
-        const uint64_t v = sh.stage[i];
-        const uint32_t p = (hash32((uint32_t)v) >> shift) & mask;
-        const uint64_t g = (uint64_t)sh.gbase[p] + (i - sh.loff[p]);
-        // Plain stores: runs from neighbouring tiles complete each other's
-        // lines in L2 before they are written back.
-        if (g < bcap) bucket[(size_t)(bucket0 + p) * bcap + g] = v;
-        else overflow(v);
-    }
-}
-
-// Split A: columns (key, payload[, filter]) -> 2^L1 buckets.
-template <bool HAS_FILTER, class Overflow>
-__global__ __launch_bounds__(256) void k_split_cols(const int32_t *__restrict__ kcol, const int32_t *__restrict__ pcol,
-                                                    const int32_t *__restrict__ fcol, size_t n, int32_t flo,
-                                                    int32_t fhi, bool vec, int shift, uint32_t parts,
-                                                    uint64_t *__restrict__ bucket, uint64_t bcap, uint32_t *ctr,
-                                                    Overflow overflow) {
-    __shared__ SplitLds sh;
-    int32_t key[kSplitJ][4], pay[kSplitJ][4];
-    uint32_t pass[kSplitJ];
-    load_s_tile<kSplitJ, HAS_FILTER>(kcol, pcol, fcol, n, flo, fhi, vec, blockIdx.x, key, pay, pass);
-    split_block<kSplitJ>(key, pay, pass, shift, parts, 0, bucket, bcap, ctr, sh, overflow);
-}
-
-// Loads up to J*1024 (key, payload) pairs from `src` (16-byte aligned): pair
-// quad j of thread t holds pairs 2v, 2v+1 for v = (2j + h)*256 + t, h = 0, 1.
-template <int J>
-QO_DEV void load_pairs(const uint64_t *__restrict__ src, uint32_t m, int32_t (&key)[J][4], int32_t (&pay)[J][4],
-                       uint32_t (&pass)[J]) {
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        pass[j] = 0;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const uint32_t v = (uint32_t)(2 * j + h) * 256 + threadIdx.x;
-            const uint32_t e = 2 * v;
-            key[j][2 * h] = pay[j][2 * h] = key[j][2 * h + 1] = pay[j][2 * h + 1] = 0;
-            if (e + 2 <= m) {
-                const uint4 q = ld_stream(reinterpret_cast<const uint4 *>(src) + v);
-                key[j][2 * h] = (int32_t)q.x;
-                pay[j][2 * h] = (int32_t)q.y;
-                key[j][2 * h + 1] = (int32_t)q.z;
-                pay[j][2 * h + 1] = (int32_t)q.w;
-                pass[j] |= 3u << (2 * h);
-            } else if (e < m) {
-                const uint64_t w = src[e];
-                key[j][2 * h] = (int32_t)(uint32_t)w;
-                pay[j][2 * h] = (int32_t)(w >> 32);
-                pass[j] |= 1u << (2 * h);
-            }
-        }
-    }
-}
-
-// Split B: tile t of A bucket p1 -> fine buckets (p1 << L2) + part.
-template <class Overflow>
-__global__ __launch_bounds__(256) void k_split_pairs(const uint64_t *__restrict__ in, uint64_t in_cap,
-                                                     const uint32_t *in_ctr, uint32_t tiles_per_bucket, int shift,
-                                                     uint32_t parts, uint64_t *__restrict__ bucket, uint64_t bcap,
-                                                     uint32_t *ctr, Overflow overflow) {
-    __shared__ SplitLds sh;
-    const uint32_t p1 = blockIdx.x / tiles_per_bucket;
-    const uint32_t t = blockIdx.x % tiles_per_bucket;
-    uint64_t n = in_ctr[p1 * kCtrStride];
-    if (n > in_cap) n = in_cap;
-    const uint64_t e0 = (uint64_t)t * kSplitTile;
-    if (e0 >= n) return;  // whole block: no barrier has been reached
-    const uint32_t m = (uint32_t)(n - e0 < kSplitTile ? n - e0 : kSplitTile);
-    int32_t key[kSplitJ][4], pay[kSplitJ][4];
-    uint32_t pass[kSplitJ];
-    load_pairs<kSplitJ>(in + (size_t)p1 * in_cap + e0, m, key, pay, pass);
-    split_block<kSplitJ>(key, pay, pass, shift, parts, p1 * parts, bucket, bcap, ctr, sh, overflow);
-}
 
 // ---------------------------------------------------------------------------
 // Per-slice probe and build in LDS.
@@ -440,45 +293,6 @@ __global__ void k_spill_insert(const uint64_t *spill, const unsigned long long *
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-template <class Overflow>
-static int launch_splits(const PartPlan &p, const PartBufs &b, const int32_t *kcol, const int32_t *pcol,
-                         const int32_t *fcol, size_t n, int32_t flo, int32_t fhi, const Overflow &overflow,
-                         hipStream_t st) {
-    const bool vec = ((uintptr_t)kcol % 16 == 0) && ((uintptr_t)pcol % 16 == 0) && (!fcol || (uintptr_t)fcol % 16 == 0);
-    const uint32_t partsA = 1u << p.L1;
-    const size_t tiles = (n + kSplitTile - 1) / kSplitTile;
-    QO_REQUIRE(tiles < 0x7FFFFFFFull, "input too large for the partitioned join");
-    if (fcol)
-        k_split_cols<true, Overflow><<<(unsigned)tiles, 256, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
-                                                            b.bucketA, p.bcapA, b.ctr, overflow);
-    else
-        k_split_cols<false, Overflow><<<(unsigned)tiles, 256, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
-                                                             b.bucketA, p.bcapA, b.ctr, overflow);
-    QO_LAUNCHED();
-    if (p.L2 > 0) {
-        const uint32_t tpb = (uint32_t)((p.bcapA + kSplitTile - 1) / kSplitTile);
-        QO_REQUIRE((uint64_t)tpb * partsA < 0x7FFFFFFFull, "input too large for the partitioned join");
-        k_split_pairs<Overflow><<<tpb * partsA, 256, 0, st>>>(b.bucketA, p.bcapA, b.ctr, tpb, 32 - p.L, 1u << p.L2, b.bucketB,
-                                                   p.bcapB, b.ctr + (size_t)partsA * kCtrStride, overflow);
-        QO_LAUNCHED();
-    }
-    return QO_OK;
-}
-
-// Fine buckets: split B's when there is one, else split A's.
-static void fine_buckets(const PartPlan &p, const PartBufs &b, const uint64_t *&bucket, uint64_t &bcap,
-                         const uint32_t *&ctr) {
-    if (p.L2 > 0) {
-        bucket = b.bucketB;
-        bcap = p.bcapB;
-        ctr = b.ctr + ((size_t)1 << p.L1) * kCtrStride;
-    } else {
-        bucket = b.bucketA;
-        bcap = p.bcapA;
-        ctr = b.ctr;
-    }
-}
-
 int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                            void *ws, hipStream_t st, bool *used) {
     *used = false;
@@ -488,13 +302,13 @@ int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey,
     Carver cv(ws);
     const PartBufs b = part_carve(p, nr, true, cv);
     JoinHeader *hdr = static_cast<JoinHeader *>(table);
-    QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));
+    QO_TRY(launch_fill_u32(b.sb.ctr, 0, b.sb.ctr_words, st));
     QO_TRY(launch_fill_u64(b.spill_n, 0, 1, st));
-    QO_TRY(launch_splits(p, b, rkey, rpay, nullptr, nr, 0, 0, BuildOverflow{b.spill, b.spill_n}, st));
+    QO_TRY(launch_splits(p.s, b.sb, rkey, rpay, nullptr, nr, 0, 0, AppendOverflow{b.spill, b.spill_n}, st));
     const uint64_t *fb;
     uint64_t fcap;
     const uint32_t *fctr;
-    fine_buckets(p, b, fb, fcap, fctr);
+    split_fine(p.s, b.sb, fb, fcap, fctr);
     k_slice_build<<<1u << p.L, 256, (size_t)p.slice_max * 8, st>>>(fb, fcap, fctr, p.L, hdr, b.spill, b.spill_n);
     QO_LAUNCHED();
     k_spill_insert<<<(unsigned)props().cu_count * 4, 256, 0, st>>>(b.spill, b.spill_n, hdr);
@@ -514,13 +328,13 @@ int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t 
     const PartBufs b = part_carve(p, ns, false, cv);
     const JoinHeader *hdr = static_cast<const JoinHeader *>(table);
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(d_count);
-    QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));
-    QO_TRY(launch_splits(p, b, skey, spay, sfilt, ns, flo, fhi,
+    QO_TRY(launch_fill_u32(b.sb.ctr, 0, b.sb.ctr_words, st));
+    QO_TRY(launch_splits(p.s, b.sb, skey, spay, sfilt, ns, flo, fhi,
                          ProbeOverflow{hdr, counter, SoaEmit{okey, orpay, ospay, out_cap}}, st));
     const uint64_t *fb;
     uint64_t fcap;
     const uint32_t *fctr;
-    fine_buckets(p, b, fb, fcap, fctr);
+    split_fine(p.s, b.sb, fb, fcap, fctr);
     k_slice_probe<<<1u << p.L, 256, (size_t)(p.slice_max + kSliceMargin) * 8, st>>>(
         fb, fcap, fctr, p.L, hdr, counter, SoaEmit{okey, orpay, ospay, out_cap});
     QO_LAUNCHED();

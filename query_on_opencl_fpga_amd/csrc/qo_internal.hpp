@@ -58,11 +58,12 @@ enum TuneOp {
     TUNE_PROBE = 2,
     TUNE_GROUPBY = 3,
     TUNE_SORT = 4,
-    TUNE_JOIN_PART = 5,  // partitioned build/probe: items = table slice KiB per partition, param = mode
-    TUNE_NUM_OPS = 6
+    TUNE_JOIN_PART = 5,     // partitioned build/probe: items = LDS table slice KiB, param = PartMode
+    TUNE_GROUPBY_PART = 6,  // partitioned group-by: items = groups per fine bucket, param = PartMode
+    TUNE_NUM_OPS = 7
 };
-// TUNE_JOIN_PART modes: partition when the table outgrows one L2, always, never.
-enum JoinPartMode { JOIN_PART_AUTO = 0, JOIN_PART_ALWAYS = 1, JOIN_PART_NEVER = 2 };
+// Partitioned-path modes: by size (the default), always, never.
+enum PartMode { PART_AUTO = 0, PART_ALWAYS = 1, PART_NEVER = 2 };
 TuneParams tune_get(int op, size_t n);
 
 inline int size_class(size_t n) {
@@ -99,7 +100,7 @@ size_t sort_join_rows_ws_bytes(size_t m);
 int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void *ws, hipStream_t st);
 
 // Hash group-by with caller workspace (groupby.hip).
-size_t groupby_ws_bytes(size_t group_capacity);
+size_t groupby_ws_bytes(size_t group_capacity, size_t n);
 int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity, int32_t *okey,
                    int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax, uint64_t *d_num, void *ws,
                    hipStream_t st);

@@ -1,0 +1,402 @@
+// Hash split of (key, payload) tuples into 2^L buckets by the top L bits of
+// hash32(key): the partitioning step shared by the partitioned join and the
+// partitioned group-by.
+//
+// Split A streams columns (an optional range filter fused) into 2^L1 buckets;
+// split B splits each A bucket by the next L2 bits (L = L1 + L2, skipped when
+// L2 = 0).  A split tile of 8192 tuples is ranked in LDS, reserves one range
+// per non-empty bucket with one atomic add and writes contiguous runs.  Bucket
+// capacities are fixed up front from the input size; tuples that find their
+// bucket full go to an Overflow functor (skewed keys).
+#pragma once
+#include <algorithm>
+
+#include "qo_internal.hpp"
+#include "qo_join.hpp"
+
+namespace qo {
+
+constexpr int kSplitJ = 8;                       // int4 groups (4 tuples each) per thread
+constexpr uint32_t kSplitTile = kSplitJ * 1024;  // tuples per split tile
+constexpr int kMaxSplitBits = 7;                 // buckets per split pass <= 128
+constexpr uint32_t kMaxSplit = 1u << kMaxSplitBits;
+constexpr uint32_t kCtrStride = 16;              // bucket counters 64 bytes apart
+
+struct SplitPlan {
+    int L = 0, L1 = 0, L2 = 0;      // fine bucket bits = L1 (split A) + L2 (split B)
+    uint64_t bcapA = 0, bcapB = 0;  // pairs per bucket of split A / split B
+};
+
+// Plan for n tuples into 2^L fine buckets, 1 <= L <= 2 * kMaxSplitBits.  One
+// split when it suffices; else the first split takes the smaller fan-out (it
+// is the filtered one) so that both write runs of similar length.
+inline SplitPlan split_plan(int L, size_t n) {
+    SplitPlan p;
+    p.L = L;
+    p.L1 = L <= kMaxSplitBits ? L : L / 2;
+    p.L2 = L - p.L1;
+    // Hash buckets of uniform keys stay within a few standard deviations of
+    // their mean; the slack leaves only skewed inputs to the overflow path.
+    const uint64_t perA = (n + (uint64_t(1) << p.L1) - 1) >> p.L1;
+    p.bcapA = (perA + perA / 16 + 1024 + 31) / 32 * 32;
+    const uint64_t perB = (n + (uint64_t(1) << L) - 1) >> L;
+    p.bcapB = (perB + perB / 8 + 256 + 31) / 32 * 32;
+    return p;
+}
+
+struct SplitBufs {
+    uint32_t *ctr = nullptr;      // (2^L1 + 2^L) counters, kCtrStride apart: A buckets, then fine buckets
+    uint64_t *bucketA = nullptr;
+    uint64_t *bucketB = nullptr;  // only when L2 > 0
+    size_t ctr_words = 0;
+};
+
+// Carves the split buffers out of a workspace (nullptr base: sizing only).
+inline SplitBufs split_carve(const SplitPlan &p, Carver &cv) {
+    SplitBufs b;
+    b.ctr_words = ((size_t(1) << p.L1) + (size_t(1) << p.L)) * kCtrStride;
+    b.ctr = cv.take<uint32_t>(b.ctr_words);
+    b.bucketA = cv.take<uint64_t>((size_t(1) << p.L1) * p.bcapA);
+    if (p.L2 > 0) b.bucketB = cv.take<uint64_t>((size_t(1) << p.L) * p.bcapB);
+    return b;
+}
+
+// The 2^L fine buckets: split B's when there is one, else split A's.
+inline void split_fine(const SplitPlan &p, const SplitBufs &b, const uint64_t *&bucket, uint64_t &bcap,
+                       const uint32_t *&ctr) {
+    if (p.L2 > 0) {
+        bucket = b.bucketB;
+        bcap = p.bcapB;
+        ctr = b.ctr + ((size_t)1 << p.L1) * kCtrStride;
+    } else {
+        bucket = b.bucketA;
+        bcap = p.bcapA;
+        ctr = b.ctr;
+    }
+}
+
+// Overflow that appends the tuple to a list (capacity: the input size; a tuple
+// overflows at most once).
+struct AppendOverflow {
+    uint64_t *list;
+    unsigned long long *n;
+    QO_DEV void operator()(uint64_t v) const { list[atomicAdd(n, 1ull)] = v; }
+};
+
+
+// Where bucket b's tuples go in a split's output array: [start(b), start(b) +
+// cap(b)); count(b, ctr) is how many it holds after the split ran.
+// Fixed: equal capacity per bucket, fill counts in the counters.
+struct FixedLayout {
+    uint64_t bcap;
+    QO_DEV uint64_t start(uint32_t b) const { return (uint64_t)b * bcap; }
+    QO_DEV uint64_t cap(uint32_t) const { return bcap; }
+    QO_DEV uint64_t count(uint32_t b, const uint32_t *ctr) const {
+        const uint64_t c = ctr[b * kCtrStride];
+        return c < bcap ? c : bcap;
+    }
+};
+// Exact: regions from the exclusive scan `off` (2^L + 1 entries) of the
+// fine-bucket histogram; a bucket of this split covers fine buckets
+// [b << sh, (b + 1) << sh).  Nothing overflows.
+struct ExactLayout {
+    const uint32_t *off;
+    int sh;
+    QO_DEV uint64_t start(uint32_t b) const { return off[(size_t)b << sh]; }
+    QO_DEV uint64_t cap(uint32_t b) const { return off[(size_t)(b + 1) << sh] - off[(size_t)b << sh]; }
+    QO_DEV uint64_t count(uint32_t b, const uint32_t *) const { return cap(b); }
+};
+
+struct SplitLds {
+    uint64_t stage[kSplitTile];
+    uint32_t cnt[kMaxSplit], loff[kMaxSplit], gbase[kMaxSplit];
+    uint32_t w[4];
+};
+
+// Splits the block's live tuples (J groups of 4 per thread, bit k of pass[j]
+// marks element k of group j) into `parts` buckets by
+// part = (hash32(key) >> shift) & (parts - 1); bucket index = bucket0 + part,
+// placed by `out` in `bucket`; ctr holds one fill cursor per bucket.
+template <int J, class Layout, class Overflow>
+QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], const uint32_t (&pass)[J],
+                        int shift, uint32_t parts, uint32_t bucket0, uint64_t *__restrict__ bucket,
+                        const Layout &out, uint32_t *ctr, SplitLds &sh, const Overflow &overflow) {
+    const uint32_t mask = parts - 1;
+    for (uint32_t i = threadIdx.x; i < parts; i += 256) sh.cnt[i] = 0;
+    __syncthreads();
+    // Rank inside the tile: (part << 16) | arrival order within the part.
+    uint32_t rank[J][4];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            rank[j][k] = 0;
+            if ((pass[j] >> k) & 1u) {
+                const uint32_t p = (hash32((uint32_t)key[j][k]) >> shift) & mask;
+                rank[j][k] = (p << 16) | atomicAdd(&sh.cnt[p], 1u);
+            }
+        }
+    __syncthreads();
+    // Part offsets inside the tile and one bucket reservation per non-empty part.
+    const uint32_t c0 = threadIdx.x < parts ? sh.cnt[threadIdx.x] : 0u;
+    const uint32_t ex = block_excl_scan256(c0, sh.w);
+    if (threadIdx.x < parts) {
+        sh.loff[threadIdx.x] = ex;
+        sh.gbase[threadIdx.x] = c0 ? atomicAdd(&ctr[(bucket0 + threadIdx.x) * kCtrStride], c0) : 0u;
+    }
+    __syncthreads();
+    const uint32_t total = sh.loff[parts - 1] + sh.cnt[parts - 1];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if ((pass[j] >> k) & 1u)
+                sh.stage[sh.loff[rank[j][k] >> 16] + (rank[j][k] & 0xFFFFu)] = join_pack(key[j][k], pay[j][k]);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < total; i += 256) {
+        const uint64_t v = sh.stage[i];
+        const uint32_t p = (hash32((uint32_t)v) >> shift) & mask;
+        const uint32_t b = bucket0 + p;
+        const uint64_t g = (uint64_t)sh.gbase[p] + (i - sh.loff[p]);
+        // Plain stores: runs from neighbouring tiles complete each other's
+        // lines in L2 before they are written back.
+        if (g < out.cap(b)) bucket[out.start(b) + g] = v;
+        else overflow(v);
+    }
+}
+
+// Split A: columns (key, payload[, filter]) -> 2^L1 buckets.
+template <bool HAS_FILTER, class Layout, class Overflow>
+__global__ __launch_bounds__(256) void k_split_cols(const int32_t *__restrict__ kcol, const int32_t *__restrict__ pcol,
+                                                    const int32_t *__restrict__ fcol, size_t n, int32_t flo,
+                                                    int32_t fhi, bool vec, int shift, uint32_t parts,
+                                                    uint64_t *__restrict__ bucket, Layout out, uint32_t *ctr,
+                                                    Overflow overflow) {
+    __shared__ SplitLds sh;
+    int32_t key[kSplitJ][4], pay[kSplitJ][4];
+    uint32_t pass[kSplitJ];
+    load_s_tile<kSplitJ, HAS_FILTER>(kcol, pcol, fcol, n, flo, fhi, vec, blockIdx.x, key, pay, pass);
+    split_block<kSplitJ>(key, pay, pass, shift, parts, 0, bucket, out, ctr, sh, overflow);
+}
+
+// Loads up to J*1024 (key, payload) pairs from `src` (16-byte aligned): pair
+// quad j of thread t holds pairs 2v, 2v+1 for v = (2j + h)*256 + t, h = 0, 1.
+template <int J>
+QO_DEV void load_pairs(const uint64_t *__restrict__ src, uint32_t m, int32_t (&key)[J][4], int32_t (&pay)[J][4],
+                       uint32_t (&pass)[J]) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        pass[j] = 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t v = (uint32_t)(2 * j + h) * 256 + threadIdx.x;
+            const uint32_t e = 2 * v;
+            key[j][2 * h] = pay[j][2 * h] = key[j][2 * h + 1] = pay[j][2 * h + 1] = 0;
+            if (e + 2 <= m) {
+                const uint4 q = ld_stream(reinterpret_cast<const uint4 *>(src) + v);
+                key[j][2 * h] = (int32_t)q.x;
+                pay[j][2 * h] = (int32_t)q.y;
+                key[j][2 * h + 1] = (int32_t)q.z;
+                pay[j][2 * h + 1] = (int32_t)q.w;
+                pass[j] |= 3u << (2 * h);
+            } else if (e < m) {
+                const uint64_t w = src[e];
+                key[j][2 * h] = (int32_t)(uint32_t)w;
+                pay[j][2 * h] = (int32_t)(w >> 32);
+                pass[j] |= 1u << (2 * h);
+            }
+        }
+    }
+}
+
+// Split B on tile t of input bucket p1 (placed by `in`): -> buckets
+// (p1 << L2) + part.  An input bucket starting at an odd position (exact
+// layout) is read with element loads.
+template <class InLayout, class Layout, class Overflow>
+QO_DEV void split_pairs_tile(const uint64_t *__restrict__ input, const InLayout &in, const uint32_t *in_ctr,
+                             uint32_t p1, uint32_t t, int shift, uint32_t parts, uint64_t *__restrict__ bucket,
+                             const Layout &out, uint32_t *ctr, SplitLds &sh, const Overflow &overflow) {
+    const uint64_t n = in.count(p1, in_ctr);
+    const uint64_t e0 = (uint64_t)t * kSplitTile;
+    if (e0 >= n) return;  // whole block: no barrier has been reached
+    const uint32_t m = (uint32_t)(n - e0 < kSplitTile ? n - e0 : kSplitTile);
+    int32_t key[kSplitJ][4], pay[kSplitJ][4];
+    uint32_t pass[kSplitJ];
+    const uint64_t s0 = in.start(p1) + e0;
+    if ((s0 & 1u) == 0) {
+        load_pairs<kSplitJ>(input + s0, m, key, pay, pass);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kSplitJ; ++j) {
+            pass[j] = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t e = 2 * ((uint32_t)(2 * j + (k >> 1)) * 256 + threadIdx.x) + (k & 1);
+                key[j][k] = pay[j][k] = 0;
+                if (e < m) {
+                    const uint64_t w = input[s0 + e];
+                    key[j][k] = (int32_t)(uint32_t)w;
+                    pay[j][k] = (int32_t)(w >> 32);
+                    pass[j] |= 1u << k;
+                }
+            }
+        }
+    }
+    split_block<kSplitJ>(key, pay, pass, shift, parts, p1 * parts, bucket, out, ctr, sh, overflow);
+}
+
+// Split B with fixed-capacity input buckets: block = (bucket, tile).
+template <class Layout, class Overflow>
+__global__ __launch_bounds__(256) void k_split_pairs(const uint64_t *__restrict__ input, FixedLayout in,
+                                                     const uint32_t *in_ctr, uint32_t tiles_per_bucket, int shift,
+                                                     uint32_t parts, uint64_t *__restrict__ bucket, Layout out,
+                                                     uint32_t *ctr, Overflow overflow) {
+    __shared__ SplitLds sh;
+    split_pairs_tile(input, in, in_ctr, blockIdx.x / tiles_per_bucket, blockIdx.x % tiles_per_bucket, shift, parts,
+                     bucket, out, ctr, sh, overflow);
+}
+
+// Split B of the exact split: the blocks enumerate the tiles of all input
+// buckets in order (bucket b has ceil(size(b) / kSplitTile) tiles).
+template <class Overflow>
+__global__ __launch_bounds__(256) void k_split_pairs_exact(const uint64_t *__restrict__ input, ExactLayout in,
+                                                           uint32_t nin, int shift, uint32_t parts,
+                                                           uint64_t *__restrict__ bucket, ExactLayout out,
+                                                           uint32_t *ctr, Overflow overflow) {
+    __shared__ SplitLds sh;
+    __shared__ uint32_t s_b, s_t;
+    if (threadIdx.x == 0) {
+        uint32_t t = blockIdx.x, b = nin;
+        for (uint32_t i = 0; i < nin; ++i) {
+            const uint32_t nt = (uint32_t)((in.cap(i) + kSplitTile - 1) / kSplitTile);
+            if (t < nt) {
+                b = i;
+                break;
+            }
+            t -= nt;
+        }
+        s_b = b;
+        s_t = t;
+    }
+    __syncthreads();
+    if (s_b >= nin) return;  // past the last tile (the grid is an upper bound)
+    split_pairs_tile(input, in, nullptr, s_b, s_t, shift, parts, bucket, out, ctr, sh, overflow);
+}
+
+// Launches split A (and B) with fixed bucket capacities: kcol/pcol (and fcol
+// when not null, with the range [flo, fhi]) -> fine buckets.  The counters must
+// have been zeroed.
+template <class Overflow>
+inline int launch_splits(const SplitPlan &p, const SplitBufs &b, const int32_t *kcol, const int32_t *pcol,
+                         const int32_t *fcol, size_t n, int32_t flo, int32_t fhi, const Overflow &overflow,
+                         hipStream_t st) {
+    const bool vec = ((uintptr_t)kcol % 16 == 0) && ((uintptr_t)pcol % 16 == 0) && (!fcol || (uintptr_t)fcol % 16 == 0);
+    const uint32_t partsA = 1u << p.L1;
+    const size_t tiles = (n + kSplitTile - 1) / kSplitTile;
+    QO_REQUIRE(tiles < 0x7FFFFFFFull, "input too large for a hash split");
+    const FixedLayout la{p.bcapA}, lb{p.bcapB};
+    if (fcol)
+        k_split_cols<true><<<(unsigned)tiles, 256, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
+                                                            b.bucketA, la, b.ctr, overflow);
+    else
+        k_split_cols<false><<<(unsigned)tiles, 256, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
+                                                             b.bucketA, la, b.ctr, overflow);
+    QO_LAUNCHED();
+    if (p.L2 > 0) {
+        const uint32_t tpb = (uint32_t)((p.bcapA + kSplitTile - 1) / kSplitTile);
+        QO_REQUIRE((uint64_t)tpb * partsA < 0x7FFFFFFFull, "input too large for a hash split");
+        k_split_pairs<<<tpb * partsA, 256, 0, st>>>(b.bucketA, la, b.ctr, tpb, 32 - p.L, 1u << p.L2, b.bucketB, lb,
+                                                   b.ctr + (size_t)partsA * kCtrStride, overflow);
+        QO_LAUNCHED();
+    }
+    return QO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Exact split: a histogram pass over the keys sizes every bucket exactly, so
+// skewed keys cannot overflow.  Output: the fine buckets back to back in
+// `fine` (bucket f at [off[f], off[f + 1])), with `mid` holding split A's.
+// ---------------------------------------------------------------------------
+// Histogram of fine buckets (top L bits of hash32) over the key column.
+// Dynamic LDS: nb counters.
+static __global__ __launch_bounds__(512) void k_split_hist(const int32_t *__restrict__ key, size_t n, int shift, uint32_t nb,
+                                                           uint32_t *hist) {
+    extern __shared__ uint32_t s_hist[];
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) s_hist[i] = 0;
+    __syncthreads();
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const bool vec = ((uintptr_t)key % 16) == 0;
+    size_t from = 0;
+    if (vec) {
+        const size_t nv = n / 4;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+            const int4 k = ld_stream(reinterpret_cast<const int4 *>(key) + i);
+            atomicAdd(&s_hist[hash32((uint32_t)k.x) >> shift], 1u);
+            atomicAdd(&s_hist[hash32((uint32_t)k.y) >> shift], 1u);
+            atomicAdd(&s_hist[hash32((uint32_t)k.z) >> shift], 1u);
+            atomicAdd(&s_hist[hash32((uint32_t)k.w) >> shift], 1u);
+        }
+        from = nv * 4;
+    }
+    for (size_t i = from + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        atomicAdd(&s_hist[hash32((uint32_t)key[i]) >> shift], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+        if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
+}
+
+struct ExactSplitBufs {
+    uint32_t *ctr = nullptr;   // fill cursors: 2^L1 (split A), then 2^L (split B), kCtrStride apart
+    uint32_t *hist = nullptr;  // 2^L + 1 fine-bucket counts (last entry stays 0)
+    uint32_t *off = nullptr;   // 2^L + 1 exclusive offsets; off[2^L] = n
+    void *scan_ws = nullptr;
+    uint64_t *mid = nullptr;   // split A output (n pairs), when L2 > 0
+    uint64_t *fine = nullptr;  // fine buckets (n pairs)
+    size_t ctr_words = 0;
+};
+
+inline ExactSplitBufs exact_split_carve(const SplitPlan &p, size_t n, Carver &cv) {
+    ExactSplitBufs b;
+    const size_t nb = size_t(1) << p.L;
+    b.ctr_words = ((size_t(1) << p.L1) + nb) * kCtrStride;
+    b.ctr = cv.take<uint32_t>(b.ctr_words);
+    b.hist = cv.take<uint32_t>(nb + 1);
+    b.off = cv.take<uint32_t>(nb + 1);
+    b.scan_ws = cv.take<char>(scan_workspace_bytes(nb + 1));
+    if (p.L2 > 0) b.mid = cv.take<uint64_t>(n ? n : 1);
+    b.fine = cv.take<uint64_t>(n ? n : 1);
+    return b;
+}
+
+// Histogram, offsets and the exact split(s) of (kcol, pcol) into b.fine.
+inline int launch_exact_splits(const SplitPlan &p, const ExactSplitBufs &b, const int32_t *kcol,
+                               const int32_t *pcol, size_t n, hipStream_t st) {
+    const uint32_t nb = 1u << p.L;
+    QO_REQUIRE(n < 0xFFFFFFFFull && nb <= 16384, "exact split too large");
+    QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));
+    QO_TRY(launch_fill_u32(b.hist, 0, nb + 1, st));
+    const unsigned hg = grid_for(n, 512 * 16, (unsigned)props().cu_count * 2);
+    k_split_hist<<<hg, 512, (size_t)nb * 4, st>>>(kcol, n, 32 - p.L, nb, b.hist);
+    QO_LAUNCHED();
+    QO_TRY(launch_scan_excl_u32(b.hist, b.off, nb + 1, b.scan_ws, st));
+    const bool vec = ((uintptr_t)kcol % 16 == 0) && ((uintptr_t)pcol % 16 == 0);
+    const uint32_t partsA = 1u << p.L1;
+    const size_t tiles = (n + kSplitTile - 1) / kSplitTile;
+    const ExactLayout la{b.off, p.L2}, lf{b.off, 0};
+    AppendOverflow none{nullptr, nullptr};  // exact regions: never called
+    uint64_t *outA = p.L2 > 0 ? b.mid : b.fine;
+    k_split_cols<false><<<(unsigned)tiles, 256, 0, st>>>(kcol, pcol, nullptr, n, 0, 0, vec, 32 - p.L1, partsA, outA,
+                                                         la, b.ctr, none);
+    QO_LAUNCHED();
+    if (p.L2 > 0) {
+        // At most one partial tile per A bucket beyond n / kSplitTile full ones.
+        const size_t tiles_b = tiles + partsA;
+        QO_REQUIRE(tiles_b < 0x7FFFFFFFull, "input too large for a hash split");
+        k_split_pairs_exact<<<(unsigned)tiles_b, 256, 0, st>>>(b.mid, la, partsA, 32 - p.L, 1u << p.L2, b.fine, lf,
+                                                               b.ctr + (size_t)partsA * kCtrStride, none);
+        QO_LAUNCHED();
+    }
+    return QO_OK;
+}
+
+}  // namespace qo

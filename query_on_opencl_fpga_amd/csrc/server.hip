@@ -240,7 +240,7 @@ static size_t request_bytes(qo_server *sv, const qo_request &r) {
     if (r.kind == QO_Q_FILTER) return 4096;
     if (r.kind == QO_Q_GROUPBY) {
         size_t G = A.d.group_capacity ? A.d.group_capacity : 1;
-        return G * 28 + groupby_ws_bytes(G) + 8192;
+        return G * 28 + groupby_ws_bytes(G, A.d.n) + 8192;
     }
     const Table &B = sv->tables[r.table_b];
     qo_join_query q{};
@@ -307,7 +307,7 @@ static int launch_batch(qo_server *sv, Batch &b, hipStream_t st) {
         size_t G = sv->tables[live[i]->r.table_a].d.group_capacity;
         cv.take<int32_t>(G); cv.take<int64_t>(G); cv.take<int64_t>(G); cv.take<int32_t>(G); cv.take<int32_t>(G);
         cv.take<uint64_t>(1);
-        cv.take<char>(groupby_ws_bytes(G));
+        cv.take<char>(groupby_ws_bytes(G, sv->tables[live[i]->r.table_a].d.n));
         gsz.push_back(G);
     }
     for (auto &q : b.jq) {
@@ -367,7 +367,7 @@ static int launch_batch(qo_server *sv, Batch &b, hipStream_t st) {
         int32_t *omin = c2.take<int32_t>(G);
         int32_t *omax = c2.take<int32_t>(G);
         uint64_t *num = c2.take<uint64_t>(1);
-        void *ws = c2.take<char>(groupby_ws_bytes(G));
+        void *ws = c2.take<char>(groupby_ws_bytes(G, A.n));
         QO_TRY(launch_groupby(A.grp, A.val, A.n, G, okey, ocnt, osum, omin, omax, num, ws, st));
         k_s_checksum_groupby<<<std::min<unsigned>(kGrid, (unsigned)((G + 255) / 256) + 1), 256, 0, st>>>(
             okey, ocnt, osum, omin, omax, num, G, b.d_res + slot_of(live[gi[k]]));

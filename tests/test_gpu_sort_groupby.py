@@ -84,6 +84,75 @@ def test_groupby_capacity_overflow_reports(qo):
         qo.operators.groupby_agg(keys, keys, 100)
 
 
+# ---------------------------------------------------------------------------
+# Direct (LDS cache + global table) vs partitioned (hash split, per-bucket LDS
+# tables) group-by; the partitioned path is forced on small inputs with 16
+# groups per bucket so that both split levels and the overflow list are used.
+# ---------------------------------------------------------------------------
+TUNE_GROUPBY_PART = 6
+
+
+@pytest.fixture(params=["direct", "partitioned"])
+def gb_mode(request, qo):
+    import ctypes as C
+    lib = qo._lib.load()
+    mode, target = (2, 1024) if request.param == "direct" else (1, 16)
+    saved = []
+    for cls in range(40):
+        b, i, p = C.c_int(), C.c_int(), C.c_int()
+        qo._lib.call("qo_tune_get", TUNE_GROUPBY_PART, cls, C.byref(b), C.byref(i), C.byref(p))
+        saved.append((cls, b.value, i.value, p.value))
+        qo._lib.call("qo_tune_set", TUNE_GROUPBY_PART, cls, 512, target, mode)
+    yield request.param
+    for cls, b, i, p in saved:
+        lib.qo_tune_set(TUNE_GROUPBY_PART, cls, b, i, p)
+
+
+@pytest.mark.parametrize("G,z,n", [(100, 0.0, 1000), (5000, 1.0, 300_000), (70_000, 0.5, 1_000_003),
+                                   (1 << 18, 0.0, 2_000_000)])
+def test_groupby_modes_match_oracle(qo, orc, gb_mode, G, z, n):
+    t = orc.zipf_table(G, z)
+    keys = orc.gen_zipf_keys(n, t, 11, 1, 29).astype(np.int32)
+    vals = orc.gen_uniform_u32(n, 11, 2, 0, 1 << 16).astype(np.int32)
+    got = qo.operators.groupby_agg(keys, vals, G)
+    exp = orc.groupby_agg(keys, vals)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_groupby_modes_signed_keys_and_sentinel(qo, orc, gb_mode):
+    rng = np.random.default_rng(14)
+    n = 400_000
+    keys = rng.integers(-20_000, 20_000, n, dtype=np.int64).astype(np.int32)
+    keys[::9] = -1
+    keys[5::11] = INT_MIN
+    vals = rng.integers(INT_MIN, INT_MAX, n, dtype=np.int64).astype(np.int32)
+    got = qo.operators.groupby_agg(keys, vals, 40_001)
+    exp = orc.groupby_agg(keys, vals)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_groupby_modes_heavy_hitter(qo, orc, gb_mode):
+    """60% of the tuples carry one key: its bucket overflows and the excess is
+    aggregated from the overflow list."""
+    rng = np.random.default_rng(15)
+    n = 1_500_000
+    keys = rng.integers(0, 30_000, n, dtype=np.int64).astype(np.int32)
+    keys[rng.random(n) < 0.6] = 12_345
+    vals = rng.integers(-(1 << 20), 1 << 20, n, dtype=np.int64).astype(np.int32)
+    got = qo.operators.groupby_agg(keys, vals, 30_000)
+    exp = orc.groupby_agg(keys, vals)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_groupby_modes_capacity_overflow_reports(qo, gb_mode):
+    keys = np.arange(20_000, dtype=np.int32)
+    with pytest.raises(qo._lib.QoError):
+        qo.operators.groupby_agg(keys, keys, 100)
+
+
 def test_startpos_and_aggregate_known_answers(qo):
     from query_on_opencl_fpga_amd.device import DeviceBuffer
     c = KA["parallel_aggregate"]
