@@ -504,12 +504,15 @@ def run_groupby(args, qo, lib) -> dict:
                 b.free()
             t.free()
     worst = min(rows, key=lambda r: r["tuples_per_s"])
+    kernel = ("qo_groupby_agg (partitioned: k_split_hist + k_split_cols + k_split_pairs_exact + k_groupby_chunks"
+              " + compaction/sort/gather)" if worst["G"] > 1024 else "qo_groupby_agg (k_groupby_hash + compaction)")
     return {"metric": METRIC, "value": worst["tuples_per_s"], "unit": "tuples/s (slowest of 6 cases)",
             "ms_per_step": worst["ms"], "dtype": "int32",
             "config": {"workload": "hash group-by (COUNT, SUM, MIN, MAX), key Zipf(z) over G groups", "tuples": n,
                        "cases": rows},
-            "roofline": {"bound": "hbm", "kernel": "k_groupby_hash", "achieved": worst["GBps_alg"],
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": worst["frac_hbm"], "traffic": None}}
+            "roofline": {"bound": "hbm", "kernel": kernel, "achieved": worst["GBps_alg"],
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": worst["frac_hbm"], "traffic": None,
+                         "alg_bytes_formula": "8*N + 28*G", "avg_launch_ms": worst["ms"]}}
 
 
 def main() -> int:
