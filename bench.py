@@ -160,14 +160,21 @@ def run_join(args, qo, lib) -> dict:
     probe_bytes = 12 * ns + 12 * m
     achieved = probe_bytes / (probe_avg * 1e-3) / 1e9
     import ctypes as C
+    mode, used, dups = C.c_int(), C.c_uint64(), C.c_int()
+    qo._lib.call("qo_join_table_info", table.ptr, C.byref(mode), C.byref(used), C.byref(dups), st.handle)
     part, bits, splits = C.c_int(), C.c_int(), C.c_int()
-    qo._lib.call("qo_join_strategy", tb, ns, C.byref(part), C.byref(bits), C.byref(splits))
-    if part.value:
+    qo._lib.call("qo_join_strategy", tb, ns, mode.value, C.byref(part), C.byref(bits), C.byref(splits))
+    table_kind = (f"key-range table ({used.value} slots, home slot = key - min key)" if mode.value == 1
+                  else f"hash-mixed table ({used.value} slots)")
+    if part.value and mode.value == 1:
+        kernel = "probe stage (one qo_hash_probe call): k_split_cols + k_range_probe"
+        strategy = f"{table_kind}; probe: one split into {1 << bits.value} buckets, XCD-local L2 lookups"
+    elif part.value:
         kernel = ("probe stage (one qo_hash_probe call): k_split_cols"
                   + (" + k_split_pairs" if splits.value == 2 else "") + " + k_slice_probe")
-        strategy = f"partitioned, {1 << bits.value} LDS slices, {splits.value} split pass(es)"
+        strategy = f"{table_kind}; probe: {1 << bits.value} LDS slices, {splits.value} split pass(es)"
     else:
-        kernel, strategy = "k_join_probe", "direct"
+        kernel, strategy = "k_join_probe", f"{table_kind}; probe: direct"
     traffic = args.traffic
     if traffic is None:
         traffic = profile_traffic("join_probe", {"R_tuples": nr, "S_tuples": ns, "strategy": strategy})
@@ -526,6 +533,8 @@ def main() -> int:
     ap.add_argument("--join-mode", choices=["auto", "partitioned", "direct"], default="auto",
                     help="join build/probe strategy (default: partitioned once the table outgrows one L2)")
     ap.add_argument("--slice-kib", type=int, default=0, help="partitioned join: table slice per partition")
+    ap.add_argument("--no-key-range", action="store_true",
+                    help="join: always build the hash-mixed table (no key-range table)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--copy-nt", action="store_true", help="copy with non-temporal loads")
     ap.add_argument("--queries", type=int, default=512)
@@ -563,6 +572,9 @@ def main() -> int:
             b, i, p = C.c_int(), C.c_int(), C.c_int()
             qo._lib.call("qo_tune_get", 5, cls, C.byref(b), C.byref(i), C.byref(p))
             qo._lib.call("qo_tune_set", 5, cls, b.value, args.slice_kib or i.value, mode)
+    if args.no_key_range:
+        for cls in range(40):
+            qo._lib.call("qo_tune_set", 7, cls, 256, 256, 2)
     run = {"join": run_join, "filter_agg": run_filter_agg, "groupby": run_groupby, "copy": run_copy,
            "batched": run_batched, "served": run_served}
     if args.config not in run:

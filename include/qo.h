@@ -148,13 +148,21 @@ int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t grou
 /* ---- hash join ------------------------------------------------------------ */
 /* Bytes of device memory a build table for `build_rows` rows needs. */
 size_t qo_join_table_bytes(size_t build_rows);
-/* Strategy the build/probe of n rows against a table of table_bytes takes:
- * *partitioned = 1 for the per-slice LDS path (then *bits = log2 of the slice
- * count, *splits = 1 or 2 split passes), 0 for the direct path.  Host-only. */
-int qo_join_strategy(size_t table_bytes, size_t n, int *partitioned, int *bits, int *splits);
+/* Strategy the build/probe of n rows against a table of table_bytes takes for
+ * a table of the given mode (0 hash-mixed, 1 key-range; see qo_join_table_info):
+ * *partitioned = 1 for a partitioned path (hash-mixed: per-slice LDS tables,
+ * *splits = 1 or 2 split passes; key-range: one split, then L2-resident table
+ * slices), *bits = log2 of the partition count; 0 for the direct path.
+ * Host-only. */
+int qo_join_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits);
 /* Builds the table in `table` (256-byte aligned device memory of table_bytes,
- * e.g. qo_join_table_bytes(nr)).  Tables larger than one L2 are built
- * partition by partition through a per-stream workspace. */
+ * e.g. qo_join_table_bytes(nr)).  When the build keys span a range that fits a
+ * power of two of the table's slots (and the TUNE op 7 limit), the table is
+ * direct-mapped (home slot = key - min key), the layout of the reference's
+ * key & (B - 1) buckets; duplicate keys fall back to a hash-mixed table, which
+ * is built partition by partition through a per-stream workspace when it is
+ * larger than one L2.  The choice is made on the device: the call stays
+ * asynchronous. */
 int qo_hash_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay,
                   size_t nr, qo_stream s);
 /* Probe with the S filter flo <= sfilt <= fhi fused (sfilt may be NULL: no filter).
@@ -165,6 +173,10 @@ int qo_hash_build(void *table, size_t table_bytes, const int32_t *rkey, const in
 int qo_hash_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
                   const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey,
                   int32_t *orpay, int32_t *ospay, size_t out_capacity, uint64_t *d_count, qo_stream s);
+/* Which table a build made (synchronizes stream s): *mode = 0 hash-mixed,
+ * 1 key-range; *slots_used = slots in use; *has_dups = 1 when some key has
+ * several build tuples (the probe then walks whole chains). */
+int qo_join_table_info(const void *table, int *mode, uint64_t *slots_used, int *has_dups, qo_stream s);
 /* Filter S, build on R, probe, sort output by (key, s_payload).  Synchronous:
  * *h_count receives the match count. */
 int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay,
@@ -203,7 +215,10 @@ int qo_batch_destroy(qo_batch *b);
 /* ---- autotuner -------------------------------------------------------------- */
 /* Operators: 0 filter_agg, 1 filter, 2 probe (param = table load %), 3 groupby, 4 sort,
  * 5 partitioned join (items = table slice KiB per partition, param = 0 when the
- * table outgrows one L2 / 1 always / 2 never).  Size class = floor(log2 n). */
+ * table outgrows one L2 / 1 always / 2 never), 6 partitioned group-by (items =
+ * groups per fine bucket, param = 0 auto / 1 always / 2 never), 7 key-range join
+ * table (items = largest such table in MiB, param = 0 when the keys allow /
+ * 2 never).  Size class = floor(log2 n). */
 int qo_tune_get(int op, int size_class, int *block, int *items, int *param);
 int qo_tune_set(int op, int size_class, int block, int items, int param);
 int qo_tune_load(const char *path);

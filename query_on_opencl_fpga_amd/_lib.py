@@ -102,7 +102,8 @@ PROTOS = {
     "qo_groupby_agg": (C.c_int, [vp, vp, c_size, c_size, vp, vp, vp, vp, vp, vp, vp]),
     # join
     "qo_join_table_bytes": (c_size, [c_size]),
-    "qo_join_strategy": (C.c_int, [c_size, c_size, P(C.c_int), P(C.c_int), P(C.c_int)]),
+    "qo_join_strategy": (C.c_int, [c_size, c_size, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
+    "qo_join_table_info": (C.c_int, [vp, P(C.c_int), P(C.c_uint64), P(C.c_int), vp]),
     "qo_hash_build": (C.c_int, [vp, c_size, vp, vp, c_size, vp]),
     "qo_hash_probe": (C.c_int, [vp, c_size, vp, vp, vp, c_size, c_i32, c_i32, vp, vp, vp, c_size, vp, vp]),
     "qo_hj_filter_sort": (C.c_int, [vp, c_size, vp, vp, c_size, vp, vp, vp, c_size, c_i32, c_i32,

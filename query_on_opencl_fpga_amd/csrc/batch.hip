@@ -47,7 +47,11 @@ __global__ void k_b_init(const BQuery *q, int nq) {
     h->has_dups = 0;
     h->sentinel_count = 0;
     h->overflow = 0;
+    h->mode = kJoinHashMix;
     h->nr = q[i].nr;
+    h->slots_alloc = q[i].table_cap;
+    h->kmin = 0;
+    h->shift = 0;
     *q[i].count = 0;
 }
 
@@ -90,6 +94,7 @@ __global__ __launch_bounds__(256) void k_b_probe(const BQuery *q, int nq, uint32
                                                  uint64_t *mkeys, uint32_t *mvals) {
     __shared__ uint32_t s_tot[kProbeJ * 4];
     __shared__ uint64_t s_base;
+    __shared__ JoinStage stage;
     QO_TILE_LOOP(ticket, total, t)
     const int qi = query_of<1>(q, nq, t);
     const BQuery &Q = q[qi];
@@ -97,10 +102,10 @@ __global__ __launch_bounds__(256) void k_b_probe(const BQuery *q, int nq, uint32
     PackedEmit emit{mkeys + Q.m_base, mvals + Q.m_base, Q.out_cap};
     if (Q.sfilt)
         probe_tile<true>(Q.hdr, slots, Q.skey, Q.spay, Q.sfilt, Q.ns, Q.flo, Q.fhi, Q.vec != 0, t - Q.stile0, Q.count,
-                         emit, s_tot, &s_base);
+                         emit, s_tot, &s_base, &stage);
     else
         probe_tile<false>(Q.hdr, slots, Q.skey, Q.spay, Q.sfilt, Q.ns, Q.flo, Q.fhi, Q.vec != 0, t - Q.stile0, Q.count,
-                          emit, s_tot, &s_base);
+                          emit, s_tot, &s_base, &stage);
     QO_TILE_LOOP_END
 }
 

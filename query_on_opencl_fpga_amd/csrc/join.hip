@@ -33,12 +33,156 @@ __global__ void k_join_init(JoinHeader *hdr, uint64_t cap, uint64_t nr) {
         hdr->has_dups = 0;
         hdr->sentinel_count = 0;
         hdr->overflow = 0;
+        hdr->mode = kJoinHashMix;
         hdr->nr = nr;
+        hdr->slots_alloc = cap;
+        hdr->kmin = 0;
+        hdr->shift = 0;
+        hdr->rmin = INT32_MAX;
+        hdr->rmax = INT32_MIN;
+        hdr->occupied = 0;
     }
 }
 
+// Slots [0, hdr->cap) := empty, when the table's mode is run_if.
+__global__ __launch_bounds__(256) void k_fill_slots(JoinHeader *hdr, uint32_t run_if) {
+    if (hdr->mode != run_if) return;
+    const uint64_t cap = hdr->cap;
+    uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
+    const uint4 e{~0u, ~0u, ~0u, ~0u};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap / 2; i += stride)
+        reinterpret_cast<uint4 *>(slots)[i] = e;
+    if ((cap & 1u) && blockIdx.x == 0 && threadIdx.x == 0) slots[cap - 1] = kEmptySlot;
+}
+
+// ---------------------------------------------------------------------------
+// Key-range table (direct-mapped: home slot = key - kmin).
+// ---------------------------------------------------------------------------
+// Smallest and largest build key, one atomic pair per block.
+__global__ __launch_bounds__(256) void k_key_range(const int32_t *__restrict__ rkey, size_t nr, JoinHeader *hdr) {
+    __shared__ int32_t s_mn[4], s_mx[4];
+    int32_t mn = INT32_MAX, mx = INT32_MIN;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t from = 0;
+    if (((uintptr_t)rkey & 15u) == 0) {
+        const size_t nv = nr / 4;
+        for (size_t i = t; i < nv; i += stride) {
+            const int4 k = ld_stream(reinterpret_cast<const int4 *>(rkey) + i);
+            mn = min(mn, min(min(k.x, k.y), min(k.z, k.w)));
+            mx = max(mx, max(max(k.x, k.y), max(k.z, k.w)));
+        }
+        from = nv * 4;
+    }
+    for (size_t i = from + t; i < nr; i += stride) {
+        mn = min(mn, rkey[i]);
+        mx = max(mx, rkey[i]);
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    if (lane_id() == 0) {
+        s_mn[wave_id()] = mn;
+        s_mx[wave_id()] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            mn = min(mn, s_mn[w]);
+            mx = max(mx, s_mx[w]);
+        }
+        mn = min(mn, s_mn[0]);
+        mx = max(mx, s_mx[0]);
+        atomicMin(&hdr->rmin, mn);
+        atomicMax(&hdr->rmax, mx);
+    }
+}
+
+// Picks the key-range table when the key range fits 2^p <= the allocated slots
+// (and max_bytes): cap = 2^p, h = (key - kmin) << (32 - p).
+__global__ void k_range_decide(JoinHeader *hdr, uint64_t max_bytes) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || hdr->nr == 0) return;
+    const int64_t range = (int64_t)hdr->rmax - (int64_t)hdr->rmin + 1;
+    int p = 1;
+    while (p < 31 && (int64_t(1) << p) < range) ++p;
+    const uint64_t cap = uint64_t(1) << p;
+    if ((int64_t)cap < range || cap > hdr->slots_alloc || cap * 8 > max_bytes) return;
+    hdr->kmin = hdr->rmin;
+    hdr->shift = 32u - (uint32_t)p;
+    hdr->cap = cap;
+    hdr->mode = kJoinKeyRange;
+}
+
+// Every build tuple to its home slot with a plain store (no probing): two
+// tuples with one key leave fewer occupied slots than tuples, which the count
+// below detects.
+__global__ __launch_bounds__(256) void k_range_scatter(const int32_t *__restrict__ rkey,
+                                                       const int32_t *__restrict__ rpay, size_t nr, JoinHeader *hdr) {
+    if (hdr->mode != kJoinKeyRange) return;
+    const uint32_t kmin = (uint32_t)hdr->kmin;
+    uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t sentinel = 0;
+    auto put = [&](int32_t k, int32_t v) {
+        const uint64_t w = join_pack(k, v);
+        if (w == kEmptySlot) ++sentinel;
+        else slots[(uint32_t)k - kmin] = w;
+    };
+    size_t from = 0;
+    if ((((uintptr_t)rkey | (uintptr_t)rpay) & 15u) == 0) {
+        const size_t nv = nr / 4;
+        for (size_t i = t; i < nv; i += stride) {
+            const int4 k = ld_stream(reinterpret_cast<const int4 *>(rkey) + i);
+            const int4 v = ld_stream(reinterpret_cast<const int4 *>(rpay) + i);
+            put(k.x, v.x);
+            put(k.y, v.y);
+            put(k.z, v.z);
+            put(k.w, v.w);
+        }
+        from = nv * 4;
+    }
+    for (size_t i = from + t; i < nr; i += stride) put(rkey[i], rpay[i]);
+    sentinel = wave_sum(sentinel);
+    if (lane_id() == 0 && sentinel) atomicAdd(&hdr->sentinel_count, sentinel);
+}
+
+// Occupied slots of a key-range table, one atomic per block.
+__global__ __launch_bounds__(256) void k_range_count(JoinHeader *hdr) {
+    __shared__ uint32_t s_c[4];
+    if (hdr->mode != kJoinKeyRange) return;
+    const uint64_t cap = hdr->cap;  // a power of two >= 2
+    const uint4 *slots = reinterpret_cast<const uint4 *>(hdr + 1);
+    uint32_t c = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap / 2; i += stride) {
+        const uint4 q = ld_stream(slots + i);
+        c += ((q.x & q.y) != ~0u) + ((q.z & q.w) != ~0u);
+    }
+    c = wave_sum(c);
+    if (lane_id() == 0) s_c[wave_id()] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&hdr->occupied, (unsigned long long)(s_c[0] + s_c[1] + s_c[2] + s_c[3]));
+}
+
+// A key-range build that lost tuples to duplicate keys falls back to the
+// hash-mixed table (built next).
+__global__ void k_range_commit(JoinHeader *hdr) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || hdr->mode != kJoinKeyRange) return;
+    if (hdr->occupied + hdr->sentinel_count == hdr->nr) return;
+    hdr->mode = kJoinHashMix;
+    hdr->cap = hdr->slots_alloc;
+    hdr->sentinel_count = 0;
+    hdr->kmin = 0;
+    hdr->shift = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Hash-mixed table, direct build (small tables).
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_join_build(const int32_t *__restrict__ rkey, const int32_t *__restrict__ rpay,
                                                     size_t nr, uint64_t *slots, JoinHeader *hdr, uint64_t cap) {
+    if (hdr->mode != kJoinHashMix) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < nr; i += stride) join_insert(slots, cap, hdr, rkey[i], rpay[i]);
@@ -46,6 +190,10 @@ __global__ __launch_bounds__(256) void k_join_build(const int32_t *__restrict__ 
 
 size_t join_build_ws_bytes(size_t table_bytes, size_t nr) { return join_part_ws_bytes(table_bytes, nr, true); }
 
+// Build: the key-range table when the keys allow it (decided on the device),
+// else the hash-mixed table, partitioned or direct.  Every step after the
+// decision runs or returns at once according to the table's mode, so the
+// sequence is asynchronous and the same for every input.
 int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                       void *ws, hipStream_t st) {
     const size_t cap = join_table_slots(table_bytes);
@@ -55,36 +203,61 @@ int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, cons
     uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
     k_join_init<<<1, 64, 0, st>>>(hdr, cap, nr);
     QO_LAUNCHED();
-    // The partitioned build writes every slot itself.
+    const unsigned cus = (unsigned)props().cu_count;
+    const unsigned fill_grid = grid_for(cap / 2 + 1, 256, cus * 8);
+    const TuneParams tr = tune_get(TUNE_JOIN_RANGE, nr);
+    if (nr && tr.param != PART_NEVER) {
+        k_key_range<<<cus, 256, 0, st>>>(rkey, nr, hdr);
+        QO_LAUNCHED();
+        k_range_decide<<<1, 64, 0, st>>>(hdr, (uint64_t)std::max(1, tr.items) << 20);
+        QO_LAUNCHED();
+        k_fill_slots<<<fill_grid, 256, 0, st>>>(hdr, kJoinKeyRange);
+        QO_LAUNCHED();
+        k_range_scatter<<<grid_for(nr / 4 + 1, 256, cus * 8), 256, 0, st>>>(rkey, rpay, nr, hdr);
+        QO_LAUNCHED();
+        k_range_count<<<cus * 2, 256, 0, st>>>(hdr);
+        QO_LAUNCHED();
+        k_range_commit<<<1, 64, 0, st>>>(hdr);
+        QO_LAUNCHED();
+    }
+    // Hash-mixed table (each kernel returns at once for a key-range table).
     bool used = false;
     QO_TRY(launch_join_part_build(table, table_bytes, rkey, rpay, nr, ws, st, &used));
     if (used) return QO_OK;
-    QO_TRY(launch_fill_u64(slots, kEmptySlot, cap, st));
+    k_fill_slots<<<fill_grid, 256, 0, st>>>(hdr, kJoinHashMix);
+    QO_LAUNCHED();
     if (!nr) return QO_OK;
-    unsigned g = grid_for(nr, 256, (unsigned)props().cu_count * 16);
+    unsigned g = grid_for(nr, 256, cus * 16);
     k_join_build<<<g, 256, 0, st>>>(rkey, rpay, nr, slots, hdr, cap);
     QO_LAUNCHED();
     return QO_OK;
 }
 
 // Probe.  One block per tile of kProbeTile S tuples; *d_count accumulates the
-// matches (it is zeroed first, in stream order).
+// matches (it is zeroed first, in stream order).  With a gate the kernel runs
+// only for the table mode it names.
 template <bool HAS_FILTER>
 __global__ __launch_bounds__(256) void k_join_probe(const JoinHeader *__restrict__ hdr,
                                                     const uint64_t *__restrict__ slots,
                                                     const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
                                                     const int32_t *__restrict__ sfilt, size_t ns, int32_t flo,
-                                                    int32_t fhi, bool vec, SoaEmit emit, uint64_t *d_count) {
+                                                    int32_t fhi, bool vec, SoaEmit emit, uint64_t *d_count,
+                                                    Gate gate) {
     __shared__ uint32_t s_tot[kProbeJ * 4];
     __shared__ uint64_t s_base;
+    __shared__ JoinStageCol stage;
+    if (gate.off()) return;
     probe_tile<HAS_FILTER>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, blockIdx.x,
-                           reinterpret_cast<unsigned long long *>(d_count), emit, s_tot, &s_base);
+                           reinterpret_cast<unsigned long long *>(d_count), emit, s_tot, &s_base, &stage);
 }
 
 size_t join_probe_ws_bytes(size_t table_bytes, size_t ns) {
     return std::max<size_t>(256, join_part_ws_bytes(table_bytes, ns, false));
 }
 
+// Probe: large tables through their partitioned probe (per-slice LDS for a
+// hash-mixed table, split + L2-resident slices for a key-range table), small
+// ones directly; the table's mode picks among the launched paths on the device.
 int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
                       const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
                       int32_t *ospay, size_t cap, uint64_t *d_count, void *ws, hipStream_t st) {
@@ -94,18 +267,21 @@ int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey
     QO_REQUIRE(tiles < 0xFFFFFFFFull, "probe input too large");
     const JoinHeader *hdr = static_cast<const JoinHeader *>(table);
     const uint64_t *slots = reinterpret_cast<const uint64_t *>(hdr + 1);
+    bool hash_part = false, range_part = false;
+    QO_TRY(launch_join_part_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, cap,
+                                  d_count, ws, st, &hash_part, &range_part));
+    if (hash_part && range_part) return QO_OK;
+    // The direct probe serves the mode(s) no partitioned probe was launched for.
+    const Gate gate = hash_part ? Gate{&hdr->mode, kJoinKeyRange}
+                                : range_part ? Gate{&hdr->mode, kJoinHashMix} : Gate{};
     const bool vec = ((uintptr_t)skey % 16 == 0) && ((uintptr_t)spay % 16 == 0) && (!sfilt || (uintptr_t)sfilt % 16 == 0);
     SoaEmit emit{okey, orpay, ospay, cap};
-    bool used = false;
-    QO_TRY(launch_join_part_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, cap,
-                                  d_count, ws, st, &used));
-    if (used) return QO_OK;
     if (sfilt)
         k_join_probe<true><<<(unsigned)tiles, 256, 0, st>>>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, emit,
-                                                            d_count);
+                                                            d_count, gate);
     else
         k_join_probe<false><<<(unsigned)tiles, 256, 0, st>>>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, emit,
-                                                             d_count);
+                                                             d_count, gate);
     QO_LAUNCHED();
     return QO_OK;
 }
@@ -192,9 +368,23 @@ using namespace qo;
 
 extern "C" {
 
-int qo_join_strategy(size_t table_bytes, size_t n, int *partitioned, int *bits, int *splits) {
+int qo_join_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits) {
     QO_REQUIRE(partitioned && bits && splits, "null argument");
-    join_part_strategy(table_bytes, n, partitioned, bits, splits);
+    QO_REQUIRE(mode == 0 || mode == 1, "table mode must be 0 or 1");
+    join_part_strategy(table_bytes, n, mode, partitioned, bits, splits);
+    return QO_OK;
+}
+
+int qo_join_table_info(const void *table, int *mode, uint64_t *slots_used, int *has_dups, qo_stream s) {
+    QO_ENTER();
+    QO_REQUIRE(table && mode && slots_used && has_dups, "null argument");
+    hipStream_t st = stream_of(s);
+    JoinHeader h;
+    QO_HIP(hipStreamSynchronize(st));
+    QO_HIP(hipMemcpy(&h, table, sizeof(h), hipMemcpyDeviceToHost));
+    *mode = (int)h.mode;
+    *slots_used = h.cap;
+    *has_dups = (h.has_dups | h.sentinel_count) ? 1 : 0;
     return QO_OK;
 }
 

@@ -25,6 +25,14 @@
 //    compare-and-swap, as are the tuples of a full bucket.
 // The table format is the one of the direct path (qo_join.hpp): either path
 // can probe a table built by either.
+//
+// Key-range tables (one slot per key, no chains) are probed through one split
+// instead: S is split by the top bits of key - kmin into buckets whose table
+// slices are ~1 MiB, and the probe blocks of a bucket all run on one XCD (work
+// group i runs on XCD i % 8), so the XCD's L2 holds the slice while its
+// lookups run and a lookup costs an L2 hit instead of a random 128-byte line
+// from HBM.  Which of the two partitioned probes runs is decided on the device
+// from the table's mode; the other returns at once.
 #include <algorithm>
 
 #include "qo_internal.hpp"
@@ -63,6 +71,32 @@ static PartPlan part_plan(uint64_t cap, size_t n) {
     return p;
 }
 
+constexpr uint64_t kRangeSliceBytes = uint64_t(1) << 20;  // key-range table bytes per probe bucket
+
+// Key-range probe plan: one split into 2^L buckets, L in [3, 7].
+struct RangePlan {
+    bool on = false;
+    SplitPlan s;
+    uint32_t nb = 0;   // buckets
+    uint32_t tpb = 0;  // probe tiles per bucket (upper bound)
+};
+
+static RangePlan range_plan(uint64_t slots, size_t n) {
+    RangePlan r;
+    const TuneParams tp = tune_get(TUNE_JOIN_PART, n);
+    if (tp.param == PART_NEVER || n == 0 || n >= 0xFFFFFFFFull || slots < 2) return r;
+    if (tp.param != PART_ALWAYS && (slots * 8 <= (uint64_t(4) << 20) || n < (size_t(1) << 20))) return r;
+    uint64_t cap = 2;  // a key-range table uses at most the largest power of two <= slots
+    while (cap * 2 <= slots) cap *= 2;
+    int L = 3;
+    while (L < kMaxSplitBits && ((cap * 8) >> L) > kRangeSliceBytes) ++L;
+    r.on = true;
+    r.s = split_plan(L, n);
+    r.nb = 1u << L;
+    r.tpb = (uint32_t)((r.s.bcapA + kPairStep - 1) / kPairStep);
+    return r;
+}
+
 // Workspace carve-up (same order for sizing and binding).
 struct PartBufs {
     SplitBufs sb;
@@ -80,7 +114,14 @@ static PartBufs part_carve(const PartPlan &p, size_t n, bool build, Carver &cv) 
     return b;
 }
 
-void join_part_strategy(size_t table_bytes, size_t n, int *partitioned, int *bits, int *splits) {
+void join_part_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits) {
+    if (mode == (int)kJoinKeyRange) {
+        const RangePlan r = range_plan(join_table_slots(table_bytes), n);
+        *partitioned = r.on ? 1 : 0;
+        *bits = r.on ? r.s.L : 0;
+        *splits = r.on ? 1 : 0;
+        return;
+    }
     const PartPlan p = part_plan(join_table_slots(table_bytes), n);
     *partitioned = p.on ? 1 : 0;
     *bits = p.on ? p.L : 0;
@@ -88,11 +129,22 @@ void join_part_strategy(size_t table_bytes, size_t n, int *partitioned, int *bit
 }
 
 size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build) {
-    const PartPlan p = part_plan(join_table_slots(table_bytes), n);
-    if (!p.on) return 0;
-    Carver cv(nullptr);
-    part_carve(p, n, build, cv);
-    return cv.off + 256;
+    const uint64_t slots = join_table_slots(table_bytes);
+    const PartPlan p = part_plan(slots, n);
+    size_t bytes = 0;
+    if (p.on) {
+        Carver cv(nullptr);
+        part_carve(p, n, build, cv);
+        bytes = cv.off + 256;
+    }
+    const RangePlan r = build ? RangePlan{} : range_plan(slots, n);
+    if (r.on) {
+        // The two probes share the workspace: only one of them runs.
+        Carver cv(nullptr);
+        split_carve(r.s, cv);
+        bytes = std::max(bytes, cv.off + 256);
+    }
+    return bytes;
 }
 
 // Tuples that find their bucket full.  Probe: looked up and emitted at once.
@@ -102,7 +154,8 @@ struct ProbeOverflow {
     SoaEmit emit;
     QO_DEV void operator()(uint64_t v) const {
         const uint64_t *slots = reinterpret_cast<const uint64_t *>(hdr + 1);
-        probe_one(slots, hdr->cap, hdr->sentinel_count, (int32_t)(uint32_t)v, (int32_t)(v >> 32), counter, emit);
+        probe_one(slots, hdr->cap, JoinHash::of(hdr), hdr->sentinel_count, (int32_t)(uint32_t)v, (int32_t)(v >> 32),
+                  counter, emit);
     }
 };
 
@@ -135,11 +188,12 @@ QO_DEV void slice_load(uint64_t *s_slots, const uint64_t *__restrict__ slots, ui
     if ((len & 1u) && threadIdx.x == 0) s_slots[len - 1] = slots[lo + len - 1];
 }
 
-// Finds `key` starting in the LDS copy of slots [lo, lo + len), continuing in
-// global memory if the chain leaves it.  On a hit sv is the matching slot.
+// Finds `key` (of a hash-mixed table) starting in the LDS copy of slots
+// [lo, lo + len), continuing in global memory if the chain leaves it.  On a hit
+// sv is the matching slot.
 QO_DEV bool slice_find(const uint64_t *s_slots, uint32_t len, uint64_t lo, const uint64_t *__restrict__ slots,
                        uint64_t cap, int32_t key, uint64_t &sv) {
-    uint64_t h = join_home(key, cap);
+    uint64_t h = join_home(hash32((uint32_t)key), cap);
     uint64_t probes = 0;
     if (h >= lo && h < lo + len) {
         for (uint32_t i = (uint32_t)(h - lo); i < len; ++i, ++probes) {
@@ -164,13 +218,15 @@ QO_DEV bool slice_find(const uint64_t *s_slots, uint32_t len, uint64_t lo, const
     return false;
 }
 
-// Probe of fine bucket f = blockIdx.x.  Dynamic LDS: (slice_max + margin) slots.
+// Probe of fine bucket f = blockIdx.x of a hash-mixed table (does nothing for
+// a key-range table).  Dynamic LDS: (slice_max + margin) slots.
 __global__ __launch_bounds__(256) void k_slice_probe(const uint64_t *__restrict__ bucket, uint64_t bcap,
                                                      const uint32_t *ctr, int L, const JoinHeader *__restrict__ hdr,
                                                      unsigned long long *counter, SoaEmit emit) {
     extern __shared__ __attribute__((aligned(16))) uint64_t s_slots[];
     __shared__ uint32_t s_tot[kPairJ * 4];
     __shared__ uint64_t s_excl;
+    if (hdr->mode != kJoinHashMix) return;
     const uint32_t f = blockIdx.x;
     uint64_t n = ctr[f * kCtrStride];
     if (n > bcap) n = bcap;
@@ -186,7 +242,8 @@ __global__ __launch_bounds__(256) void k_slice_probe(const uint64_t *__restrict_
             int32_t key[kPairJ][4], sp[kPairJ][4];
             uint32_t pass[kPairJ];
             load_pairs<kPairJ>(src + e0, m, key, sp, pass);
-            lookup_emit<kPairJ>(slots, cap, true, sentinel, key, sp, pass, counter, emit, s_tot, &s_excl);
+            lookup_emit<kPairJ>(slots, cap, JoinHash::of(hdr), true, sentinel, key, sp, pass, counter, emit, s_tot,
+                                &s_excl);
         }
         return;
     }
@@ -230,13 +287,15 @@ __global__ __launch_bounds__(256) void k_slice_probe(const uint64_t *__restrict_
     }
 }
 
-// Build of fine bucket f = blockIdx.x: insert into an empty LDS slice, write
-// the whole slice [lo, hi) out.  Dynamic LDS: slice_max slots.
+// Build of fine bucket f = blockIdx.x of a hash-mixed table (nothing when the
+// build chose a key-range table): insert into an empty LDS slice, write the
+// whole slice [lo, hi) out.  Dynamic LDS: slice_max slots.
 __global__ __launch_bounds__(256) void k_slice_build(const uint64_t *__restrict__ bucket, uint64_t bcap,
                                                      const uint32_t *ctr, int L, JoinHeader *hdr, uint64_t *spill,
                                                      unsigned long long *spill_n) {
     extern __shared__ __attribute__((aligned(16))) uint64_t s_slots[];
     __shared__ uint32_t s_dup, s_sentinel;
+    if (hdr->mode != kJoinHashMix) return;
     const uint32_t f = blockIdx.x;
     uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
     const uint64_t cap = hdr->cap;
@@ -256,7 +315,7 @@ __global__ __launch_bounds__(256) void k_slice_build(const uint64_t *__restrict_
             continue;
         }
         const int32_t key = (int32_t)(uint32_t)v;
-        const uint64_t h = join_home(key, cap);
+        const uint64_t h = join_home(hash32((uint32_t)key), cap);
         bool placed = false;
         if (h >= lo && h < hi) {
             for (uint32_t j = (uint32_t)(h - lo); j < len; ++j) {
@@ -281,6 +340,7 @@ __global__ __launch_bounds__(256) void k_slice_build(const uint64_t *__restrict_
 
 // Tuples set aside by the build, inserted into the finished table.
 __global__ void k_spill_insert(const uint64_t *spill, const unsigned long long *spill_n, JoinHeader *hdr) {
+    if (hdr->mode != kJoinHashMix) return;
     const uint64_t n = *spill_n;
     uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
     const uint64_t cap = hdr->cap;
@@ -288,6 +348,33 @@ __global__ void k_spill_insert(const uint64_t *spill, const unsigned long long *
         const uint64_t v = spill[i];
         join_insert(slots, cap, hdr, (int32_t)(uint32_t)v, (int32_t)(v >> 32));
     }
+}
+
+// Probe of a key-range table from its S buckets.  Block i works on XCD x =
+// i % 8: its bucket is b = (i / 8 / tpb) * 8 + x and its tile (i / 8) % tpb,
+// so each XCD walks its own buckets one after another.
+__global__ __launch_bounds__(256) void k_range_probe(const uint64_t *__restrict__ bucket, uint64_t bcap,
+                                                     const uint32_t *ctr, uint32_t nb, uint32_t tpb,
+                                                     const JoinHeader *__restrict__ hdr, unsigned long long *counter,
+                                                     SoaEmit emit) {
+    __shared__ uint32_t s_tot[kPairJ * 4];
+    __shared__ uint64_t s_excl;
+    __shared__ JoinStageCol stage;
+    if (hdr->mode != kJoinKeyRange) return;
+    const uint32_t x = blockIdx.x & 7u, q = blockIdx.x >> 3;
+    const uint32_t b = (q / tpb) * 8u + x, t = q % tpb;
+    if (b >= nb) return;
+    uint64_t n = ctr[b * kCtrStride];
+    if (n > bcap) n = bcap;
+    const uint64_t e0 = (uint64_t)t * kPairStep;
+    if (e0 >= n) return;  // whole block: no barrier has been reached
+    const uint32_t m = (uint32_t)(n - e0 < kPairStep ? n - e0 : kPairStep);
+    int32_t key[kPairJ][4], sp[kPairJ][4];
+    uint32_t pass[kPairJ];
+    load_pairs<kPairJ>(bucket + (size_t)b * bcap + e0, m, key, sp, pass);
+    const uint64_t *slots = reinterpret_cast<const uint64_t *>(hdr + 1);
+    lookup_emit<kPairJ>(slots, hdr->cap, JoinHash::of(hdr), (hdr->has_dups | hdr->sentinel_count) != 0,
+                        hdr->sentinel_count, key, sp, pass, counter, emit, s_tot, &s_excl, &stage);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,7 +391,8 @@ int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey,
     JoinHeader *hdr = static_cast<JoinHeader *>(table);
     QO_TRY(launch_fill_u32(b.sb.ctr, 0, b.sb.ctr_words, st));
     QO_TRY(launch_fill_u64(b.spill_n, 0, 1, st));
-    QO_TRY(launch_splits(p.s, b.sb, rkey, rpay, nullptr, nr, 0, 0, AppendOverflow{b.spill, b.spill_n}, st));
+    QO_TRY(launch_splits(p.s, b.sb, rkey, rpay, nullptr, nr, 0, 0, AppendOverflow{b.spill, b.spill_n},
+                         Gate{&hdr->mode, kJoinHashMix}, HashSrc{}, st));
     const uint64_t *fb;
     uint64_t fcap;
     const uint32_t *fctr;
@@ -319,26 +407,43 @@ int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey,
 
 int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
                            const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
-                           int32_t *ospay, size_t out_cap, uint64_t *d_count, void *ws, hipStream_t st, bool *used) {
-    *used = false;
-    const uint64_t cap = join_table_slots(table_bytes);
-    const PartPlan p = part_plan(cap, ns);
-    if (!p.on || !ws) return QO_OK;
-    Carver cv(ws);
-    const PartBufs b = part_carve(p, ns, false, cv);
+                           int32_t *ospay, size_t out_cap, uint64_t *d_count, void *ws, hipStream_t st,
+                           bool *hash_part, bool *range_part) {
+    *hash_part = *range_part = false;
+    if (!ws) return QO_OK;
+    const uint64_t slots = join_table_slots(table_bytes);
     const JoinHeader *hdr = static_cast<const JoinHeader *>(table);
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(d_count);
-    QO_TRY(launch_fill_u32(b.sb.ctr, 0, b.sb.ctr_words, st));
-    QO_TRY(launch_splits(p.s, b.sb, skey, spay, sfilt, ns, flo, fhi,
-                         ProbeOverflow{hdr, counter, SoaEmit{okey, orpay, ospay, out_cap}}, st));
-    const uint64_t *fb;
-    uint64_t fcap;
-    const uint32_t *fctr;
-    split_fine(p.s, b.sb, fb, fcap, fctr);
-    k_slice_probe<<<1u << p.L, 256, (size_t)(p.slice_max + kSliceMargin) * 8, st>>>(
-        fb, fcap, fctr, p.L, hdr, counter, SoaEmit{okey, orpay, ospay, out_cap});
-    QO_LAUNCHED();
-    *used = true;
+    const SoaEmit emit{okey, orpay, ospay, out_cap};
+    const PartPlan p = part_plan(slots, ns);
+    if (p.on) {  // hash-mixed table
+        Carver cv(ws);
+        const PartBufs b = part_carve(p, ns, false, cv);
+        QO_TRY(launch_fill_u32(b.sb.ctr, 0, b.sb.ctr_words, st));
+        QO_TRY(launch_splits(p.s, b.sb, skey, spay, sfilt, ns, flo, fhi, ProbeOverflow{hdr, counter, emit},
+                             Gate{&hdr->mode, kJoinHashMix}, HashSrc{}, st));
+        const uint64_t *fb;
+        uint64_t fcap;
+        const uint32_t *fctr;
+        split_fine(p.s, b.sb, fb, fcap, fctr);
+        k_slice_probe<<<1u << p.L, 256, (size_t)(p.slice_max + kSliceMargin) * 8, st>>>(fb, fcap, fctr, p.L, hdr,
+                                                                                          counter, emit);
+        QO_LAUNCHED();
+        *hash_part = true;
+    }
+    const RangePlan r = range_plan(slots, ns);
+    if (r.on) {  // key-range table
+        Carver cv(ws);
+        const SplitBufs b = split_carve(r.s, cv);
+        QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));
+        QO_TRY(launch_splits(r.s, b, skey, spay, sfilt, ns, flo, fhi, ProbeOverflow{hdr, counter, emit},
+                             Gate{&hdr->mode, kJoinKeyRange}, HashSrc{hdr}, st));
+        const uint64_t grid = (uint64_t)((r.nb + 7) / 8) * 8 * r.tpb;
+        QO_REQUIRE(grid < 0x7FFFFFFFull, "probe input too large");
+        k_range_probe<<<(unsigned)grid, 256, 0, st>>>(b.bucketA, r.s.bcapA, b.ctr, r.nb, r.tpb, hdr, counter, emit);
+        QO_LAUNCHED();
+        *range_part = true;
+    }
     return QO_OK;
 }
 

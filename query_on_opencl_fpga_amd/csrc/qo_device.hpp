@@ -35,6 +35,34 @@ QO_DEV T wave_sum(T v) {
     return v;
 }
 
+template <class T>
+QO_DEV T wave_min(T v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const T t = __shfl_xor(v, o, kWave);
+        v = t < v ? t : v;
+    }
+    return v;
+}
+
+template <class T>
+QO_DEV T wave_max(T v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const T t = __shfl_xor(v, o, kWave);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+// Device-side switch for kernels whose use is decided on the device: a kernel
+// given a gate with a mode word runs only when *mode == run_if.
+struct Gate {
+    const uint32_t *mode = nullptr;
+    uint32_t run_if = 0;
+    QO_DEV bool off() const { return mode && *mode != run_if; }
+};
+
 // Streaming loads that should not displace cache-resident data.  The HIP vector
 // structs are loaded through the equivalent native clang vector types.
 typedef int32_t nv4i __attribute__((ext_vector_type(4)));
@@ -138,9 +166,10 @@ QO_DEV uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t aggregate) {
     return excl;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.  s_w holds 4
-// words; the caller must __syncthreads() before reusing it.
-QO_DEV uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w) {
+// Exclusive scan of one value per thread over a block of NW waves.  s_w holds
+// NW words; the caller must __syncthreads() before reusing it.
+template <int NW>
+QO_DEV uint32_t block_excl_scan(uint32_t v, uint32_t *s_w) {
     uint32_t inc = wave_incl_scan(v);
     if (lane_id() == kWave - 1) s_w[wave_id()] = inc;
     __syncthreads();
@@ -148,6 +177,7 @@ QO_DEV uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w) {
     for (unsigned w = 0; w < wave_id(); ++w) add += s_w[w];
     return inc - v + add;
 }
+QO_DEV uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w) { return block_excl_scan<4>(v, s_w); }
 
 // 32-bit finaliser of MurmurHash3 (Appleby): spreads sequential keys.
 QO_DEV uint32_t hash32(uint32_t k) {

@@ -60,7 +60,8 @@ enum TuneOp {
     TUNE_SORT = 4,
     TUNE_JOIN_PART = 5,     // partitioned build/probe: items = LDS table slice KiB, param = PartMode
     TUNE_GROUPBY_PART = 6,  // partitioned group-by: items = groups per fine bucket, param = PartMode
-    TUNE_NUM_OPS = 7
+    TUNE_JOIN_RANGE = 7,    // key-range join table: items = largest such table in MiB, param = PartMode
+    TUNE_NUM_OPS = 8
 };
 // Partitioned-path modes: by size (the default), always, never.
 enum PartMode { PART_AUTO = 0, PART_ALWAYS = 1, PART_NEVER = 2 };
@@ -113,15 +114,19 @@ size_t join_build_ws_bytes(size_t table_bytes, size_t nr);
 int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                       void *ws, hipStream_t st);
 size_t join_probe_ws_bytes(size_t table_bytes, size_t ns);
-// Partitioned (per-slice LDS) build and probe, join_part.hip.  *used = false
-// when the table is small enough for the direct path (nothing launched).
+// Partitioned build and probes, join_part.hip.  The build: *used = false when
+// the table is small enough for the direct path (nothing launched).  The
+// probe launches the hash-mixed table's per-slice LDS probe and the key-range
+// table's split + L2 probe where planned (each runs only for its table mode);
+// *hash_part / *range_part say which were launched.
 size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build);
-void join_part_strategy(size_t table_bytes, size_t n, int *partitioned, int *bits, int *splits);
+void join_part_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits);
 int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                            void *ws, hipStream_t st, bool *used);
 int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
                            const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
-                           int32_t *ospay, size_t out_cap, uint64_t *d_count, void *ws, hipStream_t st, bool *used);
+                           int32_t *ospay, size_t out_cap, uint64_t *d_count, void *ws, hipStream_t st,
+                           bool *hash_part, bool *range_part);
 int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
                       const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
                       int32_t *ospay, size_t cap, uint64_t *d_count, void *ws, hipStream_t st);

@@ -1,48 +1,86 @@
 // Open-addressing join table shared by the plain and batched join paths.
 //
-// Layout in device memory: a 256-byte JoinHeader followed by `cap` 8-byte
-// slots (any count; the table size is picked for bandwidth, not rounded to a
-// power of two).  A key's home slot is floor(hash32(key) * cap / 2^32) and
-// collisions probe linearly, wrapping at cap.  A slot holds
-// (r_payload << 32) | key; kEmptySlot marks a free slot.  The one tuple whose
-// packed form equals kEmptySlot (key = -1, payload = -1) is counted in the
-// header instead of being stored.
+// Layout in device memory: a 256-byte JoinHeader followed by 8-byte slots.  A
+// slot holds (r_payload << 32) | key; kEmptySlot marks a free slot.  The one
+// tuple whose packed form equals kEmptySlot (key = -1, payload = -1) is counted
+// in the header instead of being stored.  Collisions probe linearly, wrapping
+// at `cap` (the slots in use).
 //
-// Because the home slot is monotone in hash32(key), the keys whose hash has
-// top bits p (of L) all have home slots in [p*cap/2^L, (p+1)*cap/2^L]: a
-// contiguous 1/2^L of the table.  The partitioned build and probe use this to
-// work on one table slice at a time in LDS.
+// A key's home slot is floor(h * cap / 2^32) for a 32-bit hash h, chosen by the
+// build and recorded in the header (JoinHash):
+//  * hash-mixed table: h = hash32(key), cap = every allocated slot (any count);
+//  * key-range table: when the build keys span a range of at most 2^p <= the
+//    allocated slots, h = (key - kmin) << (32 - p) and cap = 2^p, so the home
+//    slot IS key - kmin: with unique keys every lookup is one load and no
+//    chain, the direct-mapped layout of the reference's table (bucket =
+//    key & (B - 1), hj_filter_sort_op.cl:36,89).  Keys outside the range
+//    cannot be in such a table.  A build that finds a duplicate key falls back
+//    to the hash-mixed table (chains of a dense key range would run long).
+//
+// Because the home slot is monotone in h, the keys whose hash has top bits p
+// (of L) all have home slots in [p*cap/2^L, (p+1)*cap/2^L]: a contiguous 1/2^L
+// of the table.  The partitioned build and probe of the hash-mixed table use
+// this to work on one table slice at a time in LDS.
 #pragma once
 #include "qo_device.hpp"
 
 namespace qo {
 
 constexpr uint64_t kEmptySlot = ~uint64_t(0);
+constexpr uint32_t kJoinHashMix = 0;   // JoinHeader::mode
+constexpr uint32_t kJoinKeyRange = 1;
 
 struct alignas(256) JoinHeader {
-    uint64_t cap;
+    uint64_t cap;             // slots in use
     uint32_t has_dups;        // some key was inserted twice: probe emits every match
     uint32_t sentinel_count;  // number of (-1, -1) build tuples
     uint32_t overflow;        // an insert found no free slot
-    uint32_t pad0;
+    uint32_t mode;            // kJoinHashMix or kJoinKeyRange
     uint64_t nr;
-    uint64_t pad[28];
+    uint64_t slots_alloc;     // slots allocated after the header
+    int32_t kmin;             // key-range table: home slot = key - kmin
+    uint32_t shift;           // key-range table: h = (key - kmin) << shift, cap = 2^(32 - shift)
+    int32_t rmin, rmax;       // build key range (reduced by the build)
+    unsigned long long occupied;  // key-range build: occupied slots
+    uint64_t pad[24];
 };
 static_assert(sizeof(JoinHeader) == 256, "header size");
 
+// The table's hash, read from its header once per kernel.
+struct JoinHash {
+    uint32_t mode, shift;
+    int32_t kmin;
+    QO_DEV static JoinHash of(const JoinHeader *h) { return JoinHash{h->mode, h->shift, h->kmin}; }
+    // Sets h; false when `key` lies outside a key-range table (it cannot be stored there).
+    QO_DEV bool operator()(int32_t key, uint32_t &h) const {
+        if (mode != kJoinKeyRange) {
+            h = hash32((uint32_t)key);
+            return true;
+        }
+        const uint32_t d = (uint32_t)key - (uint32_t)kmin;
+        h = d << shift;
+        return (d >> (32u - shift)) == 0u;  // shift is in [1, 31]
+    }
+    // Slots a lookup may have to read: a key-range table holds each key at its
+    // home slot only (a build with duplicate keys falls back to hash mixing),
+    // while hash-mixed chains run until an empty slot.
+    QO_DEV uint64_t walk(uint64_t cap) const { return mode == kJoinKeyRange ? 1 : cap; }
+};
+
 QO_DEV uint64_t join_pack(int32_t key, int32_t pay) { return ((uint64_t)(uint32_t)pay << 32) | (uint32_t)key; }
 
-QO_DEV uint64_t join_home(int32_t key, uint64_t cap) { return ((uint64_t)hash32((uint32_t)key) * cap) >> 32; }
+QO_DEV uint64_t join_home(uint32_t h, uint64_t cap) { return ((uint64_t)h * cap) >> 32; }
 
 QO_DEV uint64_t join_next(uint64_t h, uint64_t cap) { return h + 1 == cap ? 0 : h + 1; }
 
+// Insert into a hash-mixed table.
 QO_DEV void join_insert(uint64_t *slots, uint64_t cap, JoinHeader *hdr, int32_t key, int32_t pay) {
     const uint64_t packed = join_pack(key, pay);
     if (packed == kEmptySlot) {
         atomicAdd(&hdr->sentinel_count, 1u);
         return;
     }
-    uint64_t h = join_home(key, cap);
+    uint64_t h = join_home(hash32((uint32_t)key), cap);
     for (uint64_t probe = 0; probe < cap; ++probe) {
         uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long *>(&slots[h]),
                                   (unsigned long long)kEmptySlot, (unsigned long long)packed);
@@ -54,32 +92,21 @@ QO_DEV void join_insert(uint64_t *slots, uint64_t cap, JoinHeader *hdr, int32_t 
 }
 
 // Number of matches of `key` (walks the whole chain; general path).
-QO_DEV uint32_t join_count(const uint64_t *slots, uint64_t cap, uint32_t sentinel, int32_t key) {
+QO_DEV uint32_t join_count(const uint64_t *slots, uint64_t cap, const JoinHash &hs, uint32_t sentinel, int32_t key) {
     uint32_t c = 0;
-    uint64_t h = join_home(key, cap);
-    for (uint64_t probe = 0; probe < cap; ++probe) {
-        uint64_t s = slots[h];
-        if (s == kEmptySlot) break;
-        c += ((uint32_t)s == (uint32_t)key);
-        h = join_next(h, cap);
+    uint32_t hv;
+    if (hs(key, hv)) {
+        uint64_t h = join_home(hv, cap);
+        const uint64_t walk = hs.walk(cap);
+        for (uint64_t probe = 0; probe < walk; ++probe) {
+            uint64_t s = slots[h];
+            if (s == kEmptySlot) break;
+            c += ((uint32_t)s == (uint32_t)key);
+            h = join_next(h, cap);
+        }
     }
     if (key == -1) c += sentinel;
     return c;
-}
-
-// First match of `key` in a table built from unique keys, given its home slot h
-// and that slot's value s; returns whether found.
-QO_DEV bool join_find(const uint64_t *slots, uint64_t cap, int32_t key, uint64_t h, uint64_t s, int32_t &rpay) {
-    for (uint64_t probe = 0; probe < cap; ++probe) {
-        if (s == kEmptySlot) return false;
-        if ((uint32_t)s == (uint32_t)key) {
-            rpay = (int32_t)(s >> 32);
-            return true;
-        }
-        h = join_next(h, cap);
-        s = slots[h];
-    }
-    return false;
 }
 
 constexpr int kProbeJ = 4;                                   // int4 groups per thread
@@ -88,14 +115,18 @@ constexpr size_t kProbeTile = (size_t)kProbeJ * 256 * 4;     // S tuples per til
 // Emits every match of `key` from position `pos` on, walking the whole chain
 // (table-chain order, then the (-1, -1) build tuples counted in the header).
 template <class Emit>
-QO_DEV uint64_t emit_chain(const uint64_t *slots, uint64_t cap, uint32_t sentinel, int32_t key, int32_t sp,
-                           uint64_t pos, const Emit &emit) {
-    uint64_t h = join_home(key, cap);
-    for (uint64_t probe = 0; probe < cap; ++probe) {
-        uint64_t s = slots[h];
-        if (s == kEmptySlot) break;
-        if ((uint32_t)s == (uint32_t)key) emit(pos++, key, (int32_t)(s >> 32), sp);
-        h = join_next(h, cap);
+QO_DEV uint64_t emit_chain(const uint64_t *slots, uint64_t cap, const JoinHash &hs, uint32_t sentinel, int32_t key,
+                           int32_t sp, uint64_t pos, const Emit &emit) {
+    uint32_t hv;
+    if (hs(key, hv)) {
+        uint64_t h = join_home(hv, cap);
+        const uint64_t walk = hs.walk(cap);
+        for (uint64_t probe = 0; probe < walk; ++probe) {
+            uint64_t s = slots[h];
+            if (s == kEmptySlot) break;
+            if ((uint32_t)s == (uint32_t)key) emit(pos++, key, (int32_t)(s >> 32), sp);
+            h = join_next(h, cap);
+        }
     }
     if (key == -1)
         for (uint32_t q = 0; q < sentinel; ++q) emit(pos++, key, -1, sp);
@@ -105,24 +136,102 @@ QO_DEV uint64_t emit_chain(const uint64_t *slots, uint64_t cap, uint32_t sentine
 // One S tuple on its own (the rare overflow path of the partitioned probe):
 // counts its matches, reserves them with one atomic add and emits them.
 template <class Emit>
-QO_DEV void probe_one(const uint64_t *slots, uint64_t cap, uint32_t sentinel, int32_t key, int32_t sp,
-                      unsigned long long *counter, const Emit &emit) {
-    const uint32_t m = join_count(slots, cap, sentinel, key);
+QO_DEV void probe_one(const uint64_t *slots, uint64_t cap, const JoinHash &hs, uint32_t sentinel, int32_t key,
+                      int32_t sp, unsigned long long *counter, const Emit &emit) {
+    const uint32_t m = join_count(slots, cap, hs, sentinel, key);
     if (!m) return;
     const uint64_t pos = atomicAdd(counter, (unsigned long long)m);
-    emit_chain(slots, cap, sentinel, key, sp, pos, emit);
+    emit_chain(slots, cap, hs, sentinel, key, sp, pos, emit);
+}
+
+// LDS staging of one tile's matches (unique-key path), so that the tile is
+// written out with consecutive threads on consecutive rows: all three
+// columns at once (48 KiB), or one column at a time through one buffer
+// (16 KiB, for Emit types with col(c, pos, value); more blocks fit a CU).
+constexpr uint32_t kStageRows = 1024 * 4;
+struct JoinStage {
+    int32_t key[kStageRows], rpay[kStageRows], spay[kStageRows];
+};
+struct JoinStageCol {
+    int32_t v[kStageRows];
+};
+
+template <int J>
+QO_DEV void stage_rows(JoinStage *st, const uint32_t (&ex)[J], const uint32_t (&hit)[J], const int32_t (&key)[J][4],
+                       const uint64_t (&sv)[J][4], const int32_t (&sp)[J][4]) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        uint32_t pos = ex[j];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!((hit[j] >> k) & 1u)) continue;
+            st->key[pos] = key[j][k];
+            st->rpay[pos] = (int32_t)(sv[j][k] >> 32);
+            st->spay[pos] = sp[j][k];
+            ++pos;
+        }
+    }
+}
+
+// Emits `total` staged rows at base: for JoinStage, whole rows; for
+// JoinStageCol, column c = 0 (key), 1 (r_payload), 2 (s_payload) in turn.
+// Entered after the block's barrier that follows the reservation.
+template <int J, class Emit>
+QO_DEV void emit_staged(JoinStage *st, uint64_t base, uint32_t total, const uint32_t (&)[J], const uint32_t (&)[J],
+                        const int32_t (&)[J][4], const uint64_t (&)[J][4], const int32_t (&)[J][4], const Emit &emit) {
+    for (uint32_t i = threadIdx.x; i < total; i += 256) emit(base + i, st->key[i], st->rpay[i], st->spay[i]);
+}
+
+template <int J, class Emit>
+QO_DEV void emit_staged(JoinStageCol *st, uint64_t base, uint32_t total, const uint32_t (&ex)[J],
+                        const uint32_t (&hit)[J], const int32_t (&key)[J][4], const uint64_t (&sv)[J][4],
+                        const int32_t (&sp)[J][4], const Emit &emit) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c) {
+            __syncthreads();  // the previous column has been read out
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                uint32_t pos = ex[j];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (!((hit[j] >> k) & 1u)) continue;
+                    st->v[pos++] = c == 1 ? (int32_t)(sv[j][k] >> 32) : sp[j][k];
+                }
+            }
+            __syncthreads();
+        }
+        for (uint32_t i = threadIdx.x; i < total; i += 256) emit.col(c, base + i, st->v[i]);
+    }
+}
+
+template <int J>
+QO_DEV void stage_rows(JoinStageCol *st, const uint32_t (&ex)[J], const uint32_t (&hit)[J],
+                       const int32_t (&key)[J][4], const uint64_t (&)[J][4], const int32_t (&)[J][4]) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        uint32_t pos = ex[j];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if ((hit[j] >> k) & 1u) st->v[pos++] = key[j][k];
+    }
 }
 
 // Looks up the block's tuples (J groups of 4 per thread; bit k of pass[j]
 // marks element k of group j live), reserves the block's output range with one
 // atomic add on `counter` and emits the matches there in (group, wave, lane,
 // element) order; for one tuple, in table-chain order.  Emit(pos, key,
-// r_payload, s_payload).  Must be called by all 256 threads of the block;
+// r_payload, s_payload).  With `stage` (J * 1024 <= kStageRows) the unique-key
+// path writes the matches to LDS first and emits them row by row across the
+// block (coalesced stores); without it, and on the general path, each thread
+// emits its own matches.  Must be called by all 256 threads of the block;
 // s_tot holds J*4 words.
-template <int J, class Emit>
-QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, bool general, uint32_t sentinel,
+template <int J, class Emit, class Stage = JoinStage>
+QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, const JoinHash &hsh, bool general,
+                        uint32_t sentinel,
                         const int32_t (&key)[J][4], const int32_t (&sp)[J][4], const uint32_t (&pass)[J],
-                        unsigned long long *counter, const Emit &emit, uint32_t *s_tot, uint64_t *s_excl) {
+                        unsigned long long *counter, const Emit &emit, uint32_t *s_tot, uint64_t *s_excl,
+                        Stage *stage = nullptr) {
     uint32_t c[J], ex[J];
     uint32_t hit[J];
     // On a hit of the unique-key path sv holds the matching slot (its payload
@@ -133,6 +242,7 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
         // issues its next slot load in the same round, so a wave waits one
         // memory round trip per chain step, not one per tuple and step.
         const uint32_t cap32 = (uint32_t)cap_slots;
+        const uint64_t walk = hsh.walk(cap_slots);
         uint32_t hs[J][4];
         uint32_t todo[J];
 #pragma unroll
@@ -141,8 +251,10 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
             hit[j] = 0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                hs[j][k] = (uint32_t)join_home(key[j][k], cap_slots);
-                sv[j][k] = (pass[j] >> k) & 1u ? slots[hs[j][k]] : kEmptySlot;
+                uint32_t hv;
+                const bool in = hsh(key[j][k], hv);
+                hs[j][k] = (uint32_t)join_home(hv, cap_slots);
+                sv[j][k] = ((pass[j] >> k) & 1u) && in ? slots[hs[j][k]] : kEmptySlot;
             }
         }
         for (uint32_t round = 1;; ++round) {
@@ -163,7 +275,7 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
                 }
                 left |= todo[j];
             }
-            if (!left || round >= cap32) break;  // a chain never exceeds the table
+            if (!left || round >= walk) break;  // a chain never exceeds the table
 #pragma unroll
             for (int j = 0; j < J; ++j)
 #pragma unroll
@@ -181,7 +293,7 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
             for (int k = 0; k < 4; ++k) {
                 sv[j][k] = 0;
                 if ((pass[j] >> k) & 1u) {
-                    uint32_t m = join_count(slots, cap_slots, sentinel, key[j][k]);
+                    uint32_t m = join_count(slots, cap_slots, hsh, sentinel, key[j][k]);
                     c[j] += m;
                     if (m) hit[j] |= 1u << k;
                 }
@@ -191,6 +303,13 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
 
     const uint32_t total = tile_offsets<J, 4>(c, ex, s_tot);
     if (threadIdx.x == 0) *s_excl = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+    if (stage && !general) {
+        static_assert(J * 1024 <= kStageRows, "tile larger than the stage");
+        stage_rows<J>(stage, ex, hit, key, sv, sp);
+        __syncthreads();
+        emit_staged<J>(stage, *s_excl, total, ex, hit, key, sv, sp, emit);
+        return;
+    }
     __syncthreads();
     const uint64_t base = *s_excl;
 #pragma unroll
@@ -200,21 +319,22 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
         for (int k = 0; k < 4; ++k) {
             if (!((hit[j] >> k) & 1u)) continue;
             if (!general) emit(pos++, key[j][k], (int32_t)(sv[j][k] >> 32), sp[j][k]);
-            else pos = emit_chain(slots, cap_slots, sentinel, key[j][k], sp[j][k], pos, emit);
+            else pos = emit_chain(slots, cap_slots, hsh, sentinel, key[j][k], sp[j][k], pos, emit);
         }
     }
 }
 
-// Loads J int4 groups per thread of S tile `tile` (tuples [tile*J*1024, ...))
-// and evaluates the filter: bit k of pass[j] marks element k of group j.
-template <int J, bool HAS_FILTER>
+// Loads J int4 groups per thread (NT threads) of S tile `tile` (tuples
+// [tile*J*4*NT, ...)) and evaluates the filter: bit k of pass[j] marks element
+// k of group j.
+template <int J, bool HAS_FILTER, int NT = 256>
 QO_DEV void load_s_tile(const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
                         const int32_t *__restrict__ sfilt, size_t ns, int32_t flo, int32_t fhi, bool vec,
                         uint32_t tile, int32_t (&key)[J][4], int32_t (&sp)[J][4], uint32_t (&pass)[J]) {
-    const size_t vbase = (size_t)tile * (J * 256);
+    const size_t vbase = (size_t)tile * (J * NT);
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
+        const size_t v = vbase + (size_t)j * NT + threadIdx.x;
         const size_t e0 = v * 4;
         pass[j] = 0;
         if (vec && e0 + 4 <= ns) {
@@ -257,19 +377,20 @@ QO_DEV void load_s_tile(const int32_t *__restrict__ skey, const int32_t *__restr
 // order: a look-back across tiles would make every tile wait for the slowest
 // lookup of its predecessors.  Must be called by all 256 threads of the block;
 // s_tot holds kProbeJ*4 words.
-template <bool HAS_FILTER, class Emit>
+template <bool HAS_FILTER, class Emit, class Stage = JoinStage>
 QO_DEV void probe_tile(const JoinHeader *__restrict__ hdr, const uint64_t *__restrict__ slots,
                        const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
                        const int32_t *__restrict__ sfilt, size_t ns, int32_t flo, int32_t fhi, bool vec,
                        uint32_t tile, unsigned long long *counter, const Emit &emit, uint32_t *s_tot,
-                       uint64_t *s_excl) {
+                       uint64_t *s_excl, Stage *stage = nullptr) {
     const uint64_t cap_slots = hdr->cap;
     const bool general = (hdr->has_dups | hdr->sentinel_count) != 0;
     const uint32_t sentinel = hdr->sentinel_count;
     int32_t key[kProbeJ][4], sp[kProbeJ][4];
     uint32_t pass[kProbeJ];
     load_s_tile<kProbeJ, HAS_FILTER>(skey, spay, sfilt, ns, flo, fhi, vec, tile, key, sp, pass);
-    lookup_emit<kProbeJ>(slots, cap_slots, general, sentinel, key, sp, pass, counter, emit, s_tot, s_excl);
+    lookup_emit<kProbeJ>(slots, cap_slots, JoinHash::of(hdr), general, sentinel, key, sp, pass, counter, emit, s_tot,
+                         s_excl, stage);
 }
 
 // Emits match rows into three SoA int32 columns (capacity-checked).
@@ -282,6 +403,9 @@ struct SoaEmit {
             orpay[pos] = rp;
             ospay[pos] = sp;
         }
+    }
+    QO_DEV void col(int c, uint64_t pos, int32_t v) const {
+        if (pos < cap) (c == 0 ? okey : c == 1 ? orpay : ospay)[pos] = v;
     }
 };
 

@@ -1,6 +1,7 @@
-// Hash split of (key, payload) tuples into 2^L buckets by the top L bits of
-// hash32(key): the partitioning step shared by the partitioned join and the
-// partitioned group-by.
+// Hash split of (key, payload) tuples into 2^L buckets by the top L bits of a
+// 32-bit key hash: the partitioning step shared by the partitioned join and
+// the partitioned group-by.  The hash is hash32(key), or the hash of a join
+// table read from its header on the device (key-range tables split by key).
 //
 // Split A streams columns (an optional range filter fused) into 2^L1 buckets;
 // split B splits each A bucket by the next L2 bits (L = L1 + L2, skipped when
@@ -16,8 +17,9 @@
 
 namespace qo {
 
-constexpr int kSplitJ = 8;                       // int4 groups (4 tuples each) per thread
-constexpr uint32_t kSplitTile = kSplitJ * 1024;  // tuples per split tile
+constexpr int kSplitThreads = 512;                          // threads per split block
+constexpr int kSplitJ = 4;                                  // int4 groups (4 tuples each) per thread
+constexpr uint32_t kSplitTile = kSplitJ * 4 * kSplitThreads;  // tuples per split tile
 constexpr int kMaxSplitBits = 7;                 // buckets per split pass <= 128
 constexpr uint32_t kMaxSplit = 1u << kMaxSplitBits;
 constexpr uint32_t kCtrStride = 16;              // bucket counters 64 bytes apart
@@ -107,22 +109,30 @@ struct ExactLayout {
     QO_DEV uint64_t count(uint32_t b, const uint32_t *) const { return cap(b); }
 };
 
+// Where a split kernel takes its hash from: a join table's header, or (null)
+// hash32.
+struct HashSrc {
+    const JoinHeader *hdr = nullptr;
+    QO_DEV JoinHash get() const { return hdr ? JoinHash::of(hdr) : JoinHash{kJoinHashMix, 0u, 0}; }
+};
+
 struct SplitLds {
     uint64_t stage[kSplitTile];
     uint32_t cnt[kMaxSplit], loff[kMaxSplit], gbase[kMaxSplit];
-    uint32_t w[4];
+    uint32_t w[kSplitThreads / kWave];
 };
 
 // Splits the block's live tuples (J groups of 4 per thread, bit k of pass[j]
 // marks element k of group j) into `parts` buckets by
-// part = (hash32(key) >> shift) & (parts - 1); bucket index = bucket0 + part,
+// part = (hs(key) >> shift) & (parts - 1); bucket index = bucket0 + part,
 // placed by `out` in `bucket`; ctr holds one fill cursor per bucket.
 template <int J, class Layout, class Overflow>
 QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], const uint32_t (&pass)[J],
-                        int shift, uint32_t parts, uint32_t bucket0, uint64_t *__restrict__ bucket,
-                        const Layout &out, uint32_t *ctr, SplitLds &sh, const Overflow &overflow) {
+                        const JoinHash &hs, int shift, uint32_t parts, uint32_t bucket0,
+                        uint64_t *__restrict__ bucket, const Layout &out, uint32_t *ctr, SplitLds &sh,
+                        const Overflow &overflow) {
     const uint32_t mask = parts - 1;
-    for (uint32_t i = threadIdx.x; i < parts; i += 256) sh.cnt[i] = 0;
+    for (uint32_t i = threadIdx.x; i < parts; i += kSplitThreads) sh.cnt[i] = 0;
     __syncthreads();
     // Rank inside the tile: (part << 16) | arrival order within the part.
     uint32_t rank[J][4];
@@ -132,14 +142,16 @@ QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], 
         for (int k = 0; k < 4; ++k) {
             rank[j][k] = 0;
             if ((pass[j] >> k) & 1u) {
-                const uint32_t p = (hash32((uint32_t)key[j][k]) >> shift) & mask;
+                uint32_t h;
+                hs(key[j][k], h);
+                const uint32_t p = (h >> shift) & mask;
                 rank[j][k] = (p << 16) | atomicAdd(&sh.cnt[p], 1u);
             }
         }
     __syncthreads();
     // Part offsets inside the tile and one bucket reservation per non-empty part.
     const uint32_t c0 = threadIdx.x < parts ? sh.cnt[threadIdx.x] : 0u;
-    const uint32_t ex = block_excl_scan256(c0, sh.w);
+    const uint32_t ex = block_excl_scan<kSplitThreads / kWave>(c0, sh.w);
     if (threadIdx.x < parts) {
         sh.loff[threadIdx.x] = ex;
         sh.gbase[threadIdx.x] = c0 ? atomicAdd(&ctr[(bucket0 + threadIdx.x) * kCtrStride], c0) : 0u;
@@ -153,9 +165,11 @@ QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], 
             if ((pass[j] >> k) & 1u)
                 sh.stage[sh.loff[rank[j][k] >> 16] + (rank[j][k] & 0xFFFFu)] = join_pack(key[j][k], pay[j][k]);
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < total; i += 256) {
+    for (uint32_t i = threadIdx.x; i < total; i += kSplitThreads) {
         const uint64_t v = sh.stage[i];
-        const uint32_t p = (hash32((uint32_t)v) >> shift) & mask;
+        uint32_t h;
+        hs((int32_t)(uint32_t)v, h);
+        const uint32_t p = (h >> shift) & mask;
         const uint32_t b = bucket0 + p;
         const uint64_t g = (uint64_t)sh.gbase[p] + (i - sh.loff[p]);
         // Plain stores: runs from neighbouring tiles complete each other's
@@ -167,21 +181,23 @@ QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], 
 
 // Split A: columns (key, payload[, filter]) -> 2^L1 buckets.
 template <bool HAS_FILTER, class Layout, class Overflow>
-__global__ __launch_bounds__(256) void k_split_cols(const int32_t *__restrict__ kcol, const int32_t *__restrict__ pcol,
+__global__ __launch_bounds__(kSplitThreads) void k_split_cols(const int32_t *__restrict__ kcol, const int32_t *__restrict__ pcol,
                                                     const int32_t *__restrict__ fcol, size_t n, int32_t flo,
                                                     int32_t fhi, bool vec, int shift, uint32_t parts,
                                                     uint64_t *__restrict__ bucket, Layout out, uint32_t *ctr,
-                                                    Overflow overflow) {
+                                                    Overflow overflow, Gate gate, HashSrc hsrc) {
     __shared__ SplitLds sh;
+    if (gate.off()) return;
     int32_t key[kSplitJ][4], pay[kSplitJ][4];
     uint32_t pass[kSplitJ];
-    load_s_tile<kSplitJ, HAS_FILTER>(kcol, pcol, fcol, n, flo, fhi, vec, blockIdx.x, key, pay, pass);
-    split_block<kSplitJ>(key, pay, pass, shift, parts, 0, bucket, out, ctr, sh, overflow);
+    load_s_tile<kSplitJ, HAS_FILTER, kSplitThreads>(kcol, pcol, fcol, n, flo, fhi, vec, blockIdx.x, key, pay, pass);
+    split_block<kSplitJ>(key, pay, pass, hsrc.get(), shift, parts, 0, bucket, out, ctr, sh, overflow);
 }
 
-// Loads up to J*1024 (key, payload) pairs from `src` (16-byte aligned): pair
-// quad j of thread t holds pairs 2v, 2v+1 for v = (2j + h)*256 + t, h = 0, 1.
-template <int J>
+// Loads up to J*4*NT (key, payload) pairs from `src` (16-byte aligned) with NT
+// threads: pair quad j of thread t holds pairs 2v, 2v+1 for
+// v = (2j + h)*NT + t, h = 0, 1.
+template <int J, int NT = 256>
 QO_DEV void load_pairs(const uint64_t *__restrict__ src, uint32_t m, int32_t (&key)[J][4], int32_t (&pay)[J][4],
                        uint32_t (&pass)[J]) {
 #pragma unroll
@@ -189,7 +205,7 @@ QO_DEV void load_pairs(const uint64_t *__restrict__ src, uint32_t m, int32_t (&k
         pass[j] = 0;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const uint32_t v = (uint32_t)(2 * j + h) * 256 + threadIdx.x;
+            const uint32_t v = (uint32_t)(2 * j + h) * NT + threadIdx.x;
             const uint32_t e = 2 * v;
             key[j][2 * h] = pay[j][2 * h] = key[j][2 * h + 1] = pay[j][2 * h + 1] = 0;
             if (e + 2 <= m) {
@@ -214,8 +230,9 @@ QO_DEV void load_pairs(const uint64_t *__restrict__ src, uint32_t m, int32_t (&k
 // layout) is read with element loads.
 template <class InLayout, class Layout, class Overflow>
 QO_DEV void split_pairs_tile(const uint64_t *__restrict__ input, const InLayout &in, const uint32_t *in_ctr,
-                             uint32_t p1, uint32_t t, int shift, uint32_t parts, uint64_t *__restrict__ bucket,
-                             const Layout &out, uint32_t *ctr, SplitLds &sh, const Overflow &overflow) {
+                             uint32_t p1, uint32_t t, const JoinHash &hs, int shift, uint32_t parts,
+                             uint64_t *__restrict__ bucket, const Layout &out, uint32_t *ctr, SplitLds &sh,
+                             const Overflow &overflow) {
     const uint64_t n = in.count(p1, in_ctr);
     const uint64_t e0 = (uint64_t)t * kSplitTile;
     if (e0 >= n) return;  // whole block: no barrier has been reached
@@ -224,14 +241,14 @@ QO_DEV void split_pairs_tile(const uint64_t *__restrict__ input, const InLayout 
     uint32_t pass[kSplitJ];
     const uint64_t s0 = in.start(p1) + e0;
     if ((s0 & 1u) == 0) {
-        load_pairs<kSplitJ>(input + s0, m, key, pay, pass);
+        load_pairs<kSplitJ, kSplitThreads>(input + s0, m, key, pay, pass);
     } else {
 #pragma unroll
         for (int j = 0; j < kSplitJ; ++j) {
             pass[j] = 0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint32_t e = 2 * ((uint32_t)(2 * j + (k >> 1)) * 256 + threadIdx.x) + (k & 1);
+                const uint32_t e = 2 * ((uint32_t)(2 * j + (k >> 1)) * kSplitThreads + threadIdx.x) + (k & 1);
                 key[j][k] = pay[j][k] = 0;
                 if (e < m) {
                     const uint64_t w = input[s0 + e];
@@ -242,24 +259,25 @@ QO_DEV void split_pairs_tile(const uint64_t *__restrict__ input, const InLayout 
             }
         }
     }
-    split_block<kSplitJ>(key, pay, pass, shift, parts, p1 * parts, bucket, out, ctr, sh, overflow);
+    split_block<kSplitJ>(key, pay, pass, hs, shift, parts, p1 * parts, bucket, out, ctr, sh, overflow);
 }
 
 // Split B with fixed-capacity input buckets: block = (bucket, tile).
 template <class Layout, class Overflow>
-__global__ __launch_bounds__(256) void k_split_pairs(const uint64_t *__restrict__ input, FixedLayout in,
+__global__ __launch_bounds__(kSplitThreads) void k_split_pairs(const uint64_t *__restrict__ input, FixedLayout in,
                                                      const uint32_t *in_ctr, uint32_t tiles_per_bucket, int shift,
                                                      uint32_t parts, uint64_t *__restrict__ bucket, Layout out,
-                                                     uint32_t *ctr, Overflow overflow) {
+                                                     uint32_t *ctr, Overflow overflow, Gate gate, HashSrc hsrc) {
     __shared__ SplitLds sh;
-    split_pairs_tile(input, in, in_ctr, blockIdx.x / tiles_per_bucket, blockIdx.x % tiles_per_bucket, shift, parts,
-                     bucket, out, ctr, sh, overflow);
+    if (gate.off()) return;
+    split_pairs_tile(input, in, in_ctr, blockIdx.x / tiles_per_bucket, blockIdx.x % tiles_per_bucket, hsrc.get(),
+                     shift, parts, bucket, out, ctr, sh, overflow);
 }
 
 // Split B of the exact split: the blocks enumerate the tiles of all input
 // buckets in order (bucket b has ceil(size(b) / kSplitTile) tiles).
 template <class Overflow>
-__global__ __launch_bounds__(256) void k_split_pairs_exact(const uint64_t *__restrict__ input, ExactLayout in,
+__global__ __launch_bounds__(kSplitThreads) void k_split_pairs_exact(const uint64_t *__restrict__ input, ExactLayout in,
                                                            uint32_t nin, int shift, uint32_t parts,
                                                            uint64_t *__restrict__ bucket, ExactLayout out,
                                                            uint32_t *ctr, Overflow overflow) {
@@ -280,33 +298,34 @@ __global__ __launch_bounds__(256) void k_split_pairs_exact(const uint64_t *__res
     }
     __syncthreads();
     if (s_b >= nin) return;  // past the last tile (the grid is an upper bound)
-    split_pairs_tile(input, in, nullptr, s_b, s_t, shift, parts, bucket, out, ctr, sh, overflow);
+    split_pairs_tile(input, in, nullptr, s_b, s_t, HashSrc{}.get(), shift, parts, bucket, out, ctr, sh, overflow);
 }
 
 // Launches split A (and B) with fixed bucket capacities: kcol/pcol (and fcol
 // when not null, with the range [flo, fhi]) -> fine buckets.  The counters must
-// have been zeroed.
+// have been zeroed.  The kernels run only where `gate` lets them and hash as
+// `hsrc` says.
 template <class Overflow>
 inline int launch_splits(const SplitPlan &p, const SplitBufs &b, const int32_t *kcol, const int32_t *pcol,
                          const int32_t *fcol, size_t n, int32_t flo, int32_t fhi, const Overflow &overflow,
-                         hipStream_t st) {
+                         Gate gate, HashSrc hsrc, hipStream_t st) {
     const bool vec = ((uintptr_t)kcol % 16 == 0) && ((uintptr_t)pcol % 16 == 0) && (!fcol || (uintptr_t)fcol % 16 == 0);
     const uint32_t partsA = 1u << p.L1;
     const size_t tiles = (n + kSplitTile - 1) / kSplitTile;
     QO_REQUIRE(tiles < 0x7FFFFFFFull, "input too large for a hash split");
     const FixedLayout la{p.bcapA}, lb{p.bcapB};
     if (fcol)
-        k_split_cols<true><<<(unsigned)tiles, 256, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
-                                                            b.bucketA, la, b.ctr, overflow);
+        k_split_cols<true><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
+                                                            b.bucketA, la, b.ctr, overflow, gate, hsrc);
     else
-        k_split_cols<false><<<(unsigned)tiles, 256, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
-                                                             b.bucketA, la, b.ctr, overflow);
+        k_split_cols<false><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
+                                                             b.bucketA, la, b.ctr, overflow, gate, hsrc);
     QO_LAUNCHED();
     if (p.L2 > 0) {
         const uint32_t tpb = (uint32_t)((p.bcapA + kSplitTile - 1) / kSplitTile);
         QO_REQUIRE((uint64_t)tpb * partsA < 0x7FFFFFFFull, "input too large for a hash split");
-        k_split_pairs<<<tpb * partsA, 256, 0, st>>>(b.bucketA, la, b.ctr, tpb, 32 - p.L, 1u << p.L2, b.bucketB, lb,
-                                                   b.ctr + (size_t)partsA * kCtrStride, overflow);
+        k_split_pairs<<<tpb * partsA, kSplitThreads, 0, st>>>(b.bucketA, la, b.ctr, tpb, 32 - p.L, 1u << p.L2, b.bucketB, lb,
+                                                   b.ctr + (size_t)partsA * kCtrStride, overflow, gate, hsrc);
         QO_LAUNCHED();
     }
     return QO_OK;
@@ -385,14 +404,14 @@ inline int launch_exact_splits(const SplitPlan &p, const ExactSplitBufs &b, cons
     const ExactLayout la{b.off, p.L2}, lf{b.off, 0};
     AppendOverflow none{nullptr, nullptr};  // exact regions: never called
     uint64_t *outA = p.L2 > 0 ? b.mid : b.fine;
-    k_split_cols<false><<<(unsigned)tiles, 256, 0, st>>>(kcol, pcol, nullptr, n, 0, 0, vec, 32 - p.L1, partsA, outA,
-                                                         la, b.ctr, none);
+    k_split_cols<false><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, nullptr, n, 0, 0, vec, 32 - p.L1, partsA, outA,
+                                                         la, b.ctr, none, Gate{}, HashSrc{});
     QO_LAUNCHED();
     if (p.L2 > 0) {
         // At most one partial tile per A bucket beyond n / kSplitTile full ones.
         const size_t tiles_b = tiles + partsA;
         QO_REQUIRE(tiles_b < 0x7FFFFFFFull, "input too large for a hash split");
-        k_split_pairs_exact<<<(unsigned)tiles_b, 256, 0, st>>>(b.mid, la, partsA, 32 - p.L, 1u << p.L2, b.fine, lf,
+        k_split_pairs_exact<<<(unsigned)tiles_b, kSplitThreads, 0, st>>>(b.mid, la, partsA, 32 - p.L, 1u << p.L2, b.fine, lf,
                                                                b.ctr + (size_t)partsA * kCtrStride, none);
         QO_LAUNCHED();
     }

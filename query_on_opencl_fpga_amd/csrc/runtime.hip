@@ -179,6 +179,7 @@ TuneParams tune_default(int op, int cls) {
         case TUNE_SORT: return {256, 16, 0};           // block, keys per thread
         case TUNE_JOIN_PART: return {256, 32, 0};      // block, LDS table slice KiB, mode
         case TUNE_GROUPBY_PART: return {512, 1024, 0}; // block, groups per fine bucket, mode
+        case TUNE_JOIN_RANGE: return {256, 256, 0};    // block, largest key-range table MiB, mode
         default: return {256, 4, 0};
     }
     (void)cls;

@@ -112,27 +112,137 @@ def test_hj_filter_sort_matches_oracle(qo, orc, nr, mult):
 
 
 # ---------------------------------------------------------------------------
-# Direct vs partitioned build/probe (the partitioned path is forced on small
-# inputs with a 64 KiB table slice per partition, i.e. many partitions, and
-# small buckets so skewed keys exercise the overflow path).
+# Table layouts and probe strategies:
+#  * direct: hash-mixed table, direct build and probe;
+#  * partitioned: hash-mixed table, partitioned build and probe, forced on
+#    small inputs with a 64 KiB table slice per partition (many partitions) and
+#    small buckets so skewed keys exercise the overflow path;
+#  * keyrange: key-range table where the keys allow it, with the partitioned
+#    path planned (so the key-range probe runs behind its device-side gate and
+#    builds that fall back to hashing take the partitioned path).
 # ---------------------------------------------------------------------------
 TUNE_JOIN_PART = 5
+TUNE_JOIN_RANGE = 7
 
 
-@pytest.fixture(params=["direct", "partitioned"])
-def join_mode(request, qo):
+def _set_tune(qo, op, block, items, param):
     import ctypes as C
-    lib = qo._lib.load()
-    mode, slice_kib = (2, 2048) if request.param == "direct" else (1, 64)
     saved = []
     for cls in range(40):
         b, i, p = C.c_int(), C.c_int(), C.c_int()
-        qo._lib.call("qo_tune_get", TUNE_JOIN_PART, cls, C.byref(b), C.byref(i), C.byref(p))
-        saved.append((cls, b.value, i.value, p.value))
-        qo._lib.call("qo_tune_set", TUNE_JOIN_PART, cls, 256, slice_kib, mode)
+        qo._lib.call("qo_tune_get", op, cls, C.byref(b), C.byref(i), C.byref(p))
+        saved.append((op, cls, b.value, i.value, p.value))
+        qo._lib.call("qo_tune_set", op, cls, block, items, param)
+    return saved
+
+
+def _restore_tune(qo, saved):
+    lib = qo._lib.load()
+    for op, cls, b, i, p in saved:
+        lib.qo_tune_set(op, cls, b, i, p)
+
+
+@pytest.fixture(params=["direct", "partitioned", "keyrange"])
+def join_mode(request, qo):
+    part, slice_kib, rng = {"direct": (2, 2048, 2), "partitioned": (1, 64, 2), "keyrange": (1, 64, 0)}[request.param]
+    saved = _set_tune(qo, TUNE_JOIN_PART, 256, slice_kib, part) + _set_tune(qo, TUNE_JOIN_RANGE, 256, 256, rng)
     yield request.param
-    for cls, b, i, p in saved:
-        lib.qo_tune_set(TUNE_JOIN_PART, cls, b, i, p)
+    _restore_tune(qo, saved)
+
+
+def build_info(qo, rk, rp):
+    """Builds a table from (rk, rp); returns (mode, slots in use, has_dups)."""
+    import ctypes as C
+    from query_on_opencl_fpga_amd.device import DeviceBuffer
+    rk, rp = np.asarray(rk, np.int32), np.asarray(rp, np.int32)
+    tb = qo._lib.load().qo_join_table_bytes(len(rk))
+    with DeviceBuffer(tb) as t, DeviceBuffer.from_numpy(rk) as dk, DeviceBuffer.from_numpy(rp) as dp:
+        qo._lib.call("qo_hash_build", t.ptr, tb, dk.ptr, dp.ptr, len(rk), None)
+        mode, used, dups = C.c_int(), C.c_uint64(), C.c_int()
+        qo._lib.call("qo_join_table_info", t.ptr, C.byref(mode), C.byref(used), C.byref(dups), None)
+    return mode.value, used.value, dups.value
+
+
+@pytest.mark.parametrize("base", [0, -12_345, INT_MIN, INT_MAX - 99_999, 1 << 30])
+def test_key_range_table_matches_oracle(qo, orc, base):
+    """Dense unique keys anywhere in the int32 range get the key-range table;
+    probe keys just outside the range, and far outside it, miss."""
+    nr, ns = 100_000, 1_000_003
+    rk = (orc.gen_perm(nr, 41).astype(np.int64) + base).astype(np.int32)
+    rp = orc.gen_uniform_u32(nr, 42, 0, 0, 0).astype(np.int32)
+    sk = rk[orc.gen_uniform_u32(ns, 42, 1, 0, nr)]
+    lo, hi = int(rk.min()), int(rk.max())
+    sk[::11] = lo - 1 if lo > INT_MIN else INT_MAX
+    sk[5::13] = hi + 1 if hi < INT_MAX else INT_MIN
+    sk[7::17] = INT_MIN
+    sk[9::19] = INT_MAX
+    sp = orc.gen_uniform_u32(ns, 42, 2, 0, 0).astype(np.int32)
+    sf = orc.gen_uniform_u32(ns, 42, 3, 0, 100).astype(np.int32)
+    mode, used, dups = build_info(qo, rk, rp)
+    assert mode == 1 and used == 1 << 17 and dups == 0
+    got = qo.operators.hash_join(rk, rp, sk, sp, sf, 10, 79)
+    exp = orc.hash_join(rk, rp, sk, sp, sf, 10, 79)
+    assert got[3] == len(exp[0])
+    assert np.array_equal(canon(rows(got)), canon(rows(exp)))
+
+
+def test_key_range_falls_back_on_duplicates(qo, orc):
+    """One duplicated key in a dense range: the build falls back to the
+    hash-mixed table and every match of the duplicate is emitted."""
+    nr, ns = 50_000, 400_000
+    rk = orc.gen_perm(nr, 43).astype(np.int32)
+    rk[123] = rk[4567]
+    rp = np.arange(nr, dtype=np.int32)
+    sk = orc.gen_perm(nr, 43)[orc.gen_uniform_u32(ns, 44, 1, 0, nr)].astype(np.int32)
+    sk[::50] = rk[4567]
+    sp = np.arange(ns, dtype=np.int32)
+    assert build_info(qo, rk, rp)[0] == 0
+    got = qo.operators.hash_join(rk, rp, sk, sp, capacity=2 * ns)
+    exp = orc.hash_join(rk, rp, sk, sp)
+    assert got[3] == len(exp[0])
+    assert np.array_equal(canon(rows(got)), canon(rows(exp)))
+    # the same key twice with the same payload is a duplicate too
+    rk2, rp2 = np.arange(1000, dtype=np.int32), np.arange(1000, dtype=np.int32)
+    rk2[10], rp2[10] = rk2[11], rp2[11]
+    assert build_info(qo, rk2, rp2)[0] == 0
+    got = qo.operators.hash_join(rk2, rp2, rk2, rp2, capacity=4000)
+    exp = orc.hash_join(rk2, rp2, rk2, rp2)
+    assert got[3] == len(exp[0]) == 1002
+    assert np.array_equal(canon(rows(got)), canon(rows(exp)))
+
+
+def test_key_range_with_sentinel_build_tuples(qo, orc):
+    """(-1, -1) build tuples are counted, not stored: a dense key set holding
+    them (and another payload of key -1) keeps the key-range table and emits
+    every match of key -1."""
+    rk = np.arange(-1, 20_000, dtype=np.int32)
+    rp = np.arange(rk.size, dtype=np.int32) * 7
+    rp[0] = -1  # (-1, -1)
+    rk = np.concatenate([rk, np.array([-1, -1], np.int32)])
+    rp = np.concatenate([rp, np.array([-1, 55], np.int32)])  # a second (-1, -1) and (-1, 55)
+    sk = np.concatenate([np.arange(-3, 20_003, dtype=np.int32), np.full(100, -1, np.int32)])
+    sp = np.arange(sk.size, dtype=np.int32)
+    mode, _, dups = build_info(qo, rk, rp)
+    assert mode == 1 and dups == 1
+    got = qo.operators.hash_join(rk, rp, sk, sp, capacity=100_000)
+    exp = orc.hash_join(rk, rp, sk, sp)
+    assert got[3] == len(exp[0]) == 20_000 + 101 * 3
+    assert np.array_equal(canon(rows(got)), canon(rows(exp)))
+
+
+def test_key_range_not_chosen(qo, orc):
+    """Keys spread wider than the table, or the key-range table switched off:
+    the hash-mixed table."""
+    rng = np.random.default_rng(45)
+    rk = rng.choice(np.arange(-10**7, 10**7), 30_000, replace=False).astype(np.int32)
+    assert build_info(qo, rk, rk)[0] == 0
+    dense = np.arange(30_000, dtype=np.int32)
+    assert build_info(qo, dense, dense)[0] == 1
+    saved = _set_tune(qo, TUNE_JOIN_RANGE, 256, 256, 2)
+    try:
+        assert build_info(qo, dense, dense)[0] == 0
+    finally:
+        _restore_tune(qo, saved)
 
 
 @pytest.mark.parametrize("nr,ns", [(1000, 3000), (50_000, 1_000_003), (1 << 20, 1 << 21)])
