@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_join_probe(const JoinHeader *__restrict
                                                     Gate gate) {
     __shared__ uint32_t s_tot[kProbeJ * 4];
     __shared__ uint64_t s_base;
-    __shared__ JoinStageCol stage;
+    __shared__ JoinStage stage;
     if (gate.off()) return;
     probe_tile<HAS_FILTER>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, blockIdx.x,
                            reinterpret_cast<unsigned long long *>(d_count), emit, s_tot, &s_base, &stage);

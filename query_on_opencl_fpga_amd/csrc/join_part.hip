@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void k_range_probe(const uint64_t *__restrict_
                                                      SoaEmit emit) {
     __shared__ uint32_t s_tot[kPairJ * 4];
     __shared__ uint64_t s_excl;
-    __shared__ JoinStageCol stage;
+    __shared__ JoinStage stage;
     if (hdr->mode != kJoinKeyRange) return;
     const uint32_t x = blockIdx.x & 7u, q = blockIdx.x >> 3;
     const uint32_t b = (q / tpb) * 8u + x, t = q % tpb;

@@ -145,77 +145,11 @@ QO_DEV void probe_one(const uint64_t *slots, uint64_t cap, const JoinHash &hs, u
 }
 
 // LDS staging of one tile's matches (unique-key path), so that the tile is
-// written out with consecutive threads on consecutive rows: all three
-// columns at once (48 KiB), or one column at a time through one buffer
-// (16 KiB, for Emit types with col(c, pos, value); more blocks fit a CU).
+// written out with consecutive threads on consecutive rows (coalesced stores).
 constexpr uint32_t kStageRows = 1024 * 4;
 struct JoinStage {
     int32_t key[kStageRows], rpay[kStageRows], spay[kStageRows];
 };
-struct JoinStageCol {
-    int32_t v[kStageRows];
-};
-
-template <int J>
-QO_DEV void stage_rows(JoinStage *st, const uint32_t (&ex)[J], const uint32_t (&hit)[J], const int32_t (&key)[J][4],
-                       const uint64_t (&sv)[J][4], const int32_t (&sp)[J][4]) {
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        uint32_t pos = ex[j];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (!((hit[j] >> k) & 1u)) continue;
-            st->key[pos] = key[j][k];
-            st->rpay[pos] = (int32_t)(sv[j][k] >> 32);
-            st->spay[pos] = sp[j][k];
-            ++pos;
-        }
-    }
-}
-
-// Emits `total` staged rows at base: for JoinStage, whole rows; for
-// JoinStageCol, column c = 0 (key), 1 (r_payload), 2 (s_payload) in turn.
-// Entered after the block's barrier that follows the reservation.
-template <int J, class Emit>
-QO_DEV void emit_staged(JoinStage *st, uint64_t base, uint32_t total, const uint32_t (&)[J], const uint32_t (&)[J],
-                        const int32_t (&)[J][4], const uint64_t (&)[J][4], const int32_t (&)[J][4], const Emit &emit) {
-    for (uint32_t i = threadIdx.x; i < total; i += 256) emit(base + i, st->key[i], st->rpay[i], st->spay[i]);
-}
-
-template <int J, class Emit>
-QO_DEV void emit_staged(JoinStageCol *st, uint64_t base, uint32_t total, const uint32_t (&ex)[J],
-                        const uint32_t (&hit)[J], const int32_t (&key)[J][4], const uint64_t (&sv)[J][4],
-                        const int32_t (&sp)[J][4], const Emit &emit) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        if (c) {
-            __syncthreads();  // the previous column has been read out
-#pragma unroll
-            for (int j = 0; j < J; ++j) {
-                uint32_t pos = ex[j];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    if (!((hit[j] >> k) & 1u)) continue;
-                    st->v[pos++] = c == 1 ? (int32_t)(sv[j][k] >> 32) : sp[j][k];
-                }
-            }
-            __syncthreads();
-        }
-        for (uint32_t i = threadIdx.x; i < total; i += 256) emit.col(c, base + i, st->v[i]);
-    }
-}
-
-template <int J>
-QO_DEV void stage_rows(JoinStageCol *st, const uint32_t (&ex)[J], const uint32_t (&hit)[J],
-                       const int32_t (&key)[J][4], const uint64_t (&)[J][4], const int32_t (&)[J][4]) {
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        uint32_t pos = ex[j];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if ((hit[j] >> k) & 1u) st->v[pos++] = key[j][k];
-    }
-}
 
 // Looks up the block's tuples (J groups of 4 per thread; bit k of pass[j]
 // marks element k of group j live), reserves the block's output range with one
@@ -223,15 +157,15 @@ QO_DEV void stage_rows(JoinStageCol *st, const uint32_t (&ex)[J], const uint32_t
 // element) order; for one tuple, in table-chain order.  Emit(pos, key,
 // r_payload, s_payload).  With `stage` (J * 1024 <= kStageRows) the unique-key
 // path writes the matches to LDS first and emits them row by row across the
-// block (coalesced stores); without it, and on the general path, each thread
-// emits its own matches.  Must be called by all 256 threads of the block;
+// block; without it, and on the general path, each thread emits its own
+// matches.  Must be called by all 256 threads of the block;
 // s_tot holds J*4 words.
-template <int J, class Emit, class Stage = JoinStage>
+template <int J, class Emit>
 QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, const JoinHash &hsh, bool general,
                         uint32_t sentinel,
                         const int32_t (&key)[J][4], const int32_t (&sp)[J][4], const uint32_t (&pass)[J],
                         unsigned long long *counter, const Emit &emit, uint32_t *s_tot, uint64_t *s_excl,
-                        Stage *stage = nullptr) {
+                        JoinStage *stage = nullptr) {
     uint32_t c[J], ex[J];
     uint32_t hit[J];
     // On a hit of the unique-key path sv holds the matching slot (its payload
@@ -305,9 +239,21 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
     if (threadIdx.x == 0) *s_excl = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
     if (stage && !general) {
         static_assert(J * 1024 <= kStageRows, "tile larger than the stage");
-        stage_rows<J>(stage, ex, hit, key, sv, sp);
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            uint32_t pos = ex[j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!((hit[j] >> k) & 1u)) continue;
+                stage->key[pos] = key[j][k];
+                stage->rpay[pos] = (int32_t)(sv[j][k] >> 32);
+                stage->spay[pos] = sp[j][k];
+                ++pos;
+            }
+        }
         __syncthreads();
-        emit_staged<J>(stage, *s_excl, total, ex, hit, key, sv, sp, emit);
+        const uint64_t base = *s_excl;
+        for (uint32_t i = threadIdx.x; i < total; i += 256) emit(base + i, stage->key[i], stage->rpay[i], stage->spay[i]);
         return;
     }
     __syncthreads();
@@ -377,12 +323,12 @@ QO_DEV void load_s_tile(const int32_t *__restrict__ skey, const int32_t *__restr
 // order: a look-back across tiles would make every tile wait for the slowest
 // lookup of its predecessors.  Must be called by all 256 threads of the block;
 // s_tot holds kProbeJ*4 words.
-template <bool HAS_FILTER, class Emit, class Stage = JoinStage>
+template <bool HAS_FILTER, class Emit>
 QO_DEV void probe_tile(const JoinHeader *__restrict__ hdr, const uint64_t *__restrict__ slots,
                        const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
                        const int32_t *__restrict__ sfilt, size_t ns, int32_t flo, int32_t fhi, bool vec,
                        uint32_t tile, unsigned long long *counter, const Emit &emit, uint32_t *s_tot,
-                       uint64_t *s_excl, Stage *stage = nullptr) {
+                       uint64_t *s_excl, JoinStage *stage = nullptr) {
     const uint64_t cap_slots = hdr->cap;
     const bool general = (hdr->has_dups | hdr->sentinel_count) != 0;
     const uint32_t sentinel = hdr->sentinel_count;
@@ -403,9 +349,6 @@ struct SoaEmit {
             orpay[pos] = rp;
             ospay[pos] = sp;
         }
-    }
-    QO_DEV void col(int c, uint64_t pos, int32_t v) const {
-        if (pos < cap) (c == 0 ? okey : c == 1 ? orpay : ospay)[pos] = v;
     }
 };
 
