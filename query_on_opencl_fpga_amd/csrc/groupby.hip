@@ -47,11 +47,21 @@ __global__ void k_gtable_init(GEntry *t, size_t n) {
     }
 }
 
+// Home slot of `key` in a table of 2^bits slots: multiplicative (Fibonacci)
+// hashing, the top bits of key * 2^32/phi.  Consecutive keys land almost one
+// per slot (group keys are often a dense range), and other key sets spread
+// like under any mixing hash.
+QO_DEV uint32_t fib_slot(uint32_t key, int bits) { return (key * 0x9E3779B9u) >> (32 - bits); }
+
 // Entry for `key` in the global table (index mask+1 is reserved for key -1,
 // whose bit pattern is the empty marker).  nullptr when the table is full.
 QO_DEV GEntry *g_entry(GEntry *t, uint64_t mask, uint32_t key) {
     if (key == kEmptyKey) return &t[mask + 1];
-    uint64_t h = hash32(key) & mask;
+    uint64_t h = fib_slot(key, __popcll(mask));
+    {
+        const uint32_t k = __hip_atomic_load(&t[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == key) return &t[h];
+    }
     for (uint64_t p = 0; p <= mask; ++p) {
         uint32_t k = __hip_atomic_load(&t[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (k == key) return &t[h];
@@ -77,10 +87,15 @@ QO_DEV void g_update(GEntry *t, uint64_t mask, uint32_t key, uint64_t cnt, uint6
     atomicMax(&e->mx, mx);
 }
 
-// LDS slot for `key` within `probes` probes, or -1.
+// LDS slot for `key` within `probes` probes, or -1.  The home slot is tested
+// first on its own: in the common case (the key is already there) no lane of
+// the wave enters the probe loop.
 template <int C>
 QO_DEV int l_slot(uint32_t *lkey, uint32_t key, int probes) {
-    uint32_t h = hash32(key) & (C - 1);
+    static_assert((C & (C - 1)) == 0, "power-of-two table");
+    constexpr int kBits = __builtin_ctz(C);
+    uint32_t h = fib_slot(key, kBits);
+    if (lkey[h] == key) return (int)h;
     for (int p = 0; p < probes; ++p) {
         uint32_t k = lkey[h];
         if (k == key) return (int)h;
