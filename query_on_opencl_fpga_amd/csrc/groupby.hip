@@ -4,8 +4,9 @@
 //    partial table (key, COUNT, SUM, MIN, MAX) with linear probing; tuples whose
 //    key does not find an LDS slot within a few probes update a global
 //    open-addressing table directly.  At the end each block merges its LDS table
-//    into the global one; the global entries are compacted, sorted by key and
-//    gathered into the SoA outputs.
+//    into the global one (or, when it saw every tuple of its keys, appends its
+//    groups as finished rows); the rows are sorted by key and gathered into the
+//    SoA outputs.
 // 2. The reference's sort-based pipeline for drop-in callers: group start
 //    positions over Records sorted by .y (scanGroupLabel_kernel +
 //    groupByImpl_write_kernel, kernel/aggregation/1/group_by_cu_op_dp.cl:293-327,
@@ -87,14 +88,16 @@ QO_DEV void g_update(GEntry *t, uint64_t mask, uint32_t key, uint64_t cnt, uint6
     atomicMax(&e->mx, mx);
 }
 
-// LDS slot for `key` within `probes` probes, or -1.  The home slot is tested
-// first on its own: in the common case (the key is already there) no lane of
-// the wave enters the probe loop.
+// LDS slot for `key` within `probes` probes, or -1.  The home slot is the
+// next log2(C) bits of the multiplicative hash after the `skip` top bits (a
+// split by those bits put the tuple in this table).  It is tested first on its
+// own: in the common case (the key is already there) no lane of the wave
+// enters the probe loop.
 template <int C>
-QO_DEV int l_slot(uint32_t *lkey, uint32_t key, int probes) {
+QO_DEV int l_slot(uint32_t *lkey, uint32_t key, int probes, int skip) {
     static_assert((C & (C - 1)) == 0, "power-of-two table");
     constexpr int kBits = __builtin_ctz(C);
-    uint32_t h = fib_slot(key, kBits);
+    uint32_t h = ((key * 0x9E3779B9u) << skip) >> (32 - kBits);
     if (lkey[h] == key) return (int)h;
     for (int p = 0; p < probes; ++p) {
         uint32_t k = lkey[h];
@@ -130,17 +133,43 @@ QO_DEV void ltable_init(LTable<C> &L) {
 
 // One tuple into the block's LDS table, or into the global table when its key
 // finds no LDS slot within `probes` probes (or is the empty-marker pattern).
-template <int C>
+//
+// COMBINE (for inputs where one key can fill whole waves, such as the buckets
+// of a heavy hitter): when every lane of the wave is active and at least a
+// quarter of the lanes hold the key of the first lane, those lanes combine
+// (COUNT, SUM, MIN, MAX) across the wave first and one of them updates the
+// table, instead of all of them updating one LDS address one after another.
+template <int C, bool COMBINE = false>
 QO_DEV void ltable_add(LTable<C> &L, int32_t k, int32_t v, int probes, GEntry *gt, uint64_t gmask,
-                       uint32_t *overflow) {
-    int s = (uint32_t)k == kEmptyKey ? -1 : l_slot<C>(L.key, (uint32_t)k, probes);
+                       uint32_t *overflow, int skip = 0) {
+    uint32_t cnt = 1;
+    uint64_t sum = (uint64_t)(int64_t)v;
+    int32_t mn = v, mx = v;
+    if (COMBINE && ballot(true) == ~uint64_t(0)) {
+        const int32_t k0 = __builtin_amdgcn_readfirstlane(k);
+        const bool in = k == k0;
+        const uint64_t m = ballot(in);
+        if (popc(m) >= kWave / 4) {  // uniform across the wave
+            const uint64_t s2 = wave_sum(in ? sum : uint64_t(0));
+            const int32_t mn2 = wave_min(in ? v : INT_MAX);
+            const int32_t mx2 = wave_max(in ? v : INT_MIN);
+            if (in) {
+                if (lane_id() != (unsigned)__builtin_ctzll(m)) return;
+                cnt = popc(m);
+                sum = s2;
+                mn = mn2;
+                mx = mx2;
+            }
+        }
+    }
+    int s = (uint32_t)k == kEmptyKey ? -1 : l_slot<C>(L.key, (uint32_t)k, probes, skip);
     if (s >= 0) {
-        atomicAdd(&L.cnt[s], 1u);
-        atomicAdd(&L.sum[s], (unsigned long long)(long long)v);
-        atomicMin(&L.mn[s], v);
-        atomicMax(&L.mx[s], v);
+        atomicAdd(&L.cnt[s], cnt);
+        atomicAdd(&L.sum[s], (unsigned long long)sum);
+        atomicMin(&L.mn[s], mn);
+        atomicMax(&L.mx[s], mx);
     } else {
-        g_update(gt, gmask, (uint32_t)k, 1, (uint64_t)(int64_t)v, v, v, overflow);
+        g_update(gt, gmask, (uint32_t)k, cnt, sum, mn, mx, overflow);
     }
 }
 
@@ -149,6 +178,40 @@ template <int C>
 QO_DEV void ltable_flush(const LTable<C> &L, GEntry *gt, uint64_t gmask, uint32_t *overflow) {
     for (int i = threadIdx.x; i < C; i += blockDim.x)
         if (L.cnt[i]) g_update(gt, gmask, L.key[i], L.cnt[i], L.sum[i], L.mn[i], L.mx[i], overflow);
+}
+
+// Appends the block's LDS groups as finished rows (the block saw every tuple
+// of these keys): one reservation on *d_rows per block; rows past `cap` are
+// counted but not written.  After a barrier; 512 threads.
+template <int C>
+QO_DEV void ltable_append(const LTable<C> &L, GEntry *rows, unsigned long long *d_rows, size_t cap, uint32_t *s_w,
+                          unsigned long long *s_base) {
+    static_assert(C % 512 == 0, "table size");
+    constexpr int kPer = C / 512;
+    const int i0 = threadIdx.x * kPer;
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) c += L.cnt[i0 + k] != 0;
+    const uint32_t ex = block_excl_scan<8>(c, s_w);
+    if (threadIdx.x == 511) *s_base = atomicAdd(d_rows, (unsigned long long)(ex + c));
+    __syncthreads();
+    uint64_t pos = *s_base + ex;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = i0 + k;
+        if (!L.cnt[i]) continue;
+        if (pos < cap) {
+            GEntry e;
+            e.key = L.key[i];
+            e.mn = L.mn[i];
+            e.mx = L.mx[i];
+            e.pad = 0;
+            e.cnt = L.cnt[i];
+            e.sum = L.sum[i];
+            rows[pos] = e;
+        }
+        ++pos;
+    }
 }
 
 template <int C, bool VEC>
@@ -182,28 +245,39 @@ __global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict_
 
 // ---------------------------------------------------------------------------
 // Partitioned hash group-by (more groups than one LDS table holds).  The
-// (key, value) tuples are split exactly by key hash (qo_split.hpp: a histogram
-// pass sizes every bucket, so skewed keys cannot overflow) into 2^L fine
-// buckets of about `target` groups each.  One work-group per chunk of a fine
+// (key, value) tuples are split exactly by the multiplicative key hash
+// (qo_split.hpp: a histogram pass sizes every bucket, so skewed keys cannot
+// overflow) into 2^L fine buckets of about `target` groups each; the LDS table
+// of a bucket is indexed by the next hash bits.  One work-group per chunk of a fine
 // bucket aggregates in its LDS table and merges its partials into the global
 // table, so global atomics fall from four per tuple to four per (chunk, group);
 // a heavy key's bucket is simply cut into more chunks.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kGbChunk = 1u << 18;  // tuples per aggregation work item at most
+constexpr uint32_t kGbChunk = 1u << 18;  // tuples per work item of a large bucket
+constexpr uint32_t kGbWhole = 1u << 19;  // buckets up to this size are one work item
 
-// Chunks per fine bucket: cch[f] = ceil(hist[f] / kGbChunk), cch[nb] = 0.
+// Work items per fine bucket: one for a bucket of at most kGbWhole tuples,
+// else ceil(hist[f] / kGbChunk); cch[nb] = 0.
 __global__ void k_chunk_counts(const uint32_t *hist, uint32_t nb, uint32_t *cch) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f <= nb) cch[f] = f < nb ? (hist[f] + kGbChunk - 1) / kGbChunk : 0u;
+    if (f > nb) return;
+    const uint32_t h = f < nb ? hist[f] : 0u;
+    cch[f] = h == 0 ? 0u : h <= kGbWhole ? 1u : (h + kGbChunk - 1) / kGbChunk;
 }
 
-// Work item w = chunk (w - cpre[f]) of fine bucket f, cpre[f] <= w < cpre[f + 1].
+// Work item w of fine bucket f (cpre[f] <= w < cpre[f + 1]).  A bucket that
+// is one work item holds every tuple of its keys, so the block appends its
+// groups as finished rows; a chunk of a larger bucket merges its partials
+// into the global table.
 template <int C>
 __global__ __launch_bounds__(512) void k_groupby_chunks(const uint64_t *__restrict__ fine, const uint32_t *off,
-                                                        const uint32_t *cpre, uint32_t nb, GEntry *gt, uint64_t gmask,
-                                                        int probes, uint32_t *overflow) {
+                                                        const uint32_t *cpre, uint32_t nb, int L_bits, GEntry *gt,
+                                                        uint64_t gmask, int probes, uint32_t *overflow, GEntry *rows,
+                                                        unsigned long long *d_rows, size_t rows_cap) {
     extern __shared__ unsigned char s_raw[];
     LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
+    __shared__ uint32_t s_w[8];
+    __shared__ unsigned long long s_base;
     const uint32_t w = blockIdx.x;
     if (w >= cpre[nb]) return;  // whole block: the grid is an upper bound
     int lo = 0, hi = (int)nb - 1;
@@ -212,35 +286,58 @@ __global__ __launch_bounds__(512) void k_groupby_chunks(const uint64_t *__restri
         if (cpre[mid] <= w) lo = mid;
         else hi = mid - 1;
     }
-    const uint64_t e0 = off[lo] + (uint64_t)(w - cpre[lo]) * kGbChunk;
+    const uint64_t size = off[lo + 1] - off[lo];
+    const bool whole = size <= kGbWhole;
+    const uint64_t e0 = off[lo] + (whole ? 0 : (uint64_t)(w - cpre[lo]) * kGbChunk);
     const uint64_t e1 = off[lo + 1];
-    const uint32_t m = (uint32_t)(e1 - e0 < kGbChunk ? e1 - e0 : kGbChunk);
+    const uint32_t m = whole ? (uint32_t)size : (uint32_t)(e1 - e0 < kGbChunk ? e1 - e0 : kGbChunk);
     const uint64_t *src = fine + e0;
     ltable_init<C>(L);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
         const uint64_t v = ld_stream(src + i);
-        ltable_add<C>(L, (int32_t)(uint32_t)v, (int32_t)(v >> 32), probes, gt, gmask, overflow);
+        ltable_add<C, true>(L, (int32_t)(uint32_t)v, (int32_t)(v >> 32), probes, gt, gmask, overflow, L_bits);
     }
     __syncthreads();
-    ltable_flush<C>(L, gt, gmask, overflow);
+    if (whole) ltable_append<C>(L, rows, d_rows, rows_cap, s_w, &s_base);
+    else ltable_flush<C>(L, gt, gmask, overflow);
 }
 
-// Occupied global entries -> (key ^ signbit) << 32 | index, appended by atomics
-// (the order is fixed by the sort that follows).
-__global__ void k_gtable_compact(const GEntry *t, size_t entries, uint64_t *list, uint64_t *d_count, size_t cap) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < entries; i += stride) {
-        const GEntry &e = t[i];
-        if (e.cnt == 0) continue;
-        unsigned long long p = atomicAdd(reinterpret_cast<unsigned long long *>(d_count), 1ull);
-        uint32_t key = (i == entries - 1) ? kEmptyKey : e.key;
-        if (p < cap) list[p] = ((uint64_t)(key ^ 0x80000000u) << 32) | (uint32_t)i;
+// Occupied global entries appended to `rows`: kCollectJ entries per thread,
+// one reservation per block and round.
+constexpr int kCollectJ = 8;
+__global__ __launch_bounds__(256) void k_gtable_collect(const GEntry *t, size_t entries, GEntry *rows,
+                                                        unsigned long long *d_rows, size_t cap) {
+    __shared__ uint32_t s_w[4];
+    __shared__ unsigned long long s_base;
+    constexpr size_t kRound = 256 * kCollectJ;
+    for (size_t r0 = (size_t)blockIdx.x * kRound; r0 < entries; r0 += (size_t)gridDim.x * kRound) {
+        const size_t i0 = r0 + (size_t)threadIdx.x * kCollectJ;
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < kCollectJ; ++k) c += (i0 + k < entries) && t[i0 + k].cnt != 0;
+        const uint32_t ex = block_excl_scan256(c, s_w);
+        if (threadIdx.x == 255) s_base = ex + c ? atomicAdd(d_rows, (unsigned long long)(ex + c)) : 0ull;
+        __syncthreads();
+        uint64_t pos = s_base + ex;
+#pragma unroll
+        for (int k = 0; k < kCollectJ; ++k) {
+            if (i0 + k >= entries || t[i0 + k].cnt == 0) continue;
+            if (pos < cap) rows[pos] = t[i0 + k];
+            ++pos;
+        }
+        __syncthreads();  // s_w and s_base are reused by the next round
     }
 }
 
-__global__ void k_gtable_gather(const GEntry *t, const uint64_t *list, const uint64_t *d_count, size_t cap,
+// Sort keys of the finished rows: (key ^ signbit) << 32 | row.
+__global__ void k_rows_list(const GEntry *rows, const unsigned long long *d_rows, size_t cap, uint64_t *list) {
+    const uint64_t n = *d_rows < cap ? *d_rows : cap;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        list[i] = ((uint64_t)(rows[i].key ^ 0x80000000u) << 32) | (uint32_t)i;
+}
+
+__global__ void k_gtable_gather(const GEntry *rows, const uint64_t *list, const uint64_t *d_count, size_t cap,
                                 const uint32_t *overflow, int32_t *okey, int64_t *ocount, int64_t *osum,
                                 int32_t *omin, int32_t *omax, uint64_t *d_num) {
     size_t g = d_count[0];
@@ -249,7 +346,7 @@ __global__ void k_gtable_gather(const GEntry *t, const uint64_t *list, const uin
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < lim; i += stride) {
         uint64_t v = list[i];
-        const GEntry &e = t[(uint32_t)v];
+        const GEntry &e = rows[(uint32_t)v];
         okey[i] = (int32_t)((uint32_t)(v >> 32) ^ 0x80000000u);
         ocount[i] = (int64_t)e.cnt;
         osum[i] = (int64_t)e.sum;
@@ -292,7 +389,8 @@ static GbPartPlan groupby_part_plan(size_t group_capacity, size_t n) {
 
 struct GroupbyBufs {
     GEntry *gt;
-    uint64_t *counters;  // [0] groups found, [1] table-full flag (low word)
+    uint64_t *counters;  // [0] groups found (finished rows), [1] table-full flag (low word)
+    GEntry *rows;        // finished groups, unordered
     uint64_t *list;
     void *rws;
     ExactSplitBufs sb;   // partitioned path only
@@ -305,6 +403,7 @@ static GroupbyBufs groupby_carve(const GroupbyPlan &p, const GbPartPlan &pp, siz
     GroupbyBufs b;
     b.gt = cv.take<GEntry>(p.entries);
     b.counters = cv.take<uint64_t>(4);
+    b.rows = cv.take<GEntry>(group_capacity ? group_capacity : 1);
     b.list = cv.take<uint64_t>(group_capacity ? group_capacity : 1);
     b.rws = cv.take<char>(radix_workspace_bytes(group_capacity, false));
     if (pp.on) {
@@ -348,8 +447,9 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         // Upper bound on the chunk count: every bucket one partial chunk.
         const uint64_t items = (uint64_t)nb + (n + kGbChunk - 1) / kGbChunk;
         QO_REQUIRE(items < 0x7FFFFFFFull, "group-by input too large");
-        k_groupby_chunks<2048><<<(unsigned)items, 512, sizeof(LTable<2048>), st>>>(b.sb.fine, b.sb.off, b.cpre, nb, gt,
-                                                                                    p.gmask, 64, overflow);
+        k_groupby_chunks<2048><<<(unsigned)items, 512, sizeof(LTable<2048>), st>>>(
+            b.sb.fine, b.sb.off, b.cpre, nb, pp.s.L, gt, p.gmask, 64, overflow, b.rows,
+            reinterpret_cast<unsigned long long *>(counters), group_capacity);
         QO_LAUNCHED();
     } else if (n) {
         TuneParams tp = tune_get(TUNE_GROUPBY, n);
@@ -377,12 +477,16 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         }
         QO_LAUNCHED();
     }
-    k_gtable_compact<<<ig, 256, 0, st>>>(gt, p.entries, list, counters, group_capacity);
+    unsigned long long *d_rows = reinterpret_cast<unsigned long long *>(counters);
+    const unsigned cg = grid_for(p.entries, 256 * kCollectJ, (unsigned)props().cu_count * 4);
+    k_gtable_collect<<<cg, 256, 0, st>>>(gt, p.entries, b.rows, d_rows, group_capacity);
+    QO_LAUNCHED();
+    const unsigned gg = grid_for(group_capacity, 256, (unsigned)props().cu_count * 8);
+    k_rows_list<<<gg, 256, 0, st>>>(b.rows, d_rows, group_capacity, list);
     QO_LAUNCHED();
     if (group_capacity > 1) QO_TRY(launch_radix_sort_u64_dev(list, nullptr, counters, group_capacity, 32, 64, rws, st));
-    const unsigned gg = grid_for(group_capacity, 256, (unsigned)props().cu_count * 8);
-    k_gtable_gather<<<gg, 256, 0, st>>>(gt, list, counters, group_capacity, overflow, okey, ocount, osum, omin, omax,
-                                        d_num);
+    k_gtable_gather<<<gg, 256, 0, st>>>(b.rows, list, counters, group_capacity, overflow, okey, ocount, osum, omin,
+                                        omax, d_num);
     QO_LAUNCHED();
     return QO_OK;
 }

@@ -29,6 +29,7 @@ namespace qo {
 constexpr uint64_t kEmptySlot = ~uint64_t(0);
 constexpr uint32_t kJoinHashMix = 0;   // JoinHeader::mode
 constexpr uint32_t kJoinKeyRange = 1;
+constexpr uint32_t kHashFib = 2;       // multiplicative hash: splits of the group-by only, never a table mode
 
 struct alignas(256) JoinHeader {
     uint64_t cap;             // slots in use
@@ -53,6 +54,10 @@ struct JoinHash {
     QO_DEV static JoinHash of(const JoinHeader *h) { return JoinHash{h->mode, h->shift, h->kmin}; }
     // Sets h; false when `key` lies outside a key-range table (it cannot be stored there).
     QO_DEV bool operator()(int32_t key, uint32_t &h) const {
+        if (mode == kHashFib) {
+            h = (uint32_t)key * 0x9E3779B9u;
+            return true;
+        }
         if (mode != kJoinKeyRange) {
             h = hash32((uint32_t)key);
             return true;

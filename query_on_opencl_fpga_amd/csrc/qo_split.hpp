@@ -110,10 +110,13 @@ struct ExactLayout {
 };
 
 // Where a split kernel takes its hash from: a join table's header, or (null)
-// hash32.
+// hash32, or with `fib` the multiplicative hash key * 2^32/phi.
 struct HashSrc {
     const JoinHeader *hdr = nullptr;
-    QO_DEV JoinHash get() const { return hdr ? JoinHash::of(hdr) : JoinHash{kJoinHashMix, 0u, 0}; }
+    uint32_t fib = 0;
+    QO_DEV JoinHash get() const {
+        return hdr ? JoinHash::of(hdr) : JoinHash{fib ? kHashFib : kJoinHashMix, 0u, 0};
+    }
 };
 
 struct SplitLds {
@@ -298,7 +301,8 @@ __global__ __launch_bounds__(kSplitThreads) void k_split_pairs_exact(const uint6
     }
     __syncthreads();
     if (s_b >= nin) return;  // past the last tile (the grid is an upper bound)
-    split_pairs_tile(input, in, nullptr, s_b, s_t, HashSrc{}.get(), shift, parts, bucket, out, ctr, sh, overflow);
+    split_pairs_tile(input, in, nullptr, s_b, s_t, HashSrc{nullptr, 1}.get(), shift, parts, bucket, out, ctr, sh,
+                     overflow);
 }
 
 // Launches split A (and B) with fixed bucket capacities: kcol/pcol (and fcol
@@ -332,11 +336,14 @@ inline int launch_splits(const SplitPlan &p, const SplitBufs &b, const int32_t *
 }
 
 // ---------------------------------------------------------------------------
-// Exact split: a histogram pass over the keys sizes every bucket exactly, so
+// Exact split (by the multiplicative hash, key * 2^32/phi, so that a dense key
+// range spreads evenly over the buckets and, within a bucket, over the next
+// hash bits): a histogram pass over the keys sizes every bucket exactly, so
 // skewed keys cannot overflow.  Output: the fine buckets back to back in
 // `fine` (bucket f at [off[f], off[f + 1])), with `mid` holding split A's.
 // ---------------------------------------------------------------------------
-// Histogram of fine buckets (top L bits of hash32) over the key column.
+// Histogram of fine buckets (top L bits of the multiplicative hash) over the key
+// column.
 // Dynamic LDS: nb counters.
 static __global__ __launch_bounds__(512) void k_split_hist(const int32_t *__restrict__ key, size_t n, int shift, uint32_t nb,
                                                            uint32_t *hist) {
@@ -350,15 +357,15 @@ static __global__ __launch_bounds__(512) void k_split_hist(const int32_t *__rest
         const size_t nv = n / 4;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
             const int4 k = ld_stream(reinterpret_cast<const int4 *>(key) + i);
-            atomicAdd(&s_hist[hash32((uint32_t)k.x) >> shift], 1u);
-            atomicAdd(&s_hist[hash32((uint32_t)k.y) >> shift], 1u);
-            atomicAdd(&s_hist[hash32((uint32_t)k.z) >> shift], 1u);
-            atomicAdd(&s_hist[hash32((uint32_t)k.w) >> shift], 1u);
+            atomicAdd(&s_hist[((uint32_t)k.x * 0x9E3779B9u) >> shift], 1u);
+            atomicAdd(&s_hist[((uint32_t)k.y * 0x9E3779B9u) >> shift], 1u);
+            atomicAdd(&s_hist[((uint32_t)k.z * 0x9E3779B9u) >> shift], 1u);
+            atomicAdd(&s_hist[((uint32_t)k.w * 0x9E3779B9u) >> shift], 1u);
         }
         from = nv * 4;
     }
     for (size_t i = from + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        atomicAdd(&s_hist[hash32((uint32_t)key[i]) >> shift], 1u);
+        atomicAdd(&s_hist[((uint32_t)key[i] * 0x9E3779B9u) >> shift], 1u);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
         if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
@@ -405,7 +412,7 @@ inline int launch_exact_splits(const SplitPlan &p, const ExactSplitBufs &b, cons
     AppendOverflow none{nullptr, nullptr};  // exact regions: never called
     uint64_t *outA = p.L2 > 0 ? b.mid : b.fine;
     k_split_cols<false><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, nullptr, n, 0, 0, vec, 32 - p.L1, partsA, outA,
-                                                         la, b.ctr, none, Gate{}, HashSrc{});
+                                                         la, b.ctr, none, Gate{}, HashSrc{nullptr, 1});
     QO_LAUNCHED();
     if (p.L2 > 0) {
         // At most one partial tile per A bucket beyond n / kSplitTile full ones.
