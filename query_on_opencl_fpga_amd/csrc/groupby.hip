@@ -270,8 +270,8 @@ __global__ void k_chunk_counts(const uint32_t *hist, uint32_t nb, uint32_t *cch)
 // groups as finished rows; a chunk of a larger bucket merges its partials
 // into the global table.
 template <int C>
-__global__ __launch_bounds__(512) void k_groupby_chunks(const uint64_t *__restrict__ fine, const uint32_t *off,
-                                                        const uint32_t *cpre, uint32_t nb, int L_bits, GEntry *gt,
+__global__ __launch_bounds__(512) void k_groupby_chunks(const uint64_t *__restrict__ fine, const uint32_t *offp,
+                                                        const uint32_t *off, const uint32_t *cpre, uint32_t nb, int L_bits, GEntry *gt,
                                                         uint64_t gmask, int probes, uint32_t *overflow, GEntry *rows,
                                                         unsigned long long *d_rows, size_t rows_cap) {
     extern __shared__ unsigned char s_raw[];
@@ -286,10 +286,10 @@ __global__ __launch_bounds__(512) void k_groupby_chunks(const uint64_t *__restri
         if (cpre[mid] <= w) lo = mid;
         else hi = mid - 1;
     }
-    const uint64_t size = off[lo + 1] - off[lo];
+    const uint64_t size = off[lo + 1] - off[lo];  // bucket lo: [offp[lo], offp[lo] + size) of `fine`
     const bool whole = size <= kGbWhole;
-    const uint64_t e0 = off[lo] + (whole ? 0 : (uint64_t)(w - cpre[lo]) * kGbChunk);
-    const uint64_t e1 = off[lo + 1];
+    const uint64_t e0 = offp[lo] + (whole ? 0 : (uint64_t)(w - cpre[lo]) * kGbChunk);
+    const uint64_t e1 = offp[lo] + size;
     const uint32_t m = whole ? (uint32_t)size : (uint32_t)(e1 - e0 < kGbChunk ? e1 - e0 : kGbChunk);
     const uint64_t *src = fine + e0;
     ltable_init<C>(L);
@@ -448,7 +448,7 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         const uint64_t items = (uint64_t)nb + (n + kGbChunk - 1) / kGbChunk;
         QO_REQUIRE(items < 0x7FFFFFFFull, "group-by input too large");
         k_groupby_chunks<2048><<<(unsigned)items, 512, sizeof(LTable<2048>), st>>>(
-            b.sb.fine, b.sb.off, b.cpre, nb, pp.s.L, gt, p.gmask, 64, overflow, b.rows,
+            b.sb.fine, b.sb.offp, b.sb.off, b.cpre, nb, pp.s.L, gt, p.gmask, 64, overflow, b.rows,
             reinterpret_cast<unsigned long long *>(counters), group_capacity);
         QO_LAUNCHED();
     } else if (n) {

@@ -98,15 +98,19 @@ struct FixedLayout {
         return c < bcap ? c : bcap;
     }
 };
-// Exact: regions from the exclusive scan `off` (2^L + 1 entries) of the
-// fine-bucket histogram; a bucket of this split covers fine buckets
-// [b << sh, (b + 1) << sh).  Nothing overflows.
+// Exact: a bucket of this split covers fine buckets [b << sh, (b + 1) << sh).
+// Its region starts at offp (the exclusive scan of the fine-bucket counts,
+// each rounded up to even, so that every region starts 16-byte aligned) and it
+// holds as many tuples as the exclusive scan `off` of the exact counts says.
+// Nothing overflows.
 struct ExactLayout {
-    const uint32_t *off;
+    const uint32_t *offp, *off;
     int sh;
-    QO_DEV uint64_t start(uint32_t b) const { return off[(size_t)b << sh]; }
-    QO_DEV uint64_t cap(uint32_t b) const { return off[(size_t)(b + 1) << sh] - off[(size_t)b << sh]; }
-    QO_DEV uint64_t count(uint32_t b, const uint32_t *) const { return cap(b); }
+    QO_DEV uint64_t start(uint32_t b) const { return offp[(size_t)b << sh]; }
+    QO_DEV uint64_t cap(uint32_t b) const { return offp[(size_t)(b + 1) << sh] - offp[(size_t)b << sh]; }
+    QO_DEV uint64_t count(uint32_t b, const uint32_t *) const {
+        return off[(size_t)(b + 1) << sh] - off[(size_t)b << sh];
+    }
 };
 
 // Where a split kernel takes its hash from: a join table's header, or (null)
@@ -286,18 +290,15 @@ __global__ __launch_bounds__(kSplitThreads) void k_split_pairs_exact(const uint6
                                                            uint32_t *ctr, Overflow overflow) {
     __shared__ SplitLds sh;
     __shared__ uint32_t s_b, s_t;
-    if (threadIdx.x == 0) {
-        uint32_t t = blockIdx.x, b = nin;
-        for (uint32_t i = 0; i < nin; ++i) {
-            const uint32_t nt = (uint32_t)((in.cap(i) + kSplitTile - 1) / kSplitTile);
-            if (t < nt) {
-                b = i;
-                break;
-            }
-            t -= nt;
-        }
-        s_b = b;
-        s_t = t;
+    // Tile counts of all input buckets at once (nin <= kSplitThreads), their
+    // prefix sums, and the bucket whose tile range holds blockIdx.x.
+    const uint32_t nt =
+        threadIdx.x < nin ? (uint32_t)((in.count(threadIdx.x, nullptr) + kSplitTile - 1) / kSplitTile) : 0u;
+    if (threadIdx.x == 0) s_b = nin;
+    const uint32_t first = block_excl_scan<kSplitThreads / kWave>(nt, sh.w);
+    if (nt && blockIdx.x >= first && blockIdx.x < first + nt) {
+        s_b = threadIdx.x;
+        s_t = blockIdx.x - first;
     }
     __syncthreads();
     if (s_b >= nin) return;  // past the last tile (the grid is an upper bound)
@@ -374,10 +375,12 @@ static __global__ __launch_bounds__(512) void k_split_hist(const int32_t *__rest
 struct ExactSplitBufs {
     uint32_t *ctr = nullptr;   // fill cursors: 2^L1 (split A), then 2^L (split B), kCtrStride apart
     uint32_t *hist = nullptr;  // 2^L + 1 fine-bucket counts (last entry stays 0)
-    uint32_t *off = nullptr;   // 2^L + 1 exclusive offsets; off[2^L] = n
+    uint32_t *off = nullptr;   // 2^L + 1 exclusive offsets of the counts; off[2^L] = n
+    uint32_t *hpad = nullptr;  // counts rounded up to even
+    uint32_t *offp = nullptr;  // 2^L + 1 exclusive offsets of hpad: where each fine bucket starts
     void *scan_ws = nullptr;
-    uint64_t *mid = nullptr;   // split A output (n pairs), when L2 > 0
-    uint64_t *fine = nullptr;  // fine buckets (n pairs)
+    uint64_t *mid = nullptr;   // split A output (n + 2^L pairs), when L2 > 0
+    uint64_t *fine = nullptr;  // fine buckets (n + 2^L pairs)
     size_t ctr_words = 0;
 };
 
@@ -388,10 +391,17 @@ inline ExactSplitBufs exact_split_carve(const SplitPlan &p, size_t n, Carver &cv
     b.ctr = cv.take<uint32_t>(b.ctr_words);
     b.hist = cv.take<uint32_t>(nb + 1);
     b.off = cv.take<uint32_t>(nb + 1);
+    b.hpad = cv.take<uint32_t>(nb + 1);
+    b.offp = cv.take<uint32_t>(nb + 1);
     b.scan_ws = cv.take<char>(scan_workspace_bytes(nb + 1));
-    if (p.L2 > 0) b.mid = cv.take<uint64_t>(n ? n : 1);
-    b.fine = cv.take<uint64_t>(n ? n : 1);
+    if (p.L2 > 0) b.mid = cv.take<uint64_t>(n + nb);
+    b.fine = cv.take<uint64_t>(n + nb);
     return b;
+}
+
+static __global__ void k_round_even(const uint32_t *h, uint32_t m, uint32_t *hp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) hp[i] = (h[i] + 1u) & ~1u;
 }
 
 // Histogram, offsets and the exact split(s) of (kcol, pcol) into b.fine.
@@ -405,10 +415,13 @@ inline int launch_exact_splits(const SplitPlan &p, const ExactSplitBufs &b, cons
     k_split_hist<<<hg, 512, (size_t)nb * 4, st>>>(kcol, n, 32 - p.L, nb, b.hist);
     QO_LAUNCHED();
     QO_TRY(launch_scan_excl_u32(b.hist, b.off, nb + 1, b.scan_ws, st));
+    k_round_even<<<(nb + 1 + 255) / 256, 256, 0, st>>>(b.hist, nb + 1, b.hpad);
+    QO_LAUNCHED();
+    QO_TRY(launch_scan_excl_u32(b.hpad, b.offp, nb + 1, b.scan_ws, st));
     const bool vec = ((uintptr_t)kcol % 16 == 0) && ((uintptr_t)pcol % 16 == 0);
     const uint32_t partsA = 1u << p.L1;
     const size_t tiles = (n + kSplitTile - 1) / kSplitTile;
-    const ExactLayout la{b.off, p.L2}, lf{b.off, 0};
+    const ExactLayout la{b.offp, b.off, p.L2}, lf{b.offp, b.off, 0};
     AppendOverflow none{nullptr, nullptr};  // exact regions: never called
     uint64_t *outA = p.L2 > 0 ? b.mid : b.fine;
     k_split_cols<false><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, nullptr, n, 0, 0, vec, 32 - p.L1, partsA, outA,
