@@ -249,6 +249,128 @@ def run_filter_agg(args, qo, lib) -> dict:
     return res
 
 
+def _time_op(st, steps, warmup, fn):
+    """Average ms of fn() on stream st over `steps` calls (HIP events around each)."""
+    from query_on_opencl_fpga_amd.device import Event
+    for _ in range(warmup):
+        fn()
+    st.sync()
+    evs = [(Event(), Event()) for _ in range(steps)]
+    for a, b in evs:
+        a.record(st)
+        fn()
+        b.record(st)
+    st.sync()
+    return float(np.mean([a.elapsed_ms(b) for a, b in evs]))
+
+
+def run_ops(args, qo, lib) -> dict:
+    """The remaining operators of the reference's path, each at a large size:
+    filter compaction, exclusive scan, Record sort, the reference's sort-based
+    group-by pipeline (sort -> startPos -> four aggregates), and filter + join +
+    sort (hj_filter_sort) on the join workload.  Each row: ms per call (HIP
+    events on the call's stream) and algorithmic GB/s against the HBM peak."""
+    import ctypes as C
+    from query_on_opencl_fpga_amd import data
+    from query_on_opencl_fpga_amd.device import DeviceBuffer, Stream
+
+    st = Stream()
+    rows = []
+
+    def row(name, n, ms, alg, note):
+        gbs = alg / (ms * 1e-3) / 1e9
+        rows.append({"op": name, "n": n, "ms": ms, "alg_bytes": alg, "GBps_alg": gbs,
+                     "frac_hbm": gbs / HBM_PEAK_GBS, "alg_bytes_formula": note})
+        log(f"[bench] ops: {name} n={n}: {ms:.3f} ms, {gbs:.0f} GB/s alg")
+
+    seed = SEED_BASE + 5
+    # Filter compaction over Records (.y uniform in [0, 2^31), ~50 % selected).
+    n = args.n_filter
+    x = data.gen_uniform(n, seed, 0, 0, 0)
+    y = data.gen_uniform(n, seed, 1, 0, 1 << 31)
+    rec = DeviceBuffer(n * 8)
+    qo._lib.call("qo_pack_records", x.ptr, y.ptr, n, rec.ptr, None)
+    x.free()
+    out = DeviceBuffer(n * 8)
+    cnt = DeviceBuffer(8)
+    ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+        "qo_filter_records", rec.ptr, n, 0, (1 << 30) - 1, out.ptr, cnt.ptr, st.handle))
+    m = int(cnt.to_numpy(np.uint64, 1)[0])
+    row("filter_records (map + scan + stable compaction)", n, ms, 8 * n + 8 * m, "8*N + 8*selected")
+    out.free()
+    rec.free()
+    # Exclusive scan of int32.
+    sc = DeviceBuffer(n * 4)
+    ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+        "qo_scan_exclusive_i32", y.ptr, sc.ptr, n, st.handle))
+    row("scan_exclusive_i32", n, ms, 8 * n, "4*N read + 4*N written")
+    sc.free()
+    y.free()
+    # Record sort and the reference's sort-based group-by pipeline.
+    ns = args.n_sort
+    G = 1 << 20
+    x = data.gen_uniform(ns, seed, 2, 0, 1 << 16)
+    y = data.gen_uniform(ns, seed, 3, 0, G)
+    src = DeviceBuffer(ns * 8)
+    qo._lib.call("qo_pack_records", x.ptr, y.ptr, ns, src.ptr, None)
+    x.free()
+    y.free()
+    work = DeviceBuffer(ns * 8)
+
+    def sort_once():
+        qo._lib.call("qo_copy", work.ptr, src.ptr, ns * 8, st.handle)
+        qo._lib.call("qo_sort_records", work.ptr, ns, st.handle)
+
+    ms_copy = _time_op(st, args.steps, args.warmup,
+                       lambda: qo._lib.call("qo_copy", work.ptr, src.ptr, ns * 8, st.handle))
+    ms = _time_op(st, args.steps, args.warmup, sort_once) - ms_copy
+    row("sort_records (LSD radix on .y, 4 passes; the copy restoring the input is subtracted)", ns, ms,
+        16 * ns, "8*N read + 8*N written")
+    sp = DeviceBuffer(ns * 4)
+    num = DeviceBuffer(8)
+    aggs = [DeviceBuffer(G * 4) for _ in range(4)]
+
+    def groupby_once():
+        sort_once()
+        qo._lib.call("qo_group_startpos", work.ptr, ns, sp.ptr, num.ptr, st.handle)
+        for op in range(4):
+            qo._lib.call("qo_aggregate_groups", work.ptr, ns, sp.ptr, G, op, aggs[op].ptr, st.handle)
+
+    groupby_once()
+    st.sync()
+    g = int(num.to_numpy(np.uint64, 1)[0])
+    if g != G:
+        raise SystemExit(f"sort-based group-by found {g} groups, expected {G}")
+    ms = _time_op(st, args.steps, args.warmup, groupby_once) - ms_copy
+    row("sort-based group-by (sort + startPos + SUM/MAX/MIN/AVG), G = 2^20", ns, ms, 8 * ns + 28 * G,
+        "8*N + 28*G")
+    for b in aggs + [sp, num, work, src]:
+        b.free()
+    # Filter + join + sort (hj_filter_sort) on the join workload.
+    nr, nsj = args.nr, args.ns
+    t = data.JoinTables(nr, nsj, SEED_BASE + 2)
+    tb = lib.qo_join_table_bytes(nr)
+    table = DeviceBuffer(tb)
+    outs = [DeviceBuffer(nsj * 4) for _ in range(3)]
+    mm = C.c_uint64()
+    ms = _time_op(st, max(1, args.steps // 2), 1, lambda: qo._lib.call(
+        "qo_hj_filter_sort", table.ptr, tb, t.rkey.ptr, t.rpay.ptr, nr, t.skey.ptr, t.spay.ptr, t.sfilt.ptr, nsj,
+        INT_MIN, 49, outs[0].ptr, outs[1].ptr, outs[2].ptr, nsj, C.byref(mm), st.handle))
+    mj = mm.value
+    row("hj_filter_sort (build, filtered probe, sort by (key, s_payload))", nsj, ms,
+        16 * nr + 12 * nsj + 12 * mj + 24 * mj, "16*|R| + 12*|S| + 12*M + 24*M")
+    for b in outs + [table]:
+        b.free()
+    t.free()
+    worst = min(rows, key=lambda r: r["frac_hbm"])
+    return {"metric": METRIC, "value": worst["GBps_alg"] * 1e9, "unit": "algorithmic bytes/s (slowest op)",
+            "ms_per_step": worst["ms"], "dtype": "int32",
+            "config": {"workload": "operator set of the reference path at scale", "ops": rows},
+            "roofline": {"bound": "hbm", "kernel": worst["op"], "achieved": worst["GBps_alg"],
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": worst["frac_hbm"], "traffic": None,
+                         "avg_launch_ms": worst["ms"]}}
+
+
 def run_copy(args, qo, lib) -> dict:
     """Plain device copy of a known size: the achievable-HBM reference and the
     calibration kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE."""
@@ -528,7 +650,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="join",
-                    choices=["join", "filter_agg", "groupby", "batched", "served", "copy"])
+                    choices=["join", "filter_agg", "groupby", "batched", "served", "copy", "ops"])
     ap.add_argument("--load", type=int, default=0, help="join table load factor in percent (default: tuned)")
     ap.add_argument("--join-mode", choices=["auto", "partitioned", "direct"], default="auto",
                     help="join build/probe strategy (default: partitioned once the table outgrows one L2)")
@@ -550,6 +672,7 @@ def main() -> int:
     ap.add_argument("--ns", type=int, default=1 << 28)
     ap.add_argument("--n-filter", type=int, default=1 << 28)
     ap.add_argument("--n-groupby", type=int, default=1 << 28)
+    ap.add_argument("--n-sort", type=int, default=1 << 26)
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", type=float, default=None,
@@ -576,7 +699,7 @@ def main() -> int:
         for cls in range(40):
             qo._lib.call("qo_tune_set", 7, cls, 256, 256, 2)
     run = {"join": run_join, "filter_agg": run_filter_agg, "groupby": run_groupby, "copy": run_copy,
-           "batched": run_batched, "served": run_served}
+           "batched": run_batched, "served": run_served, "ops": run_ops}
     if args.config not in run:
         print(f"config {args.config} is not available", file=sys.stderr)
         return 2

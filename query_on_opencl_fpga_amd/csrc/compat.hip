@@ -19,7 +19,7 @@ int launch_group_startpos(const qo_record *sorted, size_t n, int32_t *startPos, 
 size_t group_startpos_ws_bytes(size_t n);
 int launch_aggregate_groups(const qo_record *s, size_t n, const int32_t *startPos, size_t G, int op, int32_t *out,
                             void *ws, hipStream_t st);
-size_t aggregate_ws_bytes(size_t G);
+size_t aggregate_ws_bytes(size_t G, size_t n);
 
 // The reference exits on any OpenCL error (AOCL_Utils.cpp:228-251); here that is opt-in.
 static void compat_fail(const char *where) {
@@ -290,7 +290,7 @@ static int agg_range(const qo_record *s, size_t rLen, const int32_t *startPos, s
         last_end = (size_t)(nxt > 0 ? nxt : 0);
     }
     int status = QO_OK;
-    void *ws = workspace(st, aggregate_ws_bytes(count), &status);
+    void *ws = workspace(st, aggregate_ws_bytes(count, (size_t)rLen), &status);
     if (!ws) return status;
     QO_TRY(launch_aggregate_groups(s, last_end, startPos + first, count, op, out + first, ws, st));
     QO_HIP(hipStreamSynchronize(st));

@@ -548,24 +548,40 @@ __global__ void k_agg_chunks(const int32_t *startPos, uint32_t G, size_t n, uint
 
 QO_DEV int32_t agg_identity(int op) { return op == QO_REDUCE_MIN ? (1 << 30) : 0; }
 
+// Owner group of every work unit below umax: units [cofs[g], cofs[g] +
+// chunks[g]) belong to group g (one thread per group writes its units).
+__global__ void k_agg_units(const uint32_t *cofs, const uint32_t *chunks, uint32_t G, uint32_t *unit_group,
+                            uint32_t umax) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    for (uint32_t q = 0; q < chunks[g] && cofs[g] + q < umax; ++q) unit_group[cofs[g] + q] = (uint32_t)g;
+}
+
+// Group of work unit w: from the owner table, or, for units past it (start
+// positions that overlap, which only a malformed input produces), the last g
+// with cofs[g] <= w.  An empty group shares its cofs with the following group,
+// so the last such g is the non-empty group that owns unit w.
+QO_DEV uint32_t unit_owner(const uint32_t *unit_group, uint32_t umax, const uint32_t *cofs, uint32_t G, size_t w) {
+    if (w < umax) return unit_group[w];
+    uint32_t lo = 0, hi = G - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (cofs[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(256) void k_agg_groups(const qo_record *__restrict__ s, size_t n,
                                                     const int32_t *__restrict__ startPos, uint32_t G,
                                                     const uint32_t *__restrict__ cofs, const uint32_t *__restrict__ chunks,
-                                                    int op, int32_t *acc) {
+                                                    const uint32_t *__restrict__ unit_group, uint32_t umax, int op,
+                                                    int32_t *acc) {
     const uint32_t total = cofs[G - 1] + chunks[G - 1];
     const unsigned lane = lane_id();
     const size_t waves = (size_t)gridDim.x * (blockDim.x / kWave);
     for (size_t w = (size_t)blockIdx.x * (blockDim.x / kWave) + wave_id(); w < total; w += waves) {
-        // Group of work unit w: the last g with cofs[g] <= w.  An empty group
-        // shares its cofs with the following group, so the last such g is the
-        // non-empty group that owns unit w.
-        uint32_t lo = 0, hi = G - 1;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (cofs[mid] <= w) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t g = lo;
+        const uint32_t g = unit_owner(unit_group, umax, cofs, G, w);
         int64_t gs, ge;
         group_range(startPos, g, G, n, gs, ge);
         const int64_t a = gs + (int64_t)(w - cofs[g]) * kAggChunk;
@@ -601,7 +617,12 @@ __global__ void k_agg_average(const int32_t *startPos, uint32_t G, size_t n, con
     out[g] = cnt > 0 ? sum[g] / cnt : 0;
 }
 
-size_t aggregate_ws_bytes(size_t G) { return G * 12 + scan_workspace_bytes(G) + 1024; }
+// Work units: at most one partial 512-row chunk per group beyond n / 512 full ones.
+static size_t agg_units_max(size_t n, size_t G) { return n / kAggChunk + G + 1; }
+
+size_t aggregate_ws_bytes(size_t G, size_t n) {
+    return G * 12 + agg_units_max(n, G) * 4 + scan_workspace_bytes(G) + 2048;
+}
 
 int launch_aggregate_groups(const qo_record *s, size_t n, const int32_t *startPos, size_t G, int op, int32_t *out,
                             void *ws, hipStream_t st) {
@@ -610,6 +631,7 @@ int launch_aggregate_groups(const qo_record *s, size_t n, const int32_t *startPo
     uint32_t *chunks = cv.take<uint32_t>(G);
     uint32_t *cofs = cv.take<uint32_t>(G);
     int32_t *sum = cv.take<int32_t>(G);
+    uint32_t *unit_group = cv.take<uint32_t>(agg_units_max(n, G));
     void *sws = cv.take<char>(scan_workspace_bytes(G));
     int32_t *acc = op == QO_REDUCE_AVERAGE ? sum : out;
     QO_TRY(launch_fill_u32(acc, op == QO_REDUCE_MIN ? (1u << 30) : 0u, G, st));
@@ -617,8 +639,12 @@ int launch_aggregate_groups(const qo_record *s, size_t n, const int32_t *startPo
     k_agg_chunks<<<gb, 256, 0, st>>>(startPos, (uint32_t)G, n, chunks);
     QO_LAUNCHED();
     QO_TRY(launch_scan_excl_u32(chunks, cofs, G, sws, st));
+    const size_t umax = std::min<size_t>(agg_units_max(n, G), 0xFFFFFFFFull);
+    k_agg_units<<<gb, 256, 0, st>>>(cofs, chunks, (uint32_t)G, unit_group, (uint32_t)umax);
+    QO_LAUNCHED();
     const unsigned grid = grid_for((size_t)n / kAggChunk + G, 4, (unsigned)props().cu_count * 16);
-    k_agg_groups<<<grid, 256, 0, st>>>(s, n, startPos, (uint32_t)G, cofs, chunks, op, acc);
+    k_agg_groups<<<grid, 256, 0, st>>>(s, n, startPos, (uint32_t)G, cofs, chunks, unit_group, (uint32_t)umax, op,
+                                       acc);
     QO_LAUNCHED();
     if (op == QO_REDUCE_AVERAGE) {
         k_agg_average<<<gb, 256, 0, st>>>(startPos, (uint32_t)G, n, sum, out);
@@ -681,7 +707,7 @@ int qo_aggregate_groups(const qo_record *sorted, size_t n, const int32_t *startP
     QO_REQUIRE(sorted && startPos && out && numGroups < 0xFFFFFFFFull, "bad aggregate arguments");
     hipStream_t st = stream_of(s);
     int status = QO_OK;
-    void *ws = workspace(st, aggregate_ws_bytes(numGroups), &status);
+    void *ws = workspace(st, aggregate_ws_bytes(numGroups, n), &status);
     if (!ws) return status;
     return launch_aggregate_groups(sorted, n, startPos, numGroups, op, out, ws, st);
 }
