@@ -187,25 +187,31 @@ struct RecVec<1> {
     using T = uint2;
 };
 
-template <int J, int VEC>
-__global__ __launch_bounds__(256) void k_filter_records(const qo_record *__restrict__ rin, size_t n, int32_t lo,
-                                                        int32_t hi, qo_record *__restrict__ rout,
-                                                        uint64_t *status, uint32_t *ticket, uint64_t *d_count,
-                                                        uint32_t num_tiles) {
-    constexpr int NW = 4;
+// Tiles of J * NT * VEC records (8192 with the launch below): one ticket per
+// tile keeps the same-address ticket atomics few.  The selected records are
+// staged in LDS in input order and written out row by row across the block.
+constexpr int kFiltJ = 8;
+constexpr int kFiltThreads = 512;
+
+template <int J, int VEC, int NT>
+__global__ __launch_bounds__(NT) void k_filter_records(const qo_record *__restrict__ rin, size_t n, int32_t lo,
+                                                       int32_t hi, qo_record *__restrict__ rout, uint64_t *status,
+                                                       uint32_t *ticket, uint64_t *d_count, uint32_t num_tiles) {
+    constexpr int NW = NT / kWave;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_tot[J * NW];
     __shared__ uint64_t s_excl;
+    __shared__ qo_record s_out[J * NT * VEC];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t vbase = (size_t)tile * (J * 256);
+    const size_t vbase = (size_t)tile * (J * NT);
     qo_record r[J][VEC];
     uint32_t c[J], ex[J];
     uint32_t fl[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
+        const size_t v = vbase + (size_t)j * NT + threadIdx.x;
         const size_t e0 = v * VEC;
         fl[j] = 0;
         if (e0 + VEC <= n) {
@@ -239,15 +245,17 @@ __global__ __launch_bounds__(256) void k_filter_records(const qo_record *__restr
         uint64_t e = lookback(status, tile, total);
         if (threadIdx.x == 0) s_excl = e;
     }
-    __syncthreads();
-    const uint64_t base = s_excl;
+    // Stage the selected records at their tile-local rank (input order).
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        uint64_t pos = base + ex[j];
+        uint32_t pos = ex[j];
 #pragma unroll
         for (int k = 0; k < VEC; ++k)
-            if (fl[j] & (1u << k)) rout[pos++] = r[j][k];
+            if (fl[j] & (1u << k)) s_out[pos++] = r[j][k];
     }
+    __syncthreads();
+    const uint64_t base = s_excl;
+    for (uint32_t i = threadIdx.x; i < total; i += NT) rout[base + i] = s_out[i];
     if (tile == num_tiles - 1 && threadIdx.x == 0) *d_count = base + total;
 }
 
@@ -255,10 +263,10 @@ static size_t filter_ws_bytes(size_t tiles) { return tiles * 8 + 256; }
 
 static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi, qo_record *rout,
                                  uint64_t *d_count, hipStream_t st) {
-    constexpr int J = 4;
+    constexpr int J = kFiltJ, NT = kFiltThreads;
     const bool vec = ((uintptr_t)rin % 16) == 0;
     const int VEC = vec ? 2 : 1;
-    const size_t per_tile = (size_t)J * 256 * VEC;
+    const size_t per_tile = (size_t)J * NT * VEC;
     const size_t tiles = (n + per_tile - 1) / per_tile;
     QO_REQUIRE(tiles < 0xFFFFFFFFull, "input too large");
     int status = QO_OK;
@@ -269,11 +277,11 @@ static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int
     uint32_t *ticket = cv.take<uint32_t>(1);
     QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
     if (vec)
-        k_filter_records<J, 2><<<(unsigned)tiles, 256, 0, st>>>(rin, n, lo, hi, rout, stat, ticket, d_count,
-                                                                (uint32_t)tiles);
+        k_filter_records<J, 2, NT><<<(unsigned)tiles, NT, 0, st>>>(rin, n, lo, hi, rout, stat, ticket, d_count,
+                                                                   (uint32_t)tiles);
     else
-        k_filter_records<J, 1><<<(unsigned)tiles, 256, 0, st>>>(rin, n, lo, hi, rout, stat, ticket, d_count,
-                                                                (uint32_t)tiles);
+        k_filter_records<J, 1, NT><<<(unsigned)tiles, NT, 0, st>>>(rin, n, lo, hi, rout, stat, ticket, d_count,
+                                                                   (uint32_t)tiles);
     QO_LAUNCHED();
     return QO_OK;
 }
@@ -281,10 +289,13 @@ static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int
 // ---------------------------------------------------------------------------
 // Exclusive scan of 32-bit integers (modulo 2^32), single pass.
 // ---------------------------------------------------------------------------
-template <int J, bool VEC>
-__global__ __launch_bounds__(256) void k_scan_excl(const uint32_t *in, uint32_t *out, size_t n, uint64_t *status,
-                                                   uint32_t *ticket) {
-    constexpr int NW = 4;
+constexpr int kScanJ = 8;
+constexpr int kScanThreads = 512;
+
+template <int J, bool VEC, int NT>
+__global__ __launch_bounds__(NT) void k_scan_excl(const uint32_t *in, uint32_t *out, size_t n, uint64_t *status,
+                                                  uint32_t *ticket) {
+    constexpr int NW = NT / kWave;
     constexpr int VW = VEC ? 4 : 1;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_tot[J * NW];
@@ -292,12 +303,12 @@ __global__ __launch_bounds__(256) void k_scan_excl(const uint32_t *in, uint32_t 
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t vbase = (size_t)tile * (J * 256);
+    const size_t vbase = (size_t)tile * (J * NT);
     uint32_t x[J][VW];
     uint32_t c[J], ex[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
+        const size_t v = vbase + (size_t)j * NT + threadIdx.x;
         const size_t e0 = v * VW;
         if (VEC && e0 + VW <= n) {
             uint4 q = reinterpret_cast<const uint4 *>(in)[v];
@@ -325,7 +336,7 @@ __global__ __launch_bounds__(256) void k_scan_excl(const uint32_t *in, uint32_t 
     const uint32_t base = (uint32_t)s_excl;
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const size_t v = vbase + (size_t)j * 256 + threadIdx.x;
+        const size_t v = vbase + (size_t)j * NT + threadIdx.x;
         const size_t e0 = v * VW;
         uint32_t run = base + ex[j];
         uint32_t o[VW];
@@ -347,24 +358,24 @@ __global__ __launch_bounds__(256) void k_scan_excl(const uint32_t *in, uint32_t 
 
 size_t scan_workspace_bytes(size_t n) {
     // Sized for the unaligned path (one element per load), which has the most tiles.
-    const size_t per_tile = 4 * 256;
+    const size_t per_tile = (size_t)kScanJ * kScanThreads;
     const size_t tiles = (n + per_tile - 1) / per_tile + 1;
     return tiles * 8 + 512;
 }
 
 int launch_scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, void *ws, hipStream_t st) {
     if (n == 0) return QO_OK;
-    constexpr int J = 4;
+    constexpr int J = kScanJ, NT = kScanThreads;
     const bool vec = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
-    const size_t per_tile = (size_t)J * 256 * (vec ? 4 : 1);
+    const size_t per_tile = (size_t)J * NT * (vec ? 4 : 1);
     const size_t tiles = (n + per_tile - 1) / per_tile;
     QO_REQUIRE(tiles < 0xFFFFFFFFull, "scan input too large");
     Carver cv(ws);
     uint64_t *stat = cv.take<uint64_t>(tiles);
     uint32_t *ticket = cv.take<uint32_t>(1);
     QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
-    if (vec) k_scan_excl<J, true><<<(unsigned)tiles, 256, 0, st>>>(in, out, n, stat, ticket);
-    else k_scan_excl<J, false><<<(unsigned)tiles, 256, 0, st>>>(in, out, n, stat, ticket);
+    if (vec) k_scan_excl<J, true, NT><<<(unsigned)tiles, NT, 0, st>>>(in, out, n, stat, ticket);
+    else k_scan_excl<J, false, NT><<<(unsigned)tiles, NT, 0, st>>>(in, out, n, stat, ticket);
     QO_LAUNCHED();
     return QO_OK;
 }
