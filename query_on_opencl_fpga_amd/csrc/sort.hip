@@ -154,6 +154,180 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restric
 
 size_t radix_hist_bytes(uint32_t total_tiles) { return (size_t)total_tiles * 256 * 4 + 256; }
 
+// ---------------------------------------------------------------------------
+// One-sweep passes for a single array: the digit counts of every pass come
+// from one histogram over the keys up front; each pass is then a single
+// kernel that ranks its tile, publishes the tile's per-digit counts and finds
+// the counts of all earlier tiles with a decoupled look-back per digit (thread
+// d of the block walks digit d), so no per-pass histogram read or scan runs.
+// Tiles are numbered by a ticket: every tile a block waits for belongs to a
+// block that is already running.
+// ---------------------------------------------------------------------------
+constexpr int kSweepMaxPass = 8;
+constexpr int kSweepCountBlocks = 512;
+
+// Per-block digit counts of every pass (LDS), one row of npass*256 per block.
+__global__ __launch_bounds__(256) void k_sweep_counts(const uint64_t *__restrict__ keys, RadixSeg sg, int lo_bit,
+                                                      int npass, uint32_t *partial) {
+    __shared__ uint32_t s_h[kSweepMaxPass * 256];
+    for (int i = threadIdx.x; i < npass * 256; i += 256) s_h[i] = 0;
+    __syncthreads();
+    const size_t n = seg_count(sg);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const uint64_t k = ld_stream(keys + sg.base + i);
+        for (int p = 0; p < npass; ++p) atomicAdd(&s_h[p * 256 + ((k >> (lo_bit + 8 * p)) & 255u)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * 256; i += 256) partial[(size_t)blockIdx.x * npass * 256 + i] = s_h[i];
+}
+
+// counts[p*256 + d] = sum over blocks of partial[b][p][d]: exclusive digit starts
+// per pass (the digit order is the output order), written in place.
+__global__ __launch_bounds__(256) void k_sweep_starts(const uint32_t *partial, int nblocks, int npass,
+                                                      uint32_t *starts) {
+    __shared__ uint32_t s_w[4];
+    const int p = blockIdx.x;
+    if (p >= npass) return;
+    uint32_t c = 0;
+    for (int b = 0; b < nblocks; ++b) c += partial[(size_t)b * npass * 256 + p * 256 + threadIdx.x];
+    starts[p * 256 + threadIdx.x] = block_excl_scan256(c, s_w);
+}
+
+template <bool HAS_VAL>
+__global__ __launch_bounds__(256) void k_radix_sweep(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                     RadixSeg sg, int shift, const uint32_t *__restrict__ dstart,
+                                                     uint64_t *status, uint32_t *ticket) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_cnt[4][256];
+    __shared__ uint32_t s_dpre[256];
+    __shared__ uint32_t s_gofs[256];
+    __shared__ uint32_t s_w[4];
+    __shared__ uint64_t s_k[kSortTile];
+    __shared__ uint32_t s_v[HAS_VAL ? kSortTile : 1];
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    for (int i = threadIdx.x; i < 1024; i += 256) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t lt = s_tile;
+    const size_t n = seg_count(sg);
+    if ((size_t)lt * kSortTile >= n) return;  // whole block; no later tile waits for it
+    const unsigned w = wave_id(), lane = lane_id();
+    const size_t wbase = (size_t)lt * kSortTile + (size_t)w * (kSortTile / 4);
+    uint64_t k[kSortItems];
+    uint32_t v[kSortItems];
+    uint32_t rank[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        size_t idx = wbase + (size_t)j * kWave + lane;
+        bool valid = idx < n;
+        k[j] = valid ? kin[sg.base + idx] : ~uint64_t(0);
+        if (HAS_VAL) v[j] = valid ? vin[sg.base + idx] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        size_t idx = wbase + (size_t)j * kWave + lane;
+        bool valid = idx < n;
+        uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+        uint64_t m = match_digit(d, valid);
+        uint32_t pre = s_cnt[w][d];
+        uint32_t r = popc(m & lanemask_lt());
+        if (valid && r == 0) s_cnt[w][d] = pre + popc(m);
+        rank[j] = pre + r;
+    }
+    __syncthreads();
+    {
+        const unsigned d = threadIdx.x;
+        uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
+        uint32_t tot = c0 + c1 + c2 + c3;
+        // Publish this tile's count of digit d, then add up the earlier tiles'.
+        uint64_t *my = status + (size_t)lt * 256 + d;
+        uint64_t excl = 0;
+        if (lt == 0) {
+            st_agent(my, kFlagInc | tot);
+        } else {
+            st_agent(my, kFlagAgg | tot);
+            for (int64_t tp = (int64_t)lt - 1; tp >= 0; --tp) {
+                uint64_t st;
+                while (((st = ld_agent(status + (size_t)tp * 256 + d)) >> 62) == 0) __builtin_amdgcn_s_sleep(1);
+                excl += st & kValMask;
+                if ((st >> 62) == 2) break;
+            }
+            st_agent(my, kFlagInc | ((excl + tot) & kValMask));
+        }
+        s_gofs[d] = dstart[d] + (uint32_t)excl;
+        uint32_t ex = block_excl_scan256(tot, s_w);
+        s_dpre[d] = ex;
+        s_cnt[0][d] = 0;
+        s_cnt[1][d] = c0;
+        s_cnt[2][d] = c0 + c1;
+        s_cnt[3][d] = c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        size_t idx = wbase + (size_t)j * kWave + lane;
+        if (idx < n) {
+            uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+            uint32_t p = s_dpre[d] + s_cnt[w][d] + rank[j];
+            s_k[p] = k[j];
+            if (HAS_VAL) s_v[p] = v[j];
+        }
+    }
+    __syncthreads();
+    const size_t tstart = (size_t)lt * kSortTile;
+    const uint32_t tcount = (uint32_t)((n - tstart) < (size_t)kSortTile ? (n - tstart) : (size_t)kSortTile);
+    for (uint32_t i = threadIdx.x; i < tcount; i += 256) {
+        uint64_t kk = s_k[i];
+        uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        size_t gp = sg.base + (size_t)s_gofs[d] + (i - s_dpre[d]);
+        kout[gp] = kk;
+        if (HAS_VAL) vout[gp] = s_v[i];
+    }
+}
+
+// Workspace of the one-sweep passes over `tiles` tiles and npass passes.
+struct SweepBufs {
+    uint32_t *partial, *starts, *ticket;
+    uint64_t *status;
+};
+static SweepBufs sweep_carve(uint32_t tiles, int npass, Carver &cv) {
+    SweepBufs b;
+    b.partial = cv.take<uint32_t>((size_t)kSweepCountBlocks * npass * 256);
+    b.starts = cv.take<uint32_t>((size_t)npass * 256);
+    b.ticket = cv.take<uint32_t>(64);
+    b.status = cv.take<uint64_t>((size_t)tiles * 256);
+    return b;
+}
+
+static int launch_radix_sweep(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &sg,
+                              int lo_bit, int hi_bit, const SweepBufs &b, hipStream_t st, bool *in_alt) {
+    *in_alt = false;
+    const int npass = (hi_bit - lo_bit + 7) / 8;
+    QO_REQUIRE(npass >= 1 && npass <= kSweepMaxPass, "bad bit range");
+    k_sweep_counts<<<kSweepCountBlocks, 256, 0, st>>>(keys, sg, lo_bit, npass, b.partial);
+    QO_LAUNCHED();
+    k_sweep_starts<<<npass, 256, 0, st>>>(b.partial, kSweepCountBlocks, npass, b.starts);
+    QO_LAUNCHED();
+    uint64_t *ksrc = keys, *kdst = kalt;
+    uint32_t *vsrc = vals, *vdst = valt;
+    for (int p = 0; p < npass; ++p) {
+        QO_TRY(launch_fill_u64(b.status, 0, (size_t)sg.ntile * 256, st));
+        QO_TRY(launch_fill_u32(b.ticket, 0, 1, st));
+        const int sh = lo_bit + 8 * p;
+        if (vals)
+            k_radix_sweep<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, sh, b.starts + p * 256,
+                                                          b.status, b.ticket);
+        else
+            k_radix_sweep<false><<<sg.ntile, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, sg, sh, b.starts + p * 256,
+                                                           b.status, b.ticket);
+        QO_LAUNCHED();
+        std::swap(ksrc, kdst);
+        std::swap(vsrc, vdst);
+        *in_alt = !*in_alt;
+    }
+    return QO_OK;
+}
+
 int launch_radix_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &seg0,
                           const RadixSeg *d_segs, int nseg, uint32_t total_tiles, int lo_bit, int hi_bit,
                           uint32_t *hist, void *scan_ws, hipStream_t st, bool *in_alt) {
@@ -182,30 +356,52 @@ int launch_radix_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32
 
 static uint32_t radix_tiles(size_t n) { return (uint32_t)((n + kSortTile - 1) / kSortTile); }
 
+// Key + value sorts take the one-sweep passes; key-only sorts, whose tiles are
+// light enough that the per-digit look-back costs more than the histogram read
+// it replaces (measured: a 2^26-Record sort 2.41 vs 1.98 ms), keep the
+// histogram / scan / scatter passes.
+static void radix_carve(size_t n, bool with_vals, Carver &cv, uint64_t **kalt, uint32_t **valt, uint32_t **hist,
+                        void **scan_ws, SweepBufs *sb) {
+    const uint32_t tiles = radix_tiles(n) + 1;
+    *kalt = cv.take<uint64_t>(n ? n : 1);
+    *valt = with_vals ? cv.take<uint32_t>(n ? n : 1) : nullptr;
+    if (with_vals) {
+        *sb = sweep_carve(tiles, kSweepMaxPass, cv);
+    } else {
+        *hist = cv.take<uint32_t>((size_t)tiles * 256);
+        *scan_ws = cv.take<char>(scan_workspace_bytes((size_t)tiles * 256));
+    }
+}
+
 size_t radix_workspace_bytes(size_t n, bool with_vals) {
-    size_t tiles = radix_tiles(n) + 1;
-    size_t b = 0;
-    b += n * 8 + 256;                    // alternate keys
-    if (with_vals) b += n * 4 + 256;     // alternate values
-    b += radix_hist_bytes((uint32_t)tiles) + 256;
-    b += scan_workspace_bytes(tiles * 256) + 256;
-    return b;
+    Carver cv(nullptr);
+    uint64_t *kalt;
+    uint32_t *valt, *hist;
+    void *scan_ws;
+    SweepBufs sb;
+    radix_carve(n, with_vals, cv, &kalt, &valt, &hist, &scan_ws, &sb);
+    return cv.off + 256;
 }
 
 static int radix_impl(uint64_t *keys, uint32_t *vals, size_t n, const uint64_t *d_n, int lo_bit, int hi_bit,
                       void *ws, hipStream_t st) {
     if (n == 0) return QO_OK;
     QO_REQUIRE(n < 0xFFFFFFFFull, "sort input too large");
+    QO_REQUIRE(lo_bit >= 0 && hi_bit <= 64 && lo_bit < hi_bit, "bad bit range");
     const uint32_t tiles = radix_tiles(n);
     Carver cv(ws);
-    uint64_t *kalt = cv.take<uint64_t>(n);
-    uint32_t *valt = vals ? cv.take<uint32_t>(n) : nullptr;
-    uint32_t *hist = cv.take<uint32_t>((size_t)tiles * 256);
-    void *scan_ws = cv.take<char>(scan_workspace_bytes((size_t)tiles * 256));
+    uint64_t *kalt = nullptr;
+    uint32_t *valt = nullptr, *hist = nullptr;
+    void *scan_ws = nullptr;
+    SweepBufs sb{};
+    radix_carve(n, vals != nullptr, cv, &kalt, &valt, &hist, &scan_ws, &sb);
     RadixSeg seg{0, n, d_n, 0, tiles};
     bool in_alt = false;
-    QO_TRY(launch_radix_segments(keys, vals, kalt, valt, seg, nullptr, 1, tiles, lo_bit, hi_bit, hist, scan_ws, st,
-                                 &in_alt));
+    if (vals)
+        QO_TRY(launch_radix_sweep(keys, vals, kalt, valt, seg, lo_bit, hi_bit, sb, st, &in_alt));
+    else
+        QO_TRY(launch_radix_segments(keys, nullptr, kalt, nullptr, seg, nullptr, 1, tiles, lo_bit, hi_bit, hist,
+                                     scan_ws, st, &in_alt));
     if (in_alt) {
         // Odd number of passes: the result is in the alternate buffers.  The
         // count may only be known on the device, so copy the full capacity.
