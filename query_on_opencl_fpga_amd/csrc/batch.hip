@@ -5,9 +5,10 @@
 // persistent kernels pulling tiles from a work queue (an atomic ticket); a tile
 // maps to its query through the per-query first-tile table, so a 4K-row query
 // and a 4M-row query share the same launch.  The probe writes sort-ready
-// (key, s_payload) 64-bit keys with the r_payload beside them; a segmented LSD
-// radix sort (one segment per query, element counts read from device memory)
-// orders every query's matches, and an unpack kernel writes the SoA outputs.
+// (key, s_payload) 64-bit keys with the r_payload beside them; a segmented
+// one-sweep LSD radix sort (one segment per query, element counts read from
+// device memory, per-query digit counts up front, one kernel per pass) orders
+// every query's matches, and an unpack kernel writes the SoA outputs.
 // The stage sequence (fills included, no memset nodes) is captured once into a
 // graph at creation and replayed by qo_batch_launch.
 //
@@ -133,6 +134,8 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
     p.nq = nq;
     p.hq.assign(nq, BQuery{});
     p.hs.assign(nq, RadixSeg{});
+    p.hchunks.clear();
+    p.hchunk0.assign(1, 0u);
     p.total_slots = p.total_match = 0;
     uint64_t rt = 0, stt = 0, mt = 0;
     const int load = std::max(10, std::min(90, tune_get(TUNE_PROBE, 1 << 20).param));
@@ -161,6 +164,9 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
         rt += q.nrtile;
         stt += q.nstile;
         mt += q.nmtile;
+        for (uint32_t t0 = 0; t0 < q.nmtile; t0 += kSweepChunkTiles)
+            p.hchunks.push_back(SweepChunk{(uint32_t)i, t0, std::min(q.nmtile, t0 + kSweepChunkTiles)});
+        p.hchunk0.push_back((uint32_t)p.hchunks.size());
         p.total_slots += 32 + q.table_cap;  // 256-byte header = 32 slots
         p.total_match += (u.out_capacity + 31) / 32 * 32;
     }
@@ -181,8 +187,9 @@ static void plan_carve(JoinBatchPlan &p, Carver &cv) {
     p.mvals = cv.take<uint32_t>(p.total_match);
     p.kalt = cv.take<uint64_t>(p.total_match);
     p.valt = cv.take<uint32_t>(p.total_match);
-    p.hist = cv.take<uint32_t>((size_t)p.mt * 256 + 1);
-    p.scan_ws = cv.take<char>(scan_workspace_bytes((size_t)p.mt * 256 + 1));
+    p.d_chunks = cv.take<SweepChunk>(p.hchunks.size() + 1);
+    p.d_chunk0 = cv.take<uint32_t>(p.hchunk0.size());
+    p.sort_ws = cv.take<char>(radix_sweep_seg_bytes(p.mt, p.nq, (uint32_t)p.hchunks.size()));
 }
 
 size_t join_plan_bytes(JoinBatchPlan &p) {
@@ -208,6 +215,11 @@ void join_plan_bind(JoinBatchPlan &p, void *arena) {
 int join_plan_upload(const JoinBatchPlan &p, hipStream_t st) {
     QO_HIP(hipMemcpyAsync(p.d_q, p.hq.data(), sizeof(BQuery) * p.nq, hipMemcpyHostToDevice, st));
     QO_HIP(hipMemcpyAsync(p.d_segs, p.hs.data(), sizeof(RadixSeg) * p.nq, hipMemcpyHostToDevice, st));
+    if (!p.hchunks.empty())
+        QO_HIP(hipMemcpyAsync(p.d_chunks, p.hchunks.data(), sizeof(SweepChunk) * p.hchunks.size(),
+                              hipMemcpyHostToDevice, st));
+    QO_HIP(hipMemcpyAsync(p.d_chunk0, p.hchunk0.data(), sizeof(uint32_t) * p.hchunk0.size(), hipMemcpyHostToDevice,
+                          st));
     return QO_OK;
 }
 
@@ -229,9 +241,8 @@ int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
     }
     if (p.mt) {
         bool in_alt = false;
-        // With one query the sort takes its segment by value (hs[0]).
-        QO_TRY(launch_radix_segments(p.mkeys, p.mvals, p.kalt, p.valt, p.hs[0], p.d_segs, p.nq, p.mt, 0, 64, p.hist,
-                                     p.scan_ws, st, &in_alt));
+        QO_TRY(launch_radix_sweep_segments(p.mkeys, p.mvals, p.kalt, p.valt, p.d_segs, p.nq, p.mt, p.d_chunks,
+                                           (uint32_t)p.hchunks.size(), p.d_chunk0, 0, 64, p.sort_ws, st, &in_alt));
         k_b_unpack<<<p.mt, 256, 0, st>>>(p.d_q, p.nq, in_alt ? p.kalt : p.mkeys, in_alt ? p.valt : p.mvals);
         QO_LAUNCHED();
     }

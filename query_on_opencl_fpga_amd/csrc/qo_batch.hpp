@@ -32,6 +32,8 @@ struct JoinBatchPlan {
     int nq = 0;
     std::vector<BQuery> hq;
     std::vector<RadixSeg> hs;
+    std::vector<SweepChunk> hchunks;  // up-front digit-count blocks of the sort
+    std::vector<uint32_t> hchunk0;    // query q owns chunks [hchunk0[q], hchunk0[q + 1])
     uint64_t total_slots = 0, total_match = 0;
     uint32_t rt = 0, stt = 0, mt = 0;
     size_t small_words = 0;
@@ -41,8 +43,10 @@ struct JoinBatchPlan {
     BQuery *d_q = nullptr;
     RadixSeg *d_segs = nullptr;
     uint64_t *mkeys = nullptr, *kalt = nullptr;
-    uint32_t *mvals = nullptr, *valt = nullptr, *hist = nullptr;
-    void *scan_ws = nullptr;
+    uint32_t *mvals = nullptr, *valt = nullptr;
+    SweepChunk *d_chunks = nullptr;
+    uint32_t *d_chunk0 = nullptr;
+    void *sort_ws = nullptr;
     unsigned long long *counts = nullptr;
 };
 

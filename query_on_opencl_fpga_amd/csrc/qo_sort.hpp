@@ -22,6 +22,22 @@ struct RadixSeg {
 };
 
 size_t radix_hist_bytes(uint32_t total_tiles);
+
+// Tiles [t0, t1) (segment-local) of segment `seg`: one block of the up-front
+// digit count of a segmented one-sweep sort.
+struct SweepChunk {
+    uint32_t seg, t0, t1;
+};
+constexpr uint32_t kSweepChunkTiles = 16;
+// Workspace of launch_radix_sweep_segments.
+size_t radix_sweep_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks);
+// Segmented one-sweep sort by bits [lo_bit, hi_bit): digit counts per segment
+// from the chunks (segment s owns chunks [chunk0[s], chunk0[s + 1])), then one
+// kernel per pass.  On return *in_alt tells whether the data ended in kalt / valt.
+int launch_radix_sweep_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
+                                const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
+                                uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
+                                hipStream_t st, bool *in_alt);
 // Sorts every segment by bits [lo_bit, hi_bit) of the keys.  With nseg == 1 the
 // segment is passed by value (seg0) and d_segs may be null.  On return *in_alt
 // tells whether the sorted data ended in kalt / valt (odd pass count).

@@ -13,6 +13,8 @@
 // driving bitonicSortShared / bitonicMergeGlobal (kernel/aggregation/1/
 // group_by_cu_op_dp.cl:89-291): O(log^2 N) in-place passes over the array.
 // Here a Record sort is 4 passes over .y; ties keep input order (stable).
+#include <algorithm>
+
 #include "qo_device.hpp"
 #include "qo_internal.hpp"
 #include "qo_sort.hpp"
@@ -40,6 +42,18 @@ QO_DEV RadixSeg seg_of(const RadixSeg &seg0, const RadixSeg *segs, int nseg, uin
         else hi = mid - 1;
     }
     return segs[lo];
+}
+
+// Index of the segment that owns global tile `tile` (0 for a single segment).
+QO_DEV int seg_index_of(const RadixSeg *segs, int nseg, uint32_t tile) {
+    if (nseg <= 1) return 0;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 QO_DEV size_t seg_count(const RadixSeg &s) {
@@ -188,15 +202,54 @@ __global__ __launch_bounds__(256) void k_sweep_starts(const uint32_t *partial, i
     __shared__ uint32_t s_w[4];
     const int p = blockIdx.x;
     if (p >= npass) return;
+    const uint32_t *col = partial + p * 256 + threadIdx.x;
+    const size_t row = (size_t)npass * 256;
+    uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // independent sums: loads stay in flight together
+    int b = 0;
+    for (; b + 8 <= nblocks; b += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] += col[(size_t)(b + u) * row];
+    for (; b < nblocks; ++b) c[0] += col[(size_t)b * row];
+    const uint32_t t = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7];
+    starts[p * 256 + threadIdx.x] = block_excl_scan256(t, s_w);
+}
+
+// Segmented digit counts: block c counts tiles [t0, t1) of segment `seg` of
+// chunk c (segment-local tile numbers), one row of npass*256 per chunk.
+__global__ __launch_bounds__(256) void k_sweep_counts_seg(const uint64_t *__restrict__ keys, const RadixSeg *segs,
+                                                          const SweepChunk *chunks, int lo_bit, int npass,
+                                                          uint32_t *partial) {
+    __shared__ uint32_t s_h[kSweepMaxPass * 256];
+    for (int i = threadIdx.x; i < npass * 256; i += 256) s_h[i] = 0;
+    __syncthreads();
+    const SweepChunk ch = chunks[blockIdx.x];
+    const RadixSeg sg = segs[ch.seg];
+    const size_t n = seg_count(sg);
+    const size_t e1 = std::min<size_t>((size_t)ch.t1 * kSortTile, n);
+    for (size_t i = (size_t)ch.t0 * kSortTile + threadIdx.x; i < e1; i += 256) {
+        const uint64_t k = ld_stream(keys + sg.base + i);
+        for (int p = 0; p < npass; ++p) atomicAdd(&s_h[p * 256 + ((k >> (lo_bit + 8 * p)) & 255u)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * 256; i += 256) partial[(size_t)blockIdx.x * npass * 256 + i] = s_h[i];
+}
+
+// Per segment and pass: exclusive digit starts from the segment's chunk rows
+// [chunk0[s], chunk0[s + 1]).  Block b = s * npass + p.
+__global__ __launch_bounds__(256) void k_sweep_starts_seg(const uint32_t *partial, const uint32_t *chunk0, int npass,
+                                                          uint32_t *starts) {
+    __shared__ uint32_t s_w[4];
+    const int s = blockIdx.x / npass, p = blockIdx.x % npass;
     uint32_t c = 0;
-    for (int b = 0; b < nblocks; ++b) c += partial[(size_t)b * npass * 256 + p * 256 + threadIdx.x];
-    starts[p * 256 + threadIdx.x] = block_excl_scan256(c, s_w);
+    for (uint32_t r = chunk0[s]; r < chunk0[s + 1]; ++r) c += partial[(size_t)r * npass * 256 + p * 256 + threadIdx.x];
+    starts[(size_t)blockIdx.x * 256 + threadIdx.x] = block_excl_scan256(c, s_w);
 }
 
 template <bool HAS_VAL>
 __global__ __launch_bounds__(256) void k_radix_sweep(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                     RadixSeg sg, int shift, const uint32_t *__restrict__ dstart,
+                                                     RadixSeg seg0, const RadixSeg *segs, int nseg, int shift,
+                                                     const uint32_t *__restrict__ starts, uint32_t sstride,
                                                      uint64_t *status, uint32_t *ticket) {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_cnt[4][256];
@@ -208,9 +261,13 @@ __global__ __launch_bounds__(256) void k_radix_sweep(const uint64_t *__restrict_
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     for (int i = threadIdx.x; i < 1024; i += 256) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
-    const uint32_t lt = s_tile;
+    const uint32_t tile = s_tile;  // global tile; the segment's tiles are [sg.tile0, sg.tile0 + sg.ntile)
+    const int si = seg_index_of(segs, nseg, tile);
+    const RadixSeg sg = segs ? segs[si] : seg0;
+    const uint32_t lt = tile - sg.tile0;
+    const uint32_t *dstart = starts + (size_t)si * sstride;
     const size_t n = seg_count(sg);
-    if ((size_t)lt * kSortTile >= n) return;  // whole block; no later tile waits for it
+    if ((size_t)lt * kSortTile >= n) return;  // whole block; no later tile of the segment waits for it
     const unsigned w = wave_id(), lane = lane_id();
     const size_t wbase = (size_t)lt * kSortTile + (size_t)w * (kSortTile / 4);
     uint64_t k[kSortItems];
@@ -240,13 +297,13 @@ __global__ __launch_bounds__(256) void k_radix_sweep(const uint64_t *__restrict_
         uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
         uint32_t tot = c0 + c1 + c2 + c3;
         // Publish this tile's count of digit d, then add up the earlier tiles'.
-        uint64_t *my = status + (size_t)lt * 256 + d;
+        uint64_t *my = status + (size_t)tile * 256 + d;
         uint64_t excl = 0;
         if (lt == 0) {
             st_agent(my, kFlagInc | tot);
         } else {
             st_agent(my, kFlagAgg | tot);
-            for (int64_t tp = (int64_t)lt - 1; tp >= 0; --tp) {
+            for (int64_t tp = (int64_t)tile - 1; tp >= (int64_t)sg.tile0; --tp) {
                 uint64_t st;
                 while (((st = ld_agent(status + (size_t)tp * 256 + d)) >> 62) == 0) __builtin_amdgcn_s_sleep(1);
                 excl += st & kValMask;
@@ -304,9 +361,11 @@ static int launch_radix_sweep(uint64_t *keys, uint32_t *vals, uint64_t *kalt, ui
     *in_alt = false;
     const int npass = (hi_bit - lo_bit + 7) / 8;
     QO_REQUIRE(npass >= 1 && npass <= kSweepMaxPass, "bad bit range");
-    k_sweep_counts<<<kSweepCountBlocks, 256, 0, st>>>(keys, sg, lo_bit, npass, b.partial);
+    // At least 64K keys per counting block (capacity bound when the count is on the device).
+    const int cblocks = (int)std::min<uint64_t>(kSweepCountBlocks, std::max<uint64_t>(1, (sg.cap + 65535) / 65536));
+    k_sweep_counts<<<cblocks, 256, 0, st>>>(keys, sg, lo_bit, npass, b.partial);
     QO_LAUNCHED();
-    k_sweep_starts<<<npass, 256, 0, st>>>(b.partial, kSweepCountBlocks, npass, b.starts);
+    k_sweep_starts<<<npass, 256, 0, st>>>(b.partial, cblocks, npass, b.starts);
     QO_LAUNCHED();
     uint64_t *ksrc = keys, *kdst = kalt;
     uint32_t *vsrc = vals, *vdst = valt;
@@ -315,11 +374,11 @@ static int launch_radix_sweep(uint64_t *keys, uint32_t *vals, uint64_t *kalt, ui
         QO_TRY(launch_fill_u32(b.ticket, 0, 1, st));
         const int sh = lo_bit + 8 * p;
         if (vals)
-            k_radix_sweep<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, sh, b.starts + p * 256,
-                                                          b.status, b.ticket);
+            k_radix_sweep<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, nullptr, 1, sh,
+                                                          b.starts + p * 256, 0, b.status, b.ticket);
         else
-            k_radix_sweep<false><<<sg.ntile, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, sg, sh, b.starts + p * 256,
-                                                           b.status, b.ticket);
+            k_radix_sweep<false><<<sg.ntile, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, sg, nullptr, 1, sh,
+                                                           b.starts + p * 256, 0, b.status, b.ticket);
         QO_LAUNCHED();
         std::swap(ksrc, kdst);
         std::swap(vsrc, vdst);
@@ -355,6 +414,54 @@ int launch_radix_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32
 }
 
 static uint32_t radix_tiles(size_t n) { return (uint32_t)((n + kSortTile - 1) / kSortTile); }
+
+size_t radix_sweep_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks) {
+    Carver cv(nullptr);
+    cv.take<uint32_t>((size_t)nchunks * kSweepMaxPass * 256);
+    cv.take<uint32_t>((size_t)nseg * kSweepMaxPass * 256);
+    cv.take<uint32_t>(64);
+    cv.take<uint64_t>((size_t)total_tiles * 256 + 1);
+    return cv.off + 256;
+}
+
+int launch_radix_sweep_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
+                                const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
+                                uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
+                                hipStream_t st, bool *in_alt) {
+    *in_alt = false;
+    if (total_tiles == 0 || nchunks == 0) return QO_OK;
+    const int npass = (hi_bit - lo_bit + 7) / 8;
+    QO_REQUIRE(npass >= 1 && npass <= kSweepMaxPass && lo_bit >= 0 && hi_bit <= 64, "bad bit range");
+    QO_REQUIRE(nseg >= 1 && (size_t)nseg * npass < 0x7FFFFFFFull, "bad segment count");
+    Carver cv(ws);
+    uint32_t *partial = cv.take<uint32_t>((size_t)nchunks * npass * 256);
+    uint32_t *starts = cv.take<uint32_t>((size_t)nseg * npass * 256);
+    uint32_t *ticket = cv.take<uint32_t>(64);
+    uint64_t *status = cv.take<uint64_t>((size_t)total_tiles * 256 + 1);
+    k_sweep_counts_seg<<<nchunks, 256, 0, st>>>(keys, d_segs, d_chunks, lo_bit, npass, partial);
+    QO_LAUNCHED();
+    k_sweep_starts_seg<<<(unsigned)(nseg * npass), 256, 0, st>>>(partial, d_chunk0, npass, starts);
+    QO_LAUNCHED();
+    const RadixSeg none{0, 0, nullptr, 0, 0};
+    uint64_t *ksrc = keys, *kdst = kalt;
+    uint32_t *vsrc = vals, *vdst = valt;
+    for (int p = 0; p < npass; ++p) {
+        QO_TRY(launch_fill_u64(status, 0, (size_t)total_tiles * 256, st));
+        QO_TRY(launch_fill_u32(ticket, 0, 1, st));
+        const int sh = lo_bit + 8 * p;
+        if (vals)
+            k_radix_sweep<true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, none, d_segs, nseg, sh,
+                                                             starts + p * 256, (uint32_t)npass * 256, status, ticket);
+        else
+            k_radix_sweep<false><<<total_tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, none, d_segs, nseg, sh,
+                                                              starts + p * 256, (uint32_t)npass * 256, status, ticket);
+        QO_LAUNCHED();
+        std::swap(ksrc, kdst);
+        std::swap(vsrc, vdst);
+        *in_alt = !*in_alt;
+    }
+    return QO_OK;
+}
 
 // Key + value sorts take the one-sweep passes; key-only sorts, whose tiles are
 // light enough that the per-digit look-back costs more than the histogram read
