@@ -166,8 +166,8 @@ def run_join(args, qo, lib) -> dict:
     qo._lib.call("qo_join_strategy", tb, ns, mode.value, C.byref(part), C.byref(bits), C.byref(splits))
     table_kind = (f"key-range table ({used.value} slots, home slot = key - min key)" if mode.value == 1
                   else f"hash-mixed table ({used.value} slots)")
-    if part.value and mode.value == 1:
-        kernel = "probe stage (one qo_hash_probe call): k_split_cols + k_range_probe"
+    if part.value and splits.value == 1:
+        kernel = "probe stage (one qo_hash_probe call): k_split_cols + k_l2_probe"
         strategy = f"{table_kind}; probe: one split into {1 << bits.value} buckets, XCD-local L2 lookups"
     elif part.value:
         kernel = ("probe stage (one qo_hash_probe call): k_split_cols"
@@ -652,7 +652,7 @@ def main() -> int:
     ap.add_argument("--config", default="join",
                     choices=["join", "filter_agg", "groupby", "batched", "served", "copy", "ops"])
     ap.add_argument("--load", type=int, default=0, help="join table load factor in percent (default: tuned)")
-    ap.add_argument("--join-mode", choices=["auto", "partitioned", "direct"], default="auto",
+    ap.add_argument("--join-mode", choices=["auto", "partitioned", "direct", "lds"], default="auto",
                     help="join build/probe strategy (default: partitioned once the table outgrows one L2)")
     ap.add_argument("--slice-kib", type=int, default=0, help="partitioned join: table slice per partition")
     ap.add_argument("--no-key-range", action="store_true",
@@ -690,7 +690,7 @@ def main() -> int:
     log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']} LDS/block={info['lds_bytes_per_block']}")
     if args.join_mode != "auto" or args.slice_kib:
         import ctypes as C
-        mode = {"auto": 0, "partitioned": 1, "direct": 2}[args.join_mode]
+        mode = {"auto": 0, "partitioned": 1, "direct": 2, "lds": 3}[args.join_mode]
         for cls in range(40):
             b, i, p = C.c_int(), C.c_int(), C.c_int()
             qo._lib.call("qo_tune_get", 5, cls, C.byref(b), C.byref(i), C.byref(p))

@@ -148,12 +148,12 @@ int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t grou
 /* ---- hash join ------------------------------------------------------------ */
 /* Bytes of device memory a build table for `build_rows` rows needs. */
 size_t qo_join_table_bytes(size_t build_rows);
-/* Strategy the build/probe of n rows against a table of table_bytes takes for
- * a table of the given mode (0 hash-mixed, 1 key-range; see qo_join_table_info):
- * *partitioned = 1 for a partitioned path (hash-mixed: per-slice LDS tables,
- * *splits = 1 or 2 split passes; key-range: one split, then L2-resident table
- * slices), *bits = log2 of the partition count; 0 for the direct path.
- * Host-only. */
+/* Strategy the probe of n rows against a table of table_bytes takes for a
+ * table of the given mode (0 hash-mixed, 1 key-range; see qo_join_table_info):
+ * *partitioned = 1 for a partitioned probe, *bits = log2 of the partition
+ * count, *splits = 1 for one split and L2-resident table slices (either mode),
+ * 2 for two splits and LDS-resident slices (hash-mixed tables too large for
+ * the first); *partitioned = 0 for the direct probe.  Host-only. */
 int qo_join_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits);
 /* Builds the table in `table` (256-byte aligned device memory of table_bytes,
  * e.g. qo_join_table_bytes(nr)).  When the build keys span a range that fits a
@@ -214,8 +214,9 @@ int qo_batch_destroy(qo_batch *b);
 
 /* ---- autotuner -------------------------------------------------------------- */
 /* Operators: 0 filter_agg, 1 filter, 2 probe (param = table load %), 3 groupby, 4 sort,
- * 5 partitioned join (items = table slice KiB per partition, param = 0 when the
- * table outgrows one L2 / 1 always / 2 never), 6 partitioned group-by (items =
+ * 5 partitioned join (items = LDS table slice KiB per partition, param = 0 when
+ * the table outgrows one L2 / 1 always / 2 never / 3 always, hash-mixed probes
+ * through LDS slices), 6 partitioned group-by (items =
  * groups per fine bucket, param = 0 auto / 1 always / 2 never), 7 key-range join
  * table (items = largest such table in MiB, param = 0 when the keys allow /
  * 2 never).  Size class = floor(log2 n). */

@@ -1,38 +1,35 @@
 // Partitioned join build and probe for tables larger than one L2.
 //
-// A lookup into a table that misses L2 moves a 128-byte line to fetch an
-// 8-byte slot, and even an L2-resident table is read a line per lookup into
-// L1: random lookups run at a small fraction of HBM bandwidth either way.
-// Here every lookup and insert happens in LDS instead:
+// A lookup into a table that misses L2 moves a 128-byte line from MALL/HBM to
+// fetch an 8-byte slot: random lookups into a large table run at a small
+// fraction of HBM bandwidth.  The home slot of a key is monotone in the table's
+// 32-bit hash (qo_join.hpp), so the keys whose hash has top L bits f have their
+// home slots in the slice [f*cap >> L, (f+1)*cap >> L) of the table; splitting
+// the input by those bits gives each partition a contiguous table slice.
 //
-//  * The home slot of a key is monotone in hash32(key), so the keys whose hash
-//    has top L bits f have their home slots in the slice
-//    [f*cap >> L, (f+1)*cap >> L) of the table.  L is picked so that a slice
-//    fits in LDS (tens of KiB).
-//  * Split A: one pass over the input (filter fused) sends every tuple to one
-//    of 2^L1 buckets by its top L1 hash bits: each 8192-tuple tile is ranked in
+//  * Split: one pass over the input (filter fused) sends every tuple to one of
+//    2^L1 buckets by its top L1 hash bits: each 8192-tuple tile is ranked in
 //    LDS, reserves one range per bucket with one atomic add, and writes
-//    contiguous runs.  Split B splits each A bucket by the next L2 bits
-//    (L = L1 + L2; skipped when L1 = L).
-//  * Probe: one work-group per fine bucket loads its slice (plus a margin for
-//    chains that run past the slice end) into LDS and looks every tuple up
-//    there; a chain that leaves the loaded range continues in global memory.
-//    Matches are appended at a range reserved per 4096 tuples.
-//  * Build: one work-group per fine bucket inserts its tuples into an empty
-//    LDS slice (compare-and-swap in LDS) and writes the whole slice out, which
-//    also initialises the table.  Tuples whose chain reaches the slice end are
-//    set aside and inserted into the finished table with global
-//    compare-and-swap, as are the tuples of a full bucket.
-// The table format is the one of the direct path (qo_join.hpp): either path
-// can probe a table built by either.
-//
-// Key-range tables (one slot per key, no chains) are probed through one split
-// instead: S is split by the top bits of key - kmin into buckets whose table
-// slices are ~1 MiB, and the probe blocks of a bucket all run on one XCD (work
-// group i runs on XCD i % 8), so the XCD's L2 holds the slice while its
-// lookups run and a lookup costs an L2 hit instead of a random 128-byte line
-// from HBM.  Which of the two partitioned probes runs is decided on the device
-// from the table's mode; the other returns at once.
+//    contiguous runs (qo_split.hpp).  A second split by the next L2 bits is
+//    used where slices must be smaller still.
+//  * L2 probe (both table modes, slices up to a few MiB): one split into 2^L
+//    buckets (L <= 7); the probe blocks of a bucket all run on one XCD (work
+//    group i runs on XCD i % 8) and walk that XCD's buckets one after another,
+//    so the XCD's L2 holds the current slice and a lookup is an L2 hit.
+//  * LDS-slice probe (hash-mixed tables too large for the L2 probe): two splits
+//    into slices that fit LDS (tens of KiB); one work-group per slice loads it
+//    (plus a margin for chains that run past the slice end) and looks every
+//    tuple up there; a chain that leaves the loaded range continues in global
+//    memory.
+//  * Build (hash-mixed table): one work-group per fine bucket inserts its
+//    tuples into an empty LDS slice (compare-and-swap in LDS) and writes the
+//    whole slice out, which also initialises the table.  Tuples whose chain
+//    reaches the slice end are set aside and inserted into the finished table
+//    with global compare-and-swap, as are the tuples of a full bucket.
+// Matches are appended at a range reserved per 4096 tuples.  The table format
+// is the one of the direct path: either path can probe a table built by
+// either.  Which launched path runs is decided on the device from the table's
+// mode; a path planned for the other mode returns at once.
 #include <algorithm>
 
 #include "qo_internal.hpp"
@@ -57,7 +54,8 @@ static PartPlan part_plan(uint64_t cap, size_t n) {
     PartPlan p;
     const TuneParams tp = tune_get(TUNE_JOIN_PART, n);
     if (tp.param == PART_NEVER || n == 0 || n >= 0xFFFFFFFFull || cap < 2) return p;
-    if (tp.param != PART_ALWAYS && (cap * 8 <= (uint64_t(4) << 20) || n < (size_t(1) << 20))) return p;
+    if (tp.param != PART_ALWAYS && tp.param != PART_LDS && (cap * 8 <= (uint64_t(4) << 20) || n < (size_t(1) << 20)))
+        return p;
     // Slice target in slots (tune items = KiB per slice).
     const uint64_t target = std::min<uint64_t>(kMaxSliceSlots - kSliceMargin - 2, (uint64_t)std::max(1, tp.items) * 128);
     int L = 1;
@@ -71,30 +69,44 @@ static PartPlan part_plan(uint64_t cap, size_t n) {
     return p;
 }
 
-constexpr uint64_t kRangeSliceBytes = uint64_t(1) << 20;  // key-range table bytes per probe bucket
+constexpr uint64_t kL2SliceBytes = uint64_t(1) << 20;     // key-range table bytes per L2 probe bucket
+constexpr uint64_t kL2SliceMaxBytes = uint64_t(3) << 20;  // larger hash-mixed slices take the LDS path
 
-// Key-range probe plan: one split into 2^L buckets, L in [3, 7].
-struct RangePlan {
+// L2 probe plan: one split into 2^L buckets (L in [3, 7]) whose table slices
+// stay in the L2 of the XCD that probes them.  It always serves a key-range
+// table (at most the largest power of two <= slots, ~1 MiB slices); it serves
+// a hash-mixed table (every slot) when that table's slices stay under
+// kL2SliceMaxBytes, else the hash-mixed table takes the LDS-slice probe.
+struct L2Plan {
     bool on = false;
+    bool hash_fits = false;
     SplitPlan s;
     uint32_t nb = 0;   // buckets
     uint32_t tpb = 0;  // probe tiles per bucket (upper bound)
 };
 
-static RangePlan range_plan(uint64_t slots, size_t n) {
-    RangePlan r;
+static L2Plan l2_plan(uint64_t slots, size_t n) {
+    L2Plan r;
     const TuneParams tp = tune_get(TUNE_JOIN_PART, n);
-    if (tp.param == PART_NEVER || n == 0 || n >= 0xFFFFFFFFull || slots < 2) return r;
+    if (tp.param == PART_NEVER || tp.param == PART_LDS || n == 0 || n >= 0xFFFFFFFFull || slots < 2) return r;
     if (tp.param != PART_ALWAYS && (slots * 8 <= (uint64_t(4) << 20) || n < (size_t(1) << 20))) return r;
-    uint64_t cap = 2;  // a key-range table uses at most the largest power of two <= slots
+    uint64_t cap = 2;
     while (cap * 2 <= slots) cap *= 2;
     int L = 3;
-    while (L < kMaxSplitBits && ((cap * 8) >> L) > kRangeSliceBytes) ++L;
+    while (L < kMaxSplitBits && ((cap * 8) >> L) > kL2SliceBytes) ++L;
     r.on = true;
+    r.hash_fits = ((slots * 8) >> L) <= kL2SliceMaxBytes;
     r.s = split_plan(L, n);
     r.nb = 1u << L;
     r.tpb = (uint32_t)((r.s.bcapA + kPairStep - 1) / kPairStep);
     return r;
+}
+
+// The probe's plans: the L2 plan, and the LDS-slice plan only for a
+// hash-mixed table the L2 plan does not serve.
+static void probe_plans(uint64_t slots, size_t n, L2Plan &r, PartPlan &p) {
+    r = l2_plan(slots, n);
+    p = (r.on && r.hash_fits) ? PartPlan{} : part_plan(slots, n);
 }
 
 // Workspace carve-up (same order for sizing and binding).
@@ -115,31 +127,35 @@ static PartBufs part_carve(const PartPlan &p, size_t n, bool build, Carver &cv) 
 }
 
 void join_part_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits) {
-    if (mode == (int)kJoinKeyRange) {
-        const RangePlan r = range_plan(join_table_slots(table_bytes), n);
-        *partitioned = r.on ? 1 : 0;
-        *bits = r.on ? r.s.L : 0;
-        *splits = r.on ? 1 : 0;
-        return;
+    L2Plan r;
+    PartPlan p;
+    probe_plans(join_table_slots(table_bytes), n, r, p);
+    *partitioned = *bits = *splits = 0;
+    if (r.on && (mode == (int)kJoinKeyRange || r.hash_fits)) {
+        *partitioned = 1;
+        *bits = r.s.L;
+        *splits = 1;
+    } else if (mode != (int)kJoinKeyRange && p.on) {
+        *partitioned = 1;
+        *bits = p.L;
+        *splits = 2;  // the LDS-slice probe serves only tables too large for one split
     }
-    const PartPlan p = part_plan(join_table_slots(table_bytes), n);
-    *partitioned = p.on ? 1 : 0;
-    *bits = p.on ? p.L : 0;
-    *splits = p.on ? (p.L2 > 0 ? 2 : 1) : 0;
 }
 
 size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build) {
     const uint64_t slots = join_table_slots(table_bytes);
-    const PartPlan p = part_plan(slots, n);
+    L2Plan r;
+    PartPlan p;
+    if (build) p = part_plan(slots, n);
+    else probe_plans(slots, n, r, p);
     size_t bytes = 0;
     if (p.on) {
         Carver cv(nullptr);
         part_carve(p, n, build, cv);
         bytes = cv.off + 256;
     }
-    const RangePlan r = build ? RangePlan{} : range_plan(slots, n);
     if (r.on) {
-        // The two probes share the workspace: only one of them runs.
+        // The probes share the workspace: only one of them runs.
         Carver cv(nullptr);
         split_carve(r.s, cv);
         bytes = std::max(bytes, cv.off + 256);
@@ -350,17 +366,18 @@ __global__ void k_spill_insert(const uint64_t *spill, const unsigned long long *
     }
 }
 
-// Probe of a key-range table from its S buckets.  Block i works on XCD x =
-// i % 8: its bucket is b = (i / 8 / tpb) * 8 + x and its tile (i / 8) % tpb,
-// so each XCD walks its own buckets one after another.
-__global__ __launch_bounds__(256) void k_range_probe(const uint64_t *__restrict__ bucket, uint64_t bcap,
-                                                     const uint32_t *ctr, uint32_t nb, uint32_t tpb,
-                                                     const JoinHeader *__restrict__ hdr, unsigned long long *counter,
-                                                     SoaEmit emit) {
+// L2 probe from the S buckets (either table mode).  Block i works on XCD
+// x = i % 8: its bucket is b = (i / 8 / tpb) * 8 + x and its tile (i / 8) % tpb,
+// so each XCD walks its own buckets one after another and its L2 holds the
+// current table slice.
+__global__ __launch_bounds__(256) void k_l2_probe(const uint64_t *__restrict__ bucket, uint64_t bcap,
+                                                  const uint32_t *ctr, uint32_t nb, uint32_t tpb,
+                                                  const JoinHeader *__restrict__ hdr, unsigned long long *counter,
+                                                  SoaEmit emit, Gate gate) {
     __shared__ uint32_t s_tot[kPairJ * 4];
     __shared__ uint64_t s_excl;
     __shared__ JoinStage stage;
-    if (hdr->mode != kJoinKeyRange) return;
+    if (gate.off()) return;
     const uint32_t x = blockIdx.x & 7u, q = blockIdx.x >> 3;
     const uint32_t b = (q / tpb) * 8u + x, t = q % tpb;
     if (b >= nb) return;
@@ -415,8 +432,10 @@ int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t 
     const JoinHeader *hdr = static_cast<const JoinHeader *>(table);
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(d_count);
     const SoaEmit emit{okey, orpay, ospay, out_cap};
-    const PartPlan p = part_plan(slots, ns);
-    if (p.on) {  // hash-mixed table
+    L2Plan r;
+    PartPlan p;
+    probe_plans(slots, ns, r, p);
+    if (p.on) {  // hash-mixed table too large for the L2 plan: per-slice LDS tables
         Carver cv(ws);
         const PartBufs b = part_carve(p, ns, false, cv);
         QO_TRY(launch_fill_u32(b.sb.ctr, 0, b.sb.ctr_words, st));
@@ -431,18 +450,19 @@ int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t 
         QO_LAUNCHED();
         *hash_part = true;
     }
-    const RangePlan r = range_plan(slots, ns);
-    if (r.on) {  // key-range table
+    if (r.on) {  // one split by the table's own hash, L2-resident slices
+        const Gate gate = r.hash_fits ? Gate{} : Gate{&hdr->mode, kJoinKeyRange};
         Carver cv(ws);
         const SplitBufs b = split_carve(r.s, cv);
         QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));
-        QO_TRY(launch_splits(r.s, b, skey, spay, sfilt, ns, flo, fhi, ProbeOverflow{hdr, counter, emit},
-                             Gate{&hdr->mode, kJoinKeyRange}, HashSrc{hdr}, st));
+        QO_TRY(launch_splits(r.s, b, skey, spay, sfilt, ns, flo, fhi, ProbeOverflow{hdr, counter, emit}, gate,
+                             HashSrc{hdr}, st));
         const uint64_t grid = (uint64_t)((r.nb + 7) / 8) * 8 * r.tpb;
         QO_REQUIRE(grid < 0x7FFFFFFFull, "probe input too large");
-        k_range_probe<<<(unsigned)grid, 256, 0, st>>>(b.bucketA, r.s.bcapA, b.ctr, r.nb, r.tpb, hdr, counter, emit);
+        k_l2_probe<<<(unsigned)grid, 256, 0, st>>>(b.bucketA, r.s.bcapA, b.ctr, r.nb, r.tpb, hdr, counter, emit, gate);
         QO_LAUNCHED();
         *range_part = true;
+        if (r.hash_fits) *hash_part = true;
     }
     return QO_OK;
 }

@@ -64,7 +64,8 @@ enum TuneOp {
     TUNE_NUM_OPS = 8
 };
 // Partitioned-path modes: by size (the default), always, never.
-enum PartMode { PART_AUTO = 0, PART_ALWAYS = 1, PART_NEVER = 2 };
+// PART_LDS (join only): partition, and probe hash-mixed tables through LDS slices.
+enum PartMode { PART_AUTO = 0, PART_ALWAYS = 1, PART_NEVER = 2, PART_LDS = 3 };
 TuneParams tune_get(int op, size_t n);
 
 inline int size_class(size_t n) {

@@ -114,12 +114,14 @@ def test_hj_filter_sort_matches_oracle(qo, orc, nr, mult):
 # ---------------------------------------------------------------------------
 # Table layouts and probe strategies:
 #  * direct: hash-mixed table, direct build and probe;
-#  * partitioned: hash-mixed table, partitioned build and probe, forced on
-#    small inputs with a 64 KiB table slice per partition (many partitions) and
-#    small buckets so skewed keys exercise the overflow path;
+#  * partitioned: hash-mixed table, partitioned (LDS-slice) build and the
+#    one-split L2 probe, forced on small inputs with small buckets so skewed
+#    keys exercise the overflow path;
+#  * lds: hash-mixed table, partitioned build and the two-split LDS-slice
+#    probe, with a 64 KiB table slice per partition (many partitions);
 #  * keyrange: key-range table where the keys allow it, with the partitioned
-#    path planned (so the key-range probe runs behind its device-side gate and
-#    builds that fall back to hashing take the partitioned path).
+#    paths planned (builds that fall back to hashing take the partitioned
+#    build).
 # ---------------------------------------------------------------------------
 TUNE_JOIN_PART = 5
 TUNE_JOIN_RANGE = 7
@@ -142,9 +144,10 @@ def _restore_tune(qo, saved):
         lib.qo_tune_set(op, cls, b, i, p)
 
 
-@pytest.fixture(params=["direct", "partitioned", "keyrange"])
+@pytest.fixture(params=["direct", "partitioned", "lds", "keyrange"])
 def join_mode(request, qo):
-    part, slice_kib, rng = {"direct": (2, 2048, 2), "partitioned": (1, 64, 2), "keyrange": (1, 64, 0)}[request.param]
+    part, slice_kib, rng = {"direct": (2, 2048, 2), "partitioned": (1, 64, 2), "lds": (3, 64, 2),
+                            "keyrange": (1, 64, 0)}[request.param]
     saved = _set_tune(qo, TUNE_JOIN_PART, 256, slice_kib, part) + _set_tune(qo, TUNE_JOIN_RANGE, 256, 256, rng)
     yield request.param
     _restore_tune(qo, saved)
