@@ -93,7 +93,7 @@ struct PackedEmit {
 
 __global__ __launch_bounds__(256) void k_b_probe(const BQuery *q, int nq, uint32_t total, uint32_t *ticket,
                                                  uint64_t *mkeys, uint32_t *mvals) {
-    __shared__ uint32_t s_tot[kProbeJ * 4];
+    __shared__ uint32_t s_tot[tile_offsets_words(kProbeJ, 4)];
     __shared__ uint64_t s_base;
     __shared__ JoinStage stage;
     QO_TILE_LOOP(ticket, total, t)

@@ -189,62 +189,57 @@ struct RecVec<1> {
 
 // Tiles of J * NT * VEC records (8192 with the launch below): one ticket per
 // tile keeps the same-address ticket atomics few.  The selected records are
-// staged in LDS in input order and written out row by row across the block.
-constexpr int kFiltJ = 8;
-constexpr int kFiltThreads = 512;
+// staged in LDS in input order before the look-back and written out row by row
+// across the block.  In-tile offsets are 32-bit from the tile's base pointer:
+// with 64-bit per-element indexing the kernel needed 140 VGPRs and one block
+// per CU, which cost 1.44 ms instead of 0.86 ms on 2^28 records.
+constexpr int kFiltJ = 4;
+constexpr int kFiltThreads = 1024;
 
 template <int J, int VEC, int NT>
 __global__ __launch_bounds__(NT) void k_filter_records(const qo_record *__restrict__ rin, size_t n, int32_t lo,
                                                        int32_t hi, qo_record *__restrict__ rout, uint64_t *status,
                                                        uint32_t *ticket, uint64_t *d_count, uint32_t num_tiles) {
     constexpr int NW = NT / kWave;
+    constexpr uint32_t kTile = J * NT * VEC;
+    using V = typename RecVec<VEC>::T;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_tot[J * NW];
+    __shared__ uint32_t s_tot[tile_offsets_words(J, NW)];
     __shared__ uint64_t s_excl;
-    __shared__ qo_record s_out[J * NT * VEC];
+    __shared__ qo_record s_out[kTile];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t vbase = (size_t)tile * (J * NT);
+    const size_t ebase = (size_t)tile * kTile;
+    const qo_record *src = rin + ebase;
+    const uint32_t rem = (uint32_t)min((size_t)kTile, n - ebase);
     qo_record r[J][VEC];
-    uint32_t c[J], ex[J];
-    uint32_t fl[J];
+    uint32_t c[J], ex[J], fl[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const size_t v = vbase + (size_t)j * NT + threadIdx.x;
-        const size_t e0 = v * VEC;
-        fl[j] = 0;
-        if (e0 + VEC <= n) {
+        const uint32_t v = (uint32_t)j * NT + threadIdx.x;
+        const uint32_t e0 = v * VEC;
+        if (e0 + VEC <= rem) {
+            const V q = ld_stream(reinterpret_cast<const V *>(src) + v);
             if constexpr (VEC == 2) {
-                uint4 q = ld_stream(reinterpret_cast<const uint4 *>(rin) + v);
                 r[j][0] = {q.x, q.y};
                 r[j][1] = {q.z, q.w};
             } else {
-                uint2 q = ld_stream(reinterpret_cast<const uint2 *>(rin) + v);
                 r[j][0] = {q.x, q.y};
-            }
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) {
-                int32_t key = (int32_t)r[j][k].y;
-                fl[j] |= ((key >= lo) & (key <= hi)) ? (1u << k) : 0u;
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < VEC; ++k) {
-                if (e0 + k < n) {
-                    r[j][k] = rin[e0 + k];
-                    int32_t key = (int32_t)r[j][k].y;
-                    fl[j] |= ((key >= lo) & (key <= hi)) ? (1u << k) : 0u;
-                }
-            }
+            for (int k = 0; k < VEC; ++k) r[j][k] = (e0 + k < rem) ? src[e0 + k] : qo_record{0, 0};
+        }
+        fl[j] = 0;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const int32_t key = (int32_t)r[j][k].y;
+            fl[j] |= ((key >= lo) & (key <= hi) & (e0 + k < rem)) ? (1u << k) : 0u;
         }
         c[j] = __popc(fl[j]);
     }
     const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
-    if (threadIdx.x < kWave) {
-        uint64_t e = lookback(status, tile, total);
-        if (threadIdx.x == 0) s_excl = e;
-    }
     // Stage the selected records at their tile-local rank (input order).
 #pragma unroll
     for (int j = 0; j < J; ++j) {
@@ -253,9 +248,14 @@ __global__ __launch_bounds__(NT) void k_filter_records(const qo_record *__restri
         for (int k = 0; k < VEC; ++k)
             if (fl[j] & (1u << k)) s_out[pos++] = r[j][k];
     }
+    if (threadIdx.x < kWave) {
+        const uint64_t e = lookback(status, tile, total);
+        if (threadIdx.x == 0) s_excl = e;
+    }
     __syncthreads();
     const uint64_t base = s_excl;
-    for (uint32_t i = threadIdx.x; i < total; i += NT) rout[base + i] = s_out[i];
+    qo_record *dst = rout + base;
+    for (uint32_t i = threadIdx.x; i < total; i += NT) dst[i] = s_out[i];
     if (tile == num_tiles - 1 && threadIdx.x == 0) *d_count = base + total;
 }
 
@@ -289,29 +289,33 @@ static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int
 // ---------------------------------------------------------------------------
 // Exclusive scan of 32-bit integers (modulo 2^32), single pass.
 // ---------------------------------------------------------------------------
-constexpr int kScanJ = 8;
-constexpr int kScanThreads = 512;
+constexpr int kScanJ = 4;
+constexpr int kScanThreads = 1024;
 
 template <int J, bool VEC, int NT>
 __global__ __launch_bounds__(NT) void k_scan_excl(const uint32_t *in, uint32_t *out, size_t n, uint64_t *status,
                                                   uint32_t *ticket) {
     constexpr int NW = NT / kWave;
     constexpr int VW = VEC ? 4 : 1;
+    constexpr uint32_t kTile = J * NT * VW;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_tot[J * NW];
+    __shared__ uint32_t s_tot[tile_offsets_words(J, NW)];
     __shared__ uint64_t s_excl;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t vbase = (size_t)tile * (J * NT);
+    const size_t ebase = (size_t)tile * kTile;
+    const uint32_t *src = in + ebase;
+    uint32_t *dst = out + ebase;
+    const uint32_t rem = (uint32_t)min((size_t)kTile, n - ebase);
     uint32_t x[J][VW];
     uint32_t c[J], ex[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const size_t v = vbase + (size_t)j * NT + threadIdx.x;
-        const size_t e0 = v * VW;
-        if (VEC && e0 + VW <= n) {
-            uint4 q = reinterpret_cast<const uint4 *>(in)[v];
+        const uint32_t v = (uint32_t)j * NT + threadIdx.x;
+        const uint32_t e0 = v * VW;
+        if (VEC && e0 + VW <= rem) {
+            const uint4 q = reinterpret_cast<const uint4 *>(src)[v];
             x[j][0] = q.x;
             if constexpr (VW == 4) {
                 x[j][1] = q.y;
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_scan_excl(const uint32_t *in, uint32_t *
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < VW; ++k) x[j][k] = (e0 + k < n) ? in[e0 + k] : 0u;
+            for (int k = 0; k < VW; ++k) x[j][k] = (e0 + k < rem) ? src[e0 + k] : 0u;
         }
         uint32_t sacc = 0;
 #pragma unroll
@@ -329,15 +333,15 @@ __global__ __launch_bounds__(NT) void k_scan_excl(const uint32_t *in, uint32_t *
     }
     const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
     if (threadIdx.x < kWave) {
-        uint64_t e = lookback(status, tile, total);
+        const uint64_t e = lookback(status, tile, total);
         if (threadIdx.x == 0) s_excl = e;
     }
     __syncthreads();
     const uint32_t base = (uint32_t)s_excl;
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const size_t v = vbase + (size_t)j * NT + threadIdx.x;
-        const size_t e0 = v * VW;
+        const uint32_t v = (uint32_t)j * NT + threadIdx.x;
+        const uint32_t e0 = v * VW;
         uint32_t run = base + ex[j];
         uint32_t o[VW];
 #pragma unroll
@@ -345,13 +349,13 @@ __global__ __launch_bounds__(NT) void k_scan_excl(const uint32_t *in, uint32_t *
             o[k] = run;
             run += x[j][k];
         }
-        if (VEC && e0 + VW <= n) {
-            if constexpr (VW == 4) reinterpret_cast<uint4 *>(out)[v] = uint4{o[0], o[1], o[2], o[3]};
-            else out[e0] = o[0];
+        if (VEC && e0 + VW <= rem) {
+            if constexpr (VW == 4) reinterpret_cast<uint4 *>(dst)[v] = uint4{o[0], o[1], o[2], o[3]};
+            else dst[e0] = o[0];
         } else {
 #pragma unroll
             for (int k = 0; k < VW; ++k)
-                if (e0 + k < n) out[e0 + k] = o[k];
+                if (e0 + k < rem) dst[e0 + k] = o[k];
         }
     }
 }

@@ -500,7 +500,7 @@ __global__ __launch_bounds__(256) void k_group_startpos(const qo_record *__restr
                                                         uint32_t num_tiles) {
     constexpr int NW = 4;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_tot[J * NW];
+    __shared__ uint32_t s_tot[tile_offsets_words(J, NW)];
     __shared__ uint64_t s_excl;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();

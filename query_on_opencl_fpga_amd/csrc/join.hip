@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_join_probe(const JoinHeader *__restrict
                                                     const int32_t *__restrict__ sfilt, size_t ns, int32_t flo,
                                                     int32_t fhi, bool vec, SoaEmit emit, uint64_t *d_count,
                                                     Gate gate) {
-    __shared__ uint32_t s_tot[kProbeJ * 4];
+    __shared__ uint32_t s_tot[tile_offsets_words(kProbeJ, 4)];
     __shared__ uint64_t s_base;
     __shared__ JoinStage stage;
     if (gate.off()) return;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void k_bucket_probe(const uint32_t *__restrict
                                                       uint32_t num_tiles) {
     constexpr int NW = 4;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_tot[J * NW];
+    __shared__ uint32_t s_tot[tile_offsets_words(J, NW)];
     __shared__ uint64_t s_excl;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();

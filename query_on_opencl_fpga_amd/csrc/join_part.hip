@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_slice_probe(const uint64_t *__restrict_
                                                      const uint32_t *ctr, int L, const JoinHeader *__restrict__ hdr,
                                                      unsigned long long *counter, SoaEmit emit) {
     extern __shared__ __attribute__((aligned(16))) uint64_t s_slots[];
-    __shared__ uint32_t s_tot[kPairJ * 4];
+    __shared__ uint32_t s_tot[tile_offsets_words(kPairJ, 4)];
     __shared__ uint64_t s_excl;
     if (hdr->mode != kJoinHashMix) return;
     const uint32_t f = blockIdx.x;
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_l2_probe(const uint64_t *__restrict__ b
                                                   const uint32_t *ctr, uint32_t nb, uint32_t tpb,
                                                   const JoinHeader *__restrict__ hdr, unsigned long long *counter,
                                                   SoaEmit emit, Gate gate) {
-    __shared__ uint32_t s_tot[kPairJ * 4];
+    __shared__ uint32_t s_tot[tile_offsets_words(kPairJ, 4)];
     __shared__ uint64_t s_excl;
     __shared__ JoinStage stage;
     if (gate.off()) return;

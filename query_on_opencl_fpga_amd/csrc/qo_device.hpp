@@ -97,31 +97,47 @@ QO_DEV void st_agent(uint64_t *p, uint64_t v) {
 // c[j]; group j of thread t precedes group j of thread t+1 and all of group j+1
 // (order (j, wave, lane)).  On return excl[j] is the exclusive offset of the
 // lane's items inside the tile and the tile total is returned.  s_tot must hold
-// J*NW entries; the caller must __syncthreads() before reusing it.
+// tile_offsets_words(J, NW) entries; the caller must __syncthreads() before
+// reusing it.  Wave 0 scans the J*NW wave totals (P per lane) so that no thread
+// keeps them all in registers.
+constexpr int tile_offsets_words(int J, int NW) { return J * NW + 1; }
+
 template <int J, int NW, class T>
 QO_DEV T tile_offsets(const T (&c)[J], T (&excl)[J], T *s_tot) {
+    constexpr int K = J * NW;
+    constexpr int P = (K + kWave - 1) / kWave;
     const unsigned w = wave_id();
+    const unsigned lane = lane_id();
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         T inc = wave_incl_scan(c[j]);
         excl[j] = inc - c[j];
-        if (lane_id() == kWave - 1) s_tot[j * NW + w] = inc;
+        if (lane == kWave - 1) s_tot[j * NW + w] = inc;
     }
     __syncthreads();
-    T run = 0;
-    T mine[J];
+    if (w == 0) {
+        T v[P];
+        T sum = 0;
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-#pragma unroll
-        for (int ww = 0; ww < NW; ++ww) {
-            T v = s_tot[j * NW + ww];
-            if (ww == (int)w) mine[j] = run;
-            run += v;
+        for (int p = 0; p < P; ++p) {
+            const unsigned idx = lane * P + p;
+            v[p] = idx < (unsigned)K ? s_tot[idx] : T(0);
+            sum += v[p];
         }
-    }
+        const T inc = wave_incl_scan(sum);
+        T run = inc - sum;
 #pragma unroll
-    for (int j = 0; j < J; ++j) excl[j] += mine[j];
-    return run;
+        for (int p = 0; p < P; ++p) {
+            const unsigned idx = lane * P + p;
+            if (idx < (unsigned)K) s_tot[idx] = run;
+            run += v[p];
+        }
+        if (lane == kWave - 1) s_tot[K] = inc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < J; ++j) excl[j] += s_tot[j * NW + w];
+    return s_tot[K];
 }
 
 // Decoupled look-back status word: [63:62] flag (0 empty, 1 aggregate,

@@ -164,7 +164,7 @@ struct JoinStage {
 // path writes the matches to LDS first and emits them row by row across the
 // block; without it, and on the general path, each thread emits its own
 // matches.  Must be called by all 256 threads of the block;
-// s_tot holds J*4 words.
+// s_tot holds tile_offsets_words(J, 4) words.
 template <int J, class Emit>
 QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, const JoinHash &hsh, bool general,
                         uint32_t sentinel,
@@ -327,7 +327,7 @@ QO_DEV void load_s_tile(const int32_t *__restrict__ skey, const int32_t *__restr
 // order (for one S tuple, in table-chain order).  Tiles land in completion
 // order: a look-back across tiles would make every tile wait for the slowest
 // lookup of its predecessors.  Must be called by all 256 threads of the block;
-// s_tot holds kProbeJ*4 words.
+// s_tot holds tile_offsets_words(kProbeJ, 4) words.
 template <bool HAS_FILTER, class Emit>
 QO_DEV void probe_tile(const JoinHeader *__restrict__ hdr, const uint64_t *__restrict__ slots,
                        const int32_t *__restrict__ skey, const int32_t *__restrict__ spay,
