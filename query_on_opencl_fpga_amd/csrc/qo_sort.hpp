@@ -21,8 +21,6 @@ struct RadixSeg {
     uint32_t ntile;
 };
 
-size_t radix_hist_bytes(uint32_t total_tiles);
-
 // Tiles [t0, t1) (segment-local) of segment `seg`: one block of the up-front
 // digit count of a segmented one-sweep sort.
 struct SweepChunk {
@@ -38,13 +36,6 @@ int launch_radix_sweep_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, 
                                 const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
                                 uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
                                 hipStream_t st, bool *in_alt);
-// Sorts every segment by bits [lo_bit, hi_bit) of the keys.  With nseg == 1 the
-// segment is passed by value (seg0) and d_segs may be null.  On return *in_alt
-// tells whether the sorted data ended in kalt / valt (odd pass count).
-int launch_radix_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &seg0,
-                          const RadixSeg *d_segs, int nseg, uint32_t total_tiles, int lo_bit, int hi_bit,
-                          uint32_t *hist, void *scan_ws, hipStream_t st, bool *in_alt);
-
 // Join rows sort ascending by (key, s_payload) as signed int32: flip the sign
 // bits so unsigned 64-bit order is the signed lexicographic order.
 __host__ __device__ __forceinline__ uint64_t join_sort_key(int32_t key, int32_t spay) {

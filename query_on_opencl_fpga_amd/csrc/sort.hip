@@ -1,13 +1,18 @@
 // Sorting: a stable LSD radix sort over 64-bit keys (8-bit digits) with an
-// optional 32-bit value, three launches per digit: per-tile digit histograms,
-// one exclusive scan over the digit-major histogram, and a scatter that ranks
-// keys with wave-level match masks (8 ballots per key), stages the tile in LDS
-// in digit order and writes contiguous runs.
+// optional 32-bit value.  Every pass ranks a 4096-key tile with wave-level
+// match masks (8 ballots per key), stages the tile in LDS in digit order and
+// writes contiguous runs per digit.
 //
-// The sort is segmented: many independent arrays (one per query of a batch)
-// are sorted by the same launches.  A segment owns a contiguous tile range; its
-// element count may live in device memory (written by an earlier kernel), so a
-// whole batch can be captured in one graph without host round trips.
+// A single array takes reduce-then-scan passes: per-tile digit counts in
+// tile-major rows (LDS atomics, one read of the keys), a column scan of the
+// rows that turns them into the global output offset of every (tile, digit),
+// and the placement kernel, which reads its tile's 256 offsets as one
+// coalesced row.  A batch of many arrays (one per query) takes one-sweep
+// passes: digit counts of all passes up front, then one kernel per pass whose
+// tiles find the counts of earlier tiles with a per-digit decoupled look-back.
+// A segment owns a contiguous tile range; its element count may live in device
+// memory (written by an earlier kernel), so a whole batch can be captured in
+// one graph without host round trips.
 //
 // Reference path replaced: radixSortImpl_sm (host/host/testSort.cpp:24-136)
 // driving bitonicSortShared / bitonicMergeGlobal (kernel/aggregation/1/
@@ -32,18 +37,6 @@ QO_DEV uint64_t match_digit(uint32_t d, bool valid) {
     return m;
 }
 
-// Segment that owns global tile `tile` (segments are ordered by tile0).
-QO_DEV RadixSeg seg_of(const RadixSeg &seg0, const RadixSeg *segs, int nseg, uint32_t tile) {
-    if (nseg <= 1) return seg0;
-    int lo = 0, hi = nseg - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (segs[mid].tile0 <= tile) lo = mid;
-        else hi = mid - 1;
-    }
-    return segs[lo];
-}
-
 // Index of the segment that owns global tile `tile` (0 for a single segment).
 QO_DEV int seg_index_of(const RadixSeg *segs, int nseg, uint32_t tile) {
     if (nseg <= 1) return 0;
@@ -61,83 +54,213 @@ QO_DEV size_t seg_count(const RadixSeg &s) {
     return v < s.cap ? v : s.cap;
 }
 
-__global__ __launch_bounds__(256) void k_radix_hist(const uint64_t *__restrict__ kin, RadixSeg seg0,
-                                                    const RadixSeg *segs, int nseg, int shift, uint32_t *hist) {
-    __shared__ uint32_t s_h[4][256];
-    const uint32_t tile = blockIdx.x;
-    const RadixSeg sg = seg_of(seg0, segs, nseg, tile);
-    const uint32_t lt = tile - sg.tile0;
-    const size_t n = seg_count(sg);
-    for (int i = threadIdx.x; i < 1024; i += 256) (&s_h[0][0])[i] = 0;
+// ---------------------------------------------------------------------------
+// Reduce-then-scan passes for a single array.
+// ---------------------------------------------------------------------------
+constexpr int kHistTiles = 2;             // tiles per block of k_tile_hist
+constexpr uint32_t kRowsBlocks = 512;     // at most this many column-scan blocks
+constexpr uint32_t kRowsMinSpan = 64;     // tiles per column-scan block, at least
+
+// rows[lt * 256 + d] = keys of tile lt of the segment whose digit is d (zero
+// rows for tiles past the element count).  VEC: the segment's keys start on a
+// 16-byte boundary, so two keys are read per load.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_tile_hist(const uint64_t *__restrict__ keys, RadixSeg sg, int shift,
+                                                   uint32_t *__restrict__ rows) {
+    __shared__ uint32_t s_h[kHistTiles][256];
+    for (int i = threadIdx.x; i < kHistTiles * 256; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
-    const unsigned w = wave_id(), lane = lane_id();
-    const size_t wbase = (size_t)lt * kSortTile + (size_t)w * (kSortTile / 4);
-    if ((size_t)lt * kSortTile < n) {
-#pragma unroll 4
-        for (int j = 0; j < kSortItems; ++j) {
-            size_t idx = wbase + (size_t)j * kWave + lane;
-            bool valid = idx < n;
-            uint32_t d = valid ? (uint32_t)(kin[sg.base + idx] >> shift) & 255u : 0u;
-            uint64_t m = match_digit(d, valid);
-            if (valid && popc(m & lanemask_lt()) == 0) s_h[w][d] += popc(m);
+    const size_t n = seg_count(sg);
+    const uint32_t t0 = blockIdx.x * kHistTiles;
+#pragma unroll
+    for (int tt = 0; tt < kHistTiles; ++tt) {
+        const uint32_t lt = t0 + tt;
+        const size_t tb = (size_t)lt * kSortTile;
+        if (lt >= sg.ntile || tb >= n) break;
+        const uint32_t rem = (uint32_t)std::min<size_t>(kSortTile, n - tb);
+        const uint64_t *src = keys + sg.base + tb;
+        if constexpr (VEC) {
+            constexpr int PER = kSortTile / 512;
+            uint4 q[PER];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t v = j * 256 + threadIdx.x;
+                q[j] = (2 * v + 2 <= rem) ? ld_stream(reinterpret_cast<const uint4 *>(src) + v) : uint4{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t v = j * 256 + threadIdx.x;
+                if (2 * v + 2 <= rem) {
+                    const uint64_t k0 = ((uint64_t)q[j].y << 32) | q[j].x;
+                    const uint64_t k1 = ((uint64_t)q[j].w << 32) | q[j].z;
+                    atomicAdd(&s_h[tt][(k0 >> shift) & 255u], 1u);
+                    atomicAdd(&s_h[tt][(k1 >> shift) & 255u], 1u);
+                } else if (2 * v < rem) {
+                    atomicAdd(&s_h[tt][(src[2 * v] >> shift) & 255u], 1u);
+                }
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < rem; i += 256) atomicAdd(&s_h[tt][(src[i] >> shift) & 255u], 1u);
         }
     }
     __syncthreads();
-    const unsigned d = threadIdx.x;
-    hist[(size_t)sg.tile0 * 256 + (size_t)d * sg.ntile + lt] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+#pragma unroll
+    for (int tt = 0; tt < kHistTiles; ++tt)
+        if (t0 + tt < sg.ntile) rows[(size_t)(t0 + tt) * 256 + threadIdx.x] = s_h[tt][threadIdx.x];
 }
 
+// Column-scan layout: block b of the scan owns tiles [b * span, (b + 1) * span).
+struct RowsPlan {
+    uint32_t span, nb;
+};
+static RowsPlan rows_plan(uint32_t ntile) {
+    RowsPlan r;
+    r.span = std::max(kRowsMinSpan, (ntile + kRowsBlocks - 1) / kRowsBlocks);
+    r.nb = (ntile + r.span - 1) / r.span;
+    return r;
+}
+
+// bsum[b * 256 + d] = digit-d keys of the tiles of block b.
+__global__ __launch_bounds__(256) void k_rows_reduce(const uint32_t *__restrict__ rows, uint32_t ntile, uint32_t span,
+                                                     uint32_t *__restrict__ bsum) {
+    const uint32_t t0 = blockIdx.x * span, t1 = std::min(ntile, t0 + span);
+    const uint32_t *col = rows + threadIdx.x;
+    uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // independent sums: loads stay in flight together
+    uint32_t t = t0;
+    for (; t + 8 <= t1; t += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] += col[(size_t)(t + u) * 256];
+    for (; t < t1; ++t) c[0] += col[(size_t)t * 256];
+    bsum[(size_t)blockIdx.x * 256 + threadIdx.x] = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7];
+}
+
+// One block of 1024 threads: bsum[b][d] := start of digit d in the output +
+// digit-d keys of blocks before b (exclusive, in place).  Thread (q, d) with
+// q = threadIdx / 256 handles quarter q of the blocks for digit d.
+__global__ __launch_bounds__(1024) void k_rows_starts(uint32_t *bsum, uint32_t nb) {
+    __shared__ uint32_t s_part[4][256];
+    __shared__ uint32_t s_w[16];
+    const uint32_t d = threadIdx.x & 255u, q = threadIdx.x >> 8;
+    const uint32_t per = (nb + 3) / 4, b0 = std::min(nb, q * per), b1 = std::min(nb, b0 + per);
+    uint32_t *col = bsum + d;
+    uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t b = b0;
+    for (; b + 8 <= b1; b += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] += col[(size_t)(b + u) * 256];
+    for (; b < b1; ++b) c[0] += col[(size_t)b * 256];
+    s_part[q][d] = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7];
+    __syncthreads();
+    const uint32_t p0 = s_part[0][d], p1 = s_part[1][d], p2 = s_part[2][d], p3 = s_part[3][d];
+    // Threads 0..255 come first in the block, so their exclusive scan is the digit start.
+    const uint32_t start = block_excl_scan<16>(q == 0 ? p0 + p1 + p2 + p3 : 0u, s_w);
+    __syncthreads();
+    if (q == 0) s_part[0][d] = start;
+    __syncthreads();
+    uint32_t run = s_part[0][d] + (q > 0 ? p0 : 0u) + (q > 1 ? p1 : 0u) + (q > 2 ? p2 : 0u);
+    b = b0;
+    for (; b + 8 <= b1; b += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(b + u) * 256];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            col[(size_t)(b + u) * 256] = run;
+            run += v[u];
+        }
+    }
+    for (; b < b1; ++b) {
+        const uint32_t v = col[(size_t)b * 256];
+        col[(size_t)b * 256] = run;
+        run += v;
+    }
+}
+
+// rows[t][d] := output offset of the first digit-d key of tile t (in place).
+__global__ __launch_bounds__(256) void k_rows_apply(uint32_t *__restrict__ rows, uint32_t ntile, uint32_t span,
+                                                    const uint32_t *__restrict__ bsum) {
+    const uint32_t t0 = blockIdx.x * span, t1 = std::min(ntile, t0 + span);
+    uint32_t *col = rows + threadIdx.x;
+    uint32_t run = bsum[(size_t)blockIdx.x * 256 + threadIdx.x];
+    uint32_t t = t0;
+    for (; t + 8 <= t1; t += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(t + u) * 256];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            col[(size_t)(t + u) * 256] = run;
+            run += v[u];
+        }
+    }
+    for (; t < t1; ++t) {
+        const uint32_t v = col[(size_t)t * 256];
+        col[(size_t)t * 256] = run;
+        run += v;
+    }
+}
+
+// Tile of work-group b when each XCD walks a contiguous tile range in order:
+// work-group b runs on XCD b mod 8, which owns tiles [x*q + min(x, r), ...)
+// with q = ntile / 8, r = ntile mod 8.  Adjacent tiles then write the two
+// halves of a digit run's boundary lines through the same L2, which merges
+// them before they reach HBM.  A bijection of [0, ntile).
+QO_DEV uint32_t xcd_tile(uint32_t b, uint32_t ntile) {
+    const uint32_t x = b & 7u, i = b >> 3, q = ntile >> 3, r = ntile & 7u;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// Places tile xcd_tile(blockIdx.x) of the segment: ranks its keys by digit
+// (stable), stages them in LDS in digit order and writes each digit's run at
+// the offset from its row.
 template <bool HAS_VAL>
-__global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restrict__ kin,
-                                                       const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
-                                                       uint32_t *__restrict__ vout, RadixSeg seg0,
-                                                       const RadixSeg *segs, int nseg, int shift,
-                                                       const uint32_t *__restrict__ offs) {
+__global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                     RadixSeg sg, int shift, const uint32_t *__restrict__ offs) {
     __shared__ uint32_t s_cnt[4][256];
     __shared__ uint32_t s_dpre[256];
     __shared__ uint32_t s_gofs[256];
     __shared__ uint32_t s_w[4];
     __shared__ uint64_t s_k[kSortTile];
     __shared__ uint32_t s_v[HAS_VAL ? kSortTile : 1];
-    const uint32_t tile = blockIdx.x;
-    const RadixSeg sg = seg_of(seg0, segs, nseg, tile);
-    const uint32_t lt = tile - sg.tile0;
+    const uint32_t lt = xcd_tile(blockIdx.x, sg.ntile);
     const size_t n = seg_count(sg);
-    if ((size_t)lt * kSortTile >= n) return;
+    const size_t tb = (size_t)lt * kSortTile;
+    if (tb >= n) return;
+    const uint32_t rem = (uint32_t)std::min<size_t>(kSortTile, n - tb);
+    const uint64_t *ks = kin + sg.base + tb;
+    const uint32_t *vs = HAS_VAL ? vin + sg.base + tb : nullptr;
     for (int i = threadIdx.x; i < 1024; i += 256) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
     const unsigned w = wave_id(), lane = lane_id();
-    const size_t wbase = (size_t)lt * kSortTile + (size_t)w * (kSortTile / 4);
+    const uint32_t wb = w * (kSortTile / 4);
     uint64_t k[kSortItems];
     uint32_t v[kSortItems];
     uint32_t rank[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-        size_t idx = wbase + (size_t)j * kWave + lane;
-        bool valid = idx < n;
-        k[j] = valid ? kin[sg.base + idx] : ~uint64_t(0);
-        if (HAS_VAL) v[j] = valid ? vin[sg.base + idx] : 0u;
+        const uint32_t idx = wb + j * kWave + lane;
+        const bool valid = idx < rem;
+        k[j] = valid ? ks[idx] : ~uint64_t(0);
+        if (HAS_VAL) v[j] = valid ? vs[idx] : 0u;
     }
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-        size_t idx = wbase + (size_t)j * kWave + lane;
-        bool valid = idx < n;
-        uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
-        uint64_t m = match_digit(d, valid);
-        uint32_t pre = s_cnt[w][d];
-        uint32_t r = popc(m & lanemask_lt());
+        const uint32_t idx = wb + j * kWave + lane;
+        const bool valid = idx < rem;
+        const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+        const uint64_t m = match_digit(d, valid);
+        const uint32_t pre = s_cnt[w][d];
+        const uint32_t r = popc(m & lanemask_lt());
         if (valid && r == 0) s_cnt[w][d] = pre + popc(m);
         rank[j] = pre + r;
     }
     __syncthreads();
     {
         const unsigned d = threadIdx.x;
-        uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
-        uint32_t tot = c0 + c1 + c2 + c3;
-        const size_t h0 = (size_t)sg.tile0 * 256;
-        s_gofs[d] = offs[h0 + (size_t)d * sg.ntile + lt] - offs[h0];
-        uint32_t ex = block_excl_scan256(tot, s_w);
-        s_dpre[d] = ex;
+        const uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
+        s_gofs[d] = offs[(size_t)lt * 256 + d];
+        s_dpre[d] = block_excl_scan256(c0 + c1 + c2 + c3, s_w);
         s_cnt[0][d] = 0;
         s_cnt[1][d] = c0;
         s_cnt[2][d] = c0 + c1;
@@ -146,73 +269,81 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restric
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-        size_t idx = wbase + (size_t)j * kWave + lane;
-        if (idx < n) {
-            uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
-            uint32_t p = s_dpre[d] + s_cnt[w][d] + rank[j];
+        const uint32_t idx = wb + j * kWave + lane;
+        if (idx < rem) {
+            const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+            const uint32_t p = s_dpre[d] + s_cnt[w][d] + rank[j];
             s_k[p] = k[j];
             if (HAS_VAL) s_v[p] = v[j];
         }
     }
     __syncthreads();
-    const size_t tstart = (size_t)lt * kSortTile;
-    const uint32_t tcount = (uint32_t)((n - tstart) < (size_t)kSortTile ? (n - tstart) : (size_t)kSortTile);
-    for (uint32_t i = threadIdx.x; i < tcount; i += 256) {
-        uint64_t kk = s_k[i];
-        uint32_t d = (uint32_t)(kk >> shift) & 255u;
-        size_t gp = sg.base + (size_t)s_gofs[d] + (i - s_dpre[d]);
-        kout[gp] = kk;
-        if (HAS_VAL) vout[gp] = s_v[i];
+    uint64_t *kd = kout + sg.base;
+    uint32_t *vd = HAS_VAL ? vout + sg.base : nullptr;
+    for (uint32_t i = threadIdx.x; i < rem; i += 256) {
+        const uint64_t kk = s_k[i];
+        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        const uint32_t gp = s_gofs[d] + (i - s_dpre[d]);
+        kd[gp] = kk;
+        if (HAS_VAL) vd[gp] = s_v[i];
     }
 }
 
-size_t radix_hist_bytes(uint32_t total_tiles) { return (size_t)total_tiles * 256 * 4 + 256; }
+// Workspace of the reduce-then-scan passes over `tiles` tiles.
+struct RowsBufs {
+    uint32_t *rows, *bsum;
+};
+static RowsBufs rows_carve(uint32_t tiles, Carver &cv) {
+    RowsBufs b;
+    b.rows = cv.take<uint32_t>((size_t)tiles * 256);
+    // rows_plan(t).nb <= min(kRowsBlocks, ceil(t / kRowsMinSpan)) for every t <= tiles.
+    b.bsum = cv.take<uint32_t>((size_t)std::min(kRowsBlocks, (tiles + kRowsMinSpan - 1) / kRowsMinSpan) * 256);
+    return b;
+}
+
+static int launch_radix_rows(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &sg,
+                             int lo_bit, int hi_bit, const RowsBufs &b, hipStream_t st, bool *in_alt) {
+    *in_alt = false;
+    QO_REQUIRE(lo_bit >= 0 && hi_bit <= 64 && lo_bit < hi_bit, "bad bit range");
+    if (sg.ntile == 0) return QO_OK;
+    const RowsPlan rp = rows_plan(sg.ntile);
+    const unsigned hblocks = (sg.ntile + kHistTiles - 1) / kHistTiles;
+    uint64_t *ksrc = keys, *kdst = kalt;
+    uint32_t *vsrc = vals, *vdst = valt;
+    for (int sh = lo_bit; sh < hi_bit; sh += 8) {
+        if ((reinterpret_cast<uintptr_t>(ksrc + sg.base) & 15) == 0)
+            k_tile_hist<true><<<hblocks, 256, 0, st>>>(ksrc, sg, sh, b.rows);
+        else
+            k_tile_hist<false><<<hblocks, 256, 0, st>>>(ksrc, sg, sh, b.rows);
+        QO_LAUNCHED();
+        k_rows_reduce<<<rp.nb, 256, 0, st>>>(b.rows, sg.ntile, rp.span, b.bsum);
+        QO_LAUNCHED();
+        k_rows_starts<<<1, 1024, 0, st>>>(b.bsum, rp.nb);
+        QO_LAUNCHED();
+        k_rows_apply<<<rp.nb, 256, 0, st>>>(b.rows, sg.ntile, rp.span, b.bsum);
+        QO_LAUNCHED();
+        if (vals)
+            k_radix_place<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, sh, b.rows);
+        else
+            k_radix_place<false><<<sg.ntile, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, sg, sh, b.rows);
+        QO_LAUNCHED();
+        std::swap(ksrc, kdst);
+        std::swap(vsrc, vdst);
+        *in_alt = !*in_alt;
+    }
+    return QO_OK;
+}
 
 // ---------------------------------------------------------------------------
-// One-sweep passes for a single array: the digit counts of every pass come
+// One-sweep passes for a batch of arrays: the digit counts of every pass come
 // from one histogram over the keys up front; each pass is then a single
 // kernel that ranks its tile, publishes the tile's per-digit counts and finds
-// the counts of all earlier tiles with a decoupled look-back per digit (thread
-// d of the block walks digit d), so no per-pass histogram read or scan runs.
-// Tiles are numbered by a ticket: every tile a block waits for belongs to a
-// block that is already running.
+// the counts of all earlier tiles of its segment with a decoupled look-back
+// per digit (thread d of the block walks digit d).  Tiles are numbered by a
+// ticket: every tile a block waits for belongs to a block that is already
+// running.
 // ---------------------------------------------------------------------------
 constexpr int kSweepMaxPass = 8;
-constexpr int kSweepCountBlocks = 512;
-
-// Per-block digit counts of every pass (LDS), one row of npass*256 per block.
-__global__ __launch_bounds__(256) void k_sweep_counts(const uint64_t *__restrict__ keys, RadixSeg sg, int lo_bit,
-                                                      int npass, uint32_t *partial) {
-    __shared__ uint32_t s_h[kSweepMaxPass * 256];
-    for (int i = threadIdx.x; i < npass * 256; i += 256) s_h[i] = 0;
-    __syncthreads();
-    const size_t n = seg_count(sg);
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const uint64_t k = ld_stream(keys + sg.base + i);
-        for (int p = 0; p < npass; ++p) atomicAdd(&s_h[p * 256 + ((k >> (lo_bit + 8 * p)) & 255u)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < npass * 256; i += 256) partial[(size_t)blockIdx.x * npass * 256 + i] = s_h[i];
-}
-
-// counts[p*256 + d] = sum over blocks of partial[b][p][d]: exclusive digit starts
-// per pass (the digit order is the output order), written in place.
-__global__ __launch_bounds__(256) void k_sweep_starts(const uint32_t *partial, int nblocks, int npass,
-                                                      uint32_t *starts) {
-    __shared__ uint32_t s_w[4];
-    const int p = blockIdx.x;
-    if (p >= npass) return;
-    const uint32_t *col = partial + p * 256 + threadIdx.x;
-    const size_t row = (size_t)npass * 256;
-    uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // independent sums: loads stay in flight together
-    int b = 0;
-    for (; b + 8 <= nblocks; b += 8)
-#pragma unroll
-        for (int u = 0; u < 8; ++u) c[u] += col[(size_t)(b + u) * row];
-    for (; b < nblocks; ++b) c[0] += col[(size_t)b * row];
-    const uint32_t t = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7];
-    starts[p * 256 + threadIdx.x] = block_excl_scan256(t, s_w);
-}
 
 // Segmented digit counts: block c counts tiles [t0, t1) of segment `seg` of
 // chunk c (segment-local tile numbers), one row of npass*256 per chunk.
@@ -342,77 +473,6 @@ __global__ __launch_bounds__(256) void k_radix_sweep(const uint64_t *__restrict_
     }
 }
 
-// Workspace of the one-sweep passes over `tiles` tiles and npass passes.
-struct SweepBufs {
-    uint32_t *partial, *starts, *ticket;
-    uint64_t *status;
-};
-static SweepBufs sweep_carve(uint32_t tiles, int npass, Carver &cv) {
-    SweepBufs b;
-    b.partial = cv.take<uint32_t>((size_t)kSweepCountBlocks * npass * 256);
-    b.starts = cv.take<uint32_t>((size_t)npass * 256);
-    b.ticket = cv.take<uint32_t>(64);
-    b.status = cv.take<uint64_t>((size_t)tiles * 256);
-    return b;
-}
-
-static int launch_radix_sweep(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &sg,
-                              int lo_bit, int hi_bit, const SweepBufs &b, hipStream_t st, bool *in_alt) {
-    *in_alt = false;
-    const int npass = (hi_bit - lo_bit + 7) / 8;
-    QO_REQUIRE(npass >= 1 && npass <= kSweepMaxPass, "bad bit range");
-    // At least 64K keys per counting block (capacity bound when the count is on the device).
-    const int cblocks = (int)std::min<uint64_t>(kSweepCountBlocks, std::max<uint64_t>(1, (sg.cap + 65535) / 65536));
-    k_sweep_counts<<<cblocks, 256, 0, st>>>(keys, sg, lo_bit, npass, b.partial);
-    QO_LAUNCHED();
-    k_sweep_starts<<<npass, 256, 0, st>>>(b.partial, cblocks, npass, b.starts);
-    QO_LAUNCHED();
-    uint64_t *ksrc = keys, *kdst = kalt;
-    uint32_t *vsrc = vals, *vdst = valt;
-    for (int p = 0; p < npass; ++p) {
-        QO_TRY(launch_fill_u64(b.status, 0, (size_t)sg.ntile * 256, st));
-        QO_TRY(launch_fill_u32(b.ticket, 0, 1, st));
-        const int sh = lo_bit + 8 * p;
-        if (vals)
-            k_radix_sweep<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, nullptr, 1, sh,
-                                                          b.starts + p * 256, 0, b.status, b.ticket);
-        else
-            k_radix_sweep<false><<<sg.ntile, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, sg, nullptr, 1, sh,
-                                                           b.starts + p * 256, 0, b.status, b.ticket);
-        QO_LAUNCHED();
-        std::swap(ksrc, kdst);
-        std::swap(vsrc, vdst);
-        *in_alt = !*in_alt;
-    }
-    return QO_OK;
-}
-
-int launch_radix_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &seg0,
-                          const RadixSeg *d_segs, int nseg, uint32_t total_tiles, int lo_bit, int hi_bit,
-                          uint32_t *hist, void *scan_ws, hipStream_t st, bool *in_alt) {
-    *in_alt = false;
-    if (total_tiles == 0) return QO_OK;
-    QO_REQUIRE(lo_bit >= 0 && hi_bit <= 64 && lo_bit < hi_bit, "bad bit range");
-    QO_REQUIRE((size_t)total_tiles * 256 < 0xFFFFFFFFull, "sort input too large");
-    uint64_t *ksrc = keys, *kdst = kalt;
-    uint32_t *vsrc = vals, *vdst = valt;
-    for (int sh = lo_bit; sh < hi_bit; sh += 8) {
-        k_radix_hist<<<total_tiles, 256, 0, st>>>(ksrc, seg0, d_segs, nseg, sh, hist);
-        QO_LAUNCHED();
-        QO_TRY(launch_scan_excl_u32(hist, hist, (size_t)total_tiles * 256, scan_ws, st));
-        if (vals)
-            k_radix_scatter<true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, seg0, d_segs, nseg, sh, hist);
-        else
-            k_radix_scatter<false><<<total_tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, seg0, d_segs, nseg, sh,
-                                                               hist);
-        QO_LAUNCHED();
-        std::swap(ksrc, kdst);
-        std::swap(vsrc, vdst);
-        *in_alt = !*in_alt;
-    }
-    return QO_OK;
-}
-
 static uint32_t radix_tiles(size_t n) { return (uint32_t)((n + kSortTile - 1) / kSortTile); }
 
 size_t radix_sweep_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks) {
@@ -463,30 +523,19 @@ int launch_radix_sweep_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, 
     return QO_OK;
 }
 
-// Key + value sorts take the one-sweep passes; key-only sorts, whose tiles are
-// light enough that the per-digit look-back costs more than the histogram read
-// it replaces (measured: a 2^26-Record sort 2.41 vs 1.98 ms), keep the
-// histogram / scan / scatter passes.
-static void radix_carve(size_t n, bool with_vals, Carver &cv, uint64_t **kalt, uint32_t **valt, uint32_t **hist,
-                        void **scan_ws, SweepBufs *sb) {
+static void radix_carve(size_t n, bool with_vals, Carver &cv, uint64_t **kalt, uint32_t **valt, RowsBufs *rb) {
     const uint32_t tiles = radix_tiles(n) + 1;
     *kalt = cv.take<uint64_t>(n ? n : 1);
     *valt = with_vals ? cv.take<uint32_t>(n ? n : 1) : nullptr;
-    if (with_vals) {
-        *sb = sweep_carve(tiles, kSweepMaxPass, cv);
-    } else {
-        *hist = cv.take<uint32_t>((size_t)tiles * 256);
-        *scan_ws = cv.take<char>(scan_workspace_bytes((size_t)tiles * 256));
-    }
+    *rb = rows_carve(tiles, cv);
 }
 
 size_t radix_workspace_bytes(size_t n, bool with_vals) {
     Carver cv(nullptr);
     uint64_t *kalt;
-    uint32_t *valt, *hist;
-    void *scan_ws;
-    SweepBufs sb;
-    radix_carve(n, with_vals, cv, &kalt, &valt, &hist, &scan_ws, &sb);
+    uint32_t *valt;
+    RowsBufs rb;
+    radix_carve(n, with_vals, cv, &kalt, &valt, &rb);
     return cv.off + 256;
 }
 
@@ -498,17 +547,12 @@ static int radix_impl(uint64_t *keys, uint32_t *vals, size_t n, const uint64_t *
     const uint32_t tiles = radix_tiles(n);
     Carver cv(ws);
     uint64_t *kalt = nullptr;
-    uint32_t *valt = nullptr, *hist = nullptr;
-    void *scan_ws = nullptr;
-    SweepBufs sb{};
-    radix_carve(n, vals != nullptr, cv, &kalt, &valt, &hist, &scan_ws, &sb);
+    uint32_t *valt = nullptr;
+    RowsBufs rb{};
+    radix_carve(n, vals != nullptr, cv, &kalt, &valt, &rb);
     RadixSeg seg{0, n, d_n, 0, tiles};
     bool in_alt = false;
-    if (vals)
-        QO_TRY(launch_radix_sweep(keys, vals, kalt, valt, seg, lo_bit, hi_bit, sb, st, &in_alt));
-    else
-        QO_TRY(launch_radix_segments(keys, nullptr, kalt, nullptr, seg, nullptr, 1, tiles, lo_bit, hi_bit, hist,
-                                     scan_ws, st, &in_alt));
+    QO_TRY(launch_radix_rows(keys, vals, kalt, valt, seg, lo_bit, hi_bit, rb, st, &in_alt));
     if (in_alt) {
         // Odd number of passes: the result is in the alternate buffers.  The
         // count may only be known on the device, so copy the full capacity.
