@@ -182,18 +182,18 @@ QO_DEV void ltable_flush(const LTable<C> &L, GEntry *gt, uint64_t gmask, uint32_
 
 // Appends the block's LDS groups as finished rows (the block saw every tuple
 // of these keys): one reservation on *d_rows per block; rows past `cap` are
-// counted but not written.  After a barrier; 512 threads.
-template <int C>
+// counted but not written.  After a barrier; NT threads.
+template <int C, int NT>
 QO_DEV void ltable_append(const LTable<C> &L, GEntry *rows, unsigned long long *d_rows, size_t cap, uint32_t *s_w,
                           unsigned long long *s_base) {
-    static_assert(C % 512 == 0, "table size");
-    constexpr int kPer = C / 512;
+    static_assert(C % NT == 0, "table size");
+    constexpr int kPer = C / NT;
     const int i0 = threadIdx.x * kPer;
     uint32_t c = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) c += L.cnt[i0 + k] != 0;
-    const uint32_t ex = block_excl_scan<8>(c, s_w);
-    if (threadIdx.x == 511) *s_base = atomicAdd(d_rows, (unsigned long long)(ex + c));
+    const uint32_t ex = block_excl_scan<NT / kWave>(c, s_w);
+    if (threadIdx.x == NT - 1) *s_base = atomicAdd(d_rows, (unsigned long long)(ex + c));
     __syncthreads();
     uint64_t pos = *s_base + ex;
 #pragma unroll
@@ -228,9 +228,23 @@ __global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict_
     size_t from = 0;
     if (VEC) {
         const size_t nv = n / 4;
-        for (size_t i = tid; i < nv; i += stride) {
-            int4 kk = ld_stream(reinterpret_cast<const int4 *>(key) + i);
-            int4 vv = ld_stream(reinterpret_cast<const int4 *>(val) + i);
+        const int4 *k4 = reinterpret_cast<const int4 *>(key), *v4 = reinterpret_cast<const int4 *>(val);
+        size_t i = tid;
+        // Two rounds of loads in flight per thread.
+        for (; i + stride < nv; i += 2 * stride) {
+            const int4 ka = ld_stream(k4 + i), va = ld_stream(v4 + i);
+            const int4 kb = ld_stream(k4 + i + stride), vb = ld_stream(v4 + i + stride);
+            add(ka.x, va.x);
+            add(ka.y, va.y);
+            add(ka.z, va.z);
+            add(ka.w, va.w);
+            add(kb.x, vb.x);
+            add(kb.y, vb.y);
+            add(kb.z, vb.z);
+            add(kb.w, vb.w);
+        }
+        if (i < nv) {
+            const int4 kk = ld_stream(k4 + i), vv = ld_stream(v4 + i);
             add(kk.x, vv.x);
             add(kk.y, vv.y);
             add(kk.z, vv.z);
@@ -255,6 +269,7 @@ __global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict_
 // ---------------------------------------------------------------------------
 constexpr uint32_t kGbChunk = 1u << 18;  // tuples per work item of a large bucket
 constexpr uint32_t kGbWhole = 1u << 19;  // buckets up to this size are one work item
+constexpr uint32_t kChunkU = 8;          // tuples in flight per thread of k_groupby_chunks
 
 // Work items per fine bucket: one for a bucket of at most kGbWhole tuples,
 // else ceil(hist[f] / kGbChunk); cch[nb] = 0.
@@ -268,15 +283,18 @@ __global__ void k_chunk_counts(const uint32_t *hist, uint32_t nb, uint32_t *cch)
 // Work item w of fine bucket f (cpre[f] <= w < cpre[f + 1]).  A bucket that
 // is one work item holds every tuple of its keys, so the block appends its
 // groups as finished rows; a chunk of a larger bucket merges its partials
-// into the global table.
-template <int C>
-__global__ __launch_bounds__(512) void k_groupby_chunks(const uint64_t *__restrict__ fine, const uint32_t *offp,
+// into the global table.  NT = 1024: two 48 KB blocks fill a CU (32 waves), so
+// a grid of 2 * 256 * k work items runs in k full rounds.
+constexpr int kChunkThreads = 1024;
+
+template <int C, int NT>
+__global__ __launch_bounds__(NT) void k_groupby_chunks(const uint64_t *__restrict__ fine, const uint32_t *offp,
                                                         const uint32_t *off, const uint32_t *cpre, uint32_t nb, int L_bits, GEntry *gt,
                                                         uint64_t gmask, int probes, uint32_t *overflow, GEntry *rows,
                                                         unsigned long long *d_rows, size_t rows_cap) {
     extern __shared__ unsigned char s_raw[];
     LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
-    __shared__ uint32_t s_w[8];
+    __shared__ uint32_t s_w[NT / kWave];
     __shared__ unsigned long long s_base;
     const uint32_t w = blockIdx.x;
     if (w >= cpre[nb]) return;  // whole block: the grid is an upper bound
@@ -294,12 +312,24 @@ __global__ __launch_bounds__(512) void k_groupby_chunks(const uint64_t *__restri
     const uint64_t *src = fine + e0;
     ltable_init<C>(L);
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-        const uint64_t v = ld_stream(src + i);
-        ltable_add<C, true>(L, (int32_t)(uint32_t)v, (int32_t)(v >> 32), probes, gt, gmask, overflow, L_bits);
+    // kChunkU tuples per thread and round, all loaded before any is added, so
+    // a thread keeps that many loads in flight.
+    constexpr uint32_t kRound = NT * kChunkU;
+    for (uint32_t r = 0; r < m; r += kRound) {
+        uint64_t v[kChunkU];
+#pragma unroll
+        for (int u = 0; u < (int)kChunkU; ++u) {
+            const uint32_t i = r + u * NT + threadIdx.x;
+            v[u] = i < m ? ld_stream(src + i) : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < (int)kChunkU; ++u)
+            if (r + u * NT + threadIdx.x < m)
+                ltable_add<C, true>(L, (int32_t)(uint32_t)v[u], (int32_t)(v[u] >> 32), probes, gt, gmask, overflow,
+                                    L_bits);
     }
     __syncthreads();
-    if (whole) ltable_append<C>(L, rows, d_rows, rows_cap, s_w, &s_base);
+    if (whole) ltable_append<C, NT>(L, rows, d_rows, rows_cap, s_w, &s_base);
     else ltable_flush<C>(L, gt, gmask, overflow);
 }
 
@@ -447,7 +477,7 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         // Upper bound on the chunk count: every bucket one partial chunk.
         const uint64_t items = (uint64_t)nb + (n + kGbChunk - 1) / kGbChunk;
         QO_REQUIRE(items < 0x7FFFFFFFull, "group-by input too large");
-        k_groupby_chunks<2048><<<(unsigned)items, 512, sizeof(LTable<2048>), st>>>(
+        k_groupby_chunks<2048, kChunkThreads><<<(unsigned)items, kChunkThreads, sizeof(LTable<2048>), st>>>(
             b.sb.fine, b.sb.offp, b.sb.off, b.cpre, nb, pp.s.L, gt, p.gmask, 64, overflow, b.rows,
             reinterpret_cast<unsigned long long *>(counters), group_capacity);
         QO_LAUNCHED();

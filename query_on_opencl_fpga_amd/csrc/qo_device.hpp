@@ -195,6 +195,17 @@ QO_DEV uint32_t block_excl_scan(uint32_t v, uint32_t *s_w) {
 }
 QO_DEV uint32_t block_excl_scan256(uint32_t v, uint32_t *s_w) { return block_excl_scan<4>(v, s_w); }
 
+// Tile of work-group b when each XCD walks a contiguous tile range in order:
+// work-group b runs on XCD b mod 8, which owns tiles [x*q + min(x, r), ...)
+// with q = ntile / 8, r = ntile mod 8.  Writes that continue one another
+// (adjacent runs of one output region) then come from one XCD, and its L2
+// merges them into whole lines before they reach HBM.  A bijection of
+// [0, ntile) for b < ntile.
+QO_DEV uint32_t xcd_tile(uint32_t b, uint32_t ntile) {
+    const uint32_t x = b & 7u, i = b >> 3, q = ntile >> 3, r = ntile & 7u;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // 32-bit finaliser of MurmurHash3 (Appleby): spreads sequential keys.
 QO_DEV uint32_t hash32(uint32_t k) {
     k ^= k >> 16;

@@ -125,7 +125,8 @@ struct HashSrc {
 
 struct SplitLds {
     uint64_t stage[kSplitTile];
-    uint32_t cnt[kMaxSplit], loff[kMaxSplit], gbase[kMaxSplit];
+    uint64_t gpos[kMaxSplit];  // where the part's run starts in `bucket`
+    uint32_t cnt[kMaxSplit], loff[kMaxSplit], room[kMaxSplit];
     uint32_t w[kSplitThreads / kWave];
 };
 
@@ -160,8 +161,13 @@ QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], 
     const uint32_t c0 = threadIdx.x < parts ? sh.cnt[threadIdx.x] : 0u;
     const uint32_t ex = block_excl_scan<kSplitThreads / kWave>(c0, sh.w);
     if (threadIdx.x < parts) {
+        // The run's start and how many of its tuples fit (the rest overflow).
+        const uint32_t b = bucket0 + threadIdx.x;
+        const uint64_t g = c0 ? atomicAdd(&ctr[b * kCtrStride], c0) : 0u;
+        const uint64_t cap = out.cap(b);
         sh.loff[threadIdx.x] = ex;
-        sh.gbase[threadIdx.x] = c0 ? atomicAdd(&ctr[(bucket0 + threadIdx.x) * kCtrStride], c0) : 0u;
+        sh.gpos[threadIdx.x] = out.start(b) + g;
+        sh.room[threadIdx.x] = (uint32_t)(g < cap ? (cap - g < c0 ? cap - g : c0) : 0u);
     }
     __syncthreads();
     const uint32_t total = sh.loff[parts - 1] + sh.cnt[parts - 1];
@@ -177,11 +183,10 @@ QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], 
         uint32_t h;
         hs((int32_t)(uint32_t)v, h);
         const uint32_t p = (h >> shift) & mask;
-        const uint32_t b = bucket0 + p;
-        const uint64_t g = (uint64_t)sh.gbase[p] + (i - sh.loff[p]);
+        const uint32_t r = i - sh.loff[p];
         // Plain stores: runs from neighbouring tiles complete each other's
         // lines in L2 before they are written back.
-        if (g < out.cap(b)) bucket[out.start(b) + g] = v;
+        if (r < sh.room[p]) bucket[sh.gpos[p] + r] = v;
         else overflow(v);
     }
 }
@@ -232,21 +237,15 @@ QO_DEV void load_pairs(const uint64_t *__restrict__ src, uint32_t m, int32_t (&k
     }
 }
 
-// Split B on tile t of input bucket p1 (placed by `in`): -> buckets
-// (p1 << L2) + part.  An input bucket starting at an odd position (exact
+// Split B on m <= kSplitTile pairs of input bucket p1 starting at input[s0]:
+// -> buckets (p1 << L2) + part.  A run starting at an odd position (exact
 // layout) is read with element loads.
-template <class InLayout, class Layout, class Overflow>
-QO_DEV void split_pairs_tile(const uint64_t *__restrict__ input, const InLayout &in, const uint32_t *in_ctr,
-                             uint32_t p1, uint32_t t, const JoinHash &hs, int shift, uint32_t parts,
-                             uint64_t *__restrict__ bucket, const Layout &out, uint32_t *ctr, SplitLds &sh,
-                             const Overflow &overflow) {
-    const uint64_t n = in.count(p1, in_ctr);
-    const uint64_t e0 = (uint64_t)t * kSplitTile;
-    if (e0 >= n) return;  // whole block: no barrier has been reached
-    const uint32_t m = (uint32_t)(n - e0 < kSplitTile ? n - e0 : kSplitTile);
+template <class Layout, class Overflow>
+QO_DEV void split_pairs_run(const uint64_t *__restrict__ input, uint64_t s0, uint32_t m, uint32_t p1,
+                            const JoinHash &hs, int shift, uint32_t parts, uint64_t *__restrict__ bucket,
+                            const Layout &out, uint32_t *ctr, SplitLds &sh, const Overflow &overflow) {
     int32_t key[kSplitJ][4], pay[kSplitJ][4];
     uint32_t pass[kSplitJ];
-    const uint64_t s0 = in.start(p1) + e0;
     if ((s0 & 1u) == 0) {
         load_pairs<kSplitJ, kSplitThreads>(input + s0, m, key, pay, pass);
     } else {
@@ -269,6 +268,19 @@ QO_DEV void split_pairs_tile(const uint64_t *__restrict__ input, const InLayout 
     split_block<kSplitJ>(key, pay, pass, hs, shift, parts, p1 * parts, bucket, out, ctr, sh, overflow);
 }
 
+// Split B on tile t of input bucket p1 (placed by `in`).
+template <class InLayout, class Layout, class Overflow>
+QO_DEV void split_pairs_tile(const uint64_t *__restrict__ input, const InLayout &in, const uint32_t *in_ctr,
+                             uint32_t p1, uint32_t t, const JoinHash &hs, int shift, uint32_t parts,
+                             uint64_t *__restrict__ bucket, const Layout &out, uint32_t *ctr, SplitLds &sh,
+                             const Overflow &overflow) {
+    const uint64_t n = in.count(p1, in_ctr);
+    const uint64_t e0 = (uint64_t)t * kSplitTile;
+    if (e0 >= n) return;  // whole block: no barrier has been reached
+    const uint32_t m = (uint32_t)(n - e0 < kSplitTile ? n - e0 : kSplitTile);
+    split_pairs_run(input, in.start(p1) + e0, m, p1, hs, shift, parts, bucket, out, ctr, sh, overflow);
+}
+
 // Split B with fixed-capacity input buckets: block = (bucket, tile).
 template <class Layout, class Overflow>
 __global__ __launch_bounds__(kSplitThreads) void k_split_pairs(const uint64_t *__restrict__ input, FixedLayout in,
@@ -281,29 +293,49 @@ __global__ __launch_bounds__(kSplitThreads) void k_split_pairs(const uint64_t *_
                      shift, parts, bucket, out, ctr, sh, overflow);
 }
 
-// Split B of the exact split: the blocks enumerate the tiles of all input
-// buckets in order (bucket b has ceil(size(b) / kSplitTile) tiles).
-template <class Overflow>
-__global__ __launch_bounds__(kSplitThreads) void k_split_pairs_exact(const uint64_t *__restrict__ input, ExactLayout in,
-                                                           uint32_t nin, int shift, uint32_t parts,
-                                                           uint64_t *__restrict__ bucket, ExactLayout out,
-                                                           uint32_t *ctr, Overflow overflow) {
-    __shared__ SplitLds sh;
-    __shared__ uint32_t s_b, s_t;
-    // Tile counts of all input buckets at once (nin <= kSplitThreads), their
-    // prefix sums, and the bucket whose tile range holds blockIdx.x.
+// Tile map of split B of the exact split: entry i describes the tile of block
+// i, the tiles of all input buckets in order (bucket b has
+// ceil(size(b) / kSplitTile) tiles), as tile_map_entry(start, pairs, bucket);
+// entries past the last tile are ~0.  One block of kSplitThreads threads
+// (nin <= kSplitThreads).  The block of split B thus finds its input with one
+// load instead of a chain of dependent loads.
+__host__ __device__ inline uint64_t tile_map_entry(uint64_t s0, uint32_t m, uint32_t b) {
+    return s0 | ((uint64_t)m << 32) | ((uint64_t)b << 46);
+}
+static __global__ __launch_bounds__(kSplitThreads) void k_exact_tile_map(ExactLayout in, uint32_t nin, uint64_t *map,
+                                                                         uint32_t entries) {
+    __shared__ uint32_t s_w[kSplitThreads / kWave];
+    __shared__ uint32_t s_first[kSplitThreads + 1];
     const uint32_t nt =
         threadIdx.x < nin ? (uint32_t)((in.count(threadIdx.x, nullptr) + kSplitTile - 1) / kSplitTile) : 0u;
-    if (threadIdx.x == 0) s_b = nin;
-    const uint32_t first = block_excl_scan<kSplitThreads / kWave>(nt, sh.w);
-    if (nt && blockIdx.x >= first && blockIdx.x < first + nt) {
-        s_b = threadIdx.x;
-        s_t = blockIdx.x - first;
-    }
+    const uint32_t first = block_excl_scan<kSplitThreads / kWave>(nt, s_w);
+    s_first[threadIdx.x] = first;
+    if (threadIdx.x == kSplitThreads - 1) s_first[kSplitThreads] = first + nt;
     __syncthreads();
-    if (s_b >= nin) return;  // past the last tile (the grid is an upper bound)
-    split_pairs_tile(input, in, nullptr, s_b, s_t, HashSrc{nullptr, 1}.get(), shift, parts, bucket, out, ctr, sh,
-                     overflow);
+    for (uint32_t b = 0; b < nin; ++b) {
+        const uint32_t f = s_first[b], c = s_first[b + 1] - f;
+        for (uint32_t t = threadIdx.x; t < c; t += kSplitThreads)
+            if (f + t < entries) {
+                const uint64_t e0 = (uint64_t)t * kSplitTile, n = in.count(b, nullptr);
+                map[f + t] = tile_map_entry(in.start(b) + e0, (uint32_t)(n - e0 < kSplitTile ? n - e0 : kSplitTile), b);
+            }
+    }
+    for (uint32_t i = s_first[kSplitThreads] + threadIdx.x; i < entries; i += kSplitThreads) map[i] = ~uint64_t(0);
+}
+
+// Split B of the exact split: block i takes (input bucket, tile) from map[i].
+template <class Overflow>
+__global__ __launch_bounds__(kSplitThreads) void k_split_pairs_exact(const uint64_t *__restrict__ input,
+                                                           const uint64_t *__restrict__ map, int shift,
+                                                           uint32_t parts, uint64_t *__restrict__ bucket,
+                                                           ExactLayout out, uint32_t *ctr, Overflow overflow) {
+    __shared__ SplitLds sh;
+    // XCD-contiguous order: one XCD handles whole input buckets, so the runs
+    // appended to one output bucket come from one L2.
+    const uint64_t e = map[xcd_tile(blockIdx.x, gridDim.x)];
+    if (e == ~uint64_t(0)) return;  // past the last tile (the grid is an upper bound)
+    split_pairs_run(input, e & 0xFFFFFFFFull, (uint32_t)(e >> 32) & 0x3FFFu, (uint32_t)(e >> 46),
+                    HashSrc{nullptr, 1}.get(), shift, parts, bucket, out, ctr, sh, overflow);
 }
 
 // Launches split A (and B) with fixed bucket capacities: kcol/pcol (and fcol
@@ -381,6 +413,7 @@ struct ExactSplitBufs {
     void *scan_ws = nullptr;
     uint64_t *mid = nullptr;   // split A output (n + 2^L pairs), when L2 > 0
     uint64_t *fine = nullptr;  // fine buckets (n + 2^L pairs)
+    uint64_t *tmap = nullptr;  // split B tile map (tiles + 2^L1 entries), when L2 > 0
     size_t ctr_words = 0;
 };
 
@@ -394,7 +427,10 @@ inline ExactSplitBufs exact_split_carve(const SplitPlan &p, size_t n, Carver &cv
     b.hpad = cv.take<uint32_t>(nb + 1);
     b.offp = cv.take<uint32_t>(nb + 1);
     b.scan_ws = cv.take<char>(scan_workspace_bytes(nb + 1));
-    if (p.L2 > 0) b.mid = cv.take<uint64_t>(n + nb);
+    if (p.L2 > 0) {
+        b.mid = cv.take<uint64_t>(n + nb);
+        b.tmap = cv.take<uint64_t>((n + kSplitTile - 1) / kSplitTile + (size_t(1) << p.L1));
+    }
     b.fine = cv.take<uint64_t>(n + nb);
     return b;
 }
@@ -408,7 +444,7 @@ static __global__ void k_round_even(const uint32_t *h, uint32_t m, uint32_t *hp)
 inline int launch_exact_splits(const SplitPlan &p, const ExactSplitBufs &b, const int32_t *kcol,
                                const int32_t *pcol, size_t n, hipStream_t st) {
     const uint32_t nb = 1u << p.L;
-    QO_REQUIRE(n < 0xFFFFFFFFull && nb <= 16384, "exact split too large");
+    QO_REQUIRE(n + nb < 0xFFFFFFFFull && nb <= 16384, "exact split too large");
     QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));
     QO_TRY(launch_fill_u32(b.hist, 0, nb + 1, st));
     const unsigned hg = grid_for(n, 512 * 16, (unsigned)props().cu_count * 2);
@@ -431,8 +467,11 @@ inline int launch_exact_splits(const SplitPlan &p, const ExactSplitBufs &b, cons
         // At most one partial tile per A bucket beyond n / kSplitTile full ones.
         const size_t tiles_b = tiles + partsA;
         QO_REQUIRE(tiles_b < 0x7FFFFFFFull, "input too large for a hash split");
-        k_split_pairs_exact<<<(unsigned)tiles_b, kSplitThreads, 0, st>>>(b.mid, la, partsA, 32 - p.L, 1u << p.L2, b.fine, lf,
-                                                               b.ctr + (size_t)partsA * kCtrStride, none);
+        QO_REQUIRE(partsA <= kSplitThreads, "split A fan-out too large");
+        k_exact_tile_map<<<1, kSplitThreads, 0, st>>>(la, partsA, b.tmap, (uint32_t)tiles_b);
+        QO_LAUNCHED();
+        k_split_pairs_exact<<<(unsigned)tiles_b, kSplitThreads, 0, st>>>(b.mid, b.tmap, 32 - p.L, 1u << p.L2, b.fine,
+                                                               lf, b.ctr + (size_t)partsA * kCtrStride, none);
         QO_LAUNCHED();
     }
     return QO_OK;

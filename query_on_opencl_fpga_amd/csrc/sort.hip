@@ -60,6 +60,7 @@ QO_DEV size_t seg_count(const RadixSeg &s) {
 constexpr int kHistTiles = 2;             // tiles per block of k_tile_hist
 constexpr uint32_t kRowsBlocks = 512;     // at most this many column-scan blocks
 constexpr uint32_t kRowsMinSpan = 64;     // tiles per column-scan block, at least
+constexpr uint32_t kRowsOneBlock = 256;   // up to this many tiles, one block scans the rows
 
 // rows[lt * 256 + d] = keys of tile lt of the segment whose digit is d (zero
 // rows for tiles past the element count).  VEC: the segment's keys start on a
@@ -200,16 +201,6 @@ __global__ __launch_bounds__(256) void k_rows_apply(uint32_t *__restrict__ rows,
     }
 }
 
-// Tile of work-group b when each XCD walks a contiguous tile range in order:
-// work-group b runs on XCD b mod 8, which owns tiles [x*q + min(x, r), ...)
-// with q = ntile / 8, r = ntile mod 8.  Adjacent tiles then write the two
-// halves of a digit run's boundary lines through the same L2, which merges
-// them before they reach HBM.  A bijection of [0, ntile).
-QO_DEV uint32_t xcd_tile(uint32_t b, uint32_t ntile) {
-    const uint32_t x = b & 7u, i = b >> 3, q = ntile >> 3, r = ntile & 7u;
-    return x * q + (x < r ? x : r) + i;
-}
-
 // Places tile xcd_tile(blockIdx.x) of the segment: ranks its keys by digit
 // (stable), stages them in LDS in digit order and writes each digit's run at
 // the offset from its row.
@@ -316,12 +307,18 @@ static int launch_radix_rows(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uin
         else
             k_tile_hist<false><<<hblocks, 256, 0, st>>>(ksrc, sg, sh, b.rows);
         QO_LAUNCHED();
-        k_rows_reduce<<<rp.nb, 256, 0, st>>>(b.rows, sg.ntile, rp.span, b.bsum);
-        QO_LAUNCHED();
-        k_rows_starts<<<1, 1024, 0, st>>>(b.bsum, rp.nb);
-        QO_LAUNCHED();
-        k_rows_apply<<<rp.nb, 256, 0, st>>>(b.rows, sg.ntile, rp.span, b.bsum);
-        QO_LAUNCHED();
+        if (sg.ntile <= kRowsOneBlock) {
+            // Few tiles: one block scans the rows themselves (one launch, not three).
+            k_rows_starts<<<1, 1024, 0, st>>>(b.rows, sg.ntile);
+            QO_LAUNCHED();
+        } else {
+            k_rows_reduce<<<rp.nb, 256, 0, st>>>(b.rows, sg.ntile, rp.span, b.bsum);
+            QO_LAUNCHED();
+            k_rows_starts<<<1, 1024, 0, st>>>(b.bsum, rp.nb);
+            QO_LAUNCHED();
+            k_rows_apply<<<rp.nb, 256, 0, st>>>(b.rows, sg.ntile, rp.span, b.bsum);
+            QO_LAUNCHED();
+        }
         if (vals)
             k_radix_place<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, sh, b.rows);
         else
