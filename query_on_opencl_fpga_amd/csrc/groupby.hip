@@ -174,10 +174,16 @@ QO_DEV void ltable_add(LTable<C> &L, int32_t k, int32_t v, int probes, GEntry *g
 }
 
 // Merges the block's LDS partials into the global table (after a barrier).
+// Block b walks its slots starting at (97 b) mod C, so blocks that finish
+// together update different global entries instead of queueing on the same
+// ones (measured on 2^28 tuples, 1024 groups: 0.46 -> 0.43 ms).
 template <int C>
 QO_DEV void ltable_flush(const LTable<C> &L, GEntry *gt, uint64_t gmask, uint32_t *overflow) {
-    for (int i = threadIdx.x; i < C; i += blockDim.x)
+    const int rot = (int)((blockIdx.x * 97u) & (C - 1));
+    for (int j = threadIdx.x; j < C; j += blockDim.x) {
+        const int i = (j + rot) & (C - 1);
         if (L.cnt[i]) g_update(gt, gmask, L.key[i], L.cnt[i], L.sum[i], L.mn[i], L.mx[i], overflow);
+    }
 }
 
 // Appends the block's LDS groups as finished rows (the block saw every tuple
@@ -214,8 +220,8 @@ QO_DEV void ltable_append(const LTable<C> &L, GEntry *rows, unsigned long long *
     }
 }
 
-template <int C, bool VEC>
-__global__ __launch_bounds__(512) void k_groupby_hash(const int32_t *__restrict__ key, const int32_t *__restrict__ val,
+template <int C, bool VEC, int NT>
+__global__ __launch_bounds__(NT) void k_groupby_hash(const int32_t *__restrict__ key, const int32_t *__restrict__ val,
                                                       size_t n, GEntry *gt, uint64_t gmask, int probes,
                                                       uint32_t *overflow) {
     extern __shared__ unsigned char s_raw[];
@@ -487,7 +493,12 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         int C = tp.items >= 4096 ? 4096 : 2048;
         const size_t lds4096 = sizeof(LTable<4096>);
         if (C == 4096 && (size_t)props().lds_per_block < lds4096) C = 2048;
-        const int block = 512;
+        // One 1024-thread block per CU by default: every block merges its whole
+        // LDS table into the global one at the end, and those global atomics
+        // (blocks x groups x 4) cost more than the occupancy of more blocks buys
+        // (2^28 tuples, 1024 groups: 0.68 ms with 3 x 512 threads per CU, 0.43
+        // ms with 1 x 1024).
+        constexpr int block = 1024;
         const unsigned grid = grid_for(n, block * 4, (unsigned)props().cu_count * (unsigned)std::max(1, tp.param));
         // Low cardinality: the LDS table holds every group.  High cardinality:
         // it caches the keys seen first and the rest go straight to global.
@@ -495,15 +506,15 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         if (C == 4096) {
             const size_t lds = sizeof(LTable<4096>);
             if (vec)
-                k_groupby_hash<4096, true><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+                k_groupby_hash<4096, true, block><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
             else
-                k_groupby_hash<4096, false><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+                k_groupby_hash<4096, false, block><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
         } else {
             const size_t lds = sizeof(LTable<2048>);
             if (vec)
-                k_groupby_hash<2048, true><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+                k_groupby_hash<2048, true, block><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
             else
-                k_groupby_hash<2048, false><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+                k_groupby_hash<2048, false, block><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
         }
         QO_LAUNCHED();
     }

@@ -175,7 +175,7 @@ TuneParams tune_default(int op, int cls) {
         case TUNE_FILTER_AGG: return {256, 4, 8};      // block, unroll, blocks per CU
         case TUNE_FILTER: return {256, 4, 0};          // block, uint4 loads per thread
         case TUNE_PROBE: return {256, 4, 50};          // block, int4 groups per thread, load %
-        case TUNE_GROUPBY: return {512, 2048, 3};      // block, LDS slots, blocks per CU
+        case TUNE_GROUPBY: return {1024, 2048, 1};     // block, LDS slots, blocks per CU
         case TUNE_SORT: return {256, 16, 0};           // block, keys per thread
         case TUNE_JOIN_PART: return {256, 32, 0};      // block, LDS table slice KiB, mode
         case TUNE_GROUPBY_PART: return {512, 1024, 0}; // block, groups per fine bucket, mode
