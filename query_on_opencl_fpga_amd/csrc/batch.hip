@@ -6,9 +6,10 @@
 // maps to its query through the per-query first-tile table, so a 4K-row query
 // and a 4M-row query share the same launch.  The probe writes sort-ready
 // (key, s_payload) 64-bit keys with the r_payload beside them; a segmented
-// one-sweep LSD radix sort (one segment per query, element counts read from
-// device memory, per-query digit counts up front, one kernel per pass) orders
-// every query's matches, and an unpack kernel writes the SoA outputs.
+// LSD radix sort (one segment per query, element counts read from device
+// memory, per-query digit starts up front, then per pass a tile histogram, a
+// segmented column scan and the placement) orders every query's matches, and
+// an unpack kernel writes the SoA outputs.
 // The stage sequence (fills included, no memset nodes) is captured once into a
 // graph at creation and replayed by qo_batch_launch.
 //
@@ -189,7 +190,7 @@ static void plan_carve(JoinBatchPlan &p, Carver &cv) {
     p.valt = cv.take<uint32_t>(p.total_match);
     p.d_chunks = cv.take<SweepChunk>(p.hchunks.size() + 1);
     p.d_chunk0 = cv.take<uint32_t>(p.hchunk0.size());
-    p.sort_ws = cv.take<char>(radix_sweep_seg_bytes(p.mt, p.nq, (uint32_t)p.hchunks.size()));
+    p.sort_ws = cv.take<char>(radix_rows_seg_bytes(p.mt, p.nq, (uint32_t)p.hchunks.size()));
 }
 
 size_t join_plan_bytes(JoinBatchPlan &p) {
@@ -241,7 +242,7 @@ int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
     }
     if (p.mt) {
         bool in_alt = false;
-        QO_TRY(launch_radix_sweep_segments(p.mkeys, p.mvals, p.kalt, p.valt, p.d_segs, p.nq, p.mt, p.d_chunks,
+        QO_TRY(launch_radix_rows_segments(p.mkeys, p.mvals, p.kalt, p.valt, p.d_segs, p.nq, p.mt, p.d_chunks,
                                            (uint32_t)p.hchunks.size(), p.d_chunk0, 0, 64, p.sort_ws, st, &in_alt));
         k_b_unpack<<<p.mt, 256, 0, st>>>(p.d_q, p.nq, in_alt ? p.kalt : p.mkeys, in_alt ? p.valt : p.mvals);
         QO_LAUNCHED();

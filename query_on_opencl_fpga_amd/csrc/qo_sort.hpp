@@ -22,20 +22,21 @@ struct RadixSeg {
 };
 
 // Tiles [t0, t1) (segment-local) of segment `seg`: one block of the up-front
-// digit count of a segmented one-sweep sort.
+// digit count of a segmented sort.
 struct SweepChunk {
     uint32_t seg, t0, t1;
 };
 constexpr uint32_t kSweepChunkTiles = 16;
-// Workspace of launch_radix_sweep_segments.
-size_t radix_sweep_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks);
-// Segmented one-sweep sort by bits [lo_bit, hi_bit): digit counts per segment
-// from the chunks (segment s owns chunks [chunk0[s], chunk0[s + 1])), then one
-// kernel per pass.  On return *in_alt tells whether the data ended in kalt / valt.
-int launch_radix_sweep_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
-                                const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
-                                uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
-                                hipStream_t st, bool *in_alt);
+// Workspace of launch_radix_rows_segments.
+size_t radix_rows_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks);
+// Segmented sort by bits [lo_bit, hi_bit): digit starts per segment from the
+// chunks (segment s owns chunks [chunk0[s], chunk0[s + 1])), then per pass a
+// tile histogram, a segmented column scan and the placement.  On return
+// *in_alt tells whether the data ended in kalt / valt.
+int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
+                               const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
+                               uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
+                               hipStream_t st, bool *in_alt);
 // Join rows sort ascending by (key, s_payload) as signed int32: flip the sign
 // bits so unsigned 64-bit order is the signed lexicographic order.
 __host__ __device__ __forceinline__ uint64_t join_sort_key(int32_t key, int32_t spay) {

@@ -3,16 +3,16 @@
 // match masks (8 ballots per key), stages the tile in LDS in digit order and
 // writes contiguous runs per digit.
 //
-// A single array takes reduce-then-scan passes: per-tile digit counts in
-// tile-major rows (LDS atomics, one read of the keys), a column scan of the
-// rows that turns them into the global output offset of every (tile, digit),
-// and the placement kernel, which reads its tile's 256 offsets as one
-// coalesced row.  A batch of many arrays (one per query) takes one-sweep
-// passes: digit counts of all passes up front, then one kernel per pass whose
-// tiles find the counts of earlier tiles with a per-digit decoupled look-back.
-// A segment owns a contiguous tile range; its element count may live in device
-// memory (written by an earlier kernel), so a whole batch can be captured in
-// one graph without host round trips.
+// Every pass is reduce-then-scan: per-tile digit counts in tile-major rows
+// (LDS atomics, one read of the keys), a column scan of the rows that turns
+// them into the output offset of every (tile, digit), and the placement
+// kernel, which reads its tile's 256 offsets as one coalesced row and walks
+// the tiles in XCD-contiguous order.  A batch of many arrays (one per query)
+// sorts in the same launches: a segment owns a contiguous tile range, the
+// column scan restarts at every segment's first tile, and the digit starts of
+// every pass per segment come from one histogram up front.  A segment's
+// element count may live in device memory (written by an earlier kernel), so a
+// whole batch can be captured in one graph without host round trips.
 //
 // Reference path replaced: radixSortImpl_sm (host/host/testSort.cpp:24-136)
 // driving bitonicSortShared / bitonicMergeGlobal (kernel/aggregation/1/
@@ -62,52 +62,60 @@ constexpr uint32_t kRowsBlocks = 512;     // at most this many column-scan block
 constexpr uint32_t kRowsMinSpan = 64;     // tiles per column-scan block, at least
 constexpr uint32_t kRowsOneBlock = 256;   // up to this many tiles, one block scans the rows
 
-// rows[lt * 256 + d] = keys of tile lt of the segment whose digit is d (zero
-// rows for tiles past the element count).  VEC: the segment's keys start on a
-// 16-byte boundary, so two keys are read per load.
-template <bool VEC>
-__global__ __launch_bounds__(256) void k_tile_hist(const uint64_t *__restrict__ keys, RadixSeg sg, int shift,
+// Digit counts of one tile (rem keys at src) into 256 LDS counters; `vec`:
+// src is 16-byte aligned, so two keys are read per load.
+QO_DEV void hist_tile(const uint64_t *__restrict__ src, uint32_t rem, int shift, uint32_t *s_row, bool vec) {
+    if (vec) {
+        constexpr int PER = kSortTile / 512;
+        uint4 q[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t v = j * 256 + threadIdx.x;
+            q[j] = (2 * v + 2 <= rem) ? ld_stream(reinterpret_cast<const uint4 *>(src) + v) : uint4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t v = j * 256 + threadIdx.x;
+            if (2 * v + 2 <= rem) {
+                const uint64_t k0 = ((uint64_t)q[j].y << 32) | q[j].x;
+                const uint64_t k1 = ((uint64_t)q[j].w << 32) | q[j].z;
+                atomicAdd(&s_row[(k0 >> shift) & 255u], 1u);
+                atomicAdd(&s_row[(k1 >> shift) & 255u], 1u);
+            } else if (2 * v < rem) {
+                atomicAdd(&s_row[(src[2 * v] >> shift) & 255u], 1u);
+            }
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < rem; i += 256) atomicAdd(&s_row[(src[i] >> shift) & 255u], 1u);
+    }
+}
+
+// rows[t * 256 + d] = keys of global tile t whose digit is d (zero rows for
+// tiles past their segment's element count).  A single array passes its
+// segment as seg0 with segs = null; a batch passes the segment list.
+__global__ __launch_bounds__(256) void k_tile_hist(const uint64_t *__restrict__ keys, RadixSeg seg0,
+                                                   const RadixSeg *segs, int nseg, uint32_t ntile, int shift,
                                                    uint32_t *__restrict__ rows) {
     __shared__ uint32_t s_h[kHistTiles][256];
     for (int i = threadIdx.x; i < kHistTiles * 256; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
-    const size_t n = seg_count(sg);
     const uint32_t t0 = blockIdx.x * kHistTiles;
 #pragma unroll
     for (int tt = 0; tt < kHistTiles; ++tt) {
-        const uint32_t lt = t0 + tt;
-        const size_t tb = (size_t)lt * kSortTile;
-        if (lt >= sg.ntile || tb >= n) break;
-        const uint32_t rem = (uint32_t)std::min<size_t>(kSortTile, n - tb);
+        const uint32_t t = t0 + tt;
+        if (t >= ntile) break;
+        const RadixSeg sg = segs ? segs[seg_index_of(segs, nseg, t)] : seg0;
+        const size_t n = seg_count(sg);
+        const size_t tb = (size_t)(t - sg.tile0) * kSortTile;
+        if (tb >= n) continue;
         const uint64_t *src = keys + sg.base + tb;
-        if constexpr (VEC) {
-            constexpr int PER = kSortTile / 512;
-            uint4 q[PER];
-#pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t v = j * 256 + threadIdx.x;
-                q[j] = (2 * v + 2 <= rem) ? ld_stream(reinterpret_cast<const uint4 *>(src) + v) : uint4{0, 0, 0, 0};
-            }
-#pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t v = j * 256 + threadIdx.x;
-                if (2 * v + 2 <= rem) {
-                    const uint64_t k0 = ((uint64_t)q[j].y << 32) | q[j].x;
-                    const uint64_t k1 = ((uint64_t)q[j].w << 32) | q[j].z;
-                    atomicAdd(&s_h[tt][(k0 >> shift) & 255u], 1u);
-                    atomicAdd(&s_h[tt][(k1 >> shift) & 255u], 1u);
-                } else if (2 * v < rem) {
-                    atomicAdd(&s_h[tt][(src[2 * v] >> shift) & 255u], 1u);
-                }
-            }
-        } else {
-            for (uint32_t i = threadIdx.x; i < rem; i += 256) atomicAdd(&s_h[tt][(src[i] >> shift) & 255u], 1u);
-        }
+        hist_tile(src, (uint32_t)std::min<size_t>(kSortTile, n - tb), shift, s_h[tt],
+                  (reinterpret_cast<uintptr_t>(src) & 15) == 0);
     }
     __syncthreads();
 #pragma unroll
     for (int tt = 0; tt < kHistTiles; ++tt)
-        if (t0 + tt < sg.ntile) rows[(size_t)(t0 + tt) * 256 + threadIdx.x] = s_h[tt][threadIdx.x];
+        if (t0 + tt < ntile) rows[(size_t)(t0 + tt) * 256 + threadIdx.x] = s_h[tt][threadIdx.x];
 }
 
 // Column-scan layout: block b of the scan owns tiles [b * span, (b + 1) * span).
@@ -201,22 +209,107 @@ __global__ __launch_bounds__(256) void k_rows_apply(uint32_t *__restrict__ rows,
     }
 }
 
-// Places tile xcd_tile(blockIdx.x) of the segment: ranks its keys by digit
-// (stable), stages them in LDS in digit order and writes each digit's run at
-// the offset from its row.
+// ---- segmented column scan (a batch: one array per segment) ----
+// Walks global tiles t0, t0 + 1, ... and tells which of them start a segment.
+struct SegWalk {
+    const RadixSeg *segs;
+    int nseg, si;
+    QO_DEV SegWalk(const RadixSeg *s, int n, uint32_t t0) : segs(s), nseg(n), si(seg_index_of(s, n, t0)) {}
+    QO_DEV bool starts(uint32_t t) {
+        while (si + 1 < nseg && segs[si + 1].tile0 <= t) ++si;
+        return segs[si].tile0 == t;
+    }
+};
+
+// bsum[b][d] = digit-d keys of block b's tiles after the last segment start
+// inside the block (all of them when none starts there); bflag[b] = a segment
+// starts inside the block.
+__global__ __launch_bounds__(256) void k_rows_reduce_seg(const uint32_t *__restrict__ rows, uint32_t ntile,
+                                                         uint32_t span, const RadixSeg *segs, int nseg,
+                                                         uint32_t *__restrict__ bsum, uint32_t *__restrict__ bflag) {
+    const uint32_t t0 = blockIdx.x * span, t1 = std::min(ntile, t0 + span);
+    const uint32_t *col = rows + threadIdx.x;
+    SegWalk wk(segs, nseg, t0);
+    uint32_t tail = 0, flag = 0;
+    for (uint32_t t = t0; t < t1; ++t) {
+        if (wk.starts(t)) {
+            tail = 0;
+            flag = 1;
+        }
+        tail += col[(size_t)t * 256];
+    }
+    bsum[(size_t)blockIdx.x * 256 + threadIdx.x] = tail;
+    if (threadIdx.x == 0) bflag[blockIdx.x] = flag;
+}
+
+// One block of 1024 threads: bsum[b][d] := digit-d keys of the blocks before b
+// back to the start of b's first segment (exclusive carry, in place).  Thread
+// (q, d) walks quarter q of the blocks; the quarters' (flag, sum) pairs combine
+// in order.
+__global__ __launch_bounds__(1024) void k_rows_starts_seg(uint32_t *bsum, const uint32_t *bflag, uint32_t nb) {
+    __shared__ uint32_t s_part[4][256];
+    __shared__ uint32_t s_flag[4];
+    const uint32_t d = threadIdx.x & 255u, q = threadIdx.x >> 8;
+    const uint32_t per = (nb + 3) / 4, b0 = std::min(nb, q * per), b1 = std::min(nb, b0 + per);
+    uint32_t *col = bsum + d;
+    uint32_t agg = 0, f = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t v = col[(size_t)b * 256];
+        if (bflag[b]) {
+            agg = v;
+            f = 1;
+        } else {
+            agg += v;
+        }
+    }
+    s_part[q][d] = agg;
+    if (d == 0) s_flag[q] = f;
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t qq = 0; qq < q; ++qq) c = s_flag[qq] ? s_part[qq][d] : c + s_part[qq][d];
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t v = col[(size_t)b * 256];
+        col[(size_t)b * 256] = c;
+        c = bflag[b] ? v : c + v;
+    }
+}
+
+// rows[t][d] := digit-d keys of the tiles of t's segment before t (in place).
+__global__ __launch_bounds__(256) void k_rows_apply_seg(uint32_t *__restrict__ rows, uint32_t ntile, uint32_t span,
+                                                        const RadixSeg *segs, int nseg,
+                                                        const uint32_t *__restrict__ bsum) {
+    const uint32_t t0 = blockIdx.x * span, t1 = std::min(ntile, t0 + span);
+    uint32_t *col = rows + threadIdx.x;
+    SegWalk wk(segs, nseg, t0);
+    uint32_t run = bsum[(size_t)blockIdx.x * 256 + threadIdx.x];
+    for (uint32_t t = t0; t < t1; ++t) {
+        if (wk.starts(t)) run = 0;
+        const uint32_t v = col[(size_t)t * 256];
+        col[(size_t)t * 256] = run;
+        run += v;
+    }
+}
+
+// Places global tile xcd_tile(blockIdx.x): ranks its keys by digit (stable),
+// stages them in LDS in digit order and writes each digit's run at the offset
+// from its row (plus, for a batch, the digit's start in the segment's output).
 template <bool HAS_VAL>
 __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                     RadixSeg sg, int shift, const uint32_t *__restrict__ offs) {
+                                                     RadixSeg seg0, const RadixSeg *segs, int nseg, uint32_t ntile,
+                                                     int shift, const uint32_t *__restrict__ offs,
+                                                     const uint32_t *__restrict__ dstart, uint32_t sstride) {
     __shared__ uint32_t s_cnt[4][256];
     __shared__ uint32_t s_dpre[256];
     __shared__ uint32_t s_gofs[256];
     __shared__ uint32_t s_w[4];
     __shared__ uint64_t s_k[kSortTile];
     __shared__ uint32_t s_v[HAS_VAL ? kSortTile : 1];
-    const uint32_t lt = xcd_tile(blockIdx.x, sg.ntile);
+    const uint32_t t = xcd_tile(blockIdx.x, ntile);
+    const int si = segs ? seg_index_of(segs, nseg, t) : 0;
+    const RadixSeg sg = segs ? segs[si] : seg0;
     const size_t n = seg_count(sg);
-    const size_t tb = (size_t)lt * kSortTile;
+    const size_t tb = (size_t)(t - sg.tile0) * kSortTile;
     if (tb >= n) return;
     const uint32_t rem = (uint32_t)std::min<size_t>(kSortTile, n - tb);
     const uint64_t *ks = kin + sg.base + tb;
@@ -250,7 +343,7 @@ __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict_
     {
         const unsigned d = threadIdx.x;
         const uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
-        s_gofs[d] = offs[(size_t)lt * 256 + d];
+        s_gofs[d] = offs[(size_t)t * 256 + d] + (dstart ? dstart[(size_t)si * sstride + d] : 0u);
         s_dpre[d] = block_excl_scan256(c0 + c1 + c2 + c3, s_w);
         s_cnt[0][d] = 0;
         s_cnt[1][d] = c0;
@@ -302,10 +395,7 @@ static int launch_radix_rows(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uin
     uint64_t *ksrc = keys, *kdst = kalt;
     uint32_t *vsrc = vals, *vdst = valt;
     for (int sh = lo_bit; sh < hi_bit; sh += 8) {
-        if ((reinterpret_cast<uintptr_t>(ksrc + sg.base) & 15) == 0)
-            k_tile_hist<true><<<hblocks, 256, 0, st>>>(ksrc, sg, sh, b.rows);
-        else
-            k_tile_hist<false><<<hblocks, 256, 0, st>>>(ksrc, sg, sh, b.rows);
+        k_tile_hist<<<hblocks, 256, 0, st>>>(ksrc, sg, nullptr, 1, sg.ntile, sh, b.rows);
         QO_LAUNCHED();
         if (sg.ntile <= kRowsOneBlock) {
             // Few tiles: one block scans the rows themselves (one launch, not three).
@@ -320,9 +410,11 @@ static int launch_radix_rows(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uin
             QO_LAUNCHED();
         }
         if (vals)
-            k_radix_place<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, sh, b.rows);
+            k_radix_place<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, nullptr, 1, sg.ntile, sh,
+                                                           b.rows, nullptr, 0);
         else
-            k_radix_place<false><<<sg.ntile, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, sg, sh, b.rows);
+            k_radix_place<false><<<sg.ntile, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, sg, nullptr, 1, sg.ntile,
+                                                            sh, b.rows, nullptr, 0);
         QO_LAUNCHED();
         std::swap(ksrc, kdst);
         std::swap(vsrc, vdst);
@@ -332,13 +424,10 @@ static int launch_radix_rows(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uin
 }
 
 // ---------------------------------------------------------------------------
-// One-sweep passes for a batch of arrays: the digit counts of every pass come
-// from one histogram over the keys up front; each pass is then a single
-// kernel that ranks its tile, publishes the tile's per-digit counts and finds
-// the counts of all earlier tiles of its segment with a decoupled look-back
-// per digit (thread d of the block walks digit d).  Tiles are numbered by a
-// ticket: every tile a block waits for belongs to a block that is already
-// running.
+// A batch of arrays (one per query) in the same launches: the digit starts of
+// every pass per segment come from one histogram over the keys up front
+// (chunks of kSweepChunkTiles tiles); each pass then takes the tile
+// histogram, a segmented column scan and the placement above.
 // ---------------------------------------------------------------------------
 constexpr int kSweepMaxPass = 8;
 
@@ -373,145 +462,63 @@ __global__ __launch_bounds__(256) void k_sweep_starts_seg(const uint32_t *partia
     starts[(size_t)blockIdx.x * 256 + threadIdx.x] = block_excl_scan256(c, s_w);
 }
 
-template <bool HAS_VAL>
-__global__ __launch_bounds__(256) void k_radix_sweep(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                                     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                     RadixSeg seg0, const RadixSeg *segs, int nseg, int shift,
-                                                     const uint32_t *__restrict__ starts, uint32_t sstride,
-                                                     uint64_t *status, uint32_t *ticket) {
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_cnt[4][256];
-    __shared__ uint32_t s_dpre[256];
-    __shared__ uint32_t s_gofs[256];
-    __shared__ uint32_t s_w[4];
-    __shared__ uint64_t s_k[kSortTile];
-    __shared__ uint32_t s_v[HAS_VAL ? kSortTile : 1];
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    for (int i = threadIdx.x; i < 1024; i += 256) (&s_cnt[0][0])[i] = 0;
-    __syncthreads();
-    const uint32_t tile = s_tile;  // global tile; the segment's tiles are [sg.tile0, sg.tile0 + sg.ntile)
-    const int si = seg_index_of(segs, nseg, tile);
-    const RadixSeg sg = segs ? segs[si] : seg0;
-    const uint32_t lt = tile - sg.tile0;
-    const uint32_t *dstart = starts + (size_t)si * sstride;
-    const size_t n = seg_count(sg);
-    if ((size_t)lt * kSortTile >= n) return;  // whole block; no later tile of the segment waits for it
-    const unsigned w = wave_id(), lane = lane_id();
-    const size_t wbase = (size_t)lt * kSortTile + (size_t)w * (kSortTile / 4);
-    uint64_t k[kSortItems];
-    uint32_t v[kSortItems];
-    uint32_t rank[kSortItems];
-#pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        size_t idx = wbase + (size_t)j * kWave + lane;
-        bool valid = idx < n;
-        k[j] = valid ? kin[sg.base + idx] : ~uint64_t(0);
-        if (HAS_VAL) v[j] = valid ? vin[sg.base + idx] : 0u;
-    }
-#pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        size_t idx = wbase + (size_t)j * kWave + lane;
-        bool valid = idx < n;
-        uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
-        uint64_t m = match_digit(d, valid);
-        uint32_t pre = s_cnt[w][d];
-        uint32_t r = popc(m & lanemask_lt());
-        if (valid && r == 0) s_cnt[w][d] = pre + popc(m);
-        rank[j] = pre + r;
-    }
-    __syncthreads();
-    {
-        const unsigned d = threadIdx.x;
-        uint32_t c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
-        uint32_t tot = c0 + c1 + c2 + c3;
-        // Publish this tile's count of digit d, then add up the earlier tiles'.
-        uint64_t *my = status + (size_t)tile * 256 + d;
-        uint64_t excl = 0;
-        if (lt == 0) {
-            st_agent(my, kFlagInc | tot);
-        } else {
-            st_agent(my, kFlagAgg | tot);
-            for (int64_t tp = (int64_t)tile - 1; tp >= (int64_t)sg.tile0; --tp) {
-                uint64_t st;
-                while (((st = ld_agent(status + (size_t)tp * 256 + d)) >> 62) == 0) __builtin_amdgcn_s_sleep(1);
-                excl += st & kValMask;
-                if ((st >> 62) == 2) break;
-            }
-            st_agent(my, kFlagInc | ((excl + tot) & kValMask));
-        }
-        s_gofs[d] = dstart[d] + (uint32_t)excl;
-        uint32_t ex = block_excl_scan256(tot, s_w);
-        s_dpre[d] = ex;
-        s_cnt[0][d] = 0;
-        s_cnt[1][d] = c0;
-        s_cnt[2][d] = c0 + c1;
-        s_cnt[3][d] = c0 + c1 + c2;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-        size_t idx = wbase + (size_t)j * kWave + lane;
-        if (idx < n) {
-            uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
-            uint32_t p = s_dpre[d] + s_cnt[w][d] + rank[j];
-            s_k[p] = k[j];
-            if (HAS_VAL) s_v[p] = v[j];
-        }
-    }
-    __syncthreads();
-    const size_t tstart = (size_t)lt * kSortTile;
-    const uint32_t tcount = (uint32_t)((n - tstart) < (size_t)kSortTile ? (n - tstart) : (size_t)kSortTile);
-    for (uint32_t i = threadIdx.x; i < tcount; i += 256) {
-        uint64_t kk = s_k[i];
-        uint32_t d = (uint32_t)(kk >> shift) & 255u;
-        size_t gp = sg.base + (size_t)s_gofs[d] + (i - s_dpre[d]);
-        kout[gp] = kk;
-        if (HAS_VAL) vout[gp] = s_v[i];
-    }
-}
-
 static uint32_t radix_tiles(size_t n) { return (uint32_t)((n + kSortTile - 1) / kSortTile); }
 
-size_t radix_sweep_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks) {
+// Workspace of launch_radix_rows_segments (same order as its carve-up).
+size_t radix_rows_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks) {
     Carver cv(nullptr);
     cv.take<uint32_t>((size_t)nchunks * kSweepMaxPass * 256);
     cv.take<uint32_t>((size_t)nseg * kSweepMaxPass * 256);
-    cv.take<uint32_t>(64);
-    cv.take<uint64_t>((size_t)total_tiles * 256 + 1);
+    cv.take<uint32_t>((size_t)total_tiles * 256);
+    const uint32_t nbmax = std::min(kRowsBlocks, (total_tiles + kRowsMinSpan - 1) / kRowsMinSpan);
+    cv.take<uint32_t>((size_t)nbmax * 256);
+    cv.take<uint32_t>(nbmax);
     return cv.off + 256;
 }
 
-int launch_radix_sweep_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
-                                const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
-                                uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
-                                hipStream_t st, bool *in_alt) {
+int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
+                               const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
+                               uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
+                               hipStream_t st, bool *in_alt) {
     *in_alt = false;
     if (total_tiles == 0 || nchunks == 0) return QO_OK;
     const int npass = (hi_bit - lo_bit + 7) / 8;
     QO_REQUIRE(npass >= 1 && npass <= kSweepMaxPass && lo_bit >= 0 && hi_bit <= 64, "bad bit range");
     QO_REQUIRE(nseg >= 1 && (size_t)nseg * npass < 0x7FFFFFFFull, "bad segment count");
+    QO_REQUIRE((size_t)total_tiles * 256 < 0xFFFFFFFFull, "sort input too large");
     Carver cv(ws);
     uint32_t *partial = cv.take<uint32_t>((size_t)nchunks * npass * 256);
     uint32_t *starts = cv.take<uint32_t>((size_t)nseg * npass * 256);
-    uint32_t *ticket = cv.take<uint32_t>(64);
-    uint64_t *status = cv.take<uint64_t>((size_t)total_tiles * 256 + 1);
+    uint32_t *rows = cv.take<uint32_t>((size_t)total_tiles * 256);
+    const RowsPlan rp = rows_plan(total_tiles);
+    uint32_t *bsum = cv.take<uint32_t>((size_t)rp.nb * 256);
+    uint32_t *bflag = cv.take<uint32_t>(rp.nb);
+    // Digit starts of every pass per segment, from one read of the keys.
     k_sweep_counts_seg<<<nchunks, 256, 0, st>>>(keys, d_segs, d_chunks, lo_bit, npass, partial);
     QO_LAUNCHED();
     k_sweep_starts_seg<<<(unsigned)(nseg * npass), 256, 0, st>>>(partial, d_chunk0, npass, starts);
     QO_LAUNCHED();
     const RadixSeg none{0, 0, nullptr, 0, 0};
+    const unsigned hblocks = (total_tiles + kHistTiles - 1) / kHistTiles;
     uint64_t *ksrc = keys, *kdst = kalt;
     uint32_t *vsrc = vals, *vdst = valt;
     for (int p = 0; p < npass; ++p) {
-        QO_TRY(launch_fill_u64(status, 0, (size_t)total_tiles * 256, st));
-        QO_TRY(launch_fill_u32(ticket, 0, 1, st));
         const int sh = lo_bit + 8 * p;
+        k_tile_hist<<<hblocks, 256, 0, st>>>(ksrc, none, d_segs, nseg, total_tiles, sh, rows);
+        QO_LAUNCHED();
+        k_rows_reduce_seg<<<rp.nb, 256, 0, st>>>(rows, total_tiles, rp.span, d_segs, nseg, bsum, bflag);
+        QO_LAUNCHED();
+        k_rows_starts_seg<<<1, 1024, 0, st>>>(bsum, bflag, rp.nb);
+        QO_LAUNCHED();
+        k_rows_apply_seg<<<rp.nb, 256, 0, st>>>(rows, total_tiles, rp.span, d_segs, nseg, bsum);
+        QO_LAUNCHED();
         if (vals)
-            k_radix_sweep<true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, none, d_segs, nseg, sh,
-                                                             starts + p * 256, (uint32_t)npass * 256, status, ticket);
+            k_radix_place<true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, none, d_segs, nseg, total_tiles,
+                                                             sh, rows, starts + p * 256, (uint32_t)npass * 256);
         else
-            k_radix_sweep<false><<<total_tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, none, d_segs, nseg, sh,
-                                                              starts + p * 256, (uint32_t)npass * 256, status, ticket);
+            k_radix_place<false><<<total_tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, none, d_segs, nseg,
+                                                              total_tiles, sh, rows, starts + p * 256,
+                                                              (uint32_t)npass * 256);
         QO_LAUNCHED();
         std::swap(ksrc, kdst);
         std::swap(vsrc, vdst);
