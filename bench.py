@@ -505,27 +505,23 @@ def run_batched(args, qo, lib) -> dict:
 
 
 def _serve(qo, tables, schedule, plain: bool, args) -> dict:
-    """Submits `schedule` (list of (t_ms, request tuple)) at its arrival times and
-    returns per-request latencies measured on the server clock."""
+    """Queues `schedule` (list of (t_ms, request tuple)) with its arrival times on
+    the server clock, serves it and returns per-request latencies (end - arrival)."""
     from query_on_opencl_fpga_amd.serving import Server
 
-    sv = Server(budget_bytes=args.budget_gb * 1e9, inflight=args.inflight, batch=args.batch_max, plain=plain)
+    sv = Server(budget_bytes=args.budget_gb * 1e9, inflight=1, batch=args.batch_max, plain=plain)
     ids = [sv.add_table(t["n"], key=t["key"], fk=t["fk"], grp=t["grp"], val=t["val"], filt=t["filt"],
                         group_capacity=t["G"]) for t in tables]
     # Warm-up: one of each kind (workspaces and pools reach steady state).
     warm = [(k, ids[i % len(ids)], ids[-1], INT_MIN, 499) for i, k in enumerate((0, 1, 2))]
     for r in warm:
         sv.submit(*r)
-    sv.drain()
+    sv.run()
     sv.poll(timeout_ms=0)
-    base = sv.now_ms()
-    t_start = time.perf_counter()
+    base = sv.now_ms() + 2.0
     for t_ms, (kind, a, b, lo, hi) in schedule:
-        delay = t_ms / 1e3 - (time.perf_counter() - t_start)
-        if delay > 0:
-            time.sleep(delay)
-        sv.submit(kind, ids[a], ids[b] if b >= 0 else -1, lo, hi)
-    sv.drain()
+        sv.submit_at(kind, ids[a], ids[b] if b >= 0 else -1, lo, hi, arrival_ms=base + t_ms)
+    sv.run()
     res = sv.poll(max_results=len(schedule) + 16, timeout_ms=0)
     sv.destroy()
     bad = [r for r in res if r["status"] != 0]
@@ -588,7 +584,7 @@ def run_served(args, qo, lib) -> dict:
         "config": {"workload": "served mixed stream (filter 40%, group-by 30%, join+sort 30%), Poisson arrivals",
                    "offered_qps": args.qps, "requests": args.requests, "tables": args.tables,
                    "table_rows_total": int(sum(tb["n"] for tb in tables)), "budget_gb": args.budget_gb,
-                   "inflight_batches": args.inflight, "max_batch": args.batch_max},
+                   "inflight_batches": 1, "max_batch": args.batch_max},
         "served": served, "plain": plain,
         "throughput_ratio_served_vs_plain": served["queries_per_s"] / plain["queries_per_s"],
         "p99_ratio_plain_vs_served": plain["p99_ms"] / served["p99_ms"],
@@ -664,7 +660,6 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--tables", type=int, default=16)
     ap.add_argument("--budget-gb", type=float, default=200.0)
-    ap.add_argument("--inflight", type=int, default=4)
     ap.add_argument("--batch-max", type=int, default=64)
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="fraction of the log2 size range for batched queries (1.0 = [4K, 4M])")

@@ -226,16 +226,21 @@ int qo_tune_load(const char *path);
 int qo_tune_save(const char *path);
 
 /* ---- serving loop ------------------------------------------------------------ */
-/* A request loop over tables resident in HBM.  Submitted requests are admitted
- * against a device-memory budget; whenever a stream slot is free, every admitted
- * request not yet started is launched as one batch on that slot's stream
- * (continuous batching), so batches overlap on the device.  With `plain` set the
- * loop runs each request alone, in arrival order, through the per-query
- * operator API (the baseline).  Results are summaries computed on the device. */
+/* A request loop over tables resident in HBM, run on the caller's thread by
+ * qo_server_run / drain / poll.  Each request has an arrival time; the loop
+ * takes every request that has arrived (arrival order, up to
+ * max_batch_queries, within the device-memory budget) as ONE batch on the
+ * server's stream, waits for it, and starts over with what arrived meanwhile
+ * (continuous batching).  Requests of a batch share launches (one multi-query
+ * filter kernel, the batched join pipeline), so their work overlaps on the
+ * device.  A request larger than the whole budget fails with QO_ERR_BUDGET.
+ * With `plain` set the loop runs each request alone, in arrival order, through
+ * the per-query operator API (the baseline).  Results are summaries computed on
+ * the device.  A server is used from one thread at a time. */
 typedef struct qo_server qo_server;
 typedef struct qo_server_config {
     uint64_t memory_budget_bytes;  /* device-memory admission budget */
-    int max_inflight_batches;      /* concurrent batches, one stream each */
+    int max_inflight_batches;      /* must be 1: one batch in flight per server */
     int max_batch_queries;         /* requests launched together in one batch */
     int plain;                     /* 1: no batching or overlap (per-query path) */
 } qo_server_config;
@@ -268,12 +273,20 @@ typedef struct qo_result {
 } qo_result;
 int qo_server_create(const qo_server_config *cfg, qo_server **out);
 int qo_server_add_table(qo_server *sv, const qo_table_desc *table, int *table_id);
+/* Queues a request arriving now (server clock). */
 int qo_server_submit(qo_server *sv, const qo_request *req, uint64_t *id);
-/* Copies up to `max` finished results; waits up to timeout_ms for at least one
- * (timeout_ms < 0: wait until one is available or nothing is pending).
- * Returns the number copied (>= 0) or a negative qo_status. */
+/* Queues a request arriving at `arrival_ms` on the server clock (may be in the
+ * future: the loop does not start it earlier; submit_ms reports it). */
+int qo_server_submit_at(qo_server *sv, const qo_request *req, double arrival_ms, uint64_t *id);
+/* Serves until every queued request has finished (waiting for arrivals that
+ * are not due yet). */
+int qo_server_run(qo_server *sv);
+/* Copies up to `max` finished results.  When none is ready and timeout_ms != 0,
+ * serves first until one is, or until timeout_ms has passed (timeout_ms < 0:
+ * no limit; returns 0 when nothing is queued).  Returns the number copied
+ * (>= 0) or a negative qo_status. */
 int qo_server_poll(qo_server *sv, qo_result *out, int max, int timeout_ms);
-/* Blocks until every submitted request has finished. */
+/* Same as qo_server_run. */
 int qo_server_drain(qo_server *sv);
 int qo_server_destroy(qo_server *sv);
 /* Runs one request synchronously on the plain per-query path (same result). */

@@ -1,19 +1,27 @@
-// Serving loop: a scheduler thread keeps requests flowing over HBM-resident
-// tables.  Requests are admitted against a device-memory budget; whenever one
-// of `max_inflight_batches` stream slots is free, every admitted request not
-// yet started is launched as ONE batch on that slot's stream (continuous
-// batching): filters of the batch share one multi-query kernel, joins share
-// the batched join pipeline (qo_batch.hpp), group-bys run their kernels
-// back to back on the same stream.  Batches on different slots overlap on the
-// device.  Completion is observed through an event per batch; each request's
-// summary (count, sum, checksum of its materialised output) is computed on the
-// device and copied back with the batch.
+// Serving loop over HBM-resident tables.
+//
+// Requests carry an arrival time.  The loop runs on the caller's thread
+// (qo_server_run, or drain / poll): it takes every request that has arrived,
+// in arrival order, up to max_batch_queries and within the device-memory
+// budget, launches them as ONE batch on the server's stream, waits for the
+// stream, records the results and starts over with whatever arrived in the
+// meantime (continuous batching: a request waits at most for the batch in
+// flight).  In a batch, filters share one multi-query kernel, joins share the
+// batched join pipeline (qo_batch.hpp) and group-bys run back to back, so the
+// requests' work overlaps on the device.  Each request's summary (count, sum,
+// checksum of its materialised output) is computed on the device and copied
+// back with the batch.
+//
+// The loop has no thread of its own, one stream, no event polling and no
+// per-batch pinned allocation: the result buffer is pinned once at creation.
+// A request that can never fit the budget is rejected; an injected fault fails
+// its request only.  With `plain` set each request runs alone, in arrival
+// order, through the per-query operator API (the baseline).
 //
 // The reference has no serving layer (host/host/main.cpp runs one operator per
 // process); this is the serving mode of the operator library.
 #include <algorithm>
 #include <chrono>
-#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -138,7 +146,6 @@ struct Req {
     size_t est;
     double submit_ms = 0, start_ms = 0, end_ms = 0;
     int status = QO_OK;
-    ResSlot res{};
 };
 
 // Caching device allocator: blocks are kept after use and reused for requests
@@ -192,11 +199,9 @@ class DevPool {
 
 struct Batch {
     std::vector<std::shared_ptr<Req>> reqs;
-    int slot = -1;
     void *arena = nullptr;
     ResSlot *d_res = nullptr;
-    ResSlot *h_res = nullptr;  // pinned
-    hipEvent_t done = nullptr;
+    ResSlot *h_res = nullptr;  // the server's pinned result buffer
     JoinBatchPlan jplan;
     std::vector<qo_join_query> jq;
     std::vector<FQuery> fq;
@@ -212,20 +217,12 @@ struct qo_server {
     qo_server_config cfg{};
     std::chrono::steady_clock::time_point t0;
     std::vector<Table> tables;
-    std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<std::shared_ptr<Req>> pending, admitted;
-    std::vector<std::unique_ptr<Batch>> inflight;
-    std::vector<qo_result> results;
+    std::deque<std::shared_ptr<Req>> pending;  // not started, in arrival order
+    std::vector<qo_result> results;            // finished, not yet polled
     uint64_t next_id = 1;
-    uint64_t outstanding = 0;  // submitted, not finished
-    size_t in_use = 0;
-    bool stop = false;
-    std::vector<hipStream_t> streams;
-    std::vector<bool> slot_busy;
-    hipStream_t plain_stream = nullptr;
+    hipStream_t stream = nullptr;
+    ResSlot *h_res = nullptr;  // pinned, max_batch_queries slots
     DevPool pool;
-    std::thread worker;
 
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -393,37 +390,7 @@ static int launch_batch(qo_server *sv, Batch &b, hipStream_t st) {
         }
     }
     QO_HIP(hipMemcpyAsync(b.h_res, b.d_res, sizeof(ResSlot) * n, hipMemcpyDeviceToHost, st));
-    QO_HIP(hipEventRecord(b.done, st));
     return QO_OK;
-}
-
-static void finish_batch(qo_server *sv, Batch &b, int launch_status) {
-    const double t = sv->now();
-    std::lock_guard<std::mutex> lk(sv->mu);
-    for (size_t i = 0; i < b.reqs.size(); ++i) {
-        Req &r = *b.reqs[i];
-        qo_result out{};
-        out.id = r.id;
-        out.kind = r.r.kind;
-        out.status = r.status != QO_OK ? r.status : launch_status;
-        if (out.status == QO_OK) {
-            out.count = b.h_res[i].count;
-            out.sum = (int64_t)b.h_res[i].sum;
-            out.checksum = (int64_t)b.h_res[i].checksum;
-            if (r.r.kind == QO_Q_GROUPBY && out.count > sv->tables[r.r.table_a].d.group_capacity)
-                out.status = QO_ERR_CAPACITY;
-            if (r.r.kind == QO_Q_JOIN_SORT && out.count > sv->tables[r.r.table_b].d.n) out.status = QO_ERR_CAPACITY;
-        }
-        out.submit_ms = r.submit_ms;
-        out.start_ms = r.start_ms;
-        out.end_ms = t;
-        sv->results.push_back(out);
-        sv->outstanding--;
-    }
-    sv->in_use -= b.est;
-    if (b.slot >= 0) sv->slot_busy[b.slot] = false;
-    sv->cv_done.notify_all();
-    sv->cv_work.notify_all();
 }
 
 // Runs one request alone through the per-query operator API, synchronously.
@@ -494,104 +461,88 @@ static int run_plain(qo_server *sv, const qo_request &r, qo_result &out, hipStre
     return QO_OK;
 }
 
-static void scheduler(qo_server *sv) {
-    (void)ensure_init();
-    std::unique_lock<std::mutex> lk(sv->mu);
-    while (true) {
-        // 1. Completions.
-        bool progressed = false;
-        for (size_t i = 0; i < sv->inflight.size();) {
-            Batch &b = *sv->inflight[i];
-            hipError_t q = hipEventQuery(b.done);
-            if (q == hipErrorNotReady) {
-                ++i;
-                continue;
-            }
-            std::unique_ptr<Batch> done = std::move(sv->inflight[i]);
-            sv->inflight.erase(sv->inflight.begin() + i);
-            lk.unlock();
-            finish_batch(sv, *done, q == hipSuccess ? QO_OK : QO_ERR_HIP);
-            sv->pool.put(done->arena);
-            (void)hipEventDestroy(done->done);
-            (void)hipHostFree(done->h_res);
-            lk.lock();
-            progressed = true;
-        }
-        if (sv->stop && sv->pending.empty() && sv->admitted.empty() && sv->inflight.empty()) break;
-        // 2. Admission (FIFO) against the budget.
-        while (!sv->pending.empty()) {
-            auto &r = sv->pending.front();
-            if (r->est > sv->cfg.memory_budget_bytes) {
-                auto rej = r;
-                sv->pending.pop_front();
-                qo_result out{};
-                out.id = rej->id;
-                out.kind = rej->r.kind;
-                out.status = QO_ERR_BUDGET;
-                out.submit_ms = rej->submit_ms;
-                out.start_ms = out.end_ms = sv->now();
-                sv->results.push_back(out);
-                sv->outstanding--;
-                sv->cv_done.notify_all();
-                progressed = true;
-                continue;
-            }
-            if (sv->in_use + r->est > sv->cfg.memory_budget_bytes) break;
-            sv->in_use += r->est;
-            sv->admitted.push_back(r);
+static void finish(qo_server *sv, Req &r, const ResSlot &res, int status, double t) {
+    qo_result out{};
+    out.id = r.id;
+    out.kind = r.r.kind;
+    out.status = r.status != QO_OK ? r.status : status;
+    if (out.status == QO_OK) {
+        out.count = res.count;
+        out.sum = (int64_t)res.sum;
+        out.checksum = (int64_t)res.checksum;
+        if (r.r.kind == QO_Q_GROUPBY && out.count > sv->tables[r.r.table_a].d.group_capacity)
+            out.status = QO_ERR_CAPACITY;
+        if (r.r.kind == QO_Q_JOIN_SORT && out.count > sv->tables[r.r.table_b].d.n) out.status = QO_ERR_CAPACITY;
+    }
+    out.submit_ms = r.submit_ms;
+    out.start_ms = r.start_ms;
+    out.end_ms = t;
+    sv->results.push_back(out);
+}
+
+// Requests at the front of the queue that have arrived by `t`, up to the batch
+// limit and the budget (a request larger than the whole budget is rejected).
+static std::vector<std::shared_ptr<Req>> take_batch(qo_server *sv, double t, size_t *bytes) {
+    std::vector<std::shared_ptr<Req>> b;
+    *bytes = 0;
+    const size_t lim = sv->cfg.plain ? 1 : (size_t)std::max(1, sv->cfg.max_batch_queries);
+    while (!sv->pending.empty() && b.size() < lim && sv->pending.front()->submit_ms <= t) {
+        auto r = sv->pending.front();
+        if (r->est > sv->cfg.memory_budget_bytes) {
             sv->pending.pop_front();
-            progressed = true;
+            r->start_ms = t;
+            r->status = QO_ERR_BUDGET;
+            finish(sv, *r, ResSlot{}, QO_ERR_BUDGET, t);
+            continue;
         }
-        // 3. Launch: one batch per free slot, from everything admitted so far.
-        for (int s = 0; s < (int)sv->streams.size() && !sv->admitted.empty(); ++s) {
-            if (sv->slot_busy[s]) continue;
-            auto b = std::make_unique<Batch>();
-            const int lim = sv->cfg.plain ? 1 : std::max(1, sv->cfg.max_batch_queries);
-            while (!sv->admitted.empty() && (int)b->reqs.size() < lim) {
-                b->est += sv->admitted.front()->est;
-                b->reqs.push_back(sv->admitted.front());
-                sv->admitted.pop_front();
-            }
-            b->slot = s;
-            sv->slot_busy[s] = true;
-            const double t = sv->now();
-            for (auto &r : b->reqs) r->start_ms = t;
-            lk.unlock();
-            int rc = QO_OK;
-            if (sv->cfg.plain) {
-                // Baseline: the request alone through the per-query API.
-                qo_result out{};
-                rc = run_plain(sv, b->reqs[0]->r, out, sv->streams[s]);
-                b->reqs[0]->status = rc;
-                ResSlot h{out.count, (unsigned long long)out.sum, (unsigned long long)out.checksum, 0};
-                b->h_res = &h;
-                finish_batch(sv, *b, QO_OK);
-                b->h_res = nullptr;
-                lk.lock();
-                progressed = true;
-                continue;
-            }
-            if (hipHostMalloc(&b->h_res, sizeof(ResSlot) * b->reqs.size(), hipHostMallocDefault) != hipSuccess ||
-                hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
-                rc = set_error(QO_ERR_HIP, "batch bookkeeping");
-            if (rc == QO_OK) rc = launch_batch(sv, *b, sv->streams[s]);
-            lk.lock();
-            if (rc != QO_OK) {
-                // Launch failed before the event: make sure queued work is done, then fail the batch.
-                lk.unlock();
-                (void)hipStreamSynchronize(sv->streams[s]);
-                if (b->h_res) memset(b->h_res, 0, sizeof(ResSlot) * b->reqs.size());
-                finish_batch(sv, *b, rc);
-                sv->pool.put(b->arena);
-                if (b->done) (void)hipEventDestroy(b->done);
-                if (b->h_res) (void)hipHostFree(b->h_res);
-                lk.lock();
-            } else {
-                sv->inflight.push_back(std::move(b));
-            }
-            progressed = true;
-        }
-        if (!progressed) sv->cv_work.wait_for(lk, std::chrono::microseconds(sv->inflight.empty() ? 2000 : 50));
+        if (*bytes + r->est > sv->cfg.memory_budget_bytes) break;
+        *bytes += r->est;
+        b.push_back(r);
+        sv->pending.pop_front();
+    }
+    return b;
+}
+
+// Serves one batch of what has arrived by now; returns false when nothing has.
+static bool serve_once(qo_server *sv) {
+    size_t bytes = 0;
+    const double t = sv->now();
+    std::vector<std::shared_ptr<Req>> reqs = take_batch(sv, t, &bytes);
+    if (reqs.empty()) return false;
+    for (auto &r : reqs) r->start_ms = t;
+    if (sv->cfg.plain) {
+        Req &r = *reqs[0];
+        qo_result out{};
+        const int rc = run_plain(sv, r.r, out, sv->stream);
+        const ResSlot h{out.count, (unsigned long long)out.sum, (unsigned long long)out.checksum, 0};
+        finish(sv, r, h, rc, sv->now());
+        return true;
+    }
+    Batch b;
+    b.reqs = std::move(reqs);
+    b.est = bytes;
+    b.h_res = sv->h_res;
+    int rc = launch_batch(sv, b, sv->stream);
+    // Every launched kernel and copy has finished before results are read or
+    // the arena is reused, also after a launch error.
+    const hipError_t e = hipStreamSynchronize(sv->stream);
+    if (rc == QO_OK && e != hipSuccess) rc = set_error(QO_ERR_HIP, "server stream: %s", hipGetErrorString(e));
+    const double t1 = sv->now();
+    for (size_t i = 0; i < b.reqs.size(); ++i) finish(sv, *b.reqs[i], rc == QO_OK ? b.h_res[i] : ResSlot{}, rc, t1);
+    sv->pool.put(b.arena);
+    return true;
+}
+
+// Serves until every submitted request has finished, or (stop_at_result) until
+// a result is ready; waits for arrivals that are not due yet, up to
+// `deadline_ms` on the server clock (< 0: no limit).
+static void serve(qo_server *sv, bool stop_at_result, double deadline_ms) {
+    while (!sv->pending.empty()) {
+        if (stop_at_result && !sv->results.empty()) return;
+        if (serve_once(sv)) continue;
+        const double wait = sv->pending.front()->submit_ms - sv->now();
+        if (deadline_ms >= 0 && sv->now() + std::max(0.0, wait) > deadline_ms) return;
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait));
     }
 }
 
@@ -602,88 +553,82 @@ extern "C" {
 int qo_server_create(const qo_server_config *cfg, qo_server **out) {
     QO_TRY(ensure_init());
     QO_REQUIRE(cfg && out, "null argument");
-    // Switched off: a served mix of requests (three batches in flight) hung
-    // without a fault and no cause has been found in this code; the loop stays
-    // off until one is shown.
-    if (cfg) return set_error(QO_ERR_NOT_FOUND, "the serving loop is disabled (a served mix hung; cause not found)");
-    QO_REQUIRE(cfg->max_inflight_batches >= 1 && cfg->max_inflight_batches <= 16, "max_inflight_batches in [1, 16]");
+    // Switched off: the first batch of a served mix (filters, group-bys and
+    // joins of four tables together) never completed on the GPU, with this
+    // single-threaded loop as with an earlier multi-stream one, and no cause
+    // has been found in the batch's kernels; the loop stays off until one is.
+    if (cfg) return set_error(QO_ERR_NOT_FOUND, "the serving loop is disabled (a served batch hung; cause not found)");
+    QO_REQUIRE(cfg->max_inflight_batches == 1, "max_inflight_batches must be 1 (one batch in flight per server)");
+    QO_REQUIRE(cfg->max_batch_queries >= 1 && cfg->max_batch_queries <= 4096, "max_batch_queries in [1, 4096]");
     auto sv = new qo_server;
     sv->cfg = *cfg;
-    if (sv->cfg.plain) sv->cfg.max_inflight_batches = 1;
     sv->t0 = std::chrono::steady_clock::now();
-    for (int i = 0; i < sv->cfg.max_inflight_batches; ++i) {
-        hipStream_t st;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-            delete sv;
-            return set_error(QO_ERR_HIP, "server stream");
-        }
-        sv->streams.push_back(st);
-        sv->slot_busy.push_back(false);
-    }
-    if (hipStreamCreateWithFlags(&sv->plain_stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking) != hipSuccess) {
         delete sv;
         return set_error(QO_ERR_HIP, "server stream");
     }
-    sv->worker = std::thread(scheduler, sv);
+    if (hipHostMalloc(reinterpret_cast<void **>(&sv->h_res), sizeof(ResSlot) * cfg->max_batch_queries,
+                      hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(sv->stream);
+        delete sv;
+        return set_error(QO_ERR_HIP, "server result buffer");
+    }
     *out = sv;
     return QO_OK;
 }
 
 int qo_server_add_table(qo_server *sv, const qo_table_desc *t, int *table_id) {
     QO_REQUIRE(sv && t && table_id, "null argument");
-    std::lock_guard<std::mutex> lk(sv->mu);
     sv->tables.push_back(Table{*t});
     *table_id = (int)sv->tables.size() - 1;
     return QO_OK;
 }
 
-int qo_server_submit(qo_server *sv, const qo_request *req, uint64_t *id) {
+int qo_server_submit_at(qo_server *sv, const qo_request *req, double arrival_ms, uint64_t *id) {
     QO_REQUIRE(sv && req && id, "null argument");
+    QO_TRY(validate(sv, *req));
     auto r = std::make_shared<Req>();
-    {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        QO_TRY(validate(sv, *req));
-        r->r = *req;
-        r->est = request_bytes(sv, *req);
-        r->id = sv->next_id++;
-        r->submit_ms = sv->now();
-        *id = r->id;
-        sv->pending.push_back(r);
-        sv->outstanding++;
-    }
-    sv->cv_work.notify_all();
+    r->r = *req;
+    r->est = request_bytes(sv, *req);
+    r->id = sv->next_id++;
+    r->submit_ms = arrival_ms;
+    *id = r->id;
+    // The queue stays in arrival order (ties in submission order).
+    auto pos = std::upper_bound(sv->pending.begin(), sv->pending.end(), arrival_ms,
+                                [](double a, const std::shared_ptr<Req> &q) { return a < q->submit_ms; });
+    sv->pending.insert(pos, r);
+    return QO_OK;
+}
+
+int qo_server_submit(qo_server *sv, const qo_request *req, uint64_t *id) {
+    QO_REQUIRE(sv, "null server");
+    return qo_server_submit_at(sv, req, sv->now(), id);
+}
+
+int qo_server_run(qo_server *sv) {
+    QO_REQUIRE(sv, "null server");
+    serve(sv, false, -1.0);
     return QO_OK;
 }
 
 int qo_server_poll(qo_server *sv, qo_result *out, int max, int timeout_ms) {
     QO_REQUIRE(sv && (max == 0 || out), "null argument");
-    std::unique_lock<std::mutex> lk(sv->mu);
-    auto ready = [&] { return !sv->results.empty() || sv->outstanding == 0; };
-    if (timeout_ms < 0) sv->cv_done.wait(lk, ready);
-    else sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
-    int n = std::min<int>(max, (int)sv->results.size());
+    if (sv->results.empty() && timeout_ms != 0) serve(sv, true, timeout_ms < 0 ? -1.0 : sv->now() + timeout_ms);
+    const int n = std::min<int>(max, (int)sv->results.size());
     for (int i = 0; i < n; ++i) out[i] = sv->results[i];
     sv->results.erase(sv->results.begin(), sv->results.begin() + n);
     return n;
 }
 
-int qo_server_drain(qo_server *sv) {
-    QO_REQUIRE(sv, "null server");
-    std::unique_lock<std::mutex> lk(sv->mu);
-    sv->cv_done.wait(lk, [&] { return sv->outstanding == 0; });
-    return QO_OK;
-}
+int qo_server_drain(qo_server *sv) { return qo_server_run(sv); }
 
 int qo_server_destroy(qo_server *sv) {
     if (!sv) return QO_OK;
-    {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->stop = true;
+    if (sv->stream) {
+        (void)hipStreamSynchronize(sv->stream);
+        (void)hipStreamDestroy(sv->stream);
     }
-    sv->cv_work.notify_all();
-    if (sv->worker.joinable()) sv->worker.join();
-    for (auto st : sv->streams) (void)hipStreamDestroy(st);
-    if (sv->plain_stream) (void)hipStreamDestroy(sv->plain_stream);
+    if (sv->h_res) (void)hipHostFree(sv->h_res);
     delete sv;
     return QO_OK;
 }
@@ -693,7 +638,7 @@ int qo_server_run_plain(qo_server *sv, const qo_request *req, qo_result *out) {
     QO_REQUIRE(sv && req && out, "null argument");
     qo_result r{};
     r.submit_ms = r.start_ms = sv->now();
-    int rc = run_plain(sv, *req, r, sv->plain_stream);
+    int rc = run_plain(sv, *req, r, sv->stream);
     r.status = rc;
     r.end_ms = sv->now();
     *out = r;

@@ -1,9 +1,14 @@
 """Serving loop over HBM-resident tables (qo_server_* in include/qo.h).
 
-    sv = Server(budget_bytes=200e9, inflight=4, batch=64)
+    sv = Server(budget_bytes=200e9, batch=64)
     t = sv.add_table(key=..., fk=..., grp=..., val=..., filt=..., n=N, group_capacity=G)
-    rid = sv.submit(FILTER, t, lo=0, hi=99)
-    results = sv.poll(timeout_ms=-1)
+    rid = sv.submit(FILTER, t, lo=0, hi=99)              # arrives now
+    rid2 = sv.submit_at(GROUPBY, t, arrival_ms=5.0)      # arrives at 5 ms on the server clock
+    sv.run()                                             # serves both (continuous batching)
+    results = sv.poll(timeout_ms=0)
+
+The loop runs on the calling thread: run(), drain() and poll(timeout_ms != 0)
+serve what has arrived, one batch at a time on the server's stream.
 
 Request kinds: FILTER (COUNT, SUM(val) where lo <= filt <= hi), GROUPBY (group
 by grp: COUNT/SUM/MIN/MAX per group), JOIN_SORT (build on table_a (key, val),
@@ -46,6 +51,8 @@ _PROTOS = {
     "qo_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "qo_server_add_table": (C.c_int, [C.c_void_p, C.POINTER(TableDesc), C.POINTER(C.c_int)]),
     "qo_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "qo_server_submit_at": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_double, C.POINTER(C.c_uint64)]),
+    "qo_server_run": (C.c_int, [C.c_void_p]),
     "qo_server_poll": (C.c_int, [C.c_void_p, C.POINTER(Result), C.c_int, C.c_int]),
     "qo_server_drain": (C.c_int, [C.c_void_p]),
     "qo_server_destroy": (C.c_int, [C.c_void_p]),
@@ -67,7 +74,7 @@ def _ptr(b):
 
 
 class Server:
-    def __init__(self, budget_bytes: float = 200e9, inflight: int = 4, batch: int = 64, plain: bool = False):
+    def __init__(self, budget_bytes: float = 200e9, inflight: int = 1, batch: int = 64, plain: bool = False):
         self.lib = _lib_server()
         cfg = ServerConfig(int(budget_bytes), int(inflight), int(batch), 1 if plain else 0)
         h = C.c_void_p()
@@ -85,6 +92,16 @@ class Server:
         rid = C.c_uint64()
         _lib.call("qo_server_submit", self.handle, C.byref(r), C.byref(rid))
         return rid.value
+
+    def submit_at(self, kind: int, table_a: int, table_b: int = -1, lo: int = -(1 << 31),
+                  hi: int = (1 << 31) - 1, arrival_ms: float = 0.0) -> int:
+        r = Request(kind, table_a, table_b, lo, hi)
+        rid = C.c_uint64()
+        _lib.call("qo_server_submit_at", self.handle, C.byref(r), C.c_double(arrival_ms), C.byref(rid))
+        return rid.value
+
+    def run(self) -> None:
+        _lib.call("qo_server_run", self.handle)
 
     def poll(self, max_results: int = 4096, timeout_ms: int = -1) -> list[dict]:
         buf = (Result * max_results)()

