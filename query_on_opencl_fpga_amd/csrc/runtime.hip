@@ -172,7 +172,7 @@ std::mutex g_tune_mu;
 
 TuneParams tune_default(int op, int cls) {
     switch (op) {
-        case TUNE_FILTER_AGG: return {256, 4, 8};      // block, unroll, blocks per CU
+        case TUNE_FILTER_AGG: return {256, 4, 1};      // block, unroll, blocks per CU
         case TUNE_FILTER: return {256, 4, 0};          // block, uint4 loads per thread
         case TUNE_PROBE: return {256, 4, 50};          // block, int4 groups per thread, load %
         case TUNE_GROUPBY: return {1024, 2048, 1};     // block, LDS slots, blocks per CU
