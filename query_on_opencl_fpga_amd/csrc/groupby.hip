@@ -236,20 +236,25 @@ __global__ __launch_bounds__(NT) void k_groupby_hash(const int32_t *__restrict__
         const size_t nv = n / 4;
         const int4 *k4 = reinterpret_cast<const int4 *>(key), *v4 = reinterpret_cast<const int4 *>(val);
         size_t i = tid;
-        // Two rounds of loads in flight per thread.
-        for (; i + stride < nv; i += 2 * stride) {
-            const int4 ka = ld_stream(k4 + i), va = ld_stream(v4 + i);
-            const int4 kb = ld_stream(k4 + i + stride), vb = ld_stream(v4 + i + stride);
-            add(ka.x, va.x);
-            add(ka.y, va.y);
-            add(ka.z, va.z);
-            add(ka.w, va.w);
-            add(kb.x, vb.x);
-            add(kb.y, vb.y);
-            add(kb.z, vb.z);
-            add(kb.w, vb.w);
+        // Four rounds of loads in flight per thread (2^28 tuples, 1024 groups:
+        // 0.454 ms against 0.480 ms with two).
+        constexpr int kRounds = 4;
+        for (; i + (kRounds - 1) * stride < nv; i += kRounds * stride) {
+            int4 kk[kRounds], vv[kRounds];
+#pragma unroll
+            for (int u = 0; u < kRounds; ++u) {
+                kk[u] = ld_stream(k4 + i + u * stride);
+                vv[u] = ld_stream(v4 + i + u * stride);
+            }
+#pragma unroll
+            for (int u = 0; u < kRounds; ++u) {
+                add(kk[u].x, vv[u].x);
+                add(kk[u].y, vv[u].y);
+                add(kk[u].z, vv[u].z);
+                add(kk[u].w, vv[u].w);
+            }
         }
-        if (i < nv) {
+        for (; i < nv; i += stride) {
             const int4 kk = ld_stream(k4 + i), vv = ld_stream(v4 + i);
             add(kk.x, vv.x);
             add(kk.y, vv.y);
