@@ -259,8 +259,8 @@ size_t join_probe_ws_bytes(size_t table_bytes, size_t ns) {
 // hash-mixed table, split + L2-resident slices for a key-range table), small
 // ones directly; the table's mode picks among the launched paths on the device.
 int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
-                      const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
-                      int32_t *ospay, size_t cap, uint64_t *d_count, void *ws, hipStream_t st) {
+                      const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, const JoinOut &out,
+                      uint64_t *d_count, void *ws, hipStream_t st) {
     QO_TRY(launch_fill_u64(d_count, 0, 1, st));
     if (ns == 0) return QO_OK;
     const size_t tiles = (ns + kProbeTile - 1) / kProbeTile;
@@ -268,14 +268,14 @@ int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey
     const JoinHeader *hdr = static_cast<const JoinHeader *>(table);
     const uint64_t *slots = reinterpret_cast<const uint64_t *>(hdr + 1);
     bool hash_part = false, range_part = false;
-    QO_TRY(launch_join_part_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, cap,
-                                  d_count, ws, st, &hash_part, &range_part));
+    QO_TRY(launch_join_part_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, out, d_count, ws, st,
+                                  &hash_part, &range_part));
     if (hash_part && range_part) return QO_OK;
     // The direct probe serves the mode(s) no partitioned probe was launched for.
     const Gate gate = hash_part ? Gate{&hdr->mode, kJoinKeyRange}
                                 : range_part ? Gate{&hdr->mode, kJoinHashMix} : Gate{};
     const bool vec = ((uintptr_t)skey % 16 == 0) && ((uintptr_t)spay % 16 == 0) && (!sfilt || (uintptr_t)sfilt % 16 == 0);
-    SoaEmit emit{okey, orpay, ospay, cap};
+    const SoaEmit emit{out.okey, out.orpay, out.ospay, out.cap, out.k64, out.v32};
     if (sfilt)
         k_join_probe<true><<<(unsigned)tiles, 256, 0, st>>>(hdr, slots, skey, spay, sfilt, ns, flo, fhi, vec, emit,
                                                             d_count, gate);
@@ -423,8 +423,12 @@ int qo_hash_probe(const void *table, size_t table_bytes, const int32_t *skey, co
     int status = QO_OK;
     void *ws = workspace(st, join_probe_ws_bytes(table_bytes, ns), &status);
     if (!ws) return status;
-    return launch_join_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, out_capacity,
-                             d_count, ws, st);
+    JoinOut out;
+    out.okey = okey;
+    out.orpay = orpay;
+    out.ospay = ospay;
+    out.cap = out_capacity;
+    return launch_join_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, out, d_count, ws, st);
 }
 
 int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
@@ -434,26 +438,34 @@ int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, cons
     QO_ENTER();
     QO_REQUIRE(table && h_count, "null pointer");
     hipStream_t st = stream_of(s);
-    // One workspace serves the three stages in stream order: build buckets,
-    // probe buckets, then the sort (after the count has been read back).
+    // The probe writes sort-ready rows (join_sort_key(key, s_payload), r_payload)
+    // that the sort orders in place and unpacks into the output columns, so no
+    // separate pack pass runs.  Build buckets, probe buckets and the sort's
+    // workspace share one region in stream order.
     const size_t stage_bytes = std::max({join_build_ws_bytes(table_bytes, nr), join_probe_ws_bytes(table_bytes, ns),
-                                         sort_join_rows_ws_bytes(out_capacity)});
+                                         radix_workspace_bytes(out_capacity, true)});
+    const size_t rows = out_capacity ? out_capacity : 1;
     int status = QO_OK;
-    void *ws = workspace(st, stage_bytes + 512, &status);
+    void *ws = workspace(st, stage_bytes + rows * 12 + 1024, &status);
     if (!ws) return status;
     Carver cv(ws);
     uint64_t *d_count = cv.take<uint64_t>(1);
+    uint64_t *k64 = cv.take<uint64_t>(rows);
+    uint32_t *v32 = cv.take<uint32_t>(rows);
     void *sws = cv.take<char>(stage_bytes);
     QO_TRY(launch_join_build(table, table_bytes, rkey, rpay, nr, sws, st));
-    QO_TRY(launch_join_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, okey, orpay, ospay, out_capacity,
-                             d_count, sws, st));
+    JoinOut out;
+    out.cap = out_capacity;
+    out.k64 = k64;
+    out.v32 = v32;
+    QO_TRY(launch_join_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, out, d_count, sws, st));
     uint64_t m = 0;
     QO_HIP(hipMemcpyAsync(&m, d_count, 8, hipMemcpyDeviceToHost, st));
     QO_HIP(hipStreamSynchronize(st));
     *h_count = m;
     if (m > out_capacity) return set_error(QO_ERR_CAPACITY, "join produced %llu rows, capacity %zu",
                                            (unsigned long long)m, out_capacity);
-    QO_TRY(sort_join_rows_ws(okey, orpay, ospay, m, sws, st));
+    QO_TRY(sort_packed_join_rows(k64, v32, m, sws, okey, orpay, ospay, st));
     QO_HIP(hipStreamSynchronize(st));
     return QO_OK;
 }

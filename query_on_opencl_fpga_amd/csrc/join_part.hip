@@ -423,15 +423,14 @@ int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey,
 }
 
 int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
-                           const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
-                           int32_t *ospay, size_t out_cap, uint64_t *d_count, void *ws, hipStream_t st,
-                           bool *hash_part, bool *range_part) {
+                           const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, const JoinOut &out,
+                           uint64_t *d_count, void *ws, hipStream_t st, bool *hash_part, bool *range_part) {
     *hash_part = *range_part = false;
     if (!ws) return QO_OK;
     const uint64_t slots = join_table_slots(table_bytes);
     const JoinHeader *hdr = static_cast<const JoinHeader *>(table);
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(d_count);
-    const SoaEmit emit{okey, orpay, ospay, out_cap};
+    const SoaEmit emit{out.okey, out.orpay, out.ospay, out.cap, out.k64, out.v32};
     L2Plan r;
     PartPlan p;
     probe_plans(slots, ns, r, p);

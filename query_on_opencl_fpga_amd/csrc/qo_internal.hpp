@@ -124,13 +124,25 @@ size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build);
 void join_part_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits);
 int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                            void *ws, hipStream_t st, bool *used);
+// Sorts m sort-ready join rows (k64 = join_sort_key(key, s_payload), v32 =
+// r_payload) with workspace rws (radix_workspace_bytes(m, true)) and writes
+// them out as three int32 columns.
+int sort_packed_join_rows(uint64_t *k64, uint32_t *v32, size_t m, void *rws, int32_t *key, int32_t *rpay,
+                          int32_t *spay, hipStream_t st);
+// Output columns of a probe: three int32 columns, or (k64, v32 set) the
+// sort-ready (join_sort_key(key, s_payload), r_payload) rows.
+struct JoinOut {
+    int32_t *okey = nullptr, *orpay = nullptr, *ospay = nullptr;
+    size_t cap = 0;
+    uint64_t *k64 = nullptr;
+    uint32_t *v32 = nullptr;
+};
 int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
-                           const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
-                           int32_t *ospay, size_t out_cap, uint64_t *d_count, void *ws, hipStream_t st,
-                           bool *hash_part, bool *range_part);
+                           const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, const JoinOut &out,
+                           uint64_t *d_count, void *ws, hipStream_t st, bool *hash_part, bool *range_part);
 int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
-                      const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
-                      int32_t *ospay, size_t cap, uint64_t *d_count, void *ws, hipStream_t st);
+                      const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, const JoinOut &out,
+                      uint64_t *d_count, void *ws, hipStream_t st);
 
 }  // namespace qo
 

@@ -23,6 +23,7 @@
 // this to work on one table slice at a time in LDS.
 #pragma once
 #include "qo_device.hpp"
+#include "qo_sort.hpp"
 
 namespace qo {
 
@@ -345,11 +346,20 @@ QO_DEV void probe_tile(const JoinHeader *__restrict__ hdr, const uint64_t *__res
 }
 
 // Emits match rows into three SoA int32 columns (capacity-checked).
+// Join output row `pos` (rows past `cap` are counted, not written): three
+// int32 columns, or, when k64 is set, the sort-ready form of a join that is
+// sorted next: k64 = join_sort_key(key, s_payload), v32 = r_payload.
 struct SoaEmit {
     int32_t *okey, *orpay, *ospay;
     size_t cap;
+    uint64_t *k64 = nullptr;
+    uint32_t *v32 = nullptr;
     QO_DEV void operator()(uint64_t pos, int32_t k, int32_t rp, int32_t sp) const {
-        if (pos < cap) {
+        if (pos >= cap) return;
+        if (k64) {
+            k64[pos] = join_sort_key(k, sp);
+            v32[pos] = (uint32_t)rp;
+        } else {
             okey[pos] = k;
             orpay[pos] = rp;
             ospay[pos] = sp;

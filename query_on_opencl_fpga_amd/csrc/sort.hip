@@ -616,6 +616,16 @@ int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void
 
 size_t sort_join_rows_ws_bytes(size_t m) { return m * 12 + 1024 + radix_workspace_bytes(m, true); }
 
+int sort_packed_join_rows(uint64_t *k64, uint32_t *v32, size_t m, void *rws, int32_t *key, int32_t *rpay,
+                          int32_t *spay, hipStream_t st) {
+    if (m == 0) return QO_OK;
+    if (m > 1) QO_TRY(launch_radix_sort_u64(k64, v32, m, 0, 64, rws, st));
+    unsigned g = grid_for(m, 256, (unsigned)props().cu_count * 8);
+    k_unpack_join<<<g, 256, 0, st>>>(k64, v32, m, key, spay, rpay);
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
 }  // namespace qo
 
 using namespace qo;
