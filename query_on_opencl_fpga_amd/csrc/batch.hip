@@ -8,8 +8,8 @@
 // (key, s_payload) 64-bit keys with the r_payload beside them; a segmented
 // LSD radix sort (one segment per query, element counts read from device
 // memory, per-query digit starts up front, then per pass a tile histogram, a
-// segmented column scan and the placement) orders every query's matches, and
-// an unpack kernel writes the SoA outputs.
+// segmented column scan and the placement) orders every query's matches; its
+// last pass writes each query's output columns.
 // The stage sequence (fills included, no memset nodes) is captured once into a
 // graph at creation and replayed by qo_batch_launch.
 //
@@ -111,22 +111,6 @@ __global__ __launch_bounds__(256) void k_b_probe(const BQuery *q, int nq, uint32
     QO_TILE_LOOP_END
 }
 
-__global__ __launch_bounds__(256) void k_b_unpack(const BQuery *q, int nq, const uint64_t *mkeys, const uint32_t *mvals) {
-    const uint32_t t = blockIdx.x;
-    const int qi = query_of<2>(q, nq, t);
-    const BQuery &Q = q[qi];
-    uint64_t n = *Q.count;
-    if (n > Q.out_cap) n = Q.out_cap;
-    const uint64_t e0 = (uint64_t)(t - Q.mtile0) * kSortTile;
-    const uint64_t e1 = e0 + kSortTile < n ? e0 + kSortTile : n;
-    for (uint64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
-        const uint64_t k = mkeys[Q.m_base + i];
-        Q.okey[i] = join_sort_key_key(k);
-        Q.ospay[i] = join_sort_key_spay(k);
-        Q.orpay[i] = (int32_t)mvals[Q.m_base + i];
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Plan of one join batch: sizes, then (after binding an arena) device pointers.
 // ---------------------------------------------------------------------------
@@ -137,6 +121,7 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
     p.hs.assign(nq, RadixSeg{});
     p.hchunks.clear();
     p.hchunk0.assign(1, 0u);
+    p.houts.assign(nq, RowOut{});
     p.total_slots = p.total_match = 0;
     uint64_t rt = 0, stt = 0, mt = 0;
     const int load = std::max(10, std::min(90, tune_get(TUNE_PROBE, 1 << 20).param));
@@ -150,6 +135,7 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
         q.rkey = u.rkey; q.rpay = u.rpay; q.skey = u.skey; q.spay = u.spay; q.sfilt = u.sfilt;
         q.nr = u.nr; q.ns = u.ns; q.flo = u.flo; q.fhi = u.fhi;
         q.okey = u.okey; q.orpay = u.orpay; q.ospay = u.ospay; q.out_cap = u.out_capacity;
+        p.houts[i] = RowOut{u.okey, u.orpay, u.ospay};
         // Rounded to 32 slots so every query's 256-byte header stays 256-byte aligned.
         q.table_cap = (std::max<uint64_t>(64, (uint64_t)((double)u.nr * 100.0 / load) + 1) + 31) / 32 * 32;
         QO_REQUIRE(q.table_cap <= 0xFFFFFFFFull, "query %d: build side too large", i);
@@ -190,6 +176,7 @@ static void plan_carve(JoinBatchPlan &p, Carver &cv) {
     p.valt = cv.take<uint32_t>(p.total_match);
     p.d_chunks = cv.take<SweepChunk>(p.hchunks.size() + 1);
     p.d_chunk0 = cv.take<uint32_t>(p.hchunk0.size());
+    p.d_outs = cv.take<RowOut>(p.nq);
     p.sort_ws = cv.take<char>(radix_rows_seg_bytes(p.mt, p.nq, (uint32_t)p.hchunks.size()));
 }
 
@@ -221,6 +208,7 @@ int join_plan_upload(const JoinBatchPlan &p, hipStream_t st) {
                               hipMemcpyHostToDevice, st));
     QO_HIP(hipMemcpyAsync(p.d_chunk0, p.hchunk0.data(), sizeof(uint32_t) * p.hchunk0.size(), hipMemcpyHostToDevice,
                           st));
+    QO_HIP(hipMemcpyAsync(p.d_outs, p.houts.data(), sizeof(RowOut) * p.nq, hipMemcpyHostToDevice, st));
     return QO_OK;
 }
 
@@ -242,10 +230,10 @@ int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
     }
     if (p.mt) {
         bool in_alt = false;
+        // The sort's last pass writes every query's output columns.
         QO_TRY(launch_radix_rows_segments(p.mkeys, p.mvals, p.kalt, p.valt, p.d_segs, p.nq, p.mt, p.d_chunks,
-                                           (uint32_t)p.hchunks.size(), p.d_chunk0, 0, 64, p.sort_ws, st, &in_alt));
-        k_b_unpack<<<p.mt, 256, 0, st>>>(p.d_q, p.nq, in_alt ? p.kalt : p.mkeys, in_alt ? p.valt : p.mvals);
-        QO_LAUNCHED();
+                                           (uint32_t)p.hchunks.size(), p.d_chunk0, 0, 64, p.sort_ws, st, &in_alt,
+                                           p.d_outs));
     }
     return QO_OK;
 }

@@ -34,6 +34,7 @@ struct JoinBatchPlan {
     std::vector<RadixSeg> hs;
     std::vector<SweepChunk> hchunks;  // up-front digit-count blocks of the sort
     std::vector<uint32_t> hchunk0;    // query q owns chunks [hchunk0[q], hchunk0[q + 1])
+    std::vector<RowOut> houts;        // query q's output columns (the sort's last pass writes them)
     uint64_t total_slots = 0, total_match = 0;
     uint32_t rt = 0, stt = 0, mt = 0;
     size_t small_words = 0;
@@ -46,6 +47,7 @@ struct JoinBatchPlan {
     uint32_t *mvals = nullptr, *valt = nullptr;
     SweepChunk *d_chunks = nullptr;
     uint32_t *d_chunk0 = nullptr;
+    RowOut *d_outs = nullptr;
     void *sort_ws = nullptr;
     unsigned long long *counts = nullptr;
 };

@@ -21,6 +21,12 @@ struct RadixSeg {
     uint32_t ntile;
 };
 
+// Output columns of a sorted join-row segment: the last pass of a join-row
+// sort writes (key, r_payload, s_payload) here instead of the packed rows.
+struct RowOut {
+    int32_t *key, *rpay, *spay;
+};
+
 // Tiles [t0, t1) (segment-local) of segment `seg`: one block of the up-front
 // digit count of a segmented sort.
 struct SweepChunk {
@@ -33,10 +39,13 @@ size_t radix_rows_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks);
 // chunks (segment s owns chunks [chunk0[s], chunk0[s + 1])), then per pass a
 // tile histogram, a segmented column scan and the placement.  On return
 // *in_alt tells whether the data ended in kalt / valt.
+// With d_outs (one RowOut per segment, device memory) the keys are join
+// sort keys with r_payload values and the last pass writes each segment's
+// rows unpacked into its RowOut; the packed result is then not kept.
 int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
                                const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
                                uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
-                               hipStream_t st, bool *in_alt);
+                               hipStream_t st, bool *in_alt, const RowOut *d_outs = nullptr);
 // Join rows sort ascending by (key, s_payload) as signed int32: flip the sign
 // bits so unsigned 64-bit order is the signed lexicographic order.
 __host__ __device__ __forceinline__ uint64_t join_sort_key(int32_t key, int32_t spay) {

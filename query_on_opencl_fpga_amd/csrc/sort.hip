@@ -293,12 +293,13 @@ __global__ __launch_bounds__(256) void k_rows_apply_seg(uint32_t *__restrict__ r
 // Places global tile xcd_tile(blockIdx.x): ranks its keys by digit (stable),
 // stages them in LDS in digit order and writes each digit's run at the offset
 // from its row (plus, for a batch, the digit's start in the segment's output).
-template <bool HAS_VAL>
+template <bool HAS_VAL, bool UNPACK = false>
 __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                      RadixSeg seg0, const RadixSeg *segs, int nseg, uint32_t ntile,
                                                      int shift, const uint32_t *__restrict__ offs,
-                                                     const uint32_t *__restrict__ dstart, uint32_t sstride) {
+                                                     const uint32_t *__restrict__ dstart, uint32_t sstride,
+                                                     RowOut out0 = RowOut{}, const RowOut *outs = nullptr) {
     __shared__ uint32_t s_cnt[4][256];
     __shared__ uint32_t s_dpre[256];
     __shared__ uint32_t s_gofs[256];
@@ -362,6 +363,19 @@ __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict_
         }
     }
     __syncthreads();
+    if constexpr (UNPACK) {
+        // Last pass of a join-row sort: the segment's output columns.
+        const RowOut o = outs ? outs[si] : out0;
+        for (uint32_t i = threadIdx.x; i < rem; i += 256) {
+            const uint64_t kk = s_k[i];
+            const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+            const uint32_t gp = s_gofs[d] + (i - s_dpre[d]);
+            o.key[gp] = join_sort_key_key(kk);
+            o.spay[gp] = join_sort_key_spay(kk);
+            o.rpay[gp] = (int32_t)s_v[i];
+        }
+        return;
+    }
     uint64_t *kd = kout + sg.base;
     uint32_t *vd = HAS_VAL ? vout + sg.base : nullptr;
     for (uint32_t i = threadIdx.x; i < rem; i += 256) {
@@ -386,7 +400,8 @@ static RowsBufs rows_carve(uint32_t tiles, Carver &cv) {
 }
 
 static int launch_radix_rows(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, const RadixSeg &sg,
-                             int lo_bit, int hi_bit, const RowsBufs &b, hipStream_t st, bool *in_alt) {
+                             int lo_bit, int hi_bit, const RowsBufs &b, hipStream_t st, bool *in_alt,
+                             const RowOut *unpack = nullptr) {
     *in_alt = false;
     QO_REQUIRE(lo_bit >= 0 && hi_bit <= 64 && lo_bit < hi_bit, "bad bit range");
     if (sg.ntile == 0) return QO_OK;
@@ -409,7 +424,10 @@ static int launch_radix_rows(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uin
             k_rows_apply<<<rp.nb, 256, 0, st>>>(b.rows, sg.ntile, rp.span, b.bsum);
             QO_LAUNCHED();
         }
-        if (vals)
+        if (vals && unpack && sh + 8 >= hi_bit)
+            k_radix_place<true, true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, nullptr, 1, sg.ntile, sh,
+                                                                 b.rows, nullptr, 0, *unpack, nullptr);
+        else if (vals)
             k_radix_place<true><<<sg.ntile, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, sg, nullptr, 1, sg.ntile, sh,
                                                            b.rows, nullptr, 0);
         else
@@ -479,9 +497,10 @@ size_t radix_rows_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks) {
 int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
                                const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
                                uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
-                               hipStream_t st, bool *in_alt) {
+                               hipStream_t st, bool *in_alt, const RowOut *d_outs) {
     *in_alt = false;
     if (total_tiles == 0 || nchunks == 0) return QO_OK;
+    QO_REQUIRE(!d_outs || vals, "unpacked output needs values");
     const int npass = (hi_bit - lo_bit + 7) / 8;
     QO_REQUIRE(npass >= 1 && npass <= kSweepMaxPass && lo_bit >= 0 && hi_bit <= 64, "bad bit range");
     QO_REQUIRE(nseg >= 1 && (size_t)nseg * npass < 0x7FFFFFFFull, "bad segment count");
@@ -512,7 +531,11 @@ int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, u
         QO_LAUNCHED();
         k_rows_apply_seg<<<rp.nb, 256, 0, st>>>(rows, total_tiles, rp.span, d_segs, nseg, bsum);
         QO_LAUNCHED();
-        if (vals)
+        if (vals && d_outs && p == npass - 1)
+            k_radix_place<true, true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, none, d_segs, nseg,
+                                                                   total_tiles, sh, rows, starts + p * 256,
+                                                                   (uint32_t)npass * 256, RowOut{}, d_outs);
+        else if (vals)
             k_radix_place<true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, none, d_segs, nseg, total_tiles,
                                                              sh, rows, starts + p * 256, (uint32_t)npass * 256);
         else
@@ -544,7 +567,7 @@ size_t radix_workspace_bytes(size_t n, bool with_vals) {
 }
 
 static int radix_impl(uint64_t *keys, uint32_t *vals, size_t n, const uint64_t *d_n, int lo_bit, int hi_bit,
-                      void *ws, hipStream_t st) {
+                      void *ws, hipStream_t st, const RowOut *unpack = nullptr) {
     if (n == 0) return QO_OK;
     QO_REQUIRE(n < 0xFFFFFFFFull, "sort input too large");
     QO_REQUIRE(lo_bit >= 0 && hi_bit <= 64 && lo_bit < hi_bit, "bad bit range");
@@ -556,8 +579,8 @@ static int radix_impl(uint64_t *keys, uint32_t *vals, size_t n, const uint64_t *
     radix_carve(n, vals != nullptr, cv, &kalt, &valt, &rb);
     RadixSeg seg{0, n, d_n, 0, tiles};
     bool in_alt = false;
-    QO_TRY(launch_radix_rows(keys, vals, kalt, valt, seg, lo_bit, hi_bit, rb, st, &in_alt));
-    if (in_alt) {
+    QO_TRY(launch_radix_rows(keys, vals, kalt, valt, seg, lo_bit, hi_bit, rb, st, &in_alt, unpack));
+    if (in_alt && !unpack) {
         // Odd number of passes: the result is in the alternate buffers.  The
         // count may only be known on the device, so copy the full capacity.
         QO_HIP(hipMemcpyAsync(keys, kalt, n * 8, hipMemcpyDeviceToDevice, st));
@@ -608,10 +631,7 @@ int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void
     unsigned g = grid_for(m, 256, (unsigned)props().cu_count * 8);
     k_pack_join<<<g, 256, 0, st>>>(key, spay, rpay, m, k64, v32);
     QO_LAUNCHED();
-    QO_TRY(launch_radix_sort_u64(k64, v32, m, 0, 64, rws, st));
-    k_unpack_join<<<g, 256, 0, st>>>(k64, v32, m, key, spay, rpay);
-    QO_LAUNCHED();
-    return QO_OK;
+    return sort_packed_join_rows(k64, v32, m, rws, key, rpay, spay, st);
 }
 
 size_t sort_join_rows_ws_bytes(size_t m) { return m * 12 + 1024 + radix_workspace_bytes(m, true); }
@@ -619,11 +639,14 @@ size_t sort_join_rows_ws_bytes(size_t m) { return m * 12 + 1024 + radix_workspac
 int sort_packed_join_rows(uint64_t *k64, uint32_t *v32, size_t m, void *rws, int32_t *key, int32_t *rpay,
                           int32_t *spay, hipStream_t st) {
     if (m == 0) return QO_OK;
-    if (m > 1) QO_TRY(launch_radix_sort_u64(k64, v32, m, 0, 64, rws, st));
-    unsigned g = grid_for(m, 256, (unsigned)props().cu_count * 8);
-    k_unpack_join<<<g, 256, 0, st>>>(k64, v32, m, key, spay, rpay);
-    QO_LAUNCHED();
-    return QO_OK;
+    if (m == 1) {
+        k_unpack_join<<<1, 256, 0, st>>>(k64, v32, m, key, spay, rpay);
+        QO_LAUNCHED();
+        return QO_OK;
+    }
+    // The last radix pass writes the output columns (no separate unpack pass).
+    const RowOut out{key, rpay, spay};
+    return radix_impl(k64, v32, m, nullptr, 0, 64, rws, st, &out);
 }
 
 }  // namespace qo
