@@ -460,12 +460,23 @@ int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, cons
     out.v32 = v32;
     QO_TRY(launch_join_probe(table, table_bytes, skey, spay, sfilt, ns, flo, fhi, out, d_count, sws, st));
     uint64_t m = 0;
+    int32_t range[2] = {INT32_MAX, INT32_MIN};  // build key range, when the build measured it
+    JoinHeader *hdr = static_cast<JoinHeader *>(table);
     QO_HIP(hipMemcpyAsync(&m, d_count, 8, hipMemcpyDeviceToHost, st));
+    QO_HIP(hipMemcpyAsync(range, &hdr->rmin, sizeof(range), hipMemcpyDeviceToHost, st));
     QO_HIP(hipStreamSynchronize(st));
     *h_count = m;
     if (m > out_capacity) return set_error(QO_ERR_CAPACITY, "join produced %llu rows, capacity %zu",
                                            (unsigned long long)m, out_capacity);
-    QO_TRY(sort_packed_join_rows(k64, v32, m, sws, okey, orpay, ospay, st));
+    // Every output key is a build key, so key bits that all build keys share
+    // need no radix pass: with keys in [rmin, rmax] only the bits below the
+    // highest bit where (rmin, rmax) differ vary (as sort keys, key ^ 2^31).
+    int hi_bit = 64;
+    if (range[0] <= range[1]) {
+        const uint32_t d = ((uint32_t)range[0] ^ 0x80000000u) ^ ((uint32_t)range[1] ^ 0x80000000u);
+        hi_bit = 32 + (d ? 32 - __builtin_clz(d) : 0);
+    }
+    QO_TRY(sort_packed_join_rows(k64, v32, m, sws, okey, orpay, ospay, st, hi_bit));
     QO_HIP(hipStreamSynchronize(st));
     return QO_OK;
 }

@@ -637,7 +637,7 @@ int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void
 size_t sort_join_rows_ws_bytes(size_t m) { return m * 12 + 1024 + radix_workspace_bytes(m, true); }
 
 int sort_packed_join_rows(uint64_t *k64, uint32_t *v32, size_t m, void *rws, int32_t *key, int32_t *rpay,
-                          int32_t *spay, hipStream_t st) {
+                          int32_t *spay, hipStream_t st, int hi_bit) {
     if (m == 0) return QO_OK;
     if (m == 1) {
         k_unpack_join<<<1, 256, 0, st>>>(k64, v32, m, key, spay, rpay);
@@ -646,7 +646,7 @@ int sort_packed_join_rows(uint64_t *k64, uint32_t *v32, size_t m, void *rws, int
     }
     // The last radix pass writes the output columns (no separate unpack pass).
     const RowOut out{key, rpay, spay};
-    return radix_impl(k64, v32, m, nullptr, 0, 64, rws, st, &out);
+    return radix_impl(k64, v32, m, nullptr, 0, std::max(1, std::min(64, hi_bit)), rws, st, &out);
 }
 
 }  // namespace qo

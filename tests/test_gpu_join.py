@@ -111,6 +111,25 @@ def test_hj_filter_sort_matches_oracle(qo, orc, nr, mult):
         assert np.array_equal(g, e)
 
 
+@pytest.mark.parametrize("base,nr", [(0, 1 << 16), (1000, 256), (-5000, 4096), (-100, 201), (INT_MAX - 4095, 4096),
+                                     (INT_MIN, 4096), (77, 1)])
+def test_hj_filter_sort_key_ranges(qo, orc, base, nr):
+    """The sort skips radix passes over key bits that every build key shares;
+    build keys in narrow ranges (positive, negative, across zero, at both ends
+    of int32, a single key) must still come out fully ordered by (key, s_payload)."""
+    rng = np.random.default_rng(nr)
+    rk = (np.int64(base) + rng.permutation(nr)).astype(np.int32)
+    rp = rng.integers(INT_MIN, INT_MAX, nr, dtype=np.int64).astype(np.int32)
+    ns = max(64, nr * 8)
+    sk = rk[rng.integers(0, nr, ns)]
+    sp = rng.integers(INT_MIN, INT_MAX, ns, dtype=np.int64).astype(np.int32)
+    sf = rng.integers(0, 1000, ns).astype(np.int32)
+    got = qo.operators.hj_filter_sort(rk, rp, sk, sp, sf, INT_MIN, 599)
+    exp = orc.sort_join_rows(*orc.hash_join(rk, rp, sk, sp, sf, INT_MIN, 599))
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
 # ---------------------------------------------------------------------------
 # Table layouts and probe strategies:
 #  * direct: hash-mixed table, direct build and probe;
