@@ -92,12 +92,18 @@ struct PackedEmit {
     }
 };
 
+// One ticket covers kProbeTicketTiles consecutive S tiles (of any queries):
+// fewer same-address ticket atomics for a batch of ~10^6 tiles.
+constexpr uint32_t kProbeTicketTiles = 4;
+
 __global__ __launch_bounds__(256) void k_b_probe(const BQuery *q, int nq, uint32_t total, uint32_t *ticket,
                                                  uint64_t *mkeys, uint32_t *mvals) {
     __shared__ uint32_t s_tot[tile_offsets_words(kProbeJ, 4)];
     __shared__ uint64_t s_base;
     __shared__ JoinStage stage;
-    QO_TILE_LOOP(ticket, total, t)
+    const uint32_t groups = (total + kProbeTicketTiles - 1) / kProbeTicketTiles;
+    QO_TILE_LOOP(ticket, groups, tg)
+    for (uint32_t t = tg * kProbeTicketTiles; t < total && t < (tg + 1) * kProbeTicketTiles; ++t) {
     const int qi = query_of<1>(q, nq, t);
     const BQuery &Q = q[qi];
     const uint64_t *slots = reinterpret_cast<const uint64_t *>(Q.hdr + 1);
@@ -108,6 +114,8 @@ __global__ __launch_bounds__(256) void k_b_probe(const BQuery *q, int nq, uint32
     else
         probe_tile<false>(Q.hdr, slots, Q.skey, Q.spay, Q.sfilt, Q.ns, Q.flo, Q.fhi, Q.vec != 0, t - Q.stile0, Q.count,
                           emit, s_tot, &s_base, &stage);
+    __syncthreads();  // s_tot, s_base and the stage are reused by the next tile
+    }
     QO_TILE_LOOP_END
 }
 
