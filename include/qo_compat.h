@@ -1,6 +1,6 @@
 /*
  * qo_compat.h -- drop-in replacements for the reference's host operator entry
- * points (Xtra-Computing/Query_on_OpenCL_FPGA, host/host/*.cpp), backed by
+ * points (Xtra-Computing/Query_on_OpenCL_FPGA, host/host/ *.cpp), backed by
  * libqo's gfx950 kernels.
  *
  * Names, parameter order and meanings follow the reference.  `cl_mem` becomes

@@ -28,13 +28,13 @@
 
 namespace qo {
 
-// Last query whose first tile (selected by `which`) is <= t.
+// Last query whose first R tile (WHICH = 0) or S tile (WHICH = 1) is <= t.
 template <int WHICH>
 QO_DEV int query_of(const BQuery *q, int nq, uint32_t t) {
     int lo = 0, hi = nq - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        uint32_t t0 = WHICH == 0 ? q[mid].rtile0 : WHICH == 1 ? q[mid].stile0 : q[mid].mtile0;
+        uint32_t t0 = WHICH == 0 ? q[mid].rtile0 : q[mid].stile0;
         if (t0 <= t) lo = mid;
         else hi = mid - 1;
     }

@@ -89,8 +89,8 @@ void *workspace(hipStream_t s, size_t bytes, int *status) {
     if (w.bytes >= bytes) return w.ptr;
     if (w.ptr) {
         // The old workspace may still be read by queued work on this stream.
-        hipStreamSynchronize(s);
-        hipFree(w.ptr);
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(w.ptr);
         w.ptr = nullptr;
         w.bytes = 0;
     }
@@ -304,8 +304,8 @@ int qo_stream_destroy(qo_stream s) {
         std::lock_guard<std::mutex> lk(g_mu);
         auto it = g_ws.find(st);
         if (it != g_ws.end()) {
-            hipStreamSynchronize(st);
-            if (it->second.ptr) hipFree(it->second.ptr);
+            (void)hipStreamSynchronize(st);
+            if (it->second.ptr) (void)hipFree(it->second.ptr);
             g_ws.erase(it);
         }
     }
@@ -371,7 +371,7 @@ int qo_release_workspaces(void) {
     std::lock_guard<std::mutex> lk(g_mu);
     QO_HIP(hipDeviceSynchronize());
     for (auto &kv : g_ws)
-        if (kv.second.ptr) hipFree(kv.second.ptr);
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     g_ws.clear();
     return QO_OK;
 }
