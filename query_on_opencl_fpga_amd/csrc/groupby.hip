@@ -282,6 +282,219 @@ constexpr uint32_t kGbChunk = 1u << 18;  // tuples per work item of a large buck
 constexpr uint32_t kGbWhole = 1u << 19;  // buckets up to this size are one work item
 constexpr uint32_t kChunkU = 8;          // tuples in flight per thread of k_groupby_chunks
 
+// ---------------------------------------------------------------------------
+// One-pass exact partition into up to kOnePassParts fine buckets: per-tile
+// bucket counts in tile-major rows, a column scan of the rows into the output
+// offset of every (tile, bucket), and one placement pass whose tiles run in
+// XCD-contiguous order, so the short runs that consecutive tiles append to a
+// bucket are merged into whole lines by one L2.  No reservation atomics and no
+// second split pass; the bucket of a key is the top L bits of key * 2^32/phi,
+// as in the two-pass exact split.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kOnePassParts = 1024;
+constexpr int kPartThreads = 1024;
+constexpr int kPartJ = 8;  // tuples per thread and tile
+constexpr uint32_t kPartTile = (uint32_t)kPartThreads * kPartJ;
+constexpr uint32_t kPartScanBlocks = 128;
+constexpr uint32_t kPartMinSpan = 64;
+
+QO_DEV uint32_t fib_part(int32_t key, int L) { return ((uint32_t)key * 0x9E3779B9u) >> (32 - L); }
+
+// rows[t * nb + f] = tuples of tile t in bucket f.  VEC: the key column is
+// 16-byte aligned.
+template <bool VEC>
+__global__ __launch_bounds__(kPartThreads) void k_part_hist(const int32_t *__restrict__ key, size_t n, int L,
+                                                            uint32_t nb, uint32_t *__restrict__ rows) {
+    __shared__ uint32_t s_c[kOnePassParts];
+    for (uint32_t i = threadIdx.x; i < nb; i += kPartThreads) s_c[i] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kPartTile;
+    const uint32_t rem = (uint32_t)min((size_t)kPartTile, n - base);
+    const int32_t *src = key + base;
+    if (VEC && rem == kPartTile) {
+        // Full tile, 16-byte aligned: kPartJ / 4 int4 loads per thread.
+        int4 q[kPartJ / 4];
+#pragma unroll
+        for (int j = 0; j < kPartJ / 4; ++j)
+            q[j] = ld_stream(reinterpret_cast<const int4 *>(src) + (uint32_t)j * kPartThreads + threadIdx.x);
+#pragma unroll
+        for (int j = 0; j < kPartJ / 4; ++j) {
+            atomicAdd(&s_c[fib_part(q[j].x, L)], 1u);
+            atomicAdd(&s_c[fib_part(q[j].y, L)], 1u);
+            atomicAdd(&s_c[fib_part(q[j].z, L)], 1u);
+            atomicAdd(&s_c[fib_part(q[j].w, L)], 1u);
+        }
+    } else {
+        int32_t k[kPartJ];
+#pragma unroll
+        for (int j = 0; j < kPartJ; ++j) {
+            const uint32_t i = (uint32_t)j * kPartThreads + threadIdx.x;
+            k[j] = i < rem ? ld_stream(src + i) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kPartJ; ++j)
+            if ((uint32_t)j * kPartThreads + threadIdx.x < rem) atomicAdd(&s_c[fib_part(k[j], L)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < nb; f += kPartThreads) rows[(size_t)blockIdx.x * nb + f] = s_c[f];
+}
+
+// bsum[b * nb + f] = bucket-f tuples of tiles [b * span, (b + 1) * span).
+__global__ __launch_bounds__(kPartThreads) void k_part_reduce(const uint32_t *__restrict__ rows, uint32_t T, uint32_t nb,
+                                                              uint32_t span, uint32_t *__restrict__ bsum) {
+    const uint32_t f = threadIdx.x;
+    if (f >= nb) return;
+    const uint32_t t0 = blockIdx.x * span, t1 = min(T, t0 + span);
+    uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t t = t0;
+    for (; t + 8 <= t1; t += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] += rows[(size_t)(t + u) * nb + f];
+    for (; t < t1; ++t) c[0] += rows[(size_t)t * nb + f];
+    bsum[(size_t)blockIdx.x * nb + f] = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7];
+}
+
+// One block: total[f] = tuples in bucket f, starts[f] = where bucket f begins
+// (starts[nb] = n), and bsum[b][f] := starts[f] + bucket-f tuples of the tiles
+// of blocks before b (in place).
+__global__ __launch_bounds__(kPartThreads) void k_part_starts(uint32_t *bsum, uint32_t nbk, uint32_t nb,
+                                                              uint32_t *total, uint32_t *starts) {
+    __shared__ uint32_t s_w[kPartThreads / kWave];
+    const uint32_t f = threadIdx.x;
+    uint32_t tot = 0;
+    if (f < nb) {
+        uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t b = 0;
+        for (; b + 8 <= nbk; b += 8)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c[u] += bsum[(size_t)(b + u) * nb + f];
+        for (; b < nbk; ++b) c[0] += bsum[(size_t)b * nb + f];
+        tot = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7];
+    }
+    const uint32_t start = block_excl_scan<kPartThreads / kWave>(tot, s_w);
+    if (f >= nb) return;
+    total[f] = tot;
+    starts[f] = start;
+    if (f == nb - 1) starts[nb] = start + tot;
+    uint32_t run = start;
+    for (uint32_t b = 0; b < nbk; ++b) {
+        const uint32_t v = bsum[(size_t)b * nb + f];
+        bsum[(size_t)b * nb + f] = run;
+        run += v;
+    }
+}
+
+// rows[t][f] := output offset of the first bucket-f tuple of tile t.
+__global__ __launch_bounds__(kPartThreads) void k_part_apply(uint32_t *__restrict__ rows, uint32_t T, uint32_t nb,
+                                                             uint32_t span, const uint32_t *__restrict__ bsum) {
+    const uint32_t f = threadIdx.x;
+    if (f >= nb) return;
+    const uint32_t t0 = blockIdx.x * span, t1 = min(T, t0 + span);
+    uint32_t run = bsum[(size_t)blockIdx.x * nb + f];
+    uint32_t t = t0;
+    for (; t + 8 <= t1; t += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = rows[(size_t)(t + u) * nb + f];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            rows[(size_t)(t + u) * nb + f] = run;
+            run += v[u];
+        }
+    }
+    for (; t < t1; ++t) {
+        const uint32_t v = rows[(size_t)t * nb + f];
+        rows[(size_t)t * nb + f] = run;
+        run += v;
+    }
+}
+
+// Places tile xcd_tile(blockIdx.x): ranks its tuples by bucket (LDS atomics;
+// order inside a bucket is free for aggregation), stages them in bucket order
+// and writes each bucket's run at the offset from its row.
+__global__ __launch_bounds__(kPartThreads) void k_part_place(const int32_t *__restrict__ key,
+                                                             const int32_t *__restrict__ val, size_t n, int L,
+                                                             uint32_t nb, uint32_t T, const uint32_t *__restrict__ offs,
+                                                             uint64_t *__restrict__ out) {
+    __shared__ uint64_t s_stage[kPartTile];
+    __shared__ uint32_t s_cnt[kOnePassParts], s_loff[kOnePassParts], s_g[kOnePassParts];
+    __shared__ uint32_t s_w[kPartThreads / kWave];
+    const uint32_t t = xcd_tile(blockIdx.x, T);
+    const size_t base = (size_t)t * kPartTile;
+    const uint32_t rem = (uint32_t)min((size_t)kPartTile, n - base);
+    for (uint32_t i = threadIdx.x; i < nb; i += kPartThreads) s_cnt[i] = 0;
+    __syncthreads();
+    int32_t k[kPartJ], v[kPartJ];
+    uint32_t r[kPartJ];
+#pragma unroll
+    for (int j = 0; j < kPartJ; ++j) {
+        const uint32_t i = (uint32_t)j * kPartThreads + threadIdx.x;
+        k[j] = i < rem ? ld_stream(key + base + i) : 0;
+        v[j] = i < rem ? ld_stream(val + base + i) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPartJ; ++j)
+        r[j] = (uint32_t)j * kPartThreads + threadIdx.x < rem ? atomicAdd(&s_cnt[fib_part(k[j], L)], 1u) : 0u;
+    __syncthreads();
+    {
+        const uint32_t f = threadIdx.x;
+        const uint32_t c = f < nb ? s_cnt[f] : 0u;
+        const uint32_t ex = block_excl_scan<kPartThreads / kWave>(c, s_w);
+        if (f < nb) {
+            s_loff[f] = ex;
+            s_g[f] = offs[(size_t)t * nb + f];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPartJ; ++j)
+        if ((uint32_t)j * kPartThreads + threadIdx.x < rem)
+            s_stage[s_loff[fib_part(k[j], L)] + r[j]] = ((uint64_t)(uint32_t)v[j] << 32) | (uint32_t)k[j];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < rem; i += kPartThreads) {
+        const uint64_t e = s_stage[i];
+        const uint32_t f = fib_part((int32_t)(uint32_t)e, L);
+        out[s_g[f] + (i - s_loff[f])] = e;
+    }
+}
+
+struct OnePassBufs {
+    uint32_t *rows = nullptr, *bsum = nullptr, *total = nullptr, *starts = nullptr;
+    uint64_t *fine = nullptr;
+    uint32_t T = 0, span = 0, nbk = 0;
+};
+
+static OnePassBufs onepass_carve(size_t n, uint32_t nb, Carver &cv) {
+    OnePassBufs b;
+    b.T = (uint32_t)((n + kPartTile - 1) / kPartTile);
+    b.span = std::max(kPartMinSpan, (b.T + kPartScanBlocks - 1) / kPartScanBlocks);
+    b.nbk = (b.T + b.span - 1) / b.span;
+    b.rows = cv.take<uint32_t>((size_t)std::max(1u, b.T) * nb);
+    b.bsum = cv.take<uint32_t>((size_t)std::max(1u, b.nbk) * nb);
+    b.total = cv.take<uint32_t>(nb + 1);
+    b.starts = cv.take<uint32_t>(nb + 1);
+    b.fine = cv.take<uint64_t>(n ? n : 1);
+    return b;
+}
+
+static int launch_onepass_partition(const OnePassBufs &b, const int32_t *key, const int32_t *val, size_t n, int L,
+                                    uint32_t nb, hipStream_t st) {
+    QO_REQUIRE(n < 0xFFFFFFFFull && nb <= kOnePassParts && (1u << L) == nb, "one-pass partition too large");
+    if (b.T == 0) return QO_OK;
+    if ((reinterpret_cast<uintptr_t>(key) & 15) == 0) k_part_hist<true><<<b.T, kPartThreads, 0, st>>>(key, n, L, nb, b.rows);
+    else k_part_hist<false><<<b.T, kPartThreads, 0, st>>>(key, n, L, nb, b.rows);
+    QO_LAUNCHED();
+    k_part_reduce<<<b.nbk, kPartThreads, 0, st>>>(b.rows, b.T, nb, b.span, b.bsum);
+    QO_LAUNCHED();
+    k_part_starts<<<1, kPartThreads, 0, st>>>(b.bsum, b.nbk, nb, b.total, b.starts);
+    QO_LAUNCHED();
+    k_part_apply<<<b.nbk, kPartThreads, 0, st>>>(b.rows, b.T, nb, b.span, b.bsum);
+    QO_LAUNCHED();
+    k_part_place<<<b.T, kPartThreads, 0, st>>>(key, val, n, L, nb, b.T, b.rows, b.fine);
+    QO_LAUNCHED();
+    return QO_OK;
+}
+
 // Work items per fine bucket: one for a bucket of at most kGbWhole tuples,
 // else ceil(hist[f] / kGbChunk); cch[nb] = 0.
 __global__ void k_chunk_counts(const uint32_t *hist, uint32_t nb, uint32_t *cch) {
@@ -434,7 +647,8 @@ struct GroupbyBufs {
     GEntry *rows;        // finished groups, unordered
     uint64_t *list;
     void *rws;
-    ExactSplitBufs sb;   // partitioned path only
+    ExactSplitBufs sb;   // partitioned path, two-pass split (more than kOnePassParts buckets)
+    OnePassBufs op;      // partitioned path, one-pass partition
     uint32_t *cch = nullptr, *cpre = nullptr;
     void *scan_ws = nullptr;
 };
@@ -449,7 +663,8 @@ static GroupbyBufs groupby_carve(const GroupbyPlan &p, const GbPartPlan &pp, siz
     b.rws = cv.take<char>(radix_workspace_bytes(group_capacity, false));
     if (pp.on) {
         const size_t nb = size_t(1) << pp.s.L;
-        b.sb = exact_split_carve(pp.s, n, cv);
+        if (nb <= kOnePassParts) b.op = onepass_carve(n, (uint32_t)nb, cv);
+        else b.sb = exact_split_carve(pp.s, n, cv);
         b.cch = cv.take<uint32_t>(nb + 1);
         b.cpre = cv.take<uint32_t>(nb + 1);
         b.scan_ws = cv.take<char>(scan_workspace_bytes(nb + 1));
@@ -481,15 +696,30 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
     QO_LAUNCHED();
     if (n && pp.on) {
         const uint32_t nb = 1u << pp.s.L;
-        QO_TRY(launch_exact_splits(pp.s, b.sb, key, val, n, st));
-        k_chunk_counts<<<(nb + 1 + 255) / 256, 256, 0, st>>>(b.sb.hist, nb, b.cch);
+        const bool onepass = nb <= kOnePassParts;
+        // Fine buckets back to back: bucket f at [offp[f], offp[f] + off[f + 1] - off[f]) of `fine`.
+        const uint32_t *hist, *offp, *off;
+        const uint64_t *fine;
+        if (onepass) {
+            QO_TRY(launch_onepass_partition(b.op, key, val, n, pp.s.L, nb, st));
+            hist = b.op.total;
+            offp = off = b.op.starts;
+            fine = b.op.fine;
+        } else {
+            QO_TRY(launch_exact_splits(pp.s, b.sb, key, val, n, st));
+            hist = b.sb.hist;
+            offp = b.sb.offp;
+            off = b.sb.off;
+            fine = b.sb.fine;
+        }
+        k_chunk_counts<<<(nb + 1 + 255) / 256, 256, 0, st>>>(hist, nb, b.cch);
         QO_LAUNCHED();
         QO_TRY(launch_scan_excl_u32(b.cch, b.cpre, nb + 1, b.scan_ws, st));
         // Upper bound on the chunk count: every bucket one partial chunk.
         const uint64_t items = (uint64_t)nb + (n + kGbChunk - 1) / kGbChunk;
         QO_REQUIRE(items < 0x7FFFFFFFull, "group-by input too large");
         k_groupby_chunks<2048, kChunkThreads><<<(unsigned)items, kChunkThreads, sizeof(LTable<2048>), st>>>(
-            b.sb.fine, b.sb.offp, b.sb.off, b.cpre, nb, pp.s.L, gt, p.gmask, 64, overflow, b.rows,
+            fine, offp, off, b.cpre, nb, pp.s.L, gt, p.gmask, 64, overflow, b.rows,
             reinterpret_cast<unsigned long long *>(counters), group_capacity);
         QO_LAUNCHED();
     } else if (n) {
