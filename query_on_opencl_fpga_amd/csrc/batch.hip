@@ -93,10 +93,13 @@ struct PackedEmit {
 };
 
 // One ticket covers kProbeTicketTiles consecutive S tiles (of any queries):
-// fewer same-address ticket atomics for a batch of ~10^6 tiles.
+// fewer same-address ticket atomics for a batch of ~10^6 tiles.  The launch
+// bound asks for 3 waves per SIMD (the 48 KB output stage allows no more): at
+// 184 VGPRs the kernel ran 2 and took 48.3 ms per 512-query batch, with the
+// bound (168 VGPRs, a few spilled) 39.7 ms.
 constexpr uint32_t kProbeTicketTiles = 4;
 
-__global__ __launch_bounds__(256) void k_b_probe(const BQuery *q, int nq, uint32_t total, uint32_t *ticket,
+__global__ __launch_bounds__(256, 3) void k_b_probe(const BQuery *q, int nq, uint32_t total, uint32_t *ticket,
                                                  uint64_t *mkeys, uint32_t *mvals) {
     __shared__ uint32_t s_tot[tile_offsets_words(kProbeJ, 4)];
     __shared__ uint64_t s_base;
