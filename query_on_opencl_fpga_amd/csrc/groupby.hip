@@ -13,6 +13,7 @@
 //    505-528) and the per-group reduction with the reference's operator
 //    semantics (parallelAggregate_kernel, :547-716).
 #include <algorithm>
+#include <type_traits>
 #include <climits>
 
 #include "qo_device.hpp"
@@ -90,15 +91,27 @@ QO_DEV void g_update(GEntry *t, uint64_t mask, uint32_t key, uint64_t cnt, uint6
 
 // LDS slot for `key` within `probes` probes, or -1.  The home slot is the
 // next log2(C) bits of the multiplicative hash after the `skip` top bits (a
-// split by those bits put the tuple in this table).  It is tested first on its
-// own: in the common case (the key is already there) no lane of the wave
-// enters the probe loop.
-template <int C>
+// split by those bits put the tuple in this table).  The home slot is tested
+// first on its own, or with TWO the home slot and the one after it, read
+// together: in a table at most half full nearly every key sits in one of them
+// (2^20 dense keys over 1024 tables of 2048 slots: 73 % at home, 27 % one slot
+// on, 0.02 % further), so the wave enters the probe loop only when one of its
+// lanes is further out.  `lkey` holds C + 2 words; the two past the end stay
+// kEmptyKey, which no looked-up key equals, so the second read needs no wrap.
+// TWO costs LDS bank cycles where keys do sit at home (a few groups).
+template <int C, bool TWO = false>
 QO_DEV int l_slot(uint32_t *lkey, uint32_t key, int probes, int skip) {
     static_assert((C & (C - 1)) == 0, "power-of-two table");
     constexpr int kBits = __builtin_ctz(C);
     uint32_t h = ((key * 0x9E3779B9u) << skip) >> (32 - kBits);
-    if (lkey[h] == key) return (int)h;
+    if (TWO) {
+        typedef uint32_t Pair __attribute__((ext_vector_type(2), aligned(4)));
+        const Pair kk = *reinterpret_cast<const Pair *>(lkey + h);  // one two-word LDS read
+        if (kk.x == key) return (int)h;
+        if (kk.y == key) return (int)(h + 1);
+    } else if (lkey[h] == key) {
+        return (int)h;
+    }
     for (int p = 0; p < probes; ++p) {
         uint32_t k = lkey[h];
         if (k == key) return (int)h;
@@ -111,9 +124,11 @@ QO_DEV int l_slot(uint32_t *lkey, uint32_t key, int probes, int skip) {
     return -1;
 }
 
+// Block-local aggregation table; key[C] and key[C + 1] are padding (always
+// kEmptyKey) so that l_slot can read two slots from any home slot.
 template <int C>
 struct LTable {
-    uint32_t key[C];
+    uint32_t key[C + 2];
     uint32_t cnt[C];
     int32_t mn[C];
     int32_t mx[C];
@@ -122,6 +137,7 @@ struct LTable {
 
 template <int C>
 QO_DEV void ltable_init(LTable<C> &L) {
+    if (threadIdx.x < 2) L.key[C + threadIdx.x] = kEmptyKey;
     for (int i = threadIdx.x; i < C; i += blockDim.x) {
         L.key[i] = kEmptyKey;
         L.cnt[i] = 0;
@@ -139,7 +155,7 @@ QO_DEV void ltable_init(LTable<C> &L) {
 // quarter of the lanes hold the key of the first lane, those lanes combine
 // (COUNT, SUM, MIN, MAX) across the wave first and one of them updates the
 // table, instead of all of them updating one LDS address one after another.
-template <int C, bool COMBINE = false>
+template <int C, bool COMBINE = false, bool TWO = false>
 QO_DEV void ltable_add(LTable<C> &L, int32_t k, int32_t v, int probes, GEntry *gt, uint64_t gmask,
                        uint32_t *overflow, int skip = 0) {
     uint32_t cnt = 1;
@@ -162,7 +178,7 @@ QO_DEV void ltable_add(LTable<C> &L, int32_t k, int32_t v, int probes, GEntry *g
             }
         }
     }
-    int s = (uint32_t)k == kEmptyKey ? -1 : l_slot<C>(L.key, (uint32_t)k, probes, skip);
+    int s = (uint32_t)k == kEmptyKey ? -1 : l_slot<C, TWO>(L.key, (uint32_t)k, probes, skip);
     if (s >= 0) {
         atomicAdd(&L.cnt[s], cnt);
         atomicAdd(&L.sum[s], (unsigned long long)sum);
@@ -220,50 +236,144 @@ QO_DEV void ltable_append(const LTable<C> &L, GEntry *rows, unsigned long long *
     }
 }
 
+// Dominant key of a block: the most common key among 64 sampled tuples (one
+// wave; kEmptyKey for lanes without a tuple), and how often it occurs there.
+QO_DEV uint32_t sample_dominant(uint32_t k, unsigned *count) {
+    uint32_t best = k;
+    unsigned bc = 0;
+#pragma unroll
+    for (int c = 0; c < kWave; c += 8) {
+        const uint32_t cand = (uint32_t)__shfl((int)k, c, kWave);
+        const unsigned n = popc(ballot(k == cand));
+        if (n > bc) {
+            bc = n;
+            best = cand;
+        }
+    }
+    *count = bc;
+    return best;
+}
+
+// Per-thread COUNT / SUM / MIN / MAX of the dominant key's tuples.
+struct DomAcc {
+    uint32_t c = 0;
+    uint64_t s = 0;
+    int32_t mn = INT_MAX, mx = INT_MIN;
+    QO_DEV void add(int32_t x) {
+        ++c;
+        s += (uint64_t)(int64_t)x;
+        mn = x < mn ? x : mn;
+        mx = x > mx ? x : mx;
+    }
+};
+
+// Block totals of the dominant key, added once to its LDS entry (or to the
+// global table when it has none).  All NT threads call it; the caller
+// synchronises before reading the table.
+template <int C, int NT>
+QO_DEV void dom_merge(LTable<C> &L, const DomAcc &a, uint32_t dom, int probes, int skip, GEntry *gt, uint64_t gmask,
+                      uint32_t *overflow) {
+    __shared__ uint64_t s_ds[NT / kWave];
+    __shared__ uint32_t s_dc[NT / kWave];
+    __shared__ int32_t s_dmn[NT / kWave], s_dmx[NT / kWave];
+    const uint64_t wc = wave_sum((uint64_t)a.c), ws = wave_sum(a.s);
+    const int32_t wmn = wave_min(a.mn), wmx = wave_max(a.mx);
+    if (lane_id() == 0) {
+        s_dc[wave_id()] = (uint32_t)wc;
+        s_ds[wave_id()] = ws;
+        s_dmn[wave_id()] = wmn;
+        s_dmx[wave_id()] = wmx;
+    }
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        const unsigned i = threadIdx.x;
+        const bool in = i < NT / kWave;
+        const uint32_t tc = (uint32_t)wave_sum(in ? (uint64_t)s_dc[in ? i : 0] : 0ull);
+        const uint64_t ts = wave_sum(in ? s_ds[in ? i : 0] : 0ull);
+        const int32_t tmn = wave_min(in ? s_dmn[in ? i : 0] : INT_MAX);
+        const int32_t tmx = wave_max(in ? s_dmx[in ? i : 0] : INT_MIN);
+        if (i == 0 && tc) {
+            const int sl = dom == kEmptyKey ? -1 : l_slot<C, true>(L.key, dom, probes, skip);
+            if (sl >= 0) {
+                atomicAdd(&L.cnt[sl], tc);
+                atomicAdd(&L.sum[sl], (unsigned long long)ts);
+                atomicMin(&L.mn[sl], tmn);
+                atomicMax(&L.mx[sl], tmx);
+            } else {
+                g_update(gt, gmask, dom, tc, ts, tmn, tmx, overflow);
+            }
+        }
+    }
+}
+
+// Direct hash group-by: every block aggregates a strided share of the input in
+// its LDS table and merges it into the global table at the end.  A block whose
+// 64-tuple sample holds one key at least 4 times (a skewed input) sums that
+// key's tuples in registers instead of queueing its lanes on one LDS address.
 template <int C, bool VEC, int NT>
 __global__ __launch_bounds__(NT) void k_groupby_hash(const int32_t *__restrict__ key, const int32_t *__restrict__ val,
                                                       size_t n, GEntry *gt, uint64_t gmask, int probes,
                                                       uint32_t *overflow) {
     extern __shared__ unsigned char s_raw[];
     LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
+    __shared__ uint32_t s_dom, s_heavy;
+    if (threadIdx.x < kWave) {
+        unsigned bc;
+        const uint32_t d = sample_dominant((uint32_t)key[((size_t)blockIdx.x * kWave + threadIdx.x) % n], &bc);
+        if (threadIdx.x == 0) {
+            s_dom = d;
+            s_heavy = bc >= 4;
+        }
+    }
     ltable_init<C>(L);
     __syncthreads();
-    auto add = [&](int32_t k, int32_t v) { ltable_add<C>(L, k, v, probes, gt, gmask, overflow); };
-    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t from = 0;
-    if (VEC) {
-        const size_t nv = n / 4;
-        const int4 *k4 = reinterpret_cast<const int4 *>(key), *v4 = reinterpret_cast<const int4 *>(val);
-        size_t i = tid;
-        // Four rounds of loads in flight per thread (2^28 tuples, 1024 groups:
-        // 0.454 ms against 0.480 ms with two).
-        constexpr int kRounds = 4;
-        for (; i + (kRounds - 1) * stride < nv; i += kRounds * stride) {
-            int4 kk[kRounds], vv[kRounds];
+    const uint32_t dom = s_dom;
+    auto body = [&](auto heavy_tag) {
+        constexpr bool kHeavy = decltype(heavy_tag)::value;
+        DomAcc acc;
+        auto add = [&](int32_t k, int32_t v) {
+            if (kHeavy && (uint32_t)k == dom) acc.add(v);
+            else ltable_add<C>(L, k, v, probes, gt, gmask, overflow);
+        };
+        const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const size_t stride = (size_t)gridDim.x * blockDim.x;
+        size_t from = 0;
+        if (VEC) {
+            const size_t nv = n / 4;
+            const int4 *k4 = reinterpret_cast<const int4 *>(key), *v4 = reinterpret_cast<const int4 *>(val);
+            size_t i = tid;
+            // Four rounds of loads in flight per thread (2^28 tuples, 1024 groups:
+            // 0.454 ms against 0.480 ms with two).
+            constexpr int kRounds = 4;
+            for (; i + (kRounds - 1) * stride < nv; i += kRounds * stride) {
+                int4 kk[kRounds], vv[kRounds];
 #pragma unroll
-            for (int u = 0; u < kRounds; ++u) {
-                kk[u] = ld_stream(k4 + i + u * stride);
-                vv[u] = ld_stream(v4 + i + u * stride);
-            }
+                for (int u = 0; u < kRounds; ++u) {
+                    kk[u] = ld_stream(k4 + i + u * stride);
+                    vv[u] = ld_stream(v4 + i + u * stride);
+                }
 #pragma unroll
-            for (int u = 0; u < kRounds; ++u) {
-                add(kk[u].x, vv[u].x);
-                add(kk[u].y, vv[u].y);
-                add(kk[u].z, vv[u].z);
-                add(kk[u].w, vv[u].w);
+                for (int u = 0; u < kRounds; ++u) {
+                    add(kk[u].x, vv[u].x);
+                    add(kk[u].y, vv[u].y);
+                    add(kk[u].z, vv[u].z);
+                    add(kk[u].w, vv[u].w);
+                }
             }
+            for (; i < nv; i += stride) {
+                const int4 kk = ld_stream(k4 + i), vv = ld_stream(v4 + i);
+                add(kk.x, vv.x);
+                add(kk.y, vv.y);
+                add(kk.z, vv.z);
+                add(kk.w, vv.w);
+            }
+            from = nv * 4;
         }
-        for (; i < nv; i += stride) {
-            const int4 kk = ld_stream(k4 + i), vv = ld_stream(v4 + i);
-            add(kk.x, vv.x);
-            add(kk.y, vv.y);
-            add(kk.z, vv.z);
-            add(kk.w, vv.w);
-        }
-        from = nv * 4;
-    }
-    for (size_t e = from + tid; e < n; e += stride) add(key[e], val[e]);
+        for (size_t e = from + tid; e < n; e += stride) add(key[e], val[e]);
+        if (kHeavy) dom_merge<C, NT>(L, acc, dom, probes, 0, gt, gmask, overflow);
+    };
+    if (s_heavy) body(std::true_type{});
+    else body(std::false_type{});
     __syncthreads();
     ltable_flush<C>(L, gt, gmask, overflow);
 }
@@ -512,7 +622,7 @@ __global__ void k_chunk_counts(const uint32_t *hist, uint32_t nb, uint32_t *cch)
 constexpr int kChunkThreads = 1024;
 
 template <int C, int NT>
-__global__ __launch_bounds__(NT) void k_groupby_chunks(const uint64_t *__restrict__ fine, const uint32_t *offp,
+__global__ __launch_bounds__(NT, 8) void k_groupby_chunks(const uint64_t *__restrict__ fine, const uint32_t *offp,
                                                         const uint32_t *off, const uint32_t *cpre, uint32_t nb, int L_bits, GEntry *gt,
                                                         uint64_t gmask, int probes, uint32_t *overflow, GEntry *rows,
                                                         unsigned long long *d_rows, size_t rows_cap) {
@@ -534,23 +644,57 @@ __global__ __launch_bounds__(NT) void k_groupby_chunks(const uint64_t *__restric
     const uint64_t e1 = offp[lo] + size;
     const uint32_t m = whole ? (uint32_t)size : (uint32_t)(e1 - e0 < kGbChunk ? e1 - e0 : kGbChunk);
     const uint64_t *src = fine + e0;
+    // A bucket well above the mean size holds a heavy key.  Its block takes the
+    // most common key of its first 64 tuples as the dominant key K: tuples of
+    // K are summed in registers and merged once at the end, the others go to
+    // the LDS table.  (2^28 tuples, 2^20 groups, Zipf z = 1: the key of rank 1
+    // alone is 7 % of all tuples.)
+    const bool heavy = !whole || size * 4 * nb > (uint64_t)off[nb] * 5;
+    __shared__ uint32_t s_dom;
+    if (heavy && threadIdx.x < kWave) {
+        unsigned bc;
+        const uint32_t d = sample_dominant(threadIdx.x < m ? (uint32_t)src[threadIdx.x] : kEmptyKey, &bc);
+        if (threadIdx.x == 0) s_dom = d;
+    }
     ltable_init<C>(L);
     __syncthreads();
     // kChunkU tuples per thread and round, all loaded before any is added, so
     // a thread keeps that many loads in flight.
     constexpr uint32_t kRound = NT * kChunkU;
-    for (uint32_t r = 0; r < m; r += kRound) {
-        uint64_t v[kChunkU];
+    if (!heavy) {
+        for (uint32_t r = 0; r < m; r += kRound) {
+            uint64_t v[kChunkU];
 #pragma unroll
-        for (int u = 0; u < (int)kChunkU; ++u) {
-            const uint32_t i = r + u * NT + threadIdx.x;
-            v[u] = i < m ? ld_stream(src + i) : 0ull;
+            for (int u = 0; u < (int)kChunkU; ++u) {
+                const uint32_t i = r + u * NT + threadIdx.x;
+                v[u] = i < m ? ld_stream(src + i) : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < (int)kChunkU; ++u)
+                if (r + u * NT + threadIdx.x < m)
+                    ltable_add<C, false, true>(L, (int32_t)(uint32_t)v[u], (int32_t)(v[u] >> 32), probes, gt, gmask,
+                                               overflow, L_bits);
         }
+    } else {
+        const uint32_t dom = s_dom;
+        DomAcc acc;
+        for (uint32_t r = 0; r < m; r += kRound) {
+            uint64_t v[kChunkU];
 #pragma unroll
-        for (int u = 0; u < (int)kChunkU; ++u)
-            if (r + u * NT + threadIdx.x < m)
-                ltable_add<C, true>(L, (int32_t)(uint32_t)v[u], (int32_t)(v[u] >> 32), probes, gt, gmask, overflow,
-                                    L_bits);
+            for (int u = 0; u < (int)kChunkU; ++u) {
+                const uint32_t i = r + u * NT + threadIdx.x;
+                v[u] = i < m ? ld_stream(src + i) : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < (int)kChunkU; ++u) {
+                if (r + u * NT + threadIdx.x >= m) continue;
+                const uint32_t k = (uint32_t)v[u];
+                const int32_t x = (int32_t)(v[u] >> 32);
+                if (k == dom) acc.add(x);
+                else ltable_add<C, false, true>(L, (int32_t)k, x, probes, gt, gmask, overflow, L_bits);
+            }
+        }
+        dom_merge<C, NT>(L, acc, dom, probes, L_bits, gt, gmask, overflow);
     }
     __syncthreads();
     if (whole) ltable_append<C, NT>(L, rows, d_rows, rows_cap, s_w, &s_base);

@@ -147,6 +147,24 @@ def test_groupby_modes_heavy_hitter(qo, orc, gb_mode):
         assert np.array_equal(g, e)
 
 
+def test_groupby_modes_heavy_sentinel_keys(qo, orc, gb_mode):
+    """Heavy keys that are the empty-slot pattern (-1) and INT_MIN, plus a third
+    heavy key: a bucket's dominant key is summed in registers and merged once,
+    also when it is the key that never enters an LDS table."""
+    rng = np.random.default_rng(16)
+    n = 2_000_000
+    keys = rng.integers(-50_000, 50_000, n, dtype=np.int64).astype(np.int32)
+    u = rng.random(n)
+    keys[u < 0.4] = -1
+    keys[(u >= 0.4) & (u < 0.5)] = INT_MIN
+    keys[(u >= 0.5) & (u < 0.6)] = 7
+    vals = rng.integers(INT_MIN, INT_MAX, n, dtype=np.int64).astype(np.int32)
+    got = qo.operators.groupby_agg(keys, vals, 100_003)
+    exp = orc.groupby_agg(keys, vals)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
 def test_groupby_modes_capacity_overflow_reports(qo, gb_mode):
     keys = np.arange(20_000, dtype=np.int32)
     with pytest.raises(qo._lib.QoError):
