@@ -236,21 +236,33 @@ QO_DEV void ltable_append(const LTable<C> &L, GEntry *rows, unsigned long long *
     }
 }
 
-// Dominant key of a block: the most common key among 64 sampled tuples (one
-// wave; kEmptyKey for lanes without a tuple), and how often it occurs there.
-QO_DEV uint32_t sample_dominant(uint32_t k, unsigned *count) {
-    uint32_t best = k;
-    unsigned bc = 0;
+// Dominant keys of a block: the most common key among 64 sampled tuples (one
+// wave; kEmptyKey for lanes without a tuple) and how often it occurs there,
+// and the most common other key and its count.
+QO_DEV uint32_t sample_dominant(uint32_t k, unsigned *count, uint32_t *second = nullptr, unsigned *count2 = nullptr) {
+    uint32_t best = k, best2 = k;
+    unsigned bc = 0, bc2 = 0;
 #pragma unroll
-    for (int c = 0; c < kWave; c += 8) {
+    for (int c = 0; c < kWave; c += 4) {
         const uint32_t cand = (uint32_t)__shfl((int)k, c, kWave);
         const unsigned n = popc(ballot(k == cand));
         if (n > bc) {
+            if (cand != best) {
+                bc2 = bc;
+                best2 = best;
+            }
             bc = n;
             best = cand;
+        } else if (n > bc2 && cand != best) {
+            bc2 = n;
+            best2 = cand;
         }
     }
     *count = bc;
+    if (second) {
+        *second = best2;
+        *count2 = best2 == best ? 0u : bc2;
+    }
     return best;
 }
 
@@ -308,31 +320,36 @@ QO_DEV void dom_merge(LTable<C> &L, const DomAcc &a, uint32_t dom, int probes, i
 
 // Direct hash group-by: every block aggregates a strided share of the input in
 // its LDS table and merges it into the global table at the end.  A block whose
-// 64-tuple sample holds one key at least 4 times (a skewed input) sums that
-// key's tuples in registers instead of queueing its lanes on one LDS address.
+// 64-tuple sample holds a key at least 4 times (a skewed input) sums that
+// key's tuples, and a second such key's, in registers instead of queueing its
+// lanes on one LDS address.
 template <int C, bool VEC, int NT>
 __global__ __launch_bounds__(NT) void k_groupby_hash(const int32_t *__restrict__ key, const int32_t *__restrict__ val,
                                                       size_t n, GEntry *gt, uint64_t gmask, int probes,
                                                       uint32_t *overflow) {
     extern __shared__ unsigned char s_raw[];
     LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
-    __shared__ uint32_t s_dom, s_heavy;
+    __shared__ uint32_t s_dom, s_dom2, s_heavy;
     if (threadIdx.x < kWave) {
-        unsigned bc;
-        const uint32_t d = sample_dominant((uint32_t)key[((size_t)blockIdx.x * kWave + threadIdx.x) % n], &bc);
+        unsigned bc, bc2;
+        uint32_t d2;
+        const uint32_t d =
+            sample_dominant((uint32_t)key[((size_t)blockIdx.x * kWave + threadIdx.x) % n], &bc, &d2, &bc2);
         if (threadIdx.x == 0) {
             s_dom = d;
+            s_dom2 = bc2 >= 4 ? d2 : d;
             s_heavy = bc >= 4;
         }
     }
     ltable_init<C>(L);
     __syncthreads();
-    const uint32_t dom = s_dom;
+    const uint32_t dom = s_dom, dom2 = s_dom2;
     auto body = [&](auto heavy_tag) {
         constexpr bool kHeavy = decltype(heavy_tag)::value;
-        DomAcc acc;
+        DomAcc acc, acc2;
         auto add = [&](int32_t k, int32_t v) {
             if (kHeavy && (uint32_t)k == dom) acc.add(v);
+            else if (kHeavy && (uint32_t)k == dom2) acc2.add(v);
             else ltable_add<C>(L, k, v, probes, gt, gmask, overflow);
         };
         const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,7 +387,13 @@ __global__ __launch_bounds__(NT) void k_groupby_hash(const int32_t *__restrict__
             from = nv * 4;
         }
         for (size_t e = from + tid; e < n; e += stride) add(key[e], val[e]);
-        if (kHeavy) dom_merge<C, NT>(L, acc, dom, probes, 0, gt, gmask, overflow);
+        if (kHeavy) {
+            dom_merge<C, NT>(L, acc, dom, probes, 0, gt, gmask, overflow);
+            if (dom2 != dom) {  // block-uniform
+                __syncthreads();  // dom_merge's staging is reused
+                dom_merge<C, NT>(L, acc2, dom2, probes, 0, gt, gmask, overflow);
+            }
+        }
     };
     if (s_heavy) body(std::true_type{});
     else body(std::false_type{});
@@ -644,20 +667,28 @@ __global__ __launch_bounds__(NT, 8) void k_groupby_chunks(const uint64_t *__rest
     const uint64_t e1 = offp[lo] + size;
     const uint32_t m = whole ? (uint32_t)size : (uint32_t)(e1 - e0 < kGbChunk ? e1 - e0 : kGbChunk);
     const uint64_t *src = fine + e0;
-    // A bucket well above the mean size holds a heavy key.  Its block takes the
-    // most common key of its first 64 tuples as the dominant key K: tuples of
-    // K are summed in registers and merged once at the end, the others go to
-    // the LDS table.  (2^28 tuples, 2^20 groups, Zipf z = 1: the key of rank 1
-    // alone is 7 % of all tuples.)
-    const bool heavy = !whole || size * 4 * nb > (uint64_t)off[nb] * 5;
-    __shared__ uint32_t s_dom;
-    if (heavy && threadIdx.x < kWave) {
-        unsigned bc;
-        const uint32_t d = sample_dominant(threadIdx.x < m ? (uint32_t)src[threadIdx.x] : kEmptyKey, &bc);
-        if (threadIdx.x == 0) s_dom = d;
+    // The block takes the most common key of its first 64 tuples as the
+    // dominant key K.  When K is at least 4 of them (a skewed bucket, such as
+    // a heavy key's), tuples of K are summed in registers and merged once at
+    // the end and only the others go to the LDS table.  (2^28 tuples, 2^20
+    // groups, Zipf z = 1: the key of rank 1 alone is 7 % of all tuples, and
+    // every key of rank up to ~1000 is a sizeable share of its bucket.)
+    // A second key that is also at least 4 of the sample gets registers of its
+    // own.
+    __shared__ uint32_t s_dom, s_dom2, s_heavy;
+    if (threadIdx.x < kWave) {
+        unsigned bc, bc2;
+        uint32_t d2;
+        const uint32_t d = sample_dominant(threadIdx.x < m ? (uint32_t)src[threadIdx.x] : kEmptyKey, &bc, &d2, &bc2);
+        if (threadIdx.x == 0) {
+            s_dom = d;
+            s_dom2 = bc2 >= 4 ? d2 : d;
+            s_heavy = bc >= 4;
+        }
     }
     ltable_init<C>(L);
     __syncthreads();
+    const bool heavy = s_heavy;
     // kChunkU tuples per thread and round, all loaded before any is added, so
     // a thread keeps that many loads in flight.
     constexpr uint32_t kRound = NT * kChunkU;
@@ -676,8 +707,8 @@ __global__ __launch_bounds__(NT, 8) void k_groupby_chunks(const uint64_t *__rest
                                                overflow, L_bits);
         }
     } else {
-        const uint32_t dom = s_dom;
-        DomAcc acc;
+        const uint32_t dom = s_dom, dom2 = s_dom2;
+        DomAcc acc, acc2;
         for (uint32_t r = 0; r < m; r += kRound) {
             uint64_t v[kChunkU];
 #pragma unroll
@@ -691,10 +722,15 @@ __global__ __launch_bounds__(NT, 8) void k_groupby_chunks(const uint64_t *__rest
                 const uint32_t k = (uint32_t)v[u];
                 const int32_t x = (int32_t)(v[u] >> 32);
                 if (k == dom) acc.add(x);
+                else if (k == dom2) acc2.add(x);
                 else ltable_add<C, false, true>(L, (int32_t)k, x, probes, gt, gmask, overflow, L_bits);
             }
         }
         dom_merge<C, NT>(L, acc, dom, probes, L_bits, gt, gmask, overflow);
+        if (dom2 != dom) {  // block-uniform
+            __syncthreads();  // dom_merge's staging is reused
+            dom_merge<C, NT>(L, acc2, dom2, probes, L_bits, gt, gmask, overflow);
+        }
     }
     __syncthreads();
     if (whole) ltable_append<C, NT>(L, rows, d_rows, rows_cap, s_w, &s_base);
