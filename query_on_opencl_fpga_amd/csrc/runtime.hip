@@ -138,19 +138,26 @@ __device__ __forceinline__ uint4 copy_ld(const uint4 *p) {
     }
 }
 
+// Device copy in 32 KiB block-contiguous chunks (each block copies chunks
+// b, b + grid, ...; 8 x 16 bytes in flight per thread).  A grid-strided copy
+// whose wave loads sit a grid apart moves 4.4-4.9 TB/s of read + write traffic
+// on a 4 GiB copy; the chunked one 5.2 TB/s.
+constexpr int kCopyU = 8;
+constexpr size_t kCopyChunk = 256 * kCopyU;  // uint4 per chunk
+
 template <bool NT>
 __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i + 3 * stride < n16; i += 4 * stride) {
-        uint4 a = copy_ld<NT>(src + i), b = copy_ld<NT>(src + i + stride), c = copy_ld<NT>(src + i + 2 * stride),
-              d = copy_ld<NT>(src + i + 3 * stride);
-        dst[i] = a;
-        dst[i + stride] = b;
-        dst[i + 2 * stride] = c;
-        dst[i + 3 * stride] = d;
+    const size_t nch = n16 / kCopyChunk;
+    for (size_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const size_t base = c * kCopyChunk + threadIdx.x;
+        uint4 a[kCopyU];
+#pragma unroll
+        for (int u = 0; u < kCopyU; ++u) a[u] = copy_ld<NT>(src + base + u * 256);
+#pragma unroll
+        for (int u = 0; u < kCopyU; ++u) dst[base + u * 256] = a[u];
     }
-    for (; i < n16; i += stride) dst[i] = src[i];
+    if (blockIdx.x == 0)
+        for (size_t i = nch * kCopyChunk + threadIdx.x; i < n16; i += 256) dst[i] = src[i];
 }
 
 int launch_fill_u64(void *dst, uint64_t value, size_t count, hipStream_t st) {
@@ -273,7 +280,7 @@ static int copy_impl(void *dst, const void *src, size_t bytes, bool nt, qo_strea
         QO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
         return QO_OK;
     }
-    unsigned g = grid_for(bytes / 16, 256 * 4, (unsigned)g_props.cu_count * 8);
+    unsigned g = grid_for(bytes / 16, (unsigned)kCopyChunk, (unsigned)g_props.cu_count * 32);
     if (nt) k_copy16<true><<<g, 256, 0, st>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), bytes / 16);
     else k_copy16<false><<<g, 256, 0, st>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), bytes / 16);
     QO_LAUNCHED();
