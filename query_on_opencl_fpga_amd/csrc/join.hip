@@ -446,10 +446,11 @@ int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, cons
                                          radix_workspace_bytes(out_capacity, true)});
     const size_t rows = out_capacity ? out_capacity : 1;
     int status = QO_OK;
-    void *ws = workspace(st, stage_bytes + rows * 12 + 1024, &status);
+    void *ws = workspace(st, stage_bytes + rows * 12 + 2048, &status);
     if (!ws) return status;
     Carver cv(ws);
     uint64_t *d_count = cv.take<uint64_t>(1);
+    uint32_t *d_flag = cv.take<uint32_t>(2);
     uint64_t *k64 = cv.take<uint64_t>(rows);
     uint32_t *v32 = cv.take<uint32_t>(rows);
     void *sws = cv.take<char>(stage_bytes);
@@ -476,7 +477,7 @@ int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, cons
         const uint32_t d = ((uint32_t)range[0] ^ 0x80000000u) ^ ((uint32_t)range[1] ^ 0x80000000u);
         hi_bit = 32 + (d ? 32 - __builtin_clz(d) : 0);
     }
-    QO_TRY(sort_packed_join_rows(k64, v32, m, sws, okey, orpay, ospay, st, hi_bit));
+    QO_TRY(sort_packed_join_rows(k64, v32, m, sws, okey, orpay, ospay, st, hi_bit, d_flag));
     QO_HIP(hipStreamSynchronize(st));
     return QO_OK;
 }

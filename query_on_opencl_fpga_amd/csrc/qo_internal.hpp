@@ -127,9 +127,11 @@ int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey,
 // Sorts m sort-ready join rows (k64 = join_sort_key(key, s_payload), v32 =
 // r_payload) with workspace rws (radix_workspace_bytes(m, true)) and writes
 // them out as three int32 columns.  Only bits [0, hi_bit) of k64 are sorted:
-// the caller passes 64 unless every key shares the bits above hi_bit.
+// the caller passes 64 unless every key shares the bits above hi_bit.  With
+// `flag` (one device word) the key bits are sorted first and runs of equal key
+// by s_payload after, which synchronises the stream once to read the flag.
 int sort_packed_join_rows(uint64_t *k64, uint32_t *v32, size_t m, void *rws, int32_t *key, int32_t *rpay,
-                          int32_t *spay, hipStream_t st, int hi_bit = 64);
+                          int32_t *spay, hipStream_t st, int hi_bit = 64, uint32_t *flag = nullptr);
 // Output columns of a probe: three int32 columns, or (k64, v32 set) the
 // sort-ready (join_sort_key(key, s_payload), r_payload) rows.
 struct JoinOut {

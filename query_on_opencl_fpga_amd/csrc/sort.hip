@@ -629,24 +629,153 @@ int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void
     uint32_t *v32 = cv.take<uint32_t>(m);
     void *rws = cv.take<char>(radix_workspace_bytes(m, true));
     unsigned g = grid_for(m, 256, (unsigned)props().cu_count * 8);
+    uint32_t *flag = cv.take<uint32_t>(1);
     k_pack_join<<<g, 256, 0, st>>>(key, spay, rpay, m, k64, v32);
     QO_LAUNCHED();
-    return sort_packed_join_rows(k64, v32, m, rws, key, rpay, spay, st);
+    return sort_packed_join_rows(k64, v32, m, rws, key, rpay, spay, st, 64, flag);
 }
 
-size_t sort_join_rows_ws_bytes(size_t m) { return m * 12 + 1024 + radix_workspace_bytes(m, true); }
+size_t sort_join_rows_ws_bytes(size_t m) { return m * 12 + 2048 + radix_workspace_bytes(m, true); }
+
+// Runs of equal key after a stable sort of the packed join rows on their key
+// bits (the high 32): each block owns kFixTile positions and loads kFixRun more,
+// so that every run that starts in its range and has at most kFixRun rows lies
+// in LDS whole.  One ballot per 64-position chunk marks where runs start; a
+// row finds its run's bounds from its own chunk's marks and its neighbours',
+// counts the run's rows that order before it (smaller s_payload, or equal and
+// earlier: stable) and writes itself to the output columns at run start + that
+// count.  Rows of runs that start outside the block's range are left to the
+// block that owns the start; a run longer than kFixRun sets *long_flag and is
+// not written.
+constexpr uint32_t kFixTile = 4096;
+constexpr uint32_t kFixRun = 64;
+constexpr uint32_t kFixWin = kFixTile + kFixRun;
+constexpr uint32_t kFixThreads = 1024;  // two blocks per CU, 8 waves per SIMD
+constexpr uint32_t kFixRounds = (kFixWin + kFixThreads - 1) / kFixThreads;
+constexpr uint32_t kFixChunks = kFixWin / kWave + 2;  // marks of the window's chunks and one past it
+
+__global__ __launch_bounds__(kFixThreads) void k_fix_runs(const uint64_t *__restrict__ k, const uint32_t *__restrict__ v,
+                                                  size_t m, RowOut out, uint32_t *long_flag) {
+    static_assert(kFixWin % kWave == 0, "window of whole chunks");
+    __shared__ uint32_t s_hi[kFixWin];
+    __shared__ uint32_t s_lo[kFixWin + 4];  // 4 words of slack for the unrolled rank loop
+    __shared__ uint64_t s_mark[kFixChunks];
+    const size_t t0 = (size_t)blockIdx.x * kFixTile;
+    const uint32_t own = (uint32_t)(m - t0 < kFixTile ? m - t0 : kFixTile);
+    const uint32_t len = (uint32_t)(m - t0 < kFixWin ? m - t0 : kFixWin);
+    const bool data_end = t0 + len == m;  // the window reaches the last row
+#pragma unroll
+    for (uint32_t u = 0; u < kFixRounds; ++u) {
+        const uint32_t i = u * kFixThreads + threadIdx.x;
+        if (i < len) {
+            const uint64_t x = k[t0 + i];
+            s_hi[i] = (uint32_t)(x >> 32);
+            s_lo[i] = (uint32_t)x;
+        }
+    }
+    const uint32_t prev_key = t0 > 0 ? (uint32_t)(k[t0 - 1] >> 32) : 0u;
+    __syncthreads();
+    // Run-start marks: position p starts a run if its key differs from the row
+    // before it (the first row of all rows starts one); past the last row every
+    // position counts as a start, past a cut-off window none does.
+    const unsigned lane = lane_id();
+    for (uint32_t c = wave_id(); c < kFixChunks; c += kFixThreads / kWave) {
+        const uint32_t p = c * kWave + lane;
+        bool start;
+        if (p < len) start = p == 0 ? (t0 == 0 || prev_key != s_hi[0]) : s_hi[p] != s_hi[p - 1];
+        else start = data_end;
+        const uint64_t mk = ballot(start);
+        if (lane == 0) s_mark[c] = mk;
+    }
+    __syncthreads();
+    bool long_run = false;
+    for (uint32_t i = threadIdx.x; i < len; i += kFixThreads) {
+        const uint32_t c = i / kWave;  // uniform across the wave
+        const uint64_t at_or_below = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1;
+        const uint64_t mc = s_mark[c];
+        uint32_t s;
+        if (mc & at_or_below) {
+            s = c * kWave + 63u - (uint32_t)__builtin_clzll(mc & at_or_below);
+        } else if (c == 0) {
+            continue;  // the run began before the block's range
+        } else if (s_mark[c - 1]) {
+            s = (c - 1) * kWave + 63u - (uint32_t)__builtin_clzll(s_mark[c - 1]);
+        } else {
+            long_run = true;  // no start within 64 positions back
+            continue;
+        }
+        if (s >= own) continue;  // the next block's run
+        const uint64_t above = mc & ~at_or_below;
+        uint32_t e;
+        if (above) {
+            e = c * kWave + (uint32_t)__builtin_ctzll(above);
+        } else if (s_mark[c + 1]) {
+            e = (c + 1) * kWave + (uint32_t)__builtin_ctzll(s_mark[c + 1]);
+        } else {
+            long_run = true;
+            continue;
+        }
+        if (e - s > kFixRun) {
+            long_run = true;
+            continue;
+        }
+        const uint32_t xl = s_lo[i];
+        uint32_t rank = 0;
+        for (uint32_t j = s; j < e; j += 4) {
+            const uint32_t y0 = s_lo[j], y1 = s_lo[j + 1], y2 = s_lo[j + 2], y3 = s_lo[j + 3];
+            rank += (y0 < xl || (y0 == xl && j < i)) ? 1u : 0u;
+            rank += (j + 1 < e && (y1 < xl || (y1 == xl && j + 1 < i))) ? 1u : 0u;
+            rank += (j + 2 < e && (y2 < xl || (y2 == xl && j + 2 < i))) ? 1u : 0u;
+            rank += (j + 3 < e && (y3 < xl || (y3 == xl && j + 3 < i))) ? 1u : 0u;
+        }
+        const size_t dst = t0 + s + rank;
+        const uint64_t x = ((uint64_t)s_hi[i] << 32) | xl;
+        out.key[dst] = join_sort_key_key(x);
+        out.spay[dst] = join_sort_key_spay(x);
+        out.rpay[dst] = (int32_t)v[t0 + i];
+    }
+    if (long_run) atomicOr(long_flag, 1u);
+}
 
 int sort_packed_join_rows(uint64_t *k64, uint32_t *v32, size_t m, void *rws, int32_t *key, int32_t *rpay,
-                          int32_t *spay, hipStream_t st, int hi_bit) {
+                          int32_t *spay, hipStream_t st, int hi_bit, uint32_t *flag) {
     if (m == 0) return QO_OK;
     if (m == 1) {
         k_unpack_join<<<1, 256, 0, st>>>(k64, v32, m, key, spay, rpay);
         QO_LAUNCHED();
         return QO_OK;
     }
+    hi_bit = std::max(1, std::min(64, hi_bit));
     // The last radix pass writes the output columns (no separate unpack pass).
     const RowOut out{key, rpay, spay};
-    return radix_impl(k64, v32, m, nullptr, 0, std::max(1, std::min(64, hi_bit)), rws, st, &out);
+    if (!flag || hi_bit <= 32) return radix_impl(k64, v32, m, nullptr, 0, hi_bit, rws, st, &out);
+    // Radix passes over the key bits only (stable: a key's rows keep the probe's
+    // order), then one pass that sorts every run of equal key by s_payload and
+    // writes the output columns: for 2^24 build keys 3 passes + 1 instead of 7.
+    QO_REQUIRE(m < 0xFFFFFFFFull, "sort input too large");
+    Carver cv(rws);
+    uint64_t *kalt = nullptr;
+    uint32_t *valt = nullptr;
+    RowsBufs rb{};
+    radix_carve(m, true, cv, &kalt, &valt, &rb);
+    const RadixSeg seg{0, m, nullptr, 0, radix_tiles(m)};
+    bool in_alt = false;
+    QO_TRY(launch_radix_rows(k64, v32, kalt, valt, seg, 32, hi_bit, rb, st, &in_alt, nullptr));
+    QO_TRY(launch_fill_u32(flag, 0, 1, st));
+    const size_t blocks = (m + kFixTile - 1) / kFixTile;
+    k_fix_runs<<<(unsigned)blocks, kFixThreads, 0, st>>>(in_alt ? kalt : k64, in_alt ? valt : v32, m, out, flag);
+    QO_LAUNCHED();
+    uint32_t long_run = 0;
+    QO_HIP(hipMemcpyAsync(&long_run, flag, 4, hipMemcpyDeviceToHost, st));
+    QO_HIP(hipStreamSynchronize(st));
+    if (!long_run) return QO_OK;
+    // Some key has more than kFixRun rows: sort all rows on the full width
+    // (the result does not depend on the rows' current order).
+    if (in_alt) {
+        QO_HIP(hipMemcpyAsync(k64, kalt, m * 8, hipMemcpyDeviceToDevice, st));
+        QO_HIP(hipMemcpyAsync(v32, valt, m * 4, hipMemcpyDeviceToDevice, st));
+    }
+    return radix_impl(k64, v32, m, nullptr, 0, hi_bit, rws, st, &out);
 }
 
 }  // namespace qo

@@ -130,6 +130,50 @@ def test_hj_filter_sort_key_ranges(qo, orc, base, nr):
         assert np.array_equal(g, e)
 
 
+@pytest.mark.parametrize("longest", [64, 65])
+def test_hj_filter_sort_run_lengths(qo, orc, longest):
+    """Rows are sorted on the key bits first and every run of equal key by
+    s_payload after, in tiles of 4096 positions; runs of up to 64 rows are
+    sorted there, a longer one sends the whole output through the full-width
+    sort.  Run lengths 1..longest (many runs crossing tile edges) and repeated
+    s_payload values within runs."""
+    rng = np.random.default_rng(longest)
+    nr = 20_000
+    rk = (np.int64(-7_000) + rng.permutation(nr)).astype(np.int32)
+    rp = rng.integers(INT_MIN, INT_MAX, nr, dtype=np.int64).astype(np.int32)
+    counts = rng.integers(0, 9, nr)
+    counts[rng.choice(nr, 300, replace=False)] = longest
+    counts[rng.choice(nr, 300, replace=False)] = longest - 1
+    sk = np.repeat(rk, counts)
+    sk = sk[rng.permutation(len(sk))]
+    ns = len(sk)
+    sp = rng.integers(-40, 40, ns).astype(np.int32)
+    sf = np.zeros(ns, np.int32)
+    got = qo.operators.hj_filter_sort(rk, rp, sk, sp, sf, INT_MIN, INT_MAX)
+    exp = orc.sort_join_rows(*orc.hash_join(rk, rp, sk, sp, sf, INT_MIN, INT_MAX))
+    assert len(got[0]) == ns
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_hj_filter_sort_heavy_keys(qo, orc):
+    """A third of the probe rows carry one key and the rest few keys: runs of
+    ~10^5 rows take the full-width sort."""
+    rng = np.random.default_rng(5)
+    nr = 1000
+    rk = rng.permutation(nr).astype(np.int32) * 3
+    rp = rng.integers(INT_MIN, INT_MAX, nr, dtype=np.int64).astype(np.int32)
+    ns = 600_000
+    sk = rk[rng.integers(0, 40, ns)]
+    sk[rng.random(ns) < 0.33] = rk[7]
+    sp = rng.integers(INT_MIN, INT_MAX, ns, dtype=np.int64).astype(np.int32)
+    sf = rng.integers(0, 1000, ns).astype(np.int32)
+    got = qo.operators.hj_filter_sort(rk, rp, sk, sp, sf, INT_MIN, 499)
+    exp = orc.sort_join_rows(*orc.hash_join(rk, rp, sk, sp, sf, INT_MIN, 499))
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
 # ---------------------------------------------------------------------------
 # Table layouts and probe strategies:
 #  * direct: hash-mixed table, direct build and probe;
