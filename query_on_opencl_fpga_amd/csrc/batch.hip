@@ -240,11 +240,22 @@ int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
         QO_LAUNCHED();
     }
     if (p.mt) {
+        // Radix passes over the key bits only, then one pass that orders every
+        // run of equal key by s_payload and writes the query's output columns.
+        // Should some key have more than 64 rows, the full-width sort (its
+        // kernels gated on that flag, so they return at once otherwise)
+        // rewrites the outputs.
+        uint32_t *long_flag = p.small + 2;
         bool in_alt = false;
-        // The sort's last pass writes every query's output columns.
         QO_TRY(launch_radix_rows_segments(p.mkeys, p.mvals, p.kalt, p.valt, p.d_segs, p.nq, p.mt, p.d_chunks,
-                                           (uint32_t)p.hchunks.size(), p.d_chunk0, 0, 64, p.sort_ws, st, &in_alt,
-                                           p.d_outs));
+                                           (uint32_t)p.hchunks.size(), p.d_chunk0, 32, 64, p.sort_ws, st, &in_alt));
+        uint64_t *ks = in_alt ? p.kalt : p.mkeys, *ka = in_alt ? p.mkeys : p.kalt;
+        uint32_t *vs = in_alt ? p.valt : p.mvals, *va = in_alt ? p.mvals : p.valt;
+        QO_TRY(launch_fix_runs_segments(ks, vs, p.d_segs, p.nq, p.mt, p.d_outs, long_flag, st));
+        bool alt2 = false;
+        QO_TRY(launch_radix_rows_segments(ks, vs, ka, va, p.d_segs, p.nq, p.mt, p.d_chunks,
+                                           (uint32_t)p.hchunks.size(), p.d_chunk0, 0, 64, p.sort_ws, st, &alt2,
+                                           p.d_outs, long_flag));
     }
     return QO_OK;
 }

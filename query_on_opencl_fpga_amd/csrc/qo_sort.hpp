@@ -41,11 +41,20 @@ size_t radix_rows_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks);
 // *in_alt tells whether the data ended in kalt / valt.
 // With d_outs (one RowOut per segment, device memory) the keys are join
 // sort keys with r_payload values and the last pass writes each segment's
-// rows unpacked into its RowOut; the packed result is then not kept.
+// rows unpacked into its RowOut; the packed result is then not kept.  With
+// run_if every kernel returns at once unless *run_if != 0 (decided on the
+// device, for stream captures).
 int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
                                const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
                                uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
-                               hipStream_t st, bool *in_alt, const RowOut *d_outs = nullptr);
+                               hipStream_t st, bool *in_alt, const RowOut *d_outs = nullptr,
+                               const uint32_t *run_if = nullptr);
+// After a segmented sort of join rows on their key bits only [32, 64): sorts
+// every run of equal key by s_payload and writes each segment's rows to its
+// RowOut (segment tiles of kSortTile rows); a run longer than 64 rows sets
+// *long_flag and is not written.
+int launch_fix_runs_segments(const uint64_t *keys, const uint32_t *vals, const RadixSeg *d_segs, int nseg,
+                             uint32_t total_tiles, const RowOut *d_outs, uint32_t *long_flag, hipStream_t st);
 // Join rows sort ascending by (key, s_payload) as signed int32: flip the sign
 // bits so unsigned 64-bit order is the signed lexicographic order.
 __host__ __device__ __forceinline__ uint64_t join_sort_key(int32_t key, int32_t spay) {

@@ -95,7 +95,8 @@ QO_DEV void hist_tile(const uint64_t *__restrict__ src, uint32_t rem, int shift,
 // segment as seg0 with segs = null; a batch passes the segment list.
 __global__ __launch_bounds__(256) void k_tile_hist(const uint64_t *__restrict__ keys, RadixSeg seg0,
                                                    const RadixSeg *segs, int nseg, uint32_t ntile, int shift,
-                                                   uint32_t *__restrict__ rows) {
+                                                   uint32_t *__restrict__ rows, const uint32_t *run_if = nullptr) {
+    if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     __shared__ uint32_t s_h[kHistTiles][256];
     for (int i = threadIdx.x; i < kHistTiles * 256; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
@@ -226,7 +227,9 @@ struct SegWalk {
 // starts inside the block.
 __global__ __launch_bounds__(256) void k_rows_reduce_seg(const uint32_t *__restrict__ rows, uint32_t ntile,
                                                          uint32_t span, const RadixSeg *segs, int nseg,
-                                                         uint32_t *__restrict__ bsum, uint32_t *__restrict__ bflag) {
+                                                         uint32_t *__restrict__ bsum, uint32_t *__restrict__ bflag,
+                                                         const uint32_t *run_if = nullptr) {
+    if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     const uint32_t t0 = blockIdx.x * span, t1 = std::min(ntile, t0 + span);
     const uint32_t *col = rows + threadIdx.x;
     SegWalk wk(segs, nseg, t0);
@@ -246,7 +249,9 @@ __global__ __launch_bounds__(256) void k_rows_reduce_seg(const uint32_t *__restr
 // back to the start of b's first segment (exclusive carry, in place).  Thread
 // (q, d) walks quarter q of the blocks; the quarters' (flag, sum) pairs combine
 // in order.
-__global__ __launch_bounds__(1024) void k_rows_starts_seg(uint32_t *bsum, const uint32_t *bflag, uint32_t nb) {
+__global__ __launch_bounds__(1024) void k_rows_starts_seg(uint32_t *bsum, const uint32_t *bflag, uint32_t nb,
+                                                          const uint32_t *run_if = nullptr) {
+    if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     __shared__ uint32_t s_part[4][256];
     __shared__ uint32_t s_flag[4];
     const uint32_t d = threadIdx.x & 255u, q = threadIdx.x >> 8;
@@ -277,7 +282,9 @@ __global__ __launch_bounds__(1024) void k_rows_starts_seg(uint32_t *bsum, const 
 // rows[t][d] := digit-d keys of the tiles of t's segment before t (in place).
 __global__ __launch_bounds__(256) void k_rows_apply_seg(uint32_t *__restrict__ rows, uint32_t ntile, uint32_t span,
                                                         const RadixSeg *segs, int nseg,
-                                                        const uint32_t *__restrict__ bsum) {
+                                                        const uint32_t *__restrict__ bsum,
+                                                        const uint32_t *run_if = nullptr) {
+    if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     const uint32_t t0 = blockIdx.x * span, t1 = std::min(ntile, t0 + span);
     uint32_t *col = rows + threadIdx.x;
     SegWalk wk(segs, nseg, t0);
@@ -299,7 +306,9 @@ __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict_
                                                      RadixSeg seg0, const RadixSeg *segs, int nseg, uint32_t ntile,
                                                      int shift, const uint32_t *__restrict__ offs,
                                                      const uint32_t *__restrict__ dstart, uint32_t sstride,
-                                                     RowOut out0 = RowOut{}, const RowOut *outs = nullptr) {
+                                                     RowOut out0 = RowOut{}, const RowOut *outs = nullptr,
+                                                     const uint32_t *run_if = nullptr) {
+    if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     __shared__ uint32_t s_cnt[4][256];
     __shared__ uint32_t s_dpre[256];
     __shared__ uint32_t s_gofs[256];
@@ -453,7 +462,8 @@ constexpr int kSweepMaxPass = 8;
 // chunk c (segment-local tile numbers), one row of npass*256 per chunk.
 __global__ __launch_bounds__(256) void k_sweep_counts_seg(const uint64_t *__restrict__ keys, const RadixSeg *segs,
                                                           const SweepChunk *chunks, int lo_bit, int npass,
-                                                          uint32_t *partial) {
+                                                          uint32_t *partial, const uint32_t *run_if = nullptr) {
+    if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     __shared__ uint32_t s_h[kSweepMaxPass * 256];
     for (int i = threadIdx.x; i < npass * 256; i += 256) s_h[i] = 0;
     __syncthreads();
@@ -472,7 +482,8 @@ __global__ __launch_bounds__(256) void k_sweep_counts_seg(const uint64_t *__rest
 // Per segment and pass: exclusive digit starts from the segment's chunk rows
 // [chunk0[s], chunk0[s + 1]).  Block b = s * npass + p.
 __global__ __launch_bounds__(256) void k_sweep_starts_seg(const uint32_t *partial, const uint32_t *chunk0, int npass,
-                                                          uint32_t *starts) {
+                                                          uint32_t *starts, const uint32_t *run_if = nullptr) {
+    if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     __shared__ uint32_t s_w[4];
     const int s = blockIdx.x / npass, p = blockIdx.x % npass;
     uint32_t c = 0;
@@ -497,7 +508,7 @@ size_t radix_rows_seg_bytes(uint32_t total_tiles, int nseg, uint32_t nchunks) {
 int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt,
                                const RadixSeg *d_segs, int nseg, uint32_t total_tiles, const SweepChunk *d_chunks,
                                uint32_t nchunks, const uint32_t *d_chunk0, int lo_bit, int hi_bit, void *ws,
-                               hipStream_t st, bool *in_alt, const RowOut *d_outs) {
+                               hipStream_t st, bool *in_alt, const RowOut *d_outs, const uint32_t *run_if) {
     *in_alt = false;
     if (total_tiles == 0 || nchunks == 0) return QO_OK;
     QO_REQUIRE(!d_outs || vals, "unpacked output needs values");
@@ -513,9 +524,9 @@ int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, u
     uint32_t *bsum = cv.take<uint32_t>((size_t)rp.nb * 256);
     uint32_t *bflag = cv.take<uint32_t>(rp.nb);
     // Digit starts of every pass per segment, from one read of the keys.
-    k_sweep_counts_seg<<<nchunks, 256, 0, st>>>(keys, d_segs, d_chunks, lo_bit, npass, partial);
+    k_sweep_counts_seg<<<nchunks, 256, 0, st>>>(keys, d_segs, d_chunks, lo_bit, npass, partial, run_if);
     QO_LAUNCHED();
-    k_sweep_starts_seg<<<(unsigned)(nseg * npass), 256, 0, st>>>(partial, d_chunk0, npass, starts);
+    k_sweep_starts_seg<<<(unsigned)(nseg * npass), 256, 0, st>>>(partial, d_chunk0, npass, starts, run_if);
     QO_LAUNCHED();
     const RadixSeg none{0, 0, nullptr, 0, 0};
     const unsigned hblocks = (total_tiles + kHistTiles - 1) / kHistTiles;
@@ -523,25 +534,26 @@ int launch_radix_rows_segments(uint64_t *keys, uint32_t *vals, uint64_t *kalt, u
     uint32_t *vsrc = vals, *vdst = valt;
     for (int p = 0; p < npass; ++p) {
         const int sh = lo_bit + 8 * p;
-        k_tile_hist<<<hblocks, 256, 0, st>>>(ksrc, none, d_segs, nseg, total_tiles, sh, rows);
+        k_tile_hist<<<hblocks, 256, 0, st>>>(ksrc, none, d_segs, nseg, total_tiles, sh, rows, run_if);
         QO_LAUNCHED();
-        k_rows_reduce_seg<<<rp.nb, 256, 0, st>>>(rows, total_tiles, rp.span, d_segs, nseg, bsum, bflag);
+        k_rows_reduce_seg<<<rp.nb, 256, 0, st>>>(rows, total_tiles, rp.span, d_segs, nseg, bsum, bflag, run_if);
         QO_LAUNCHED();
-        k_rows_starts_seg<<<1, 1024, 0, st>>>(bsum, bflag, rp.nb);
+        k_rows_starts_seg<<<1, 1024, 0, st>>>(bsum, bflag, rp.nb, run_if);
         QO_LAUNCHED();
-        k_rows_apply_seg<<<rp.nb, 256, 0, st>>>(rows, total_tiles, rp.span, d_segs, nseg, bsum);
+        k_rows_apply_seg<<<rp.nb, 256, 0, st>>>(rows, total_tiles, rp.span, d_segs, nseg, bsum, run_if);
         QO_LAUNCHED();
         if (vals && d_outs && p == npass - 1)
             k_radix_place<true, true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, none, d_segs, nseg,
                                                                    total_tiles, sh, rows, starts + p * 256,
-                                                                   (uint32_t)npass * 256, RowOut{}, d_outs);
+                                                                   (uint32_t)npass * 256, RowOut{}, d_outs, run_if);
         else if (vals)
             k_radix_place<true><<<total_tiles, 256, 0, st>>>(ksrc, vsrc, kdst, vdst, none, d_segs, nseg, total_tiles,
-                                                             sh, rows, starts + p * 256, (uint32_t)npass * 256);
+                                                             sh, rows, starts + p * 256, (uint32_t)npass * 256,
+                                                             RowOut{}, nullptr, run_if);
         else
             k_radix_place<false><<<total_tiles, 256, 0, st>>>(ksrc, nullptr, kdst, nullptr, none, d_segs, nseg,
                                                               total_tiles, sh, rows, starts + p * 256,
-                                                              (uint32_t)npass * 256);
+                                                              (uint32_t)npass * 256, RowOut{}, nullptr, run_if);
         QO_LAUNCHED();
         std::swap(ksrc, kdst);
         std::swap(vsrc, vdst);
@@ -654,13 +666,13 @@ constexpr uint32_t kFixThreads = 1024;  // two blocks per CU, 8 waves per SIMD
 constexpr uint32_t kFixRounds = (kFixWin + kFixThreads - 1) / kFixThreads;
 constexpr uint32_t kFixChunks = kFixWin / kWave + 2;  // marks of the window's chunks and one past it
 
-__global__ __launch_bounds__(kFixThreads) void k_fix_runs(const uint64_t *__restrict__ k, const uint32_t *__restrict__ v,
-                                                  size_t m, RowOut out, uint32_t *long_flag) {
+// Block work of the run pass: positions [t0, t0 + kFixTile) of m rows (k, v).
+QO_DEV void fix_runs_tile(const uint64_t *__restrict__ k, const uint32_t *__restrict__ v, size_t m, size_t t0,
+                          const RowOut &out, uint32_t *long_flag) {
     static_assert(kFixWin % kWave == 0, "window of whole chunks");
     __shared__ uint32_t s_hi[kFixWin];
     __shared__ uint32_t s_lo[kFixWin + 4];  // 4 words of slack for the unrolled rank loop
     __shared__ uint64_t s_mark[kFixChunks];
-    const size_t t0 = (size_t)blockIdx.x * kFixTile;
     const uint32_t own = (uint32_t)(m - t0 < kFixTile ? m - t0 : kFixTile);
     const uint32_t len = (uint32_t)(m - t0 < kFixWin ? m - t0 : kFixWin);
     const bool data_end = t0 + len == m;  // the window reaches the last row
@@ -735,6 +747,34 @@ __global__ __launch_bounds__(kFixThreads) void k_fix_runs(const uint64_t *__rest
         out.rpay[dst] = (int32_t)v[t0 + i];
     }
     if (long_run) atomicOr(long_flag, 1u);
+}
+
+__global__ __launch_bounds__(kFixThreads) void k_fix_runs(const uint64_t *__restrict__ k, const uint32_t *__restrict__ v,
+                                                          size_t m, RowOut out, uint32_t *long_flag) {
+    fix_runs_tile(k, v, m, (size_t)blockIdx.x * kFixTile, out, long_flag);
+}
+
+// Segmented form: block b is global tile b of the segments (kSortTile rows per
+// tile; n = *d_n clamped to cap rows per segment).
+__global__ __launch_bounds__(kFixThreads) void k_fix_runs_seg(const uint64_t *__restrict__ k,
+                                                              const uint32_t *__restrict__ v, const RadixSeg *segs,
+                                                              int nseg, const RowOut *outs, uint32_t *long_flag) {
+    const int si = seg_index_of(segs, nseg, blockIdx.x);
+    const RadixSeg sg = segs[si];
+    uint64_t n = sg.d_n ? *sg.d_n : sg.cap;
+    if (n > sg.cap) n = sg.cap;
+    const size_t t0 = (size_t)(blockIdx.x - sg.tile0) * kFixTile;
+    if (t0 >= n) return;  // whole block
+    fix_runs_tile(k + sg.base, v + sg.base, n, t0, outs[si], long_flag);
+}
+
+int launch_fix_runs_segments(const uint64_t *keys, const uint32_t *vals, const RadixSeg *d_segs, int nseg,
+                             uint32_t total_tiles, const RowOut *d_outs, uint32_t *long_flag, hipStream_t st) {
+    static_assert(kFixTile == (uint32_t)kSortTile, "run pass tiles are sort tiles");
+    if (total_tiles == 0) return QO_OK;
+    k_fix_runs_seg<<<total_tiles, kFixThreads, 0, st>>>(keys, vals, d_segs, nseg, d_outs, long_flag);
+    QO_LAUNCHED();
+    return QO_OK;
 }
 
 int sort_packed_join_rows(uint64_t *k64, uint32_t *v32, size_t m, void *rws, int32_t *key, int32_t *rpay,

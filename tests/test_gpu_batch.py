@@ -42,6 +42,19 @@ def test_batch_matches_oracle_and_plain(qo, orc, queries):
                 assert np.array_equal(a, p)
 
 
+def test_batch_short_runs_match_oracle(qo, orc, queries):
+    """Without a key of more than 64 rows the batch's sort is the key-bit passes
+    plus the run pass alone (the full-width fallback stays gated off); the mix
+    above holds a 100-row key and takes the fallback."""
+    sub = [queries[i] for i in (2, 3, 4, 5, 7, 9)]
+    got = qo.operators.batch_hj_filter_sort(sub)
+    for q, g in zip(sub, got):
+        rk, rp, sk, sp, sf, flo, fhi = q
+        exp = orc.sort_join_rows(*orc.hash_join(rk, rp, sk, sp, sf, flo, fhi))
+        for a, e in zip(g, exp):
+            assert np.array_equal(a, e)
+
+
 def test_batch_alone_equals_in_mix(qo, queries):
     mix = qo.operators.batch_hj_filter_sort(queries)
     for i in (3, 5, 9):
