@@ -123,7 +123,10 @@ int qo_scan_exclusive_i32(const int32_t *in, int32_t *out, size_t n, qo_stream s
 /* ---- sort ---------------------------------------------------------------- */
 /* Sort Records ascending by .y (uint32), stable, in place. */
 int qo_sort_records(qo_record *data, size_t n, qo_stream s);
-/* Sort join rows ascending by (key, s_payload) as signed int32, stable, in place. */
+/* Sort join rows ascending by (key, s_payload) as signed int32, stable, in place.
+ * Synchronizes stream s once: the rows are sorted on the key first and each
+ * key's run by s_payload after, and a key with more than 64 rows (read back
+ * from the device) sends all rows through the full-width sort. */
 int qo_sort_join_rows(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, qo_stream s);
 
 /* ---- sort-based group-by (reference pipeline) ------------------------------ */
