@@ -662,7 +662,7 @@ size_t sort_join_rows_ws_bytes(size_t m) { return m * 12 + 2048 + radix_workspac
 constexpr uint32_t kFixTile = 4096;
 constexpr uint32_t kFixRun = 64;
 constexpr uint32_t kFixWin = kFixTile + kFixRun;
-constexpr uint32_t kFixThreads = 1024;  // two blocks per CU, 8 waves per SIMD
+constexpr uint32_t kFixThreads = 512;  // four blocks per CU (34 KB LDS each), 8 waves per SIMD
 constexpr uint32_t kFixRounds = (kFixWin + kFixThreads - 1) / kFixThreads;
 constexpr uint32_t kFixChunks = kFixWin / kWave + 2;  // marks of the window's chunks and one past it
 
