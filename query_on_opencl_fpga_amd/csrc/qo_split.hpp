@@ -26,20 +26,24 @@ constexpr uint32_t kCtrStride = 16;              // bucket counters 64 bytes apa
 
 struct SplitPlan {
     int L = 0, L1 = 0, L2 = 0;      // fine bucket bits = L1 (split A) + L2 (split B)
-    uint64_t bcapA = 0, bcapB = 0;  // pairs per bucket of split A / split B
+    uint64_t bcapA = 0, bcapB = 0;  // pairs per region of split A / per bucket of split B
+    uint32_t subs = 1;              // split A regions per bucket (kXcds: XcdLayout, one split only)
 };
+constexpr uint32_t kXcds = 8;  // work-group w runs on XCD w % 8
 
 // Plan for n tuples into 2^L fine buckets, 1 <= L <= 2 * kMaxSplitBits.  One
 // split when it suffices; else the first split takes the smaller fan-out (it
 // is the filtered one) so that both write runs of similar length.
-inline SplitPlan split_plan(int L, size_t n) {
+inline SplitPlan split_plan(int L, size_t n, uint32_t subs = 1) {
     SplitPlan p;
     p.L = L;
     p.L1 = L <= kMaxSplitBits ? L : L / 2;
     p.L2 = L - p.L1;
+    p.subs = p.L2 == 0 ? subs : 1;
     // Hash buckets of uniform keys stay within a few standard deviations of
     // their mean; the slack leaves only skewed inputs to the overflow path.
-    const uint64_t perA = (n + (uint64_t(1) << p.L1) - 1) >> p.L1;
+    const uint64_t regions = (uint64_t(1) << p.L1) * p.subs;
+    const uint64_t perA = (n + regions - 1) / regions;
     p.bcapA = (perA + perA / 16 + 1024 + 31) / 32 * 32;
     const uint64_t perB = (n + (uint64_t(1) << L) - 1) >> L;
     p.bcapB = (perB + perB / 8 + 256 + 31) / 32 * 32;
@@ -56,9 +60,9 @@ struct SplitBufs {
 // Carves the split buffers out of a workspace (nullptr base: sizing only).
 inline SplitBufs split_carve(const SplitPlan &p, Carver &cv) {
     SplitBufs b;
-    b.ctr_words = ((size_t(1) << p.L1) + (size_t(1) << p.L)) * kCtrStride;
+    b.ctr_words = ((size_t(1) << p.L1) * p.subs + (size_t(1) << p.L)) * kCtrStride;
     b.ctr = cv.take<uint32_t>(b.ctr_words);
-    b.bucketA = cv.take<uint64_t>((size_t(1) << p.L1) * p.bcapA);
+    b.bucketA = cv.take<uint64_t>((size_t(1) << p.L1) * p.subs * p.bcapA);
     if (p.L2 > 0) b.bucketB = cv.take<uint64_t>((size_t(1) << p.L) * p.bcapB);
     return b;
 }
@@ -91,12 +95,25 @@ struct AppendOverflow {
 // Fixed: equal capacity per bucket, fill counts in the counters.
 struct FixedLayout {
     uint64_t bcap;
+    QO_DEV uint32_t slot(uint32_t b) const { return b; }
     QO_DEV uint64_t start(uint32_t b) const { return (uint64_t)b * bcap; }
     QO_DEV uint64_t cap(uint32_t) const { return bcap; }
     QO_DEV uint64_t count(uint32_t b, const uint32_t *ctr) const {
         const uint64_t c = ctr[b * kCtrStride];
         return c < bcap ? c : bcap;
     }
+};
+// Per XCD: bucket b has one region and one cursor per XCD, and a block
+// appends to region (b, its XCD), slot b * 8 + blockIdx % 8 (work-group w runs
+// on XCD w % 8).  Each cursor then takes the reservations of one XCD's
+// blocks: same-address device atomics serialise, and with one cursor per
+// bucket every split tile of 2^28 tuples queued 128 reservations behind those
+// of ~500 other tiles.
+struct XcdLayout {
+    uint64_t bcap;  // per region
+    QO_DEV uint32_t slot(uint32_t b) const { return b * kXcds + (blockIdx.x & (kXcds - 1)); }
+    QO_DEV uint64_t start(uint32_t s) const { return (uint64_t)s * bcap; }
+    QO_DEV uint64_t cap(uint32_t) const { return bcap; }
 };
 // Exact: a bucket of this split covers fine buckets [b << sh, (b + 1) << sh).
 // Its region starts at offp (the exclusive scan of the fine-bucket counts,
@@ -106,6 +123,7 @@ struct FixedLayout {
 struct ExactLayout {
     const uint32_t *offp, *off;
     int sh;
+    QO_DEV uint32_t slot(uint32_t b) const { return b; }
     QO_DEV uint64_t start(uint32_t b) const { return offp[(size_t)b << sh]; }
     QO_DEV uint64_t cap(uint32_t b) const { return offp[(size_t)(b + 1) << sh] - offp[(size_t)b << sh]; }
     QO_DEV uint64_t count(uint32_t b, const uint32_t *) const {
@@ -162,7 +180,7 @@ QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], 
     const uint32_t ex = block_excl_scan<kSplitThreads / kWave>(c0, sh.w);
     if (threadIdx.x < parts) {
         // The run's start and how many of its tuples fit (the rest overflow).
-        const uint32_t b = bucket0 + threadIdx.x;
+        const uint32_t b = out.slot(bucket0 + threadIdx.x);
         const uint64_t g = c0 ? atomicAdd(&ctr[b * kCtrStride], c0) : 0u;
         const uint64_t cap = out.cap(b);
         sh.loff[threadIdx.x] = ex;
@@ -351,6 +369,20 @@ inline int launch_splits(const SplitPlan &p, const SplitBufs &b, const int32_t *
     const size_t tiles = (n + kSplitTile - 1) / kSplitTile;
     QO_REQUIRE(tiles < 0x7FFFFFFFull, "input too large for a hash split");
     const FixedLayout la{p.bcapA}, lb{p.bcapB};
+    if (p.subs > 1) {
+        QO_REQUIRE(p.L2 == 0 && p.subs == kXcds, "per-XCD regions need one split");
+        const XcdLayout lx{p.bcapA};
+        if (fcol)
+            k_split_cols<true><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1,
+                                                                         partsA, b.bucketA, lx, b.ctr, overflow, gate,
+                                                                         hsrc);
+        else
+            k_split_cols<false><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1,
+                                                                          partsA, b.bucketA, lx, b.ctr, overflow, gate,
+                                                                          hsrc);
+        QO_LAUNCHED();
+        return QO_OK;
+    }
     if (fcol)
         k_split_cols<true><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1, partsA,
                                                             b.bucketA, la, b.ctr, overflow, gate, hsrc);
