@@ -81,7 +81,7 @@ struct L2Plan {
     bool on = false;
     bool hash_fits = false;
     SplitPlan s;
-    uint32_t nb = 0;   // buckets (kXcds regions each)
+    uint32_t nb = 0;   // buckets (kXcdRegions regions each)
     uint32_t tpb = 0;  // probe tiles per region (upper bound)
 };
 
@@ -96,7 +96,7 @@ static L2Plan l2_plan(uint64_t slots, size_t n) {
     while (L < kMaxSplitBits && ((cap * 8) >> L) > kL2SliceBytes) ++L;
     r.on = true;
     r.hash_fits = ((slots * 8) >> L) <= kL2SliceMaxBytes;
-    r.s = split_plan(L, n, kXcds);
+    r.s = split_plan(L, n, kXcdRegions);
     r.nb = 1u << L;
     r.tpb = (uint32_t)((r.s.bcapA + kPairStep - 1) / kPairStep);  // per region
     return r;
@@ -366,9 +366,9 @@ __global__ void k_spill_insert(const uint64_t *spill, const unsigned long long *
     }
 }
 
-// L2 probe from the S buckets (either table mode), each bucket kXcds regions
+// L2 probe from the S buckets (either table mode), each bucket kXcdRegions regions
 // (XcdLayout).  Block i works on XCD x = i % 8: with q = i / 8 it takes bucket
-// b = (q / (kXcds * tpb)) * 8 + x, region (q / tpb) % kXcds and its tile
+// b = (q / (kXcdRegions * tpb)) * 8 + x, region (q / tpb) % kXcdRegions and its tile
 // q % tpb, so each XCD walks its own buckets one after another, every region
 // of a bucket in turn, and its L2 holds the current table slice.
 __global__ __launch_bounds__(256) void k_l2_probe(const uint64_t *__restrict__ bucket, uint64_t bcap,
@@ -380,10 +380,10 @@ __global__ __launch_bounds__(256) void k_l2_probe(const uint64_t *__restrict__ b
     __shared__ JoinStage stage;
     if (gate.off()) return;
     const uint32_t x = blockIdx.x & 7u, q = blockIdx.x >> 3;
-    const uint32_t per_bucket = kXcds * tpb;
+    const uint32_t per_bucket = kXcdRegions * tpb;
     const uint32_t b = (q / per_bucket) * 8u + x, rem = q % per_bucket, t = rem % tpb;
     if (b >= nb) return;
-    const uint32_t reg = b * kXcds + rem / tpb;
+    const uint32_t reg = b * kXcdRegions + rem / tpb;
     uint64_t n = ctr[reg * kCtrStride];
     if (n > bcap) n = bcap;
     const uint64_t e0 = (uint64_t)t * kPairStep;
@@ -459,7 +459,7 @@ int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t 
         QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));
         QO_TRY(launch_splits(r.s, b, skey, spay, sfilt, ns, flo, fhi, ProbeOverflow{hdr, counter, emit}, gate,
                              HashSrc{hdr}, st));
-        const uint64_t grid = (uint64_t)((r.nb + 7) / 8) * 8 * kXcds * r.tpb;
+        const uint64_t grid = (uint64_t)((r.nb + 7) / 8) * 8 * kXcdRegions * r.tpb;
         QO_REQUIRE(grid < 0x7FFFFFFFull, "probe input too large");
         k_l2_probe<<<(unsigned)grid, 256, 0, st>>>(b.bucketA, r.s.bcapA, b.ctr, r.nb, r.tpb, hdr, counter, emit, gate);
         QO_LAUNCHED();

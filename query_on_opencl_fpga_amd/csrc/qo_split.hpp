@@ -27,9 +27,10 @@ constexpr uint32_t kCtrStride = 16;              // bucket counters 64 bytes apa
 struct SplitPlan {
     int L = 0, L1 = 0, L2 = 0;      // fine bucket bits = L1 (split A) + L2 (split B)
     uint64_t bcapA = 0, bcapB = 0;  // pairs per region of split A / per bucket of split B
-    uint32_t subs = 1;              // split A regions per bucket (kXcds: XcdLayout, one split only)
+    uint32_t subs = 1;              // split A regions per bucket (kXcdRegions: XcdLayout, one split only)
 };
-constexpr uint32_t kXcds = 8;  // work-group w runs on XCD w % 8
+constexpr uint32_t kXcds = 8;        // work-group w runs on XCD w % 8
+constexpr uint32_t kXcdRegions = kXcds;  // regions per bucket of an XcdLayout split (16: no faster)
 
 // Plan for n tuples into 2^L fine buckets, 1 <= L <= 2 * kMaxSplitBits.  One
 // split when it suffices; else the first split takes the smaller fan-out (it
@@ -111,7 +112,7 @@ struct FixedLayout {
 // of ~500 other tiles.
 struct XcdLayout {
     uint64_t bcap;  // per region
-    QO_DEV uint32_t slot(uint32_t b) const { return b * kXcds + (blockIdx.x & (kXcds - 1)); }
+    QO_DEV uint32_t slot(uint32_t b) const { return b * kXcdRegions + (blockIdx.x & (kXcdRegions - 1)); }
     QO_DEV uint64_t start(uint32_t s) const { return (uint64_t)s * bcap; }
     QO_DEV uint64_t cap(uint32_t) const { return bcap; }
 };
@@ -370,7 +371,7 @@ inline int launch_splits(const SplitPlan &p, const SplitBufs &b, const int32_t *
     QO_REQUIRE(tiles < 0x7FFFFFFFull, "input too large for a hash split");
     const FixedLayout la{p.bcapA}, lb{p.bcapB};
     if (p.subs > 1) {
-        QO_REQUIRE(p.L2 == 0 && p.subs == kXcds, "per-XCD regions need one split");
+        QO_REQUIRE(p.L2 == 0 && p.subs == kXcdRegions, "per-XCD regions need one split");
         const XcdLayout lx{p.bcapA};
         if (fcol)
             k_split_cols<true><<<(unsigned)tiles, kSplitThreads, 0, st>>>(kcol, pcol, fcol, n, flo, fhi, vec, 32 - p.L1,
