@@ -168,7 +168,7 @@ def run_join(args, qo, lib) -> dict:
                   else f"hash-mixed table ({used.value} slots)")
     if part.value and splits.value == 1:
         kernel = "probe stage (one qo_hash_probe call): k_split_cols + k_l2_probe"
-        strategy = f"{table_kind}; probe: one split into {1 << bits.value} buckets, XCD-local L2 lookups"
+        strategy = f"{table_kind}; probe: one split into {1 << bits.value} buckets of 8 per-XCD regions, XCD-local L2 lookups"
     elif part.value:
         kernel = ("probe stage (one qo_hash_probe call): k_split_cols"
                   + (" + k_split_pairs" if splits.value == 2 else "") + " + k_slice_probe")
