@@ -305,7 +305,7 @@ static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int
 // ---------------------------------------------------------------------------
 // Exclusive scan of 32-bit integers (modulo 2^32), single pass.
 // ---------------------------------------------------------------------------
-constexpr int kScanJ = 4;
+constexpr int kScanJ = 8;
 constexpr int kScanThreads = 1024;
 
 template <int J, bool VEC, int NT>
