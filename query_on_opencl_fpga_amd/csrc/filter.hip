@@ -209,7 +209,7 @@ struct RecVec<1> {
 // across the block.  In-tile offsets are 32-bit from the tile's base pointer:
 // with 64-bit per-element indexing the kernel needed 140 VGPRs and one block
 // per CU, which cost 1.44 ms instead of 0.86 ms on 2^28 records.
-constexpr int kFiltJ = 4;
+constexpr int kFiltJ = 8;
 constexpr int kFiltThreads = 1024;
 
 template <int J, int VEC, int NT>
