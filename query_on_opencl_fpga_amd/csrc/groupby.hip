@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(NT) void k_group_startpos(const qo_record *__restri
     if (tile == num_tiles - 1 && threadIdx.x == 0) *d_num = s_excl + total;
 }
 
-constexpr int kStartJ = 4;
+constexpr int kStartJ = 8;
 constexpr int kStartThreads = 1024;
 
 // Work units of kAggLanes lanes: group g, chunk q covers startPos[g] + q*kAggChunk.
