@@ -426,7 +426,7 @@ constexpr uint32_t kChunkU = 8;          // tuples in flight per thread of k_gro
 // ---------------------------------------------------------------------------
 constexpr uint32_t kOnePassParts = 1024;
 constexpr int kPartThreads = 1024;
-constexpr int kPartJ = 8;  // tuples per thread and tile
+constexpr int kPartJ = 16;  // tuples per thread and tile
 constexpr uint32_t kPartTile = (uint32_t)kPartThreads * kPartJ;
 constexpr uint32_t kPartScanBlocks = 128;
 constexpr uint32_t kPartMinSpan = 64;
