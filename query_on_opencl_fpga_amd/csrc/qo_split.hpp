@@ -318,8 +318,10 @@ __global__ __launch_bounds__(kSplitThreads) void k_split_pairs(const uint64_t *_
 // entries past the last tile are ~0.  One block of kSplitThreads threads
 // (nin <= kSplitThreads).  The block of split B thus finds its input with one
 // load instead of a chain of dependent loads.
+// Bits 0-31 start, 32-47 pairs (<= kSplitTile), 48-63 input bucket.
+static_assert(kSplitTile < (1u << 16) && kMaxSplit <= (1u << 16), "tile map fields");
 __host__ __device__ inline uint64_t tile_map_entry(uint64_t s0, uint32_t m, uint32_t b) {
-    return s0 | ((uint64_t)m << 32) | ((uint64_t)b << 46);
+    return s0 | ((uint64_t)m << 32) | ((uint64_t)b << 48);
 }
 static __global__ __launch_bounds__(kSplitThreads) void k_exact_tile_map(ExactLayout in, uint32_t nin, uint64_t *map,
                                                                          uint32_t entries) {
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(kSplitThreads) void k_split_pairs_exact(const uint6
     // appended to one output bucket come from one L2.
     const uint64_t e = map[xcd_tile(blockIdx.x, gridDim.x)];
     if (e == ~uint64_t(0)) return;  // past the last tile (the grid is an upper bound)
-    split_pairs_run(input, e & 0xFFFFFFFFull, (uint32_t)(e >> 32) & 0x3FFFu, (uint32_t)(e >> 46),
+    split_pairs_run(input, e & 0xFFFFFFFFull, (uint32_t)(e >> 32) & 0xFFFFu, (uint32_t)(e >> 48),
                     HashSrc{nullptr, 1}.get(), shift, parts, bucket, out, ctr, sh, overflow);
 }
 
