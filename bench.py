@@ -1,18 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the relational operator path on one MI355X.
 
-Default workload (one step = one pass of the hot path over one batch):
+Headline workload (one step = one pass of the hot path over one batch):
 filtered hash join, R = 2^24 tuples (key = seeded permutation of [0, 2^24),
 payload uniform int32), S = 2^28 tuples (fk uniform over R keys, payload
 uniform int32, filter column uniform in [0, 100) with predicate < 50); a step
 builds the table on R and probes it with the filtered S, materialising
 (key, r_payload, s_payload).  Inputs are generated in HBM before timing.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config join|filter_agg|groupby|batched|served]
+The default run also measures the other configurations and reports each
+under "configs", every one with its roofline and CPU baseline: fused filter +
+aggregation at 2^28 tuples, hash group-by at 2^28 tuples for G in {2^10, 2^20}
+x Zipf z in {0, 0.5, 1}, and batched hj_filter_sort (512 queries in one graph
+launch against the plain per-query path) at the configured sizes and for
+small queries.  --headline-only skips them.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--config join|filter_agg|groupby|batched|served|copy|ops] [--headline-only]
 
 Prints ONE JSON line.  `value` is S tuples per second of the whole join step
-(build + probe); the roofline block covers the probe kernel, timed with HIP
-events on the stream it runs on.
+(build + probe); the roofline block covers the probe stage, timed with HIP
+events on the stream it runs on.  CPU baselines run the multi-threaded CPU
+restatement of the reference algorithms (oracle/qo_cpu_par.c) on the host's
+cores on the same inputs (copied back from HBM), median of 5 runs after a
+warm-up.
 """
 from __future__ import annotations
 
@@ -53,53 +64,126 @@ def profile_traffic(name: str, match: dict):
 
 
 # ---------------------------------------------------------------------------
-# CPU baseline (oracle, single thread) on a bounded sample of the workload.
+# CPU baseline: the multi-threaded restatement of the reference algorithms
+# (oracle/qo_cpu_par.c) on this host's cores, on the GPU run's own inputs.
 # ---------------------------------------------------------------------------
-def cpu_baseline_join(nr: int, ns: int, seed: int, sample: int) -> dict:
+CPU_RUNS = 5
+
+
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: the cores this process may use, at
+    most 16 (one GPU's share of the host on the benchmark machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _oracle():
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as orc
 
-    rk = orc.gen_perm(nr, seed + 1).astype(np.int32)
-    rp = orc.gen_uniform_u32(nr, seed, 1, 0, 0).astype(np.int32)
-    u = orc.gen_uniform_u32(sample, seed, 2, 0, nr)
-    sk = rk[u.astype(np.int64)]  # fk = perm(u) and perm(i) is R's key column
-    sp = orc.gen_uniform_u32(sample, seed, 3, 0, 0).astype(np.int32)
-    sf = orc.gen_uniform_u32(sample, seed, 4, 0, 100).astype(np.int32)
-    lib = orc.load()
-    # Build alone (empty probe side), then build + probe of the sample.
-    e = np.zeros(1, np.int32)
-    t0 = time.perf_counter()
-    lib.or_hash_join(rk.ctypes.data, rp.ctypes.data, nr, e.ctypes.data, e.ctypes.data, None, 0, 0, 0,
-                     None, None, None, 0)
-    t1 = time.perf_counter()
-    m = lib.or_hash_join(rk.ctypes.data, rp.ctypes.data, nr, sk.ctypes.data, sp.ctypes.data, sf.ctypes.data,
-                         sample, INT_MIN, 49, None, None, None, 0)
-    t2 = time.perf_counter()
-    build_s = t1 - t0
-    probe_s = max((t2 - t1) - build_s, 1e-9)
-    full_step_s = build_s + probe_s * (ns / sample)
-    return {
-        "value": ns / full_step_s,
-        "unit": "S tuples/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"oracle or_hash_join, 1 thread: build on full R ({nr} rows, {build_s:.2f} s) + filtered "
-                   f"probe of the first {sample} S rows ({probe_s:.2f} s, {m} matches); step time "
-                   f"extrapolated to |S|={ns} as build + probe*|S|/sample"),
-    }
+    return orc, orc.load()
 
 
-def cpu_baseline_filter_agg(n: int, seed: int, sample: int) -> dict:
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle as orc
+def _median_time(fn) -> tuple[float, list]:
+    """Median wall time of fn() over CPU_RUNS runs after one warm-up run."""
+    fn()
+    ts = []
+    for _ in range(CPU_RUNS):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
 
-    k = orc.gen_uniform_u32(sample, seed, 5, 0, 1 << 31).astype(np.int32)
-    v = orc.gen_uniform_u32(sample, seed, 6, 0, 1000).astype(np.int32)
-    t0 = time.perf_counter()
-    orc.filter_agg(k, v, 0, (1 << 30) - 1)
-    t1 = time.perf_counter()
-    return {"value": sample / (t1 - t0), "unit": "tuples/s", "cores": 1, "kind": "port",
-            "sample": f"oracle or_filter_agg, 1 thread, first {sample} of {n} tuples"}
+
+def _baseline(value, unit, sample, times, extra=None) -> dict:
+    d = {"value": value, "unit": unit, "cores": cpu_threads(), "kind": "port", "sample": sample,
+         "cpu_model": cpu_model(), "runs": CPU_RUNS, "median_s": float(np.median(times)),
+         "timer": "time.perf_counter around each call (CLOCK_MONOTONIC), data generation and copies excluded"}
+    if extra:
+        d.update(extra)
+    return d
+
+
+def cpu_baseline_join(rk, rp, sk, sp, sf, flo, fhi) -> dict:
+    """Filtered hash join (bucket-chained build on R, probe of filtered S,
+    materialised rows) on the full inputs."""
+    orc, lib = _oracle()
+    nt = cpu_threads()
+    nr, ns = rk.shape[0], sk.shape[0]
+    m = lib.pr_hash_join(rk.ctypes.data, rp.ctypes.data, nr, sk.ctypes.data, sp.ctypes.data, sf.ctypes.data, ns,
+                         flo, fhi, None, None, None, 0, nt)
+    outs = [np.empty(max(m, 1), np.int32) for _ in range(3)]
+
+    def run():
+        lib.pr_hash_join(rk.ctypes.data, rp.ctypes.data, nr, sk.ctypes.data, sp.ctypes.data, sf.ctypes.data, ns,
+                         flo, fhi, outs[0].ctypes.data, outs[1].ctypes.data, outs[2].ctypes.data, m, nt)
+
+    t, ts = _median_time(run)
+    return _baseline(ns / t, "S tuples/s", f"full workload: build on |R|={nr}, filtered probe of |S|={ns}, "
+                     f"{m} rows materialised; pr_hash_join on {nt} threads", ts, {"matches": int(m)})
+
+
+def cpu_baseline_filter_agg(k, v, lo, hi) -> dict:
+    orc, lib = _oracle()
+    nt = cpu_threads()
+    out = np.zeros(3, np.int64)
+    t, ts = _median_time(lambda: lib.pr_filter_agg(k.ctypes.data, v.ctypes.data, k.shape[0], lo, hi,
+                                                   out.ctypes.data, nt))
+    return _baseline(k.shape[0] / t, "tuples/s", f"full workload: {k.shape[0]} tuples; pr_filter_agg on {nt} "
+                     "threads", ts)
+
+
+def cpu_baseline_groupby(k, v, note) -> dict:
+    """The reference's sort-based group-by (sort -> start positions -> per-group
+    aggregates) on a bounded sample of the case's tuples."""
+    orc, lib = _oracle()
+    nt = cpu_threads()
+    n = k.shape[0]
+    outs = [np.empty(n, d) for d in (np.int32, np.int64, np.int64, np.int32, np.int32)]
+    t, ts = _median_time(lambda: lib.pr_groupby_agg(k.ctypes.data, v.ctypes.data, n, *[o.ctypes.data for o in outs],
+                                                    nt))
+    return _baseline(n / t, "tuples/s", f"{note}; pr_groupby_agg (parallel LSD radix sort + group reduce) on "
+                     f"{nt} threads", ts)
+
+
+def cpu_baseline_batch(queries) -> dict:
+    """The plain per-query path on the CPU: filtered join, then sort by
+    (key, s_payload), one query after the other, for a sample of the queries
+    (each query's inputs copied back from HBM)."""
+    orc, lib = _oracle()
+    nt = cpu_threads()
+    bufs = []
+    for q in queries:
+        rk, rp = q["rkey"].to_numpy(np.int32, q["nr"]), q["rpay"].to_numpy(np.int32, q["nr"])
+        sk, sp, sf = (q[c].to_numpy(np.int32, q["ns"]) for c in ("skey", "spay", "sfilt"))
+        cap = q["cap"]
+        bufs.append((rk, rp, sk, sp, sf, q["flo"], q["fhi"], [np.empty(cap, np.int32) for _ in range(3)], cap))
+
+    def run():
+        for rk, rp, sk, sp, sf, flo, fhi, o, cap in bufs:
+            m = lib.pr_hash_join(rk.ctypes.data, rp.ctypes.data, rk.shape[0], sk.ctypes.data, sp.ctypes.data,
+                                 sf.ctypes.data, sk.shape[0], flo, fhi, o[0].ctypes.data, o[1].ctypes.data,
+                                 o[2].ctypes.data, cap, nt)
+            lib.pr_sort_join_rows(o[0].ctypes.data, o[1].ctypes.data, o[2].ctypes.data, min(m, cap), nt)
+
+    t, ts = _median_time(run)
+    return _baseline(len(queries) / t, "queries/s", f"{len(queries)} queries of the batch (the first ones), each "
+                     f"pr_hash_join + pr_sort_join_rows on {nt} threads, one after the other", ts)
 
 
 # ---------------------------------------------------------------------------
@@ -175,9 +259,11 @@ def run_join(args, qo, lib) -> dict:
         strategy = f"{table_kind}; probe: {1 << bits.value} LDS slices, {splits.value} split pass(es)"
     else:
         kernel, strategy = "k_join_probe", f"{table_kind}; probe: direct"
-    traffic = args.traffic
+    traffic, traffic_src = args.traffic, "--traffic on the command line"
     if traffic is None:
         traffic = profile_traffic("join_probe", {"R_tuples": nr, "S_tuples": ns, "strategy": strategy})
+        traffic_src = ("stored profile profiles/join_probe_traffic.json (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE "
+                       "of this configuration; not measured in this run)") if traffic is not None else None
     res = {
         "metric": METRIC,
         "value": ns / (ms_step * 1e-3),
@@ -189,6 +275,7 @@ def run_join(args, qo, lib) -> dict:
                    "table_bytes": tb, "strategy": strategy},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "alg_bytes_per_launch": probe_bytes, "alg_bytes_formula": "12*|S| + 12*matches",
                      "avg_launch_ms": probe_avg,
                      "timed_with": "HIP events on the probe stream around each qo_hash_probe call"},
@@ -199,10 +286,17 @@ def run_join(args, qo, lib) -> dict:
     }
     for b in outs + [table, cnt, fcnt]:
         b.free()
-    t.free()
     if not args.no_cpu_baseline:
-        log("[bench] join: cpu baseline")
-        res["cpu_baseline"] = cpu_baseline_join(nr, ns, seed, args.cpu_sample)
+        log("[bench] join: cpu baseline (inputs copied back from HBM)")
+        host = [t.rkey.to_numpy(np.int32, nr), t.rpay.to_numpy(np.int32, nr)] + \
+               [c.to_numpy(np.int32, ns) for c in (t.skey, t.spay, t.sfilt)]
+        t.free()
+        res["cpu_baseline"] = cpu_baseline_join(*host, INT_MIN, 49)
+        if res["cpu_baseline"]["matches"] != m:
+            raise SystemExit(f"CPU baseline found {res['cpu_baseline']['matches']} matches, GPU {m}")
+        del host
+    else:
+        t.free()
     return res
 
 
@@ -243,9 +337,12 @@ def run_filter_agg(args, qo, lib) -> dict:
                      "alg_bytes_per_launch": 8 * n, "avg_launch_ms": k_ms},
     }
     out.free()
-    t.free()
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline_filter_agg(n, seed, min(n, args.cpu_sample * 4))
+        log("[bench] filter_agg: cpu baseline")
+        k, v = t.key.to_numpy(np.int32, n), t.val.to_numpy(np.int32, n)
+        res["cpu_baseline"] = cpu_baseline_filter_agg(k, v, 0, (1 << 30) - 1)
+        del k, v
+    t.free()
     return res
 
 
@@ -429,17 +526,29 @@ def _mk_batch_queries(args, qo, seed):
     return qs
 
 
-def run_batched(args, qo, lib) -> dict:
+def _digest(qo, q, m, buf):
+    """(checksum, key sum, order violations) of a query's first m output rows."""
+    qo._lib.call("qo_join_rows_digest", q["okey"].ptr, q["orpay"].ptr, q["ospay"].ptr, int(m), None, buf.ptr, None)
+    return tuple(int(x) for x in buf.to_numpy(np.uint64, 3))
+
+
+def run_batched(args, qo, lib, scale=None, label="configured sizes") -> dict:
     """Batched hj_filter_sort: all queries in one graph launch vs the plain path
-    (qo_hj_filter_sort per query, synchronous like the reference's host loop)."""
+    (qo_hj_filter_sort per query, synchronous like the reference's host loop).
+    Every query's batched output is checked against its plain output (match
+    count and an order-sensitive device digest of the three columns, which also
+    counts rows out of (key, s_payload) order)."""
     import ctypes as C
 
     from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
     from query_on_opencl_fpga_amd.operators import JoinBatch
 
     seed = SEED_BASE + 4
-    log(f"[bench] batched: generating {args.queries} queries")
+    if scale is not None:
+        args = argparse.Namespace(**dict(vars(args), batch_scale=scale))
+    log(f"[bench] batched ({label}): generating {args.queries} queries")
     qs = _mk_batch_queries(args, qo, seed)
+    nq = len(qs)
     tot_r = sum(q["nr"] for q in qs)
     tot_s = sum(q["ns"] for q in qs)
     log(f"[bench] batched: |R| total {tot_r}, |S| total {tot_s}; creating batch")
@@ -448,59 +557,77 @@ def run_batched(args, qo, lib) -> dict:
     for _ in range(args.warmup):
         b.launch(st)
     st.sync()
-    e0, e1 = Event(), Event()
-    e0.record(st)
-    for _ in range(args.steps):
+    evs = [(Event(), Event()) for _ in range(args.steps)]
+    for e0, e1 in evs:
+        e0.record(st)
         b.launch(st)
-    e1.record(st)
+        e1.record(st)
     st.sync()
-    t_batch = e0.elapsed_ms(e1) / args.steps
-    counts_b = b.counts()
-    sums_b = [int(qs[i]["okey"].to_numpy(np.int32, int(counts_b[i])).astype(np.int64).sum()) for i in (0, len(qs) - 1)]
+    launch_ms = np.array([e0.elapsed_ms(e1) for e0, e1 in evs])
+    t_batch = float(launch_ms.mean())
+    counts_b = [int(c) for c in b.counts()]
+    dig = DeviceBuffer(24)
+    dig_b = [_digest(qo, q, m, dig) for q, m in zip(qs, counts_b)]
     b.destroy()
     log(f"[bench] batched: {t_batch:.2f} ms per batch; plain path")
     maxr = max(q["nr"] for q in qs)
-    tb = lib.qo_join_table_bytes(maxr)
-    table = DeviceBuffer(tb)
+    table = DeviceBuffer(lib.qo_join_table_bytes(maxr))
     m = C.c_uint64()
-    lat = []
 
     def plain_all(record):
+        lat = []
         t0 = time.perf_counter()
         for q in qs:
+            t1 = time.perf_counter()
             qo._lib.call("qo_hj_filter_sort", table.ptr, lib.qo_join_table_bytes(q["nr"]), q["rkey"].ptr,
                          q["rpay"].ptr, q["nr"], q["skey"].ptr, q["spay"].ptr, q["sfilt"].ptr, q["ns"], q["flo"],
                          q["fhi"], q["okey"].ptr, q["orpay"].ptr, q["ospay"].ptr, q["cap"], C.byref(m), st.handle)
+            lat.append((time.perf_counter() - t1) * 1e3)
             if record:
-                lat.append((time.perf_counter() - t0) * 1e3)
                 counts_p.append(m.value)
-        return (time.perf_counter() - t0) * 1e3
+                dig_p.append(_digest(qo, q, m.value, dig))
+        return (time.perf_counter() - t0) * 1e3, np.array(lat)
 
-    counts_p = []
+    counts_p, dig_p = [], []
     plain_all(False)  # warm-up (workspaces grow to the largest query)
-    counts_p.clear()
-    t_plain = plain_all(True)
-    sums_p = [int(qs[i]["okey"].to_numpy(np.int32, int(counts_p[i])).astype(np.int64).sum()) for i in (0, len(qs) - 1)]
-    if list(map(int, counts_b)) != counts_p or sums_b != sums_p:
-        raise SystemExit("batched and plain results differ")
+    t_plain, lat = plain_all(False)
+    plain_all(True)   # the checked run (digests outside the timed run)
+    bad = [i for i in range(nq) if counts_b[i] != counts_p[i] or dig_b[i] != dig_p[i] or dig_b[i][2] != 0]
+    if bad:
+        i = bad[0]
+        raise SystemExit(f"batched and plain results differ for {len(bad)} queries; first {i}: "
+                         f"batched {counts_b[i]} {dig_b[i]} plain {counts_p[i]} {dig_p[i]}")
     table.free()
-    for q in qs:
-        for k, v in q.items():
-            if hasattr(v, "free"):
-                v.free()
-    nq = len(qs)
-    lat = np.array(lat)
+    dig.free()
+    alg = sum(16 * q["nr"] + 12 * q["ns"] + 36 * c for q, c in zip(qs, counts_b))
+    gbs = alg / (t_batch * 1e-3) / 1e9
     res = {
         "metric": METRIC, "value": nq / (t_batch * 1e-3), "unit": "queries/s (batched)",
         "ms_per_step": t_batch, "dtype": "int32",
-        "config": {"workload": "batched hj_filter_sort: filter S, build R, probe, sort by (key, s_payload)",
-                   "queries": nq, "R_total": tot_r, "S_total": tot_s, "matches_total": int(sum(counts_p))},
-        "batched": {"queries_per_s": nq / (t_batch * 1e-3), "p50_ms": t_batch, "p99_ms": t_batch,
-                    "launches": 1},
+        "config": {"workload": f"batched hj_filter_sort ({label}): filter S, build R, probe, sort by (key, s_payload)",
+                   "queries": nq, "R_log2_range": [12, 12 + 10 * args.batch_scale], "R_total": tot_r,
+                   "S_total": tot_s, "matches_total": int(sum(counts_p)),
+                   "checked": "every query: batched count and device digest (checksum, key sum, order) == plain"},
+        "batched": {"queries_per_s": nq / (t_batch * 1e-3), "launches": args.steps,
+                    "p50_ms": float(np.percentile(launch_ms, 50)), "p99_ms": float(np.percentile(launch_ms, 99)),
+                    "latency": "a query's latency is its batch's launch time (HIP events around each graph launch)"},
         "plain": {"queries_per_s": nq / (t_plain * 1e-3), "p50_ms": float(np.percentile(lat, 50)),
-                  "p99_ms": float(np.percentile(lat, 99)), "total_ms": t_plain},
+                  "p99_ms": float(np.percentile(lat, 99)), "total_ms": t_plain,
+                  "latency": "each query alone, host wall time of its synchronous qo_hj_filter_sort call"},
         "speedup_batched_vs_plain": t_plain / t_batch,
+        "roofline": {"bound": "hbm", "kernel": "one hipGraph launch of the whole batch (all stages)",
+                     "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes_per_launch": alg,
+                     "alg_bytes_formula": "sum over queries of 16*|R| + 12*|S| + 12*M + 24*M", "avg_launch_ms": t_batch},
     }
+    if not args.no_cpu_baseline:
+        k = min(nq, args.cpu_batch_queries)
+        log(f"[bench] batched: cpu baseline on {k} queries")
+        res["cpu_baseline"] = cpu_baseline_batch(qs[:k])
+    for q in qs:
+        for v in q.values():
+            if hasattr(v, "free"):
+                v.free()
     return res
 
 
@@ -593,6 +720,10 @@ def run_served(args, qo, lib) -> dict:
 
 
 def run_groupby(args, qo, lib) -> dict:
+    """Hash group-by (COUNT, SUM, MIN, MAX per group, sorted by key) at N tuples,
+    key Zipf(z) over G groups, for G in {2^10, 2^20} and z in {0, 0.5, 1}; every
+    case with its roofline (8*N + 28*G algorithmic bytes over the HIP-event time
+    of one qo_groupby_agg call) and a CPU baseline on a 2^24-tuple sample."""
     from query_on_opencl_fpga_amd import data
     from query_on_opencl_fpga_amd.device import DeviceBuffer, Event, Stream
 
@@ -611,33 +742,38 @@ def run_groupby(args, qo, lib) -> dict:
                 qo._lib.call("qo_groupby_agg", t.key.ptr, t.val.ptr, n, G, *[b.ptr for b in outs], num.ptr,
                              st.handle)
 
-            for _ in range(args.warmup):
-                step()
-            st.sync()
-            e0, e1 = Event(), Event()
-            e0.record(st)
-            for _ in range(args.steps):
-                step()
-            e1.record(st)
-            st.sync()
-            ms = e0.elapsed_ms(e1) / args.steps
+            ms = _time_op(st, args.steps, args.warmup, step)
             g = int(num.to_numpy(np.uint64, 1)[0])
+            cnt = outs[1].to_numpy(np.int64, g)
+            keys = outs[0].to_numpy(np.int32, g)
+            if int(cnt.sum()) != n or g > G or (g > 1 and not np.all(np.diff(keys) > 0)):
+                raise SystemExit(f"group-by G={G} z={z}: {g} groups, count sum {int(cnt.sum())} (expected {n})")
             alg = 8 * n + 28 * G
-            rows.append({"G": G, "z": z, "groups": g, "ms": ms, "tuples_per_s": n / (ms * 1e-3),
-                         "GBps_alg": alg / (ms * 1e-3) / 1e9, "frac_hbm": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+            gbs = alg / (ms * 1e-3) / 1e9
+            kernel = ("qo_groupby_agg, partitioned (k_part_hist + scan + k_part_place + k_groupby_chunks + "
+                      "collect/sort/gather)" if G > 1024 else
+                      "qo_groupby_agg, direct (k_groupby_hash + collect/sort/gather)")
+            row = {"G": G, "z": z, "groups": g, "ms": ms, "tuples_per_s": n / (ms * 1e-3),
+                   "roofline": {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                                "alg_bytes_per_launch": alg, "alg_bytes_formula": "8*N + 28*G",
+                                "avg_launch_ms": ms}}
             for b in outs + [num]:
                 b.free()
+            if not args.no_cpu_baseline:
+                sn = min(n, args.cpu_groupby_sample)
+                k, v = t.key.to_numpy(np.int32, sn), t.val.to_numpy(np.int32, sn)
+                row["cpu_baseline"] = cpu_baseline_groupby(k, v, f"first {sn} of the case's {n} tuples")
+                del k, v
             t.free()
+            rows.append(row)
     worst = min(rows, key=lambda r: r["tuples_per_s"])
-    kernel = ("qo_groupby_agg (partitioned: k_split_hist + k_split_cols + k_split_pairs_exact + k_groupby_chunks"
-              " + compaction/sort/gather)" if worst["G"] > 1024 else "qo_groupby_agg (k_groupby_hash + compaction)")
     return {"metric": METRIC, "value": worst["tuples_per_s"], "unit": "tuples/s (slowest of 6 cases)",
             "ms_per_step": worst["ms"], "dtype": "int32",
             "config": {"workload": "hash group-by (COUNT, SUM, MIN, MAX), key Zipf(z) over G groups", "tuples": n,
                        "cases": rows},
-            "roofline": {"bound": "hbm", "kernel": kernel, "achieved": worst["GBps_alg"],
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": worst["frac_hbm"], "traffic": None,
-                         "alg_bytes_formula": "8*N + 28*G", "avg_launch_ms": worst["ms"]}}
+            "roofline": dict(worst["roofline"], case=f"G={worst['G']} z={worst['z']}"),
+            "cpu_baseline": worst.get("cpu_baseline")}
 
 
 def main() -> int:
@@ -668,8 +804,13 @@ def main() -> int:
     ap.add_argument("--n-filter", type=int, default=1 << 28)
     ap.add_argument("--n-groupby", type=int, default=1 << 28)
     ap.add_argument("--n-sort", type=int, default=1 << 26)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 25)
+    ap.add_argument("--cpu-groupby-sample", type=int, default=1 << 24,
+                    help="group-by CPU baseline: tuples of each case's sample")
+    ap.add_argument("--cpu-batch-queries", type=int, default=16,
+                    help="batched CPU baseline: queries of the sample (the first ones)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="default config: measure only the headline join (no per-config sub-objects)")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per launch of the roofline kernel from rocprofv3 --pmc, if measured")
     args = ap.parse_args()
@@ -699,6 +840,22 @@ def main() -> int:
         print(f"config {args.config} is not available", file=sys.stderr)
         return 2
     res = run[args.config](args, qo, lib)
+    if args.config == "join" and not args.headline_only:
+        # The other configurations, each with its roofline and CPU baseline
+        # (fewer timed steps: every one is a steady-state kernel measurement).
+        sub = argparse.Namespace(**dict(vars(args), steps=min(args.steps, 5), warmup=min(args.warmup, 2)))
+        configs = {}
+        for name, fn in (("filter_agg_2e28", lambda: run_filter_agg(sub, qo, lib)),
+                         ("groupby_2e28", lambda: run_groupby(sub, qo, lib)),
+                         ("batched_512q", lambda: run_batched(sub, qo, lib, 1.0, "|R| log-uniform in [4K, 4M]")),
+                         ("batched_512q_small", lambda: run_batched(sub, qo, lib, 0.4,
+                                                                    "small queries, |R| log-uniform in [4K, 64K]"))):
+            r = fn()
+            r.pop("metric", None)
+            r["steps"], r["warmup"] = sub.steps, sub.warmup
+            configs[name] = r
+            qo._lib.call("qo_release_workspaces")
+        res["configs"] = configs
     line = {
         "metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"),
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),

@@ -35,7 +35,9 @@ typedef enum qo_status {
     QO_ERR_BUDGET = 5,     /* server admission: request can never fit the memory budget */
     QO_ERR_INJECTED = 6,   /* fault injection flag set by qo_debug_inject_fault */
     QO_ERR_NOT_FOUND = 7,
-    QO_ERR_CANCELLED = 8
+    QO_ERR_CANCELLED = 8,
+    QO_ERR_DEVICE_WAIT = 9, /* a kernel's bounded device-side wait ran out (results undefined) */
+    QO_ERR_TIMEOUT = 10     /* a host-side deadline passed before the device finished */
 } qo_status;
 
 /* Aggregate operators, same codes as the reference (host/host/common.h:89-92). */
@@ -70,6 +72,9 @@ int qo_copy(void *dst, const void *src, size_t bytes, qo_stream s);
 int qo_copy_nt(void *dst, const void *src, size_t bytes, qo_stream s);
 int qo_stream_create(qo_stream *s);
 int qo_stream_destroy(qo_stream s);
+/* Both synchronise, then report QO_ERR_DEVICE_WAIT when a kernel's bounded
+ * device-side wait (the look-back of the single-pass scans) ran out since the
+ * last check. */
 int qo_stream_sync(qo_stream s);
 int qo_device_sync(void);
 int qo_event_create(qo_event *e);
@@ -83,6 +88,9 @@ int qo_release_workspaces(void);
 /* Fault injection for failure-handling tests: the next `count` operator calls
  * return QO_ERR_INJECTED before launching anything. */
 int qo_debug_inject_fault(int count);
+/* Sleeps a look-back wave may wait for a predecessor tile before it gives up
+ * (default 2^22, about a second); for tests of the bounded wait. */
+int qo_debug_set_spin_limit(uint32_t sleeps);
 
 /* ---- deterministic synthetic data (identical to the CPU generator spec) ---- */
 /* out[i] = lo + floor(hi32(r_i) * range / 2^32), r_i = splitmix64 stream (seed, stream_id)
@@ -186,6 +194,14 @@ int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, cons
                       size_t nr, const int32_t *skey, const int32_t *spay, const int32_t *sfilt,
                       size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,
                       int32_t *ospay, size_t out_capacity, uint64_t *h_count, qo_stream s);
+/* Order-sensitive digest of m join rows (three int32 columns), on the device:
+ * d_digest[0] = sum of mix(key, r_payload, s_payload, row index), d_digest[1] =
+ * sum of keys (int64, wrapping), d_digest[2] = rows that order before their
+ * predecessor by (key, s_payload) (0 when sorted).  With d_count the row count
+ * is min(*d_count, m).  Two outputs are equal, order included, when their
+ * digests are (up to 64-bit mixing collisions). */
+int qo_join_rows_digest(const int32_t *key, const int32_t *rpay, const int32_t *spay, size_t m,
+                        const uint64_t *d_count, uint64_t *d_digest, qo_stream s);
 
 /* Reference-shaped bucket-chained table (hj_filter_sort_op.cl:25-109): B buckets
  * (power of two) of bucketSize (key, val) slots plus per-bucket counts.  Input
@@ -246,6 +262,8 @@ typedef struct qo_server_config {
     int max_inflight_batches;      /* must be 1: one batch in flight per server */
     int max_batch_queries;         /* requests launched together in one batch */
     int plain;                     /* 1: no batching or overlap (per-query path) */
+    double batch_timeout_ms;       /* host deadline per batch (<= 0: 60000); a batch that
+                                      misses it fails with QO_ERR_TIMEOUT */
 } qo_server_config;
 enum { QO_Q_FILTER = 0, QO_Q_GROUPBY = 1, QO_Q_JOIN_SORT = 2 };
 /* Columns of a resident table (int32 device arrays owned by the caller; a

@@ -40,6 +40,12 @@ _PROTOS = {
     "or_hash_join": (sz, [vp, vp, sz, vp, vp, vp, sz, i32, i32, vp, vp, vp, sz]),
     "or_sort_join_rows": (None, [vp, vp, vp, sz]),
     "or_probe_pairs": (sz, [vp, sz, vp, sz, vp, sz]),
+    # multi-threaded restatement (qo_cpu_par.c): the timed CPU baseline
+    "pr_threads": (C.c_int, [C.c_int]),
+    "pr_filter_agg": (None, [vp, vp, sz, i32, i32, vp, C.c_int]),
+    "pr_groupby_agg": (sz, [vp, vp, sz, vp, vp, vp, vp, vp, C.c_int]),
+    "pr_hash_join": (sz, [vp, vp, sz, vp, vp, vp, sz, i32, i32, vp, vp, vp, sz, C.c_int]),
+    "pr_sort_join_rows": (None, [vp, vp, vp, sz, C.c_int]),
 }
 
 

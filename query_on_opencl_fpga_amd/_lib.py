@@ -79,6 +79,7 @@ PROTOS = {
     "qo_event_elapsed_ms": (C.c_int, [vp, vp, P(C.c_float)]),
     "qo_release_workspaces": (C.c_int, []),
     "qo_debug_inject_fault": (C.c_int, [C.c_int]),
+    "qo_debug_set_spin_limit": (C.c_int, [c_u32]),
     # generators
     "qo_gen_uniform_u32": (C.c_int, [vp, c_size, c_u64, c_u64, c_u32, c_u32, vp]),
     "qo_gen_perm_u32": (C.c_int, [vp, c_size, c_u64, vp]),
@@ -108,6 +109,7 @@ PROTOS = {
     "qo_hash_probe": (C.c_int, [vp, c_size, vp, vp, vp, c_size, c_i32, c_i32, vp, vp, vp, c_size, vp, vp]),
     "qo_hj_filter_sort": (C.c_int, [vp, c_size, vp, vp, c_size, vp, vp, vp, c_size, c_i32, c_i32,
                                     vp, vp, vp, c_size, P(c_u64), vp]),
+    "qo_join_rows_digest": (C.c_int, [vp, vp, vp, c_size, vp, vp, vp]),
     "qo_bucket_build": (C.c_int, [vp, c_u32, vp, c_u32, c_u32, vp, vp]),
     "qo_bucket_probe": (C.c_int, [vp, vp, c_u32, c_u32, vp, c_u32, vp, c_size, vp, vp]),
     # batched

@@ -120,7 +120,7 @@ void scanImpl(qo_mem d_Src, int rLen, qo_mem d_Dst, int *, void *, void *, int *
     }
     int rc = launch_scan_excl_u32(static_cast<const uint32_t *>(d_Src->ptr), static_cast<uint32_t *>(d_Dst->ptr),
                                   (size_t)rLen, ws, st);
-    if (rc == QO_OK && hipStreamSynchronize(st) != hipSuccess) rc = set_error(QO_ERR_HIP, "scanImpl sync");
+    if (rc == QO_OK) rc = sync_and_check(st);
     if (own) qo_free(ws);
     if (rc != QO_OK) compat_fail("scanImpl");
 }
@@ -141,6 +141,7 @@ void filterImpl(qo_mem d_Rin, int /*beginPos: unused by filter_op.cl:6-31,218-23
                                static_cast<qo_record *>((*d_Rout)->ptr), d_count, nullptr);
     uint64_t cnt = 0;
     if (rc == QO_OK) rc = qo_memcpy_d2h(&cnt, d_count, 8);
+    if (rc == QO_OK) rc = check_device_error();
     qo_free(d_count);
     QO_COMPAT_CHECK(rc, "filterImpl", );
     *outSize = (int)cnt;
@@ -238,7 +239,7 @@ int groupByImpl(qo_mem d_Rin, int rLen, qo_mem d_Rout, qo_mem *d_startPos, int, 
     int rc = launch_group_startpos(static_cast<const qo_record *>(d_Rout->ptr), n, startPos, d_num, sws, st);
     if (rc == QO_OK && hipMemcpyAsync(&num, d_num, 8, hipMemcpyDeviceToHost, st) != hipSuccess)
         rc = set_error(QO_ERR_HIP, "groupByImpl readback");
-    if (rc == QO_OK && hipStreamSynchronize(st) != hipSuccess) rc = set_error(QO_ERR_HIP, "groupByImpl sync");
+    if (rc == QO_OK) rc = sync_and_check(st);
     if (rc != QO_OK) {
         qo_free(startPos);
         compat_fail("groupByImpl");
@@ -413,6 +414,7 @@ int probeHashTable(uint32_t *rHashTable, uint32_t *sTableOnDevice, uint32_t *mat
     int rc = qo_bucket_probe(rHashTable, rHashCount, rHashTableBucketNum, hashBucketSize, sTableOnDevice, size,
                              matchedTable, (size_t)size, d_count, nullptr);
     if (rc == QO_OK) rc = qo_memcpy_d2h(&cnt, d_count, 8);
+    if (rc == QO_OK) rc = check_device_error();
     qo_free(d_count);
     if (rc != QO_OK) {
         compat_fail("probeHashTable");

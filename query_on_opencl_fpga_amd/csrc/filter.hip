@@ -215,7 +215,8 @@ constexpr int kFiltThreads = 1024;
 template <int J, int VEC, int NT>
 __global__ __launch_bounds__(NT) void k_filter_records(const qo_record *__restrict__ rin, size_t n, int32_t lo,
                                                        int32_t hi, qo_record *__restrict__ rout, uint64_t *status,
-                                                       uint32_t *ticket, uint64_t *d_count, uint32_t num_tiles) {
+                                                       uint32_t *ticket, uint64_t *d_count, uint32_t num_tiles,
+                                                       DevCtl *ctl) {
     constexpr int NW = NT / kWave;
     constexpr uint32_t kTile = J * NT * VEC;
     using V = typename RecVec<VEC>::T;
@@ -226,6 +227,7 @@ __global__ __launch_bounds__(NT) void k_filter_records(const qo_record *__restri
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
+    if (tile >= num_tiles) return;  // whole block; one ticket per launched block, so never taken
     const size_t ebase = (size_t)tile * kTile;
     const qo_record *src = rin + ebase;
     const uint32_t rem = (uint32_t)min((size_t)kTile, n - ebase);
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(NT) void k_filter_records(const qo_record *__restri
             if (fl[j] & (1u << k)) s_out[pos++] = r[j][k];
     }
     if (threadIdx.x < kWave) {
-        const uint64_t e = lookback(status, tile, total);
+        const uint64_t e = lookback(status, tile, total, ctl);
         if (threadIdx.x == 0) s_excl = e;
     }
     __syncthreads();
@@ -294,10 +296,10 @@ static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int
     QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
     if (vec)
         k_filter_records<J, 2, NT><<<(unsigned)tiles, NT, 0, st>>>(rin, n, lo, hi, rout, stat, ticket, d_count,
-                                                                   (uint32_t)tiles);
+                                                                   (uint32_t)tiles, dev_ctl());
     else
         k_filter_records<J, 1, NT><<<(unsigned)tiles, NT, 0, st>>>(rin, n, lo, hi, rout, stat, ticket, d_count,
-                                                                   (uint32_t)tiles);
+                                                                   (uint32_t)tiles, dev_ctl());
     QO_LAUNCHED();
     return QO_OK;
 }
@@ -310,7 +312,7 @@ constexpr int kScanThreads = 1024;
 
 template <int J, bool VEC, int NT>
 __global__ __launch_bounds__(NT) void k_scan_excl(const uint32_t *in, uint32_t *out, size_t n, uint64_t *status,
-                                                  uint32_t *ticket) {
+                                                  uint32_t *ticket, uint32_t num_tiles, DevCtl *ctl) {
     constexpr int NW = NT / kWave;
     constexpr int VW = VEC ? 4 : 1;
     constexpr uint32_t kTile = J * NT * VW;
@@ -320,6 +322,7 @@ __global__ __launch_bounds__(NT) void k_scan_excl(const uint32_t *in, uint32_t *
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
+    if (tile >= num_tiles) return;  // whole block; one ticket per launched block, so never taken
     const size_t ebase = (size_t)tile * kTile;
     const uint32_t *src = in + ebase;
     uint32_t *dst = out + ebase;
@@ -349,7 +352,7 @@ __global__ __launch_bounds__(NT) void k_scan_excl(const uint32_t *in, uint32_t *
     }
     const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
     if (threadIdx.x < kWave) {
-        const uint64_t e = lookback(status, tile, total);
+        const uint64_t e = lookback(status, tile, total, ctl);
         if (threadIdx.x == 0) s_excl = e;
     }
     __syncthreads();
@@ -394,8 +397,8 @@ int launch_scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, void *ws, 
     uint64_t *stat = cv.take<uint64_t>(tiles);
     uint32_t *ticket = cv.take<uint32_t>(1);
     QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
-    if (vec) k_scan_excl<J, true, NT><<<(unsigned)tiles, NT, 0, st>>>(in, out, n, stat, ticket);
-    else k_scan_excl<J, false, NT><<<(unsigned)tiles, NT, 0, st>>>(in, out, n, stat, ticket);
+    if (vec) k_scan_excl<J, true, NT><<<(unsigned)tiles, NT, 0, st>>>(in, out, n, stat, ticket, (uint32_t)tiles, dev_ctl());
+    else k_scan_excl<J, false, NT><<<(unsigned)tiles, NT, 0, st>>>(in, out, n, stat, ticket, (uint32_t)tiles, dev_ctl());
     QO_LAUNCHED();
     return QO_OK;
 }
@@ -403,13 +406,14 @@ int launch_scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, void *ws, 
 // ---------------------------------------------------------------------------
 namespace {
 std::mutex g_cnt_mu;
-std::map<hipStream_t, uint32_t *> g_cnt;
+std::map<StreamKey, uint32_t *> g_cnt;
 }  // namespace
 
 static uint32_t *sync_counters(hipStream_t st, int *status) {
     *status = QO_OK;
     std::lock_guard<std::mutex> lk(g_cnt_mu);
-    auto it = g_cnt.find(st);
+    const StreamKey key = stream_key_of(st);
+    auto it = g_cnt.find(key);
     if (it != g_cnt.end()) return it->second;
     uint32_t *p = nullptr;
     hipError_t e = hipMalloc(&p, 4096);
@@ -418,7 +422,7 @@ static uint32_t *sync_counters(hipStream_t st, int *status) {
         *status = set_error(QO_ERR_NO_MEMORY, "counter block: %s", hipGetErrorString(e));
         return nullptr;
     }
-    g_cnt[st] = p;
+    g_cnt[key] = p;
     return p;
 }
 

@@ -958,7 +958,7 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
 template <int J, int NT, bool VEC>
 __global__ __launch_bounds__(NT) void k_group_startpos(const qo_record *__restrict__ s, size_t n, int32_t *startPos,
                                                        uint64_t *status, uint32_t *ticket, uint64_t *d_num,
-                                                       uint32_t num_tiles) {
+                                                       uint32_t num_tiles, DevCtl *ctl) {
     constexpr int NW = NT / kWave;
     constexpr int V = VEC ? 2 : 1;
     constexpr uint32_t kTile = J * NT * V;
@@ -968,6 +968,7 @@ __global__ __launch_bounds__(NT) void k_group_startpos(const qo_record *__restri
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
+    if (tile >= num_tiles) return;  // whole block; one ticket per launched block, so never taken
     const size_t ebase = (size_t)tile * kTile;
     const qo_record *src = s + ebase;
     const uint32_t rem = (uint32_t)min((size_t)kTile, n - ebase);
@@ -996,7 +997,7 @@ __global__ __launch_bounds__(NT) void k_group_startpos(const qo_record *__restri
     }
     const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
     if (threadIdx.x < kWave) {
-        const uint64_t e = lookback(status, tile, total);
+        const uint64_t e = lookback(status, tile, total, ctl);
         if (threadIdx.x == 0) s_excl = e;
     }
     __syncthreads();
@@ -1181,10 +1182,10 @@ int launch_group_startpos(const qo_record *sorted, size_t n, int32_t *startPos, 
     QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
     if (vec)
         k_group_startpos<J, NT, true><<<(unsigned)tiles, NT, 0, st>>>(sorted, n, startPos, stat, ticket, d_num,
-                                                                      (uint32_t)tiles);
+                                                                      (uint32_t)tiles, dev_ctl());
     else
         k_group_startpos<J, NT, false><<<(unsigned)tiles, NT, 0, st>>>(sorted, n, startPos, stat, ticket, d_num,
-                                                                       (uint32_t)tiles);
+                                                                       (uint32_t)tiles, dev_ctl());
     QO_LAUNCHED();
     return QO_OK;
 }

@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_bucket_probe(const uint32_t *__restrict
                                                       uint32_t bucketSize, const uint32_t *__restrict__ sTable,
                                                       uint32_t size, uint32_t *__restrict__ matched, size_t cap_pairs,
                                                       uint64_t *status, uint32_t *ticket, uint64_t *d_count,
-                                                      uint32_t num_tiles) {
+                                                      uint32_t num_tiles, DevCtl *ctl) {
     constexpr int NW = 4;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_tot[tile_offsets_words(J, NW)];
@@ -321,6 +321,7 @@ __global__ __launch_bounds__(256) void k_bucket_probe(const uint32_t *__restrict
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
+    if (tile >= num_tiles) return;  // whole block; one ticket per launched block, so never taken
     const size_t base_i = (size_t)tile * (J * 256);
     uint2 kv[J];
     uint32_t c[J], ex[J];
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(256) void k_bucket_probe(const uint32_t *__restrict
     }
     const uint32_t total = tile_offsets<J, NW>(c, ex, s_tot);
     if (threadIdx.x < kWave) {
-        uint64_t e = lookback(status, tile, total);
+        uint64_t e = lookback(status, tile, total, ctl);
         if (threadIdx.x == 0) s_excl = e;
     }
     __syncthreads();
@@ -524,7 +525,7 @@ int qo_bucket_probe(const uint32_t *rHashTable, const uint32_t *rHashCount, uint
     QO_TRY(launch_fill_u32(ws, 0, cv.off / 4, st));
     k_bucket_probe<J><<<(unsigned)tiles, 256, 0, st>>>(rHashTable, rHashCount, B, bucketSize, sTable, size,
                                                        matchedTable, capacity_pairs, stat, ticket, d_count,
-                                                       (uint32_t)tiles);
+                                                       (uint32_t)tiles, dev_ctl());
     QO_LAUNCHED();
     return QO_OK;
 }

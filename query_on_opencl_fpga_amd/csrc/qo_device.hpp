@@ -18,6 +18,13 @@ QO_DEV uint64_t lanemask_lt() { return (uint64_t(1) << lane_id()) - 1; }
 QO_DEV uint64_t ballot(bool p) { return __ballot(p); }
 QO_DEV unsigned popc(uint64_t m) { return __popcll(m); }
 
+// Every lane of the wave finishes the statements before this point before any
+// lane runs the ones after it.  The lanes of a wave run in lockstep, so this
+// emits no instruction; it keeps the compiler from moving memory operations
+// across it and marks code that relies on the lockstep (a lane reading a word
+// that another lane of its wave writes next).
+QO_DEV void wave_lockstep() { __builtin_amdgcn_wave_barrier(); }
+
 template <class T>
 QO_DEV T wave_incl_scan(T v) {
 #pragma unroll
@@ -146,12 +153,25 @@ constexpr uint64_t kFlagAgg = uint64_t(1) << 62;
 constexpr uint64_t kFlagInc = uint64_t(2) << 62;
 constexpr uint64_t kValMask = (uint64_t(1) << 62) - 1;
 
+// Library-wide device control block (one per device, allocated at init):
+// kernels with a bounded wait read the bound from it and report a wait that
+// ran out into `error` (the host turns that into QO_ERR_DEVICE_WAIT).
+struct DevCtl {
+    uint32_t error;       // kDevErr* bits, sticky until the host reads them
+    uint32_t spin_limit;  // sleeps a look-back may wait before giving up
+    uint32_t pad[62];
+};
+constexpr uint32_t kDevErrLookback = 1u;
+constexpr uint32_t kDefaultSpinLimit = 1u << 22;  // s_sleep(1) rounds: about a second
+
 // Called by ONE whole wave of the block that owns `tile`.  Publishes the tile
 // aggregate, accumulates predecessors 64 at a time and publishes the inclusive
 // prefix.  Returns the exclusive prefix (sum of all earlier tiles, modulo 2^62).
 // Tiles must be numbered by a ticket counter, so every predecessor is already
-// resident and the spin always ends.
-QO_DEV uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t aggregate) {
+// resident and the wait ends; it is bounded all the same: after
+// ctl->spin_limit sleeps the wave sets kDevErrLookback in ctl->error and
+// returns 0 (the call's results are then undefined, and the kernel ends).
+QO_DEV uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t aggregate, DevCtl *ctl) {
     const unsigned lane = lane_id();
     if (tile == 0) {
         if (lane == 0) st_agent(&status[0], kFlagInc | (aggregate & kValMask));
@@ -160,6 +180,8 @@ QO_DEV uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t aggregate) {
     if (lane == 0) st_agent(&status[tile], kFlagAgg | (aggregate & kValMask));
     uint64_t excl = 0;
     int64_t base = (int64_t)tile - 1;
+    const uint32_t limit = ctl->spin_limit;
+    uint32_t sleeps = 0;  // wave-uniform: the loop's branches depend on ballots only
     while (true) {
         int64_t idx = base - (int64_t)lane;
         uint64_t st = (idx >= 0) ? ld_agent(&status[idx]) : kFlagInc;
@@ -169,6 +191,10 @@ QO_DEV uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t aggregate) {
         unsigned first_inc = inc_mask ? (unsigned)__builtin_ctzll(inc_mask) : 64u;
         uint64_t relevant = first_inc >= 63 ? ~uint64_t(0) : ((uint64_t(2) << first_inc) - 1);
         if (empty_mask & relevant) {
+            if (++sleeps > limit) {
+                if (lane == 0) atomicOr(&ctl->error, kDevErrLookback);
+                return 0;
+            }
             __builtin_amdgcn_s_sleep(1);
             continue;
         }

@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <thread>
 
 #include "../../include/qo.h"
 
@@ -25,8 +26,28 @@ struct DeviceProps {
 const DeviceProps &props();
 int ensure_init();
 
+struct DevCtl;
+// The device control block of the current device (bounded-wait limit and the
+// device error word).
+DevCtl *dev_ctl();
+// Reads and clears the device error word; QO_ERR_DEVICE_WAIT when a kernel's
+// bounded wait ran out since the last check.  Call after synchronising the
+// work in question.
+int check_device_error();
+// hipStreamSynchronize(st) followed by check_device_error().
+int sync_and_check(hipStream_t st);
+
 // The stream to launch on: a caller stream, or the library's per-thread stream.
 hipStream_t stream_of(qo_stream s);
+
+// Key of per-stream library state (workspaces, counters): the stream, and for
+// hipStreamPerThread also the calling thread.
+struct StreamKey {
+    hipStream_t s;
+    std::thread::id t;
+    bool operator<(const StreamKey &o) const { return s != o.s ? s < o.s : t < o.t; }
+};
+StreamKey stream_key_of(hipStream_t s);
 
 // Per-stream grow-only workspace.  Operations on one stream are ordered by the
 // stream, so one workspace per stream is never used by two operations at once.

@@ -8,6 +8,7 @@
 #include <string>
 #include <thread>
 
+#include "qo_device.hpp"
 #include "qo_internal.hpp"
 
 namespace qo {
@@ -18,12 +19,21 @@ std::atomic<int> g_inject{0};
 std::mutex g_mu;
 bool g_init = false;
 DeviceProps g_props;
+DevCtl *g_ctl = nullptr;
 struct Ws {
     void *ptr = nullptr;
     size_t bytes = 0;
 };
-std::map<hipStream_t, Ws> g_ws;
+std::map<StreamKey, Ws> g_ws;
 }  // namespace
+
+StreamKey stream_key_of(hipStream_t s) {
+    // hipStreamPerThread is one handle for every host thread, but each thread
+    // launches on a device stream of its own: state bound to that handle is
+    // kept per thread, so two threads never share a workspace or a counter.
+    if (s == hipStreamPerThread) return StreamKey{s, std::this_thread::get_id()};
+    return StreamKey{s, std::thread::id()};
+}
 
 int set_error(int code, const char *fmt, ...) {
     char buf[1024];
@@ -56,8 +66,16 @@ static int init_locked(int device) {
     g_props.cu_count = p.multiProcessorCount;
     g_props.lds_per_block = (int)p.sharedMemPerBlock;
     g_props.hbm_bytes = p.totalGlobalMem;
-    snprintf(g_props.name, sizeof(g_props.name), "%s", p.name);
     snprintf(g_props.arch, sizeof(g_props.arch), "%s", p.gcnArchName);
+    // Some drivers report an empty marketing name: fall back to the ISA.
+    if (p.name[0]) snprintf(g_props.name, sizeof(g_props.name), "%s", p.name);
+    else snprintf(g_props.name, sizeof(g_props.name), "AMD %s (%d CUs)", p.gcnArchName, p.multiProcessorCount);
+    if (!g_ctl) {
+        QO_HIP(hipMalloc(&g_ctl, sizeof(DevCtl)));
+        DevCtl init{};
+        init.spin_limit = kDefaultSpinLimit;
+        QO_HIP(hipMemcpy(g_ctl, &init, sizeof(DevCtl), hipMemcpyHostToDevice));
+    }
     g_init = true;
     return QO_OK;
 }
@@ -76,6 +94,26 @@ int ensure_init() {
 
 const DeviceProps &props() { return g_props; }
 
+DevCtl *dev_ctl() { return g_ctl; }
+
+int check_device_error() {
+    if (!g_ctl) return QO_OK;
+    uint32_t err = 0;
+    QO_HIP(hipMemcpy(&err, &g_ctl->error, sizeof(err), hipMemcpyDeviceToHost));
+    if (!err) return QO_OK;
+    const uint32_t zero = 0;
+    QO_HIP(hipMemcpy(&g_ctl->error, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    return set_error(QO_ERR_DEVICE_WAIT,
+                     "a device-side wait ran out of its bound (error bits 0x%x): results of the calls since the "
+                     "last check are undefined",
+                     err);
+}
+
+int sync_and_check(hipStream_t st) {
+    QO_HIP(hipStreamSynchronize(st));
+    return check_device_error();
+}
+
 hipStream_t stream_of(qo_stream s) {
     if (s) return static_cast<hipStream_t>(s);
     return hipStreamPerThread;
@@ -85,7 +123,7 @@ void *workspace(hipStream_t s, size_t bytes, int *status) {
     *status = QO_OK;
     if (bytes == 0) bytes = 256;
     std::lock_guard<std::mutex> lk(g_mu);
-    Ws &w = g_ws[s];
+    Ws &w = g_ws[stream_key_of(s)];
     if (w.bytes >= bytes) return w.ptr;
     if (w.ptr) {
         // The old workspace may still be read by queued work on this stream.
@@ -309,7 +347,7 @@ int qo_stream_destroy(qo_stream s) {
     hipStream_t st = static_cast<hipStream_t>(s);
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        auto it = g_ws.find(st);
+        auto it = g_ws.find(stream_key_of(st));
         if (it != g_ws.end()) {
             (void)hipStreamSynchronize(st);
             if (it->second.ptr) (void)hipFree(it->second.ptr);
@@ -322,13 +360,19 @@ int qo_stream_destroy(qo_stream s) {
 
 int qo_stream_sync(qo_stream s) {
     QO_TRY(ensure_init());
-    QO_HIP(hipStreamSynchronize(stream_of(s)));
-    return QO_OK;
+    return sync_and_check(stream_of(s));
 }
 
 int qo_device_sync(void) {
     QO_TRY(ensure_init());
     QO_HIP(hipDeviceSynchronize());
+    return check_device_error();
+}
+
+int qo_debug_set_spin_limit(uint32_t sleeps) {
+    QO_TRY(ensure_init());
+    QO_HIP(hipDeviceSynchronize());
+    QO_HIP(hipMemcpy(&g_ctl->spin_limit, &sleeps, sizeof(sleeps), hipMemcpyHostToDevice));
     return QO_OK;
 }
 

@@ -223,6 +223,7 @@ struct qo_server {
     hipStream_t stream = nullptr;
     ResSlot *h_res = nullptr;  // pinned, max_batch_queries slots
     DevPool pool;
+    bool stuck = false;        // a batch missed its deadline; the stream may still hold work
 
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -503,6 +504,25 @@ static std::vector<std::shared_ptr<Req>> take_batch(qo_server *sv, double t, siz
     return b;
 }
 
+// Waits for the server stream, polling it until the batch deadline (server
+// clock ms since the batch started at t0).  A batch that misses it marks the
+// server stuck: its stream still holds work, so later batches fail at once.
+static int wait_stream(qo_server *sv, double t0) {
+    const double limit = sv->cfg.batch_timeout_ms > 0 ? sv->cfg.batch_timeout_ms : 60000.0;
+    int pause_us = 20;
+    for (;;) {
+        const hipError_t e = hipStreamQuery(sv->stream);
+        if (e == hipSuccess) return QO_OK;
+        if (e != hipErrorNotReady) return set_error(QO_ERR_HIP, "server stream: %s", hipGetErrorString(e));
+        if (sv->now() - t0 > limit) {
+            sv->stuck = true;
+            return set_error(QO_ERR_TIMEOUT, "batch did not finish within %.0f ms", limit);
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(pause_us));
+        pause_us = std::min(pause_us * 2, 1000);
+    }
+}
+
 // Serves one batch of what has arrived by now; returns false when nothing has.
 static bool serve_once(qo_server *sv) {
     size_t bytes = 0;
@@ -522,14 +542,18 @@ static bool serve_once(qo_server *sv) {
     b.reqs = std::move(reqs);
     b.est = bytes;
     b.h_res = sv->h_res;
-    int rc = launch_batch(sv, b, sv->stream);
+    int rc = sv->stuck ? set_error(QO_ERR_TIMEOUT, "an earlier batch never finished on the device")
+                       : launch_batch(sv, b, sv->stream);
     // Every launched kernel and copy has finished before results are read or
-    // the arena is reused, also after a launch error.
-    const hipError_t e = hipStreamSynchronize(sv->stream);
-    if (rc == QO_OK && e != hipSuccess) rc = set_error(QO_ERR_HIP, "server stream: %s", hipGetErrorString(e));
+    // the arena is reused, also after a launch error; the wait is bounded by
+    // the batch deadline.
+    const int w = wait_stream(sv, t);
+    if (rc == QO_OK) rc = w;
+    if (w == QO_OK && rc == QO_OK) rc = check_device_error();
     const double t1 = sv->now();
     for (size_t i = 0; i < b.reqs.size(); ++i) finish(sv, *b.reqs[i], rc == QO_OK ? b.h_res[i] : ResSlot{}, rc, t1);
-    sv->pool.put(b.arena);
+    // After a timeout the device may still use the arena: it is never reused.
+    if (w == QO_OK) sv->pool.put(b.arena);
     return true;
 }
 

@@ -346,6 +346,7 @@ __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict_
         const uint64_t m = match_digit(d, valid);
         const uint32_t pre = s_cnt[w][d];
         const uint32_t r = popc(m & lanemask_lt());
+        wave_lockstep();  // every lane has read its digit's count before the leaders advance it
         if (valid && r == 0) s_cnt[w][d] = pre + popc(m);
         rank[j] = pre + r;
     }

@@ -44,7 +44,7 @@ class Result(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget_bytes", C.c_uint64), ("max_inflight_batches", C.c_int),
-                ("max_batch_queries", C.c_int), ("plain", C.c_int)]
+                ("max_batch_queries", C.c_int), ("plain", C.c_int), ("batch_timeout_ms", C.c_double)]
 
 
 _PROTOS = {
@@ -74,9 +74,10 @@ def _ptr(b):
 
 
 class Server:
-    def __init__(self, budget_bytes: float = 200e9, inflight: int = 1, batch: int = 64, plain: bool = False):
+    def __init__(self, budget_bytes: float = 200e9, inflight: int = 1, batch: int = 64, plain: bool = False,
+                 timeout_ms: float = 60000.0):
         self.lib = _lib_server()
-        cfg = ServerConfig(int(budget_bytes), int(inflight), int(batch), 1 if plain else 0)
+        cfg = ServerConfig(int(budget_bytes), int(inflight), int(batch), 1 if plain else 0, float(timeout_ms))
         h = C.c_void_p()
         _lib.call("qo_server_create", C.byref(cfg), C.byref(h))
         self.handle = h.value

@@ -1,0 +1,42 @@
+"""CPU tests that run libqo's own kernels on the CPU build (tests/cpu_sim): the
+csrc sources compiled for the host with work-items as fibers, so host planning
+and kernel logic are exercised without a GPU.
+
+* runtime_checks: per-thread workspaces of the NULL stream; the bounded
+  look-back wait raising QO_ERR_DEVICE_WAIT when its predecessor never publishes.
+* serve_mix: the serving tests' four tables (scaled to 2%) and 24 mixed requests
+  in batches of 8 through the serving loop, every result equal to the same
+  request run alone on the plain path.
+"""
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+HERE = Path(__file__).resolve().parent / "cpu_sim"
+
+
+@pytest.fixture(scope="module")
+def cpu_build():
+    env = dict(os.environ, SIMT_VARIANT="fast")
+    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks"], check=True, env=env,
+                   timeout=900)
+    return HERE / "_build" / "fast"
+
+
+def _run(exe, *args, timeout=600):
+    r = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert r.stdout.strip().endswith("PASS"), r.stdout[-4000:]
+    return r.stdout
+
+
+def test_runtime_checks(cpu_build):
+    _run(cpu_build / "runtime_checks")
+
+
+def test_served_mix_equals_alone_cpu(cpu_build):
+    out = _run(cpu_build / "serve_mix", 0.02, 24)
+    assert out.count(" ok") == 24
