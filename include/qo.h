@@ -155,6 +155,15 @@ int qo_aggregate_groups(const qo_record *sorted, size_t n, const int32_t *startP
 int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity,
                    int32_t *okey, int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax,
                    uint64_t *d_numGroups, qo_stream s);
+/* Filtered group-by aggregation (the filter -> group-by -> aggregate chain of
+ * host/host/testAggAfterGB.cpp:313-427 and kernel/filter_aggregation): only
+ * rows with lo <= filt[i] <= hi are grouped and aggregated, the predicate
+ * evaluated inside the group-by kernels (no compaction pass).  filt may be the
+ * key column itself (the reference filters Record.y, the group key).  Same
+ * outputs and capacity rules as qo_groupby_agg. */
+int qo_groupby_agg_filtered(const int32_t *key, const int32_t *val, const int32_t *filt, size_t n, int32_t lo,
+                            int32_t hi, size_t group_capacity, int32_t *okey, int64_t *ocount, int64_t *osum,
+                            int32_t *omin, int32_t *omax, uint64_t *d_numGroups, qo_stream s);
 
 /* ---- hash join ------------------------------------------------------------ */
 /* Bytes of device memory a build table for `build_rows` rows needs. */
@@ -238,11 +247,24 @@ int qo_batch_destroy(qo_batch *b);
  * through LDS slices), 6 partitioned group-by (items =
  * groups per fine bucket, param = 0 auto / 1 always / 2 never), 7 key-range join
  * table (items = largest such table in MiB, param = 0 when the keys allow /
- * 2 never).  Size class = floor(log2 n). */
+ * 2 never), 8 batched join (class of the batch's mean |R|; items = S tiles per
+ * probe ticket, param = R rows per build tile).  Size class = floor(log2 n).
+ * An operator's class with no entry takes the nearest class that has one, and
+ * an operator with none its built-in default.  qo_init loads the tune file
+ * tune/<arch>.tune beside libqo.so (or $QO_TUNE_FILE; "none" loads nothing). */
 int qo_tune_get(int op, int size_class, int *block, int *items, int *param);
 int qo_tune_set(int op, int size_class, int block, int items, int param);
 int qo_tune_load(const char *path);
 int qo_tune_save(const char *path);
+/* Clears every entry (all operators back to their built-in defaults). */
+int qo_tune_reset(void);
+/* Searches operator `op`'s launch parameters at size class `size_class` on the
+ * device: builds a synthetic workload of that size in HBM, times the built-in
+ * default and every candidate (median of `reps` runs after a warm-up, HIP
+ * events) and sets the fastest into the table.  Tunable operators: 0, 2, 3, 6
+ * and 8.  Outputs the chosen parameters and both times (ms). */
+int qo_autotune(int op, int size_class, int reps, int *block, int *items, int *param, double *best_ms,
+                double *default_ms);
 
 /* ---- serving loop ------------------------------------------------------------ */
 /* A request loop over tables resident in HBM, run on the caller's thread by

@@ -43,11 +43,18 @@ void qo_mem_free(qo_mem m);
 int qo_mem_write(qo_mem m, const void *host, size_t bytes);   /* cl_writebuffer, common.cpp:93 */
 int qo_mem_read(void *host, qo_mem m, size_t bytes);          /* cl_readbuffer, common.cpp:64 */
 
-/* Scan plan (host/host/testScan.cpp:26-75, 229-270). */
+/* Scan plan (host/host/testScan.h:4-12; testScan.cpp:26-75, 229-270).  The
+ * reference's fields come first, in its order; initScan fills pass, gLength
+ * and blockSize as the reference does, while outputBuffer, blockSumBuffer and
+ * tempBuffer stay NULL (the single-pass scan needs no per-level buffers) and
+ * `ws` holds its workspace. */
 typedef struct ScanPara {
-    int gLength;
-    int blockSize;
-    int pass;
+    qo_mem *outputBuffer;    /* always NULL here */
+    qo_mem *blockSumBuffer;  /* always NULL here */
+    qo_mem tempBuffer;       /* always NULL here */
+    unsigned pass;
+    unsigned gLength;
+    unsigned blockSize;
     void *ws;
 } ScanPara;
 void initScan(int rLen, ScanPara *SP);
@@ -55,6 +62,11 @@ void closeScan(ScanPara *SP);
 /* Exclusive int32 prefix sum of d_Src into d_Dst (testScan.cpp:278). */
 void scanImpl(qo_mem d_Src, int rLen, qo_mem d_Dst, int *index, void *eventList, void *kernel,
               int *Flag_CPU_GPU, double *burden, ScanPara *SP, int _CPU_GPU);
+/* The offset overload scanImpl(d_Src, offset, rLen, d_Dst, ...) of testScan.h:16
+ * (declared there, defined nowhere in the reference) under a distinct C name:
+ * scans elements [offset, offset + rLen) of d_Src into d_Dst[0, rLen). */
+void scanImpl_offset(qo_mem d_Src, int offset, int rLen, qo_mem d_Dst, int *index, void *eventList, void *kernel,
+                     int *Flag_CPU_GPU, double *burden, ScanPara *SP, int _CPU_GPU);
 
 /* Selection (host/host/testFilter.cpp:92, host/host/Selection.cpp:6,19).
  * *d_Rout is allocated by the callee (rLen Records) and freed by the caller. */
@@ -72,6 +84,14 @@ int CL_RangeSelection(Record *h_Rin, int rLen, int rangeSmallKey, int rangeLarge
 void radixSortImpl_sm(qo_mem d_R, int rLen, int meaningless, int numThreadPB, int numBlock, int *index,
                       void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
 void CL_RadixSort(Record *h_Rin, int rLen, Record *h_Rout, int numThread, int numBlock, int _CPU_GPU);
+/* radixSortImpl (host/host/testSort.cpp:274): the same sort as radixSortImpl_sm. */
+void radixSortImpl(qo_mem d_R, int rLen, int keybits, int numThreadPB, int numBlock, int *index, void *eventList,
+                   void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+/* CL_RadixSortOnly (host/host/Sort.cpp:11): sorts the device buffer in place. */
+void CL_RadixSortOnly(qo_mem d_Rin, int rLen, int numThread, int numBlock, int _CPU_GPU);
+/* CL_BitonicSortOnly (host/host/Sort.cpp:20): as in the reference, copies d_Rin
+ * into d_Rout first and then sorts d_Rin in place (d_Rout keeps the input). */
+void CL_BitonicSortOnly(qo_mem d_Rin, int rLen, qo_mem d_Rout, int numThread, int numBlock, int _CPU_GPU);
 
 /* Group-by (host/host/testGroupBy.cpp:81, host/host/GroupBy.cpp:7).  groupByImpl
  * sorts d_Rout in place (after copying d_Rin into it when they differ) and
@@ -93,6 +113,20 @@ void parallelAggregate_init(qo_mem d_S, qo_mem d_startPos, qo_mem d_aggResults, 
 void aggAfterGroupByImpl(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
                          qo_mem d_aggResults, int OPERATOR, int numThread, int *index, void *eventList,
                          void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+/* Device-buffer forms (host/host/AggAfterGroupBy.cpp:9-24): aggAfterGroupByImpl
+ * with REDUCE_MAX / MIN / SUM / AVERAGE. */
+void agg_max_afterGroupBy(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                          qo_mem d_aggResults, int numThread, int *index, void *eventList, void *kernel,
+                          int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+void agg_min_afterGroupBy(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                          qo_mem d_aggResults, int numThread, int *index, void *eventList, void *kernel,
+                          int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+void agg_sum_afterGroupBy(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                          qo_mem d_aggResults, int numThread, int *index, void *eventList, void *kernel,
+                          int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+void agg_avg_afterGroupBy(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                          qo_mem d_aggResults, int numThread, int *index, void *eventList, void *kernel,
+                          int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
 void CL_agg_max_afterGroupBy(Record *h_Rin, int rLen, int *h_startPos, int numGroups, Record *h_Ragg,
                              int *h_aggResults, int numThread, int _CPU_GPU);
 void CL_agg_min_afterGroupBy(Record *h_Rin, int rLen, int *h_startPos, int numGroups, Record *h_Ragg,

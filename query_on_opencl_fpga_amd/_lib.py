@@ -48,7 +48,9 @@ class QoJoinQuery(C.Structure):
 
 
 class ScanPara(C.Structure):
-    _fields_ = [("gLength", C.c_int), ("blockSize", C.c_int), ("pass_", C.c_int), ("ws", vp)]
+    # host/host/testScan.h:4-12 field order, then the single-pass scan's workspace
+    _fields_ = [("outputBuffer", vp), ("blockSumBuffer", vp), ("tempBuffer", vp), ("pass_", C.c_uint),
+                ("gLength", C.c_uint), ("blockSize", C.c_uint), ("ws", vp)]
 
 
 # name -> (restype, argtypes).  restype C.c_int means a qo_status.
@@ -101,6 +103,7 @@ PROTOS = {
     "qo_aggregate_groups": (C.c_int, [vp, c_size, vp, c_size, C.c_int, vp, vp]),
     # hash group-by
     "qo_groupby_agg": (C.c_int, [vp, vp, c_size, c_size, vp, vp, vp, vp, vp, vp, vp]),
+    "qo_groupby_agg_filtered": (C.c_int, [vp, vp, vp, c_size, c_i32, c_i32, c_size, vp, vp, vp, vp, vp, vp, vp]),
     # join
     "qo_join_table_bytes": (c_size, [c_size]),
     "qo_join_strategy": (C.c_int, [c_size, c_size, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
@@ -122,6 +125,9 @@ PROTOS = {
     "qo_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "qo_tune_load": (C.c_int, [C.c_char_p]),
     "qo_tune_save": (C.c_int, [C.c_char_p]),
+    "qo_tune_reset": (C.c_int, []),
+    "qo_autotune": (C.c_int, [C.c_int, C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_double),
+                              P(C.c_double)]),
     # server prototypes: serving.py
     # compat (reference-named)
     "qo_mem_create": (vp, [c_size]),
@@ -134,6 +140,7 @@ PROTOS = {
     "initScan": (None, [C.c_int, P(ScanPara)]),
     "closeScan": (None, [P(ScanPara)]),
     "scanImpl": (None, [vp, C.c_int, vp, vp, vp, vp, vp, vp, P(ScanPara), C.c_int]),
+    "scanImpl_offset": (None, [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, P(ScanPara), C.c_int]),
     "filterImpl": (None, [vp, C.c_int, C.c_int, P(vp), P(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int,
                           vp, vp, vp, vp, vp, C.c_int]),
     "range_selection": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(vp), C.c_int, C.c_int,
@@ -141,12 +148,19 @@ PROTOS = {
     "CL_RangeSelection": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(vp), C.c_int, C.c_int, C.c_int]),
     "radixSortImpl_sm": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
     "CL_RadixSort": (None, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int]),
+    "radixSortImpl": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
+    "CL_RadixSortOnly": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "CL_BitonicSortOnly": (None, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int]),
     "groupByImpl": (C.c_int, [vp, C.c_int, vp, P(vp), C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
     "CL_GroupBy": (C.c_int, [vp, C.c_int, vp, P(vp), C.c_int, C.c_int, C.c_int]),
     "parallelAggregate_init": (None, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_int, vp, vp, vp, vp, vp, C.c_int]),
     "aggAfterGroupByImpl": (None, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, C.c_int,
                                    vp, vp, vp, vp, vp, C.c_int]),
+    "agg_max_afterGroupBy": (None, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
+    "agg_min_afterGroupBy": (None, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
+    "agg_sum_afterGroupBy": (None, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
+    "agg_avg_afterGroupBy": (None, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
     "CL_agg_max_afterGroupBy": (None, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, C.c_int]),
     "CL_agg_min_afterGroupBy": (None, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, C.c_int]),
     "CL_agg_sum_afterGroupBy": (None, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, C.c_int]),

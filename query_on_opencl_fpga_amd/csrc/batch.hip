@@ -69,12 +69,13 @@ __global__ void k_b_init(const BQuery *q, int nq) {
 
 #define QO_TILE_LOOP_END }
 
-__global__ __launch_bounds__(256) void k_b_build(const BQuery *q, int nq, uint32_t total, uint32_t *ticket) {
+__global__ __launch_bounds__(256) void k_b_build(const BQuery *q, int nq, uint32_t total, uint32_t *ticket,
+                                                 uint32_t build_tile) {
     QO_TILE_LOOP(ticket, total, t)
     const int qi = query_of<0>(q, nq, t);
     const BQuery &Q = q[qi];
-    const uint64_t r0 = (uint64_t)(t - Q.rtile0) * kBuildTile;
-    const uint64_t r1 = r0 + kBuildTile < Q.nr ? r0 + kBuildTile : Q.nr;
+    const uint64_t r0 = (uint64_t)(t - Q.rtile0) * build_tile;
+    const uint64_t r1 = r0 + build_tile < Q.nr ? r0 + build_tile : Q.nr;
     uint64_t *slots = reinterpret_cast<uint64_t *>(Q.hdr + 1);
     for (uint64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) join_insert(slots, Q.table_cap, Q.hdr, Q.rkey[i], Q.rpay[i]);
     QO_TILE_LOOP_END
@@ -92,21 +93,19 @@ struct PackedEmit {
     }
 };
 
-// One ticket covers kProbeTicketTiles consecutive S tiles (of any queries):
-// fewer same-address ticket atomics for a batch of ~10^6 tiles.  The launch
-// bound asks for 3 waves per SIMD (the 48 KB output stage allows no more): at
-// 184 VGPRs the kernel ran 2 and took 48.3 ms per 512-query batch, with the
-// bound (168 VGPRs, a few spilled) 39.7 ms.
-constexpr uint32_t kProbeTicketTiles = 4;
-
+// One ticket covers `group` consecutive S tiles (of any queries): fewer
+// same-address ticket atomics for a batch of ~10^6 tiles.  The launch bound
+// asks for 3 waves per SIMD (the 48 KB output stage allows no more): at 184
+// VGPRs the kernel ran 2 and took 48.3 ms per 512-query batch, with the bound
+// (168 VGPRs, a few spilled) 39.7 ms.
 __global__ __launch_bounds__(256, 3) void k_b_probe(const BQuery *q, int nq, uint32_t total, uint32_t *ticket,
-                                                 uint64_t *mkeys, uint32_t *mvals) {
+                                                 uint64_t *mkeys, uint32_t *mvals, uint32_t group) {
     __shared__ uint32_t s_tot[tile_offsets_words(kProbeJ, 4)];
     __shared__ uint64_t s_base;
     __shared__ JoinStage stage;
-    const uint32_t groups = (total + kProbeTicketTiles - 1) / kProbeTicketTiles;
+    const uint32_t groups = (total + group - 1) / group;
     QO_TILE_LOOP(ticket, groups, tg)
-    for (uint32_t t = tg * kProbeTicketTiles; t < total && t < (tg + 1) * kProbeTicketTiles; ++t) {
+    for (uint32_t t = tg * group; t < total && t < (tg + 1) * group; ++t) {
     const int qi = query_of<1>(q, nq, t);
     const BQuery &Q = q[qi];
     const uint64_t *slots = reinterpret_cast<const uint64_t *>(Q.hdr + 1);
@@ -135,7 +134,14 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
     p.houts.assign(nq, RowOut{});
     p.total_slots = p.total_match = 0;
     uint64_t rt = 0, stt = 0, mt = 0;
-    const int load = std::max(10, std::min(90, tune_get(TUNE_PROBE, 1 << 20).param));
+    // Launch parameters by the size class of the batch's mean build side.
+    uint64_t rsum = 0;
+    for (int i = 0; i < nq; ++i) rsum += queries[i].nr;
+    const size_t rmean = std::max<size_t>(1, (size_t)(rsum / (uint64_t)nq));
+    const int load = std::max(10, std::min(90, tune_get(TUNE_PROBE, rmean).param));
+    const TuneParams tb = tune_get(TUNE_BATCH, rmean);
+    p.build_tile = (uint32_t)std::max(256, std::min(1 << 16, tb.param));
+    p.probe_group = (uint32_t)std::max(1, std::min(64, tb.items));
     for (int i = 0; i < nq; ++i) {
         const qo_join_query &u = queries[i];
         QO_REQUIRE((u.nr == 0 || (u.rkey && u.rpay)) && (u.ns == 0 || (u.skey && u.spay)),
@@ -153,7 +159,7 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
         q.vec = ((uintptr_t)u.skey % 16 == 0 && (uintptr_t)u.spay % 16 == 0 &&
                  (!u.sfilt || (uintptr_t)u.sfilt % 16 == 0)) ? 1u : 0u;
         q.rtile0 = (uint32_t)rt;
-        q.nrtile = (uint32_t)((u.nr + kBuildTile - 1) / kBuildTile);
+        q.nrtile = (uint32_t)((u.nr + p.build_tile - 1) / p.build_tile);
         q.stile0 = (uint32_t)stt;
         q.nstile = (uint32_t)((u.ns + kProbeTile - 1) / kProbeTile);
         q.mtile0 = (uint32_t)mt;
@@ -231,12 +237,12 @@ int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
     QO_LAUNCHED();
     uint32_t *tickets = p.small;
     if (p.rt) {
-        k_b_build<<<std::min<unsigned>(p.rt, persist), 256, 0, st>>>(p.d_q, p.nq, p.rt, tickets + 0);
+        k_b_build<<<std::min<unsigned>(p.rt, persist), 256, 0, st>>>(p.d_q, p.nq, p.rt, tickets + 0, p.build_tile);
         QO_LAUNCHED();
     }
     if (p.stt) {
         k_b_probe<<<std::min<unsigned>(p.stt, persist), 256, 0, st>>>(p.d_q, p.nq, p.stt, tickets + 1, p.mkeys,
-                                                                      p.mvals);
+                                                                      p.mvals, p.probe_group);
         QO_LAUNCHED();
     }
     if (p.mt) {

@@ -91,9 +91,23 @@ int qo_mem_read(void *host, qo_mem m, size_t bytes) {
 // ---- scan -------------------------------------------------------------------
 void initScan(int rLen, ScanPara *SP) {
     if (!SP) return;
-    SP->gLength = rLen;
+    // The reference's plan (testScan.cpp:26-75): pass = floor(log N / log 256),
+    // one less when that is integral; its per-level buffers are not needed by
+    // the single-pass scan, so outputBuffer / blockSumBuffer / tempBuffer stay
+    // NULL and `ws` holds the look-back workspace instead.
+    SP->outputBuffer = nullptr;
+    SP->blockSumBuffer = nullptr;
+    SP->tempBuffer = nullptr;
+    SP->gLength = rLen > 0 ? (unsigned)rLen : 0u;
     SP->blockSize = 256;
-    SP->pass = 1;
+    unsigned pass = 0;
+    for (double x = rLen > 0 ? (double)rLen : 1.0; x >= 256.0; x /= 256.0) ++pass;
+    if (rLen > 1 && pass > 0) {
+        double v = 1.0;
+        for (unsigned i = 0; i < pass; ++i) v *= 256.0;
+        if (v == (double)rLen) --pass;
+    }
+    SP->pass = pass;
     SP->ws = nullptr;
     if (qo_malloc(&SP->ws, scan_workspace_bytes(rLen > 0 ? (size_t)rLen : 1)) != QO_OK) compat_fail("initScan");
 }
@@ -104,25 +118,41 @@ void closeScan(ScanPara *SP) {
     SP->ws = nullptr;
 }
 
-void scanImpl(qo_mem d_Src, int rLen, qo_mem d_Dst, int *, void *, void *, int *, double *, ScanPara *SP, int) {
+// Exclusive scan of `rLen` int32 from d_Src (starting at element `offset`) into d_Dst.
+static void scan_compat(const char *who, qo_mem d_Src, size_t offset, int rLen, qo_mem d_Dst, ScanPara *SP) {
     if (rLen <= 0) return;
-    if (ensure_init() != QO_OK || !d_Src || !d_Dst) {
-        set_error(QO_ERR_INVALID_ARG, "scanImpl: bad arguments");
-        compat_fail("scanImpl");
+    if (ensure_init() != QO_OK || !d_Src || !d_Dst || (offset + (size_t)rLen) * 4 > d_Src->bytes ||
+        (size_t)rLen * 4 > d_Dst->bytes) {
+        set_error(QO_ERR_INVALID_ARG, "%s: bad arguments", who);
+        compat_fail(who);
         return;
     }
     hipStream_t st = stream_of(nullptr);
-    bool own = !(SP && SP->ws && SP->gLength >= rLen);
+    bool own = !(SP && SP->ws && (int)SP->gLength >= rLen);
     void *ws = own ? nullptr : SP->ws;
     if (own && qo_malloc(&ws, scan_workspace_bytes((size_t)rLen)) != QO_OK) {
-        compat_fail("scanImpl");
+        compat_fail(who);
         return;
     }
-    int rc = launch_scan_excl_u32(static_cast<const uint32_t *>(d_Src->ptr), static_cast<uint32_t *>(d_Dst->ptr),
-                                  (size_t)rLen, ws, st);
+    int rc = launch_scan_excl_u32(static_cast<const uint32_t *>(d_Src->ptr) + offset,
+                                  static_cast<uint32_t *>(d_Dst->ptr), (size_t)rLen, ws, st);
     if (rc == QO_OK) rc = sync_and_check(st);
     if (own) qo_free(ws);
-    if (rc != QO_OK) compat_fail("scanImpl");
+    if (rc != QO_OK) compat_fail(who);
+}
+
+void scanImpl(qo_mem d_Src, int rLen, qo_mem d_Dst, int *, void *, void *, int *, double *, ScanPara *SP, int) {
+    scan_compat("scanImpl", d_Src, 0, rLen, d_Dst, SP);
+}
+
+void scanImpl_offset(qo_mem d_Src, int offset, int rLen, qo_mem d_Dst, int *, void *, void *, int *, double *,
+                     ScanPara *SP, int) {
+    if (offset < 0) {
+        set_error(QO_ERR_INVALID_ARG, "scanImpl_offset: negative offset");
+        compat_fail("scanImpl_offset");
+        return;
+    }
+    scan_compat("scanImpl_offset", d_Src, (size_t)offset, rLen, d_Dst, SP);
 }
 
 // ---- selection ----------------------------------------------------------------
@@ -194,6 +224,27 @@ void radixSortImpl_sm(qo_mem d_R, int rLen, int, int, int, int *, void *, void *
     }
     QO_COMPAT_CHECK(qo_sort_records(static_cast<qo_record *>(d_R->ptr), (size_t)rLen, nullptr), "radixSortImpl_sm", );
     QO_COMPAT_CHECK(qo_stream_sync(nullptr), "radixSortImpl_sm", );
+}
+
+void radixSortImpl(qo_mem d_R, int rLen, int keybits, int numThreadPB, int numBlock, int *index, void *eventList,
+                   void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU) {
+    radixSortImpl_sm(d_R, rLen, keybits, numThreadPB, numBlock, index, eventList, kernel, Flag_CPU_GPU, burden,
+                     _CPU_GPU);
+}
+
+void CL_RadixSortOnly(qo_mem d_Rin, int rLen, int numThread, int numBlock, int _CPU_GPU) {
+    radixSortImpl_sm(d_Rin, rLen, 32, numThread, numBlock, nullptr, nullptr, nullptr, nullptr, nullptr, _CPU_GPU);
+}
+
+void CL_BitonicSortOnly(qo_mem d_Rin, int rLen, qo_mem d_Rout, int numThread, int numBlock, int _CPU_GPU) {
+    if (!d_Rin || !d_Rout || rLen < 0 || sizeof(Record) * (size_t)rLen > d_Rout->bytes) {
+        set_error(QO_ERR_INVALID_ARG, "CL_BitonicSortOnly: bad arguments");
+        compat_fail("CL_BitonicSortOnly");
+        return;
+    }
+    if (rLen > 0) QO_COMPAT_CHECK(qo_memcpy_d2d(d_Rout->ptr, d_Rin->ptr, sizeof(Record) * (size_t)rLen, nullptr),
+                                  "CL_BitonicSortOnly", );
+    radixSortImpl_sm(d_Rin, rLen, 32, numThread, numBlock, nullptr, nullptr, nullptr, nullptr, nullptr, _CPU_GPU);
 }
 
 void CL_RadixSort(Record *h_Rin, int rLen, Record *h_Rout, int numThread, int numBlock, int _CPU_GPU) {
@@ -348,6 +399,31 @@ void aggAfterGroupByImpl(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroup
                        (size_t)numGroups, OPERATOR, static_cast<int32_t *>(d_aggResults->ptr));
     if (gathered) qo_free(gathered);
     if (rc != QO_OK) compat_fail("aggAfterGroupByImpl");
+}
+
+void agg_max_afterGroupBy(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                          qo_mem d_aggResults, int numThread, int *index, void *eventList, void *kernel,
+                          int *Flag_CPU_GPU, double *burden, int _CPU_GPU) {
+    aggAfterGroupByImpl(d_Rin, rLen, d_startPos, numGroups, d_Ragg, d_aggResults, REDUCE_MAX, numThread, index,
+                        eventList, kernel, Flag_CPU_GPU, burden, _CPU_GPU);
+}
+void agg_min_afterGroupBy(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                          qo_mem d_aggResults, int numThread, int *index, void *eventList, void *kernel,
+                          int *Flag_CPU_GPU, double *burden, int _CPU_GPU) {
+    aggAfterGroupByImpl(d_Rin, rLen, d_startPos, numGroups, d_Ragg, d_aggResults, REDUCE_MIN, numThread, index,
+                        eventList, kernel, Flag_CPU_GPU, burden, _CPU_GPU);
+}
+void agg_sum_afterGroupBy(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                          qo_mem d_aggResults, int numThread, int *index, void *eventList, void *kernel,
+                          int *Flag_CPU_GPU, double *burden, int _CPU_GPU) {
+    aggAfterGroupByImpl(d_Rin, rLen, d_startPos, numGroups, d_Ragg, d_aggResults, REDUCE_SUM, numThread, index,
+                        eventList, kernel, Flag_CPU_GPU, burden, _CPU_GPU);
+}
+void agg_avg_afterGroupBy(qo_mem d_Rin, int rLen, qo_mem d_startPos, int numGroups, qo_mem d_Ragg,
+                          qo_mem d_aggResults, int numThread, int *index, void *eventList, void *kernel,
+                          int *Flag_CPU_GPU, double *burden, int _CPU_GPU) {
+    aggAfterGroupByImpl(d_Rin, rLen, d_startPos, numGroups, d_Ragg, d_aggResults, REDUCE_AVERAGE, numThread, index,
+                        eventList, kernel, Flag_CPU_GPU, burden, _CPU_GPU);
 }
 
 static void cl_agg(Record *h_Rin, int rLen, int *h_startPos, int numGroups, Record *h_Ragg, int *h_aggResults, int op) {

@@ -323,10 +323,13 @@ QO_DEV void dom_merge(LTable<C> &L, const DomAcc &a, uint32_t dom, int probes, i
 // 64-tuple sample holds a key at least 4 times (a skewed input) sums that
 // key's tuples, and a second such key's, in registers instead of queueing its
 // lanes on one LDS address.
-template <int C, bool VEC, int NT>
+// FILT: only tuples with lo <= filt[i] <= hi are aggregated (the filter is
+// fused: its column is read beside key and value, no compaction pass).
+template <int C, bool VEC, int NT, bool FILT = false>
 __global__ __launch_bounds__(NT) void k_groupby_hash(const int32_t *__restrict__ key, const int32_t *__restrict__ val,
                                                       size_t n, GEntry *gt, uint64_t gmask, int probes,
-                                                      uint32_t *overflow) {
+                                                      uint32_t *overflow, const int32_t *__restrict__ filt = nullptr,
+                                                      int32_t flo = 0, int32_t fhi = 0) {
     extern __shared__ unsigned char s_raw[];
     LTable<C> &L = *reinterpret_cast<LTable<C> *>(s_raw);
     __shared__ uint32_t s_dom, s_dom2, s_heavy;
@@ -347,10 +350,13 @@ __global__ __launch_bounds__(NT) void k_groupby_hash(const int32_t *__restrict__
     auto body = [&](auto heavy_tag) {
         constexpr bool kHeavy = decltype(heavy_tag)::value;
         DomAcc acc, acc2;
-        auto add = [&](int32_t k, int32_t v) {
+        auto add_one = [&](int32_t k, int32_t v) {
             if (kHeavy && (uint32_t)k == dom) acc.add(v);
             else if (kHeavy && (uint32_t)k == dom2) acc2.add(v);
             else ltable_add<C>(L, k, v, probes, gt, gmask, overflow);
+        };
+        auto add = [&](int32_t k, int32_t v, int32_t f) {
+            if (!FILT || (f >= flo && f <= fhi)) add_one(k, v);
         };
         const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
         const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -362,31 +368,34 @@ __global__ __launch_bounds__(NT) void k_groupby_hash(const int32_t *__restrict__
             // Four rounds of loads in flight per thread (2^28 tuples, 1024 groups:
             // 0.454 ms against 0.480 ms with two).
             constexpr int kRounds = 4;
+            const int4 *f4 = reinterpret_cast<const int4 *>(filt);
             for (; i + (kRounds - 1) * stride < nv; i += kRounds * stride) {
-                int4 kk[kRounds], vv[kRounds];
+                int4 kk[kRounds], vv[kRounds], ff[kRounds];
 #pragma unroll
                 for (int u = 0; u < kRounds; ++u) {
                     kk[u] = ld_stream(k4 + i + u * stride);
                     vv[u] = ld_stream(v4 + i + u * stride);
+                    if (FILT) ff[u] = ld_stream(f4 + i + u * stride);
                 }
 #pragma unroll
                 for (int u = 0; u < kRounds; ++u) {
-                    add(kk[u].x, vv[u].x);
-                    add(kk[u].y, vv[u].y);
-                    add(kk[u].z, vv[u].z);
-                    add(kk[u].w, vv[u].w);
+                    add(kk[u].x, vv[u].x, FILT ? ff[u].x : 0);
+                    add(kk[u].y, vv[u].y, FILT ? ff[u].y : 0);
+                    add(kk[u].z, vv[u].z, FILT ? ff[u].z : 0);
+                    add(kk[u].w, vv[u].w, FILT ? ff[u].w : 0);
                 }
             }
             for (; i < nv; i += stride) {
                 const int4 kk = ld_stream(k4 + i), vv = ld_stream(v4 + i);
-                add(kk.x, vv.x);
-                add(kk.y, vv.y);
-                add(kk.z, vv.z);
-                add(kk.w, vv.w);
+                const int4 ff = FILT ? ld_stream(f4 + i) : int4{0, 0, 0, 0};
+                add(kk.x, vv.x, ff.x);
+                add(kk.y, vv.y, ff.y);
+                add(kk.z, vv.z, ff.z);
+                add(kk.w, vv.w, ff.w);
             }
             from = nv * 4;
         }
-        for (size_t e = from + tid; e < n; e += stride) add(key[e], val[e]);
+        for (size_t e = from + tid; e < n; e += stride) add(key[e], val[e], FILT ? filt[e] : 0);
         if (kHeavy) {
             dom_merge<C, NT>(L, acc, dom, probes, 0, gt, gmask, overflow);
             if (dom2 != dom) {  // block-uniform
@@ -435,16 +444,18 @@ QO_DEV uint32_t fib_part(int32_t key, int L) { return ((uint32_t)key * 0x9E3779B
 
 // rows[t * nb + f] = tuples of tile t in bucket f.  VEC: the key column is
 // 16-byte aligned.
-template <bool VEC>
+template <bool VEC, bool FILT = false>
 __global__ __launch_bounds__(kPartThreads) void k_part_hist(const int32_t *__restrict__ key, size_t n, int L,
-                                                            uint32_t nb, uint32_t *__restrict__ rows) {
+                                                            uint32_t nb, uint32_t *__restrict__ rows,
+                                                            const int32_t *__restrict__ filt = nullptr,
+                                                            int32_t flo = 0, int32_t fhi = 0) {
     __shared__ uint32_t s_c[kOnePassParts];
     for (uint32_t i = threadIdx.x; i < nb; i += kPartThreads) s_c[i] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * kPartTile;
     const uint32_t rem = (uint32_t)min((size_t)kPartTile, n - base);
     const int32_t *src = key + base;
-    if (VEC && rem == kPartTile) {
+    if (VEC && !FILT && rem == kPartTile) {
         // Full tile, 16-byte aligned: kPartJ / 4 int4 loads per thread.
         int4 q[kPartJ / 4];
 #pragma unroll
@@ -459,14 +470,17 @@ __global__ __launch_bounds__(kPartThreads) void k_part_hist(const int32_t *__res
         }
     } else {
         int32_t k[kPartJ];
+        bool ok[kPartJ];
 #pragma unroll
         for (int j = 0; j < kPartJ; ++j) {
             const uint32_t i = (uint32_t)j * kPartThreads + threadIdx.x;
             k[j] = i < rem ? ld_stream(src + i) : 0;
+            const int32_t f = FILT && i < rem ? ld_stream(filt + base + i) : 0;
+            ok[j] = i < rem && (!FILT || (f >= flo && f <= fhi));
         }
 #pragma unroll
         for (int j = 0; j < kPartJ; ++j)
-            if ((uint32_t)j * kPartThreads + threadIdx.x < rem) atomicAdd(&s_c[fib_part(k[j], L)], 1u);
+            if (ok[j]) atomicAdd(&s_c[fib_part(k[j], L)], 1u);
     }
     __syncthreads();
     for (uint32_t f = threadIdx.x; f < nb; f += kPartThreads) rows[(size_t)blockIdx.x * nb + f] = s_c[f];
@@ -545,10 +559,13 @@ __global__ __launch_bounds__(kPartThreads) void k_part_apply(uint32_t *__restric
 // Places tile xcd_tile(blockIdx.x): ranks its tuples by bucket (LDS atomics;
 // order inside a bucket is free for aggregation), stages them in bucket order
 // and writes each bucket's run at the offset from its row.
+template <bool FILT = false>
 __global__ __launch_bounds__(kPartThreads) void k_part_place(const int32_t *__restrict__ key,
                                                              const int32_t *__restrict__ val, size_t n, int L,
                                                              uint32_t nb, uint32_t T, const uint32_t *__restrict__ offs,
-                                                             uint64_t *__restrict__ out) {
+                                                             uint64_t *__restrict__ out,
+                                                             const int32_t *__restrict__ filt = nullptr,
+                                                             int32_t flo = 0, int32_t fhi = 0) {
     __shared__ uint64_t s_stage[kPartTile];
     __shared__ uint32_t s_cnt[kOnePassParts], s_loff[kOnePassParts], s_g[kOnePassParts];
     __shared__ uint32_t s_w[kPartThreads / kWave];
@@ -559,15 +576,17 @@ __global__ __launch_bounds__(kPartThreads) void k_part_place(const int32_t *__re
     __syncthreads();
     int32_t k[kPartJ], v[kPartJ];
     uint32_t r[kPartJ];
+    uint32_t live = 0;  // bit j: tuple j of this thread is in the tile and passes the filter
 #pragma unroll
     for (int j = 0; j < kPartJ; ++j) {
         const uint32_t i = (uint32_t)j * kPartThreads + threadIdx.x;
         k[j] = i < rem ? ld_stream(key + base + i) : 0;
         v[j] = i < rem ? ld_stream(val + base + i) : 0;
+        const int32_t f = FILT && i < rem ? ld_stream(filt + base + i) : 0;
+        live |= (i < rem && (!FILT || (f >= flo && f <= fhi))) ? (1u << j) : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < kPartJ; ++j)
-        r[j] = (uint32_t)j * kPartThreads + threadIdx.x < rem ? atomicAdd(&s_cnt[fib_part(k[j], L)], 1u) : 0u;
+    for (int j = 0; j < kPartJ; ++j) r[j] = (live >> j) & 1u ? atomicAdd(&s_cnt[fib_part(k[j], L)], 1u) : 0u;
     __syncthreads();
     {
         const uint32_t f = threadIdx.x;
@@ -581,10 +600,11 @@ __global__ __launch_bounds__(kPartThreads) void k_part_place(const int32_t *__re
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kPartJ; ++j)
-        if ((uint32_t)j * kPartThreads + threadIdx.x < rem)
+        if ((live >> j) & 1u)
             s_stage[s_loff[fib_part(k[j], L)] + r[j]] = ((uint64_t)(uint32_t)v[j] << 32) | (uint32_t)k[j];
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < rem; i += kPartThreads) {
+    const uint32_t placed = s_loff[nb - 1] + s_cnt[nb - 1];  // the tile's passing tuples
+    for (uint32_t i = threadIdx.x; i < placed; i += kPartThreads) {
         const uint64_t e = s_stage[i];
         const uint32_t f = fib_part((int32_t)(uint32_t)e, L);
         out[s_g[f] + (i - s_loff[f])] = e;
@@ -611,10 +631,12 @@ static OnePassBufs onepass_carve(size_t n, uint32_t nb, Carver &cv) {
 }
 
 static int launch_onepass_partition(const OnePassBufs &b, const int32_t *key, const int32_t *val, size_t n, int L,
-                                    uint32_t nb, hipStream_t st) {
+                                    uint32_t nb, hipStream_t st, const int32_t *filt = nullptr, int32_t flo = 0,
+                                    int32_t fhi = 0) {
     QO_REQUIRE(n < 0xFFFFFFFFull && nb <= kOnePassParts && (1u << L) == nb, "one-pass partition too large");
     if (b.T == 0) return QO_OK;
-    if ((reinterpret_cast<uintptr_t>(key) & 15) == 0) k_part_hist<true><<<b.T, kPartThreads, 0, st>>>(key, n, L, nb, b.rows);
+    if (filt) k_part_hist<false, true><<<b.T, kPartThreads, 0, st>>>(key, n, L, nb, b.rows, filt, flo, fhi);
+    else if ((reinterpret_cast<uintptr_t>(key) & 15) == 0) k_part_hist<true><<<b.T, kPartThreads, 0, st>>>(key, n, L, nb, b.rows);
     else k_part_hist<false><<<b.T, kPartThreads, 0, st>>>(key, n, L, nb, b.rows);
     QO_LAUNCHED();
     k_part_reduce<<<b.nbk, kPartThreads, 0, st>>>(b.rows, b.T, nb, b.span, b.bsum);
@@ -623,7 +645,8 @@ static int launch_onepass_partition(const OnePassBufs &b, const int32_t *key, co
     QO_LAUNCHED();
     k_part_apply<<<b.nbk, kPartThreads, 0, st>>>(b.rows, b.T, nb, b.span, b.bsum);
     QO_LAUNCHED();
-    k_part_place<<<b.T, kPartThreads, 0, st>>>(key, val, n, L, nb, b.T, b.rows, b.fine);
+    if (filt) k_part_place<true><<<b.T, kPartThreads, 0, st>>>(key, val, n, L, nb, b.T, b.rows, b.fine, filt, flo, fhi);
+    else k_part_place<false><<<b.T, kPartThreads, 0, st>>>(key, val, n, L, nb, b.T, b.rows, b.fine);
     QO_LAUNCHED();
     return QO_OK;
 }
@@ -808,14 +831,18 @@ struct GbPartPlan {
 
 // Partition when the groups outgrow one LDS table: 2^L fine buckets of about
 // `target` groups each (tune items), L <= 14.
-static GbPartPlan groupby_part_plan(size_t group_capacity, size_t n) {
+// With a filter the partition stays one pass (at most kOnePassParts buckets,
+// whose LDS tables spill to the global table beyond 2^20 groups): the filter is
+// fused into that pass.
+static GbPartPlan groupby_part_plan(size_t group_capacity, size_t n, bool filtered) {
     GbPartPlan p;
     const TuneParams tp = tune_get(TUNE_GROUPBY_PART, n);
     if (tp.param == PART_NEVER || n == 0 || n >= 0xFFFFFFFFull || group_capacity < 2) return p;
     if (tp.param != PART_ALWAYS && (group_capacity <= 1024 || n < (size_t(1) << 20))) return p;
     const size_t target = (size_t)std::max(16, tp.items);
+    const int lmax = filtered ? __builtin_ctz(kOnePassParts) : 2 * kMaxSplitBits;
     int L = 1;
-    while (L < 2 * kMaxSplitBits && (group_capacity >> L) > target) ++L;
+    while (L < lmax && (group_capacity >> L) > target) ++L;
     p.on = true;
     p.s = split_plan(L, n);
     return p;
@@ -852,17 +879,18 @@ static GroupbyBufs groupby_carve(const GroupbyPlan &p, const GbPartPlan &pp, siz
     return b;
 }
 
-size_t groupby_ws_bytes(size_t group_capacity, size_t n) {
+size_t groupby_ws_bytes(size_t group_capacity, size_t n, bool filtered) {
     Carver cv(nullptr);
-    groupby_carve(groupby_plan(group_capacity), groupby_part_plan(group_capacity, n), group_capacity, n, cv);
+    groupby_carve(groupby_plan(group_capacity), groupby_part_plan(group_capacity, n, filtered), group_capacity, n,
+                  cv);
     return cv.off + 256;
 }
 
 int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity, int32_t *okey,
                    int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax, uint64_t *d_num, void *ws,
-                   hipStream_t st) {
+                   hipStream_t st, const int32_t *filt, int32_t flo, int32_t fhi) {
     GroupbyPlan p = groupby_plan(group_capacity);
-    const GbPartPlan pp = groupby_part_plan(group_capacity, n);
+    const GbPartPlan pp = groupby_part_plan(group_capacity, n, filt != nullptr);
     Carver cv(ws);
     const GroupbyBufs b = groupby_carve(p, pp, group_capacity, n, cv);
     GEntry *gt = b.gt;
@@ -881,7 +909,7 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         const uint32_t *hist, *offp, *off;
         const uint64_t *fine;
         if (onepass) {
-            QO_TRY(launch_onepass_partition(b.op, key, val, n, pp.s.L, nb, st));
+            QO_TRY(launch_onepass_partition(b.op, key, val, n, pp.s.L, nb, st, filt, flo, fhi));
             hist = b.op.total;
             offp = off = b.op.starts;
             fine = b.op.fine;
@@ -904,7 +932,7 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         QO_LAUNCHED();
     } else if (n) {
         TuneParams tp = tune_get(TUNE_GROUPBY, n);
-        const bool vec = ((uintptr_t)key % 16 == 0) && ((uintptr_t)val % 16 == 0);
+        const bool vec = ((uintptr_t)key % 16 == 0) && ((uintptr_t)val % 16 == 0) && ((uintptr_t)filt % 16 == 0);
         int C = tp.items >= 4096 ? 4096 : 2048;
         const size_t lds4096 = sizeof(LTable<4096>);
         if (C == 4096 && (size_t)props().lds_per_block < lds4096) C = 2048;
@@ -918,19 +946,27 @@ int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t grou
         // Low cardinality: the LDS table holds every group.  High cardinality:
         // it caches the keys seen first and the rest go straight to global.
         const int probes = group_capacity * 2 <= (size_t)C ? C : 8;
+#define QO_GBH(CC, V, F)                                                                                    \
+    k_groupby_hash<CC, V, block, F><<<grid, block, sizeof(LTable<CC>), st>>>(key, val, n, gt, p.gmask, probes,  \
+                                                                             overflow, filt, flo, fhi)
         if (C == 4096) {
-            const size_t lds = sizeof(LTable<4096>);
-            if (vec)
-                k_groupby_hash<4096, true, block><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
-            else
-                k_groupby_hash<4096, false, block><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+            if (filt) {
+                if (vec) QO_GBH(4096, true, true);
+                else QO_GBH(4096, false, true);
+            } else {
+                if (vec) QO_GBH(4096, true, false);
+                else QO_GBH(4096, false, false);
+            }
         } else {
-            const size_t lds = sizeof(LTable<2048>);
-            if (vec)
-                k_groupby_hash<2048, true, block><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
-            else
-                k_groupby_hash<2048, false, block><<<grid, block, lds, st>>>(key, val, n, gt, p.gmask, probes, overflow);
+            if (filt) {
+                if (vec) QO_GBH(2048, true, true);
+                else QO_GBH(2048, false, true);
+            } else {
+                if (vec) QO_GBH(2048, true, false);
+                else QO_GBH(2048, false, false);
+            }
         }
+#undef QO_GBH
         QO_LAUNCHED();
     }
     unsigned long long *d_rows = reinterpret_cast<unsigned long long *>(counters);
@@ -1202,15 +1238,23 @@ extern "C" {
 int qo_groupby_agg(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity, int32_t *okey,
                    int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax, uint64_t *d_numGroups,
                    qo_stream s) {
+    return qo_groupby_agg_filtered(key, val, nullptr, n, 0, 0, group_capacity, okey, ocount, osum, omin, omax,
+                                   d_numGroups, s);
+}
+
+int qo_groupby_agg_filtered(const int32_t *key, const int32_t *val, const int32_t *filt, size_t n, int32_t lo,
+                            int32_t hi, size_t group_capacity, int32_t *okey, int64_t *ocount, int64_t *osum,
+                            int32_t *omin, int32_t *omax, uint64_t *d_numGroups, qo_stream s) {
     QO_ENTER();
     QO_REQUIRE(d_numGroups && (n == 0 || (key && val)), "null pointer");
     QO_REQUIRE(group_capacity == 0 || (okey && ocount && osum && omin && omax), "null output column");
     QO_REQUIRE(group_capacity < (size_t(1) << 31), "group capacity too large");
     hipStream_t st = stream_of(s);
     int status = QO_OK;
-    void *ws = workspace(st, groupby_ws_bytes(group_capacity, n), &status);
+    void *ws = workspace(st, groupby_ws_bytes(group_capacity, n, filt != nullptr), &status);
     if (!ws) return status;
-    return launch_groupby(key, val, n, group_capacity, okey, ocount, osum, omin, omax, d_numGroups, ws, st);
+    return launch_groupby(key, val, n, group_capacity, okey, ocount, osum, omin, omax, d_numGroups, ws, st, filt, lo,
+                          hi);
 }
 
 int qo_group_startpos(const qo_record *sorted, size_t n, int32_t *startPos, uint64_t *d_numGroups, qo_stream s) {

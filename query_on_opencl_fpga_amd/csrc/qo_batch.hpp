@@ -8,7 +8,8 @@
 
 namespace qo {
 
-constexpr uint32_t kBuildTile = 4096;
+constexpr uint32_t kBuildTile = 4096;       // default R rows per build tile (tune op TUNE_BATCH, param)
+constexpr uint32_t kProbeTicketTiles = 4;   // default S tiles per probe ticket (tune op TUNE_BATCH, items)
 
 // Device descriptor of one query of a batch.
 struct BQuery {
@@ -37,6 +38,8 @@ struct JoinBatchPlan {
     std::vector<RowOut> houts;        // query q's output columns (the sort's last pass writes them)
     uint64_t total_slots = 0, total_match = 0;
     uint32_t rt = 0, stt = 0, mt = 0;
+    uint32_t build_tile = kBuildTile;        // R rows per build tile
+    uint32_t probe_group = kProbeTicketTiles;  // S tiles per probe ticket
     size_t small_words = 0;
     // Bound by join_plan_bind:
     uint64_t *slots_all = nullptr;

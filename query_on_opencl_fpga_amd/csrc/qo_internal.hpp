@@ -67,7 +67,8 @@ struct Carver {
     }
 };
 
-// Autotuned launch parameters.
+// Launch parameters per operator and size class: defaults, or the values the
+// autotuner measured best (qo_autotune) and the tune file holds.
 struct TuneParams {
     int block;
     int items;
@@ -82,12 +83,16 @@ enum TuneOp {
     TUNE_JOIN_PART = 5,     // partitioned build/probe: items = LDS table slice KiB, param = PartMode
     TUNE_GROUPBY_PART = 6,  // partitioned group-by: items = groups per fine bucket, param = PartMode
     TUNE_JOIN_RANGE = 7,    // key-range join table: items = largest such table in MiB, param = PartMode
-    TUNE_NUM_OPS = 8
+    TUNE_BATCH = 8,         // batched join (class of the mean |R|): items = S tiles per probe ticket,
+                            // param = R rows per build tile
+    TUNE_NUM_OPS = 9
 };
 // Partitioned-path modes: by size (the default), always, never.
 // PART_LDS (join only): partition, and probe hash-mixed tables through LDS slices.
 enum PartMode { PART_AUTO = 0, PART_ALWAYS = 1, PART_NEVER = 2, PART_LDS = 3 };
 TuneParams tune_get(int op, size_t n);
+// The built-in default of an operator (used where no class is tuned).
+TuneParams tune_builtin(int op);
 
 inline int size_class(size_t n) {
     int c = 0;
@@ -122,11 +127,12 @@ int launch_radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, const uint64_t *d_
 size_t sort_join_rows_ws_bytes(size_t m);
 int sort_join_rows_ws(int32_t *key, int32_t *rpay, int32_t *spay, size_t m, void *ws, hipStream_t st);
 
-// Hash group-by with caller workspace (groupby.hip).
-size_t groupby_ws_bytes(size_t group_capacity, size_t n);
+// Hash group-by with caller workspace (groupby.hip); with filt, only tuples
+// with lo <= filt[i] <= hi are aggregated (the filter is fused).
+size_t groupby_ws_bytes(size_t group_capacity, size_t n, bool filtered = false);
 int launch_groupby(const int32_t *key, const int32_t *val, size_t n, size_t group_capacity, int32_t *okey,
                    int64_t *ocount, int64_t *osum, int32_t *omin, int32_t *omax, uint64_t *d_num, void *ws,
-                   hipStream_t st);
+                   hipStream_t st, const int32_t *filt = nullptr, int32_t flo = 0, int32_t fhi = 0);
 
 // Join table helpers (join.hip).
 size_t join_table_slots(size_t table_bytes);

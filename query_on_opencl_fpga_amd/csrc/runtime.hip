@@ -1,5 +1,7 @@
 // libqo runtime: error state, device selection, per-stream workspaces, memory
 // and stream plumbing, fill kernels and the autotuned parameter table.
+#include <dlfcn.h>
+
 #include <atomic>
 #include <cstdarg>
 #include <cstring>
@@ -8,6 +10,7 @@
 #include <string>
 #include <thread>
 
+#include "qo_batch.hpp"
 #include "qo_device.hpp"
 #include "qo_internal.hpp"
 
@@ -25,6 +28,7 @@ struct Ws {
     size_t bytes = 0;
 };
 std::map<StreamKey, Ws> g_ws;
+void load_tune_file(const char *arch);
 }  // namespace
 
 StreamKey stream_key_of(hipStream_t s) {
@@ -70,6 +74,7 @@ static int init_locked(int device) {
     // Some drivers report an empty marketing name: fall back to the ISA.
     if (p.name[0]) snprintf(g_props.name, sizeof(g_props.name), "%s", p.name);
     else snprintf(g_props.name, sizeof(g_props.name), "AMD %s (%d CUs)", p.gcnArchName, p.multiProcessorCount);
+    load_tune_file(g_props.arch);
     if (!g_ctl) {
         QO_HIP(hipMalloc(&g_ctl, sizeof(DevCtl)));
         DevCtl init{};
@@ -207,35 +212,69 @@ int launch_fill_u64(void *dst, uint64_t value, size_t count, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Autotuned parameter table: per operator and size class (floor(log2 n)).
-// Defaults below were chosen by sweeps on MI355X; qo_tune_load overrides them.
+// Launch-parameter table: per operator and size class (floor(log2 n)).  An
+// entry comes from the tune file loaded at init (written by the autotuner,
+// qo_autotune + qo_tune_save), from qo_tune_set / qo_tune_load, or, for a
+// class no entry covers, from the nearest tuned class of the same operator;
+// an operator with no tuned class at all uses the defaults below (measured
+// sweeps on MI355X at the benchmark sizes).
 // ---------------------------------------------------------------------------
 namespace {
 TuneParams g_tune[TUNE_NUM_OPS][64];
 bool g_tune_set[TUNE_NUM_OPS][64];
 std::mutex g_tune_mu;
 
-TuneParams tune_default(int op, int cls) {
+}  // namespace
+
+TuneParams tune_builtin(int op) {
     switch (op) {
-        case TUNE_FILTER_AGG: return {256, 4, 1};      // block, unroll, blocks per CU
+        case TUNE_FILTER_AGG: return {256, 4, 1};      // block, loads in flight per column, blocks per CU
         case TUNE_FILTER: return {256, 4, 0};          // block, uint4 loads per thread
-        case TUNE_PROBE: return {256, 4, 50};          // block, int4 groups per thread, load %
+        case TUNE_PROBE: return {256, 4, 50};          // block, int4 groups per thread, table load %
         case TUNE_GROUPBY: return {1024, 2048, 1};     // block, LDS slots, blocks per CU
         case TUNE_SORT: return {256, 16, 0};           // block, keys per thread
         case TUNE_JOIN_PART: return {256, 32, 0};      // block, LDS table slice KiB, mode
         case TUNE_GROUPBY_PART: return {512, 1024, 0}; // block, groups per fine bucket, mode
         case TUNE_JOIN_RANGE: return {256, 256, 0};    // block, largest key-range table MiB, mode
+        case TUNE_BATCH: return {256, (int)kProbeTicketTiles, (int)kBuildTile};  // block, tiles per ticket, build tile
         default: return {256, 4, 0};
     }
-    (void)cls;
+}
+
+namespace {
+// Loads <directory of libqo.so>/tune/<arch>.tune, or $QO_TUNE_FILE ("none": no file).
+void load_tune_file(const char *arch) {
+    std::string path;
+    const char *env = getenv("QO_TUNE_FILE");
+    if (env) {
+        if (!*env || !strcmp(env, "none")) return;
+        path = env;
+    } else {
+        Dl_info info;
+        if (!dladdr(reinterpret_cast<void *>(&qo_init), &info) || !info.dli_fname) return;
+        std::string lib = info.dli_fname;
+        const size_t slash = lib.rfind('/');
+        std::string isa = arch;
+        isa = isa.substr(0, isa.find(':'));
+        path = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/tune/" + isa + ".tune";
+    }
+    if (FILE *f = fopen(path.c_str(), "r")) {
+        fclose(f);
+        (void)qo_tune_load(path.c_str());
+    }
 }
 }  // namespace
 
 TuneParams tune_get(int op, size_t n) {
-    int cls = size_class(n);
+    const int cls = size_class(n);
+    if (op < 0 || op >= TUNE_NUM_OPS) return tune_builtin(op);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    if (op >= 0 && op < TUNE_NUM_OPS && g_tune_set[op][cls]) return g_tune[op][cls];
-    return tune_default(op, cls);
+    if (g_tune_set[op][cls]) return g_tune[op][cls];
+    for (int d = 1; d < 64; ++d) {  // nearest tuned class, the smaller one first
+        if (cls - d >= 0 && g_tune_set[op][cls - d]) return g_tune[op][cls - d];
+        if (cls + d < 64 && g_tune_set[op][cls + d]) return g_tune[op][cls + d];
+    }
+    return tune_builtin(op);
 }
 
 }  // namespace qo
@@ -438,6 +477,13 @@ int qo_tune_get(int op, int size_class, int *block, int *items, int *param) {
     if (block) *block = p.block;
     if (items) *items = p.items;
     if (param) *param = p.param;
+    return QO_OK;
+}
+
+int qo_tune_reset(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (int op = 0; op < TUNE_NUM_OPS; ++op)
+        for (int c = 0; c < 64; ++c) g_tune_set[op][c] = false;
     return QO_OK;
 }
 

@@ -233,12 +233,19 @@ struct qo_server {
 namespace qo {
 
 // Device bytes one request needs (its outputs, workspaces and join tables).
+// The filter column of a group-by request: the table's filt column, unless the
+// predicate is the whole int32 range (then no filter is evaluated).
+static const int32_t *groupby_filter(const qo_table_desc &A, const qo_request &r) {
+    if (!A.filt || (r.lo == INT32_MIN && r.hi == INT32_MAX)) return nullptr;
+    return A.filt;
+}
+
 static size_t request_bytes(qo_server *sv, const qo_request &r) {
     const Table &A = sv->tables[r.table_a];
     if (r.kind == QO_Q_FILTER) return 4096;
     if (r.kind == QO_Q_GROUPBY) {
         size_t G = A.d.group_capacity ? A.d.group_capacity : 1;
-        return G * 28 + groupby_ws_bytes(G, A.d.n) + 8192;
+        return G * 28 + groupby_ws_bytes(G, A.d.n, groupby_filter(A.d, r) != nullptr) + 8192;
     }
     const Table &B = sv->tables[r.table_b];
     qo_join_query q{};
@@ -305,7 +312,8 @@ static int launch_batch(qo_server *sv, Batch &b, hipStream_t st) {
         size_t G = sv->tables[live[i]->r.table_a].d.group_capacity;
         cv.take<int32_t>(G); cv.take<int64_t>(G); cv.take<int64_t>(G); cv.take<int32_t>(G); cv.take<int32_t>(G);
         cv.take<uint64_t>(1);
-        cv.take<char>(groupby_ws_bytes(G, sv->tables[live[i]->r.table_a].d.n));
+        const qo_table_desc &A = sv->tables[live[i]->r.table_a].d;
+        cv.take<char>(groupby_ws_bytes(G, A.n, groupby_filter(A, live[i]->r) != nullptr));
         gsz.push_back(G);
     }
     for (auto &q : b.jq) {
@@ -365,8 +373,9 @@ static int launch_batch(qo_server *sv, Batch &b, hipStream_t st) {
         int32_t *omin = c2.take<int32_t>(G);
         int32_t *omax = c2.take<int32_t>(G);
         uint64_t *num = c2.take<uint64_t>(1);
-        void *ws = c2.take<char>(groupby_ws_bytes(G, A.n));
-        QO_TRY(launch_groupby(A.grp, A.val, A.n, G, okey, ocnt, osum, omin, omax, num, ws, st));
+        const int32_t *filt = groupby_filter(A, r);
+        void *ws = c2.take<char>(groupby_ws_bytes(G, A.n, filt != nullptr));
+        QO_TRY(launch_groupby(A.grp, A.val, A.n, G, okey, ocnt, osum, omin, omax, num, ws, st, filt, r.lo, r.hi));
         k_s_checksum_groupby<<<std::min<unsigned>(kGrid, (unsigned)((G + 255) / 256) + 1), 256, 0, st>>>(
             okey, ocnt, osum, omin, omax, num, G, b.d_res + slot_of(live[gi[k]]));
         QO_LAUNCHED();
@@ -425,7 +434,8 @@ static int run_plain(qo_server *sv, const qo_request &r, qo_result &out, hipStre
         int32_t *omin = cv.take<int32_t>(G);
         int32_t *omax = cv.take<int32_t>(G);
         uint64_t *num = cv.take<uint64_t>(1);
-        rc = qo_groupby_agg(A.grp, A.val, A.n, G, okey, ocnt, osum, omin, omax, num, st);
+        rc = qo_groupby_agg_filtered(A.grp, A.val, groupby_filter(A, r), A.n, r.lo, r.hi, G, okey, ocnt, osum, omin,
+                                     omax, num, st);
         if (rc == QO_OK) {
             k_s_checksum_groupby<<<std::min<unsigned>(kGrid, (unsigned)((G + 255) / 256) + 1), 256, 0, st>>>(
                 okey, ocnt, osum, omin, omax, num, G, d_res);

@@ -31,7 +31,7 @@ VARIANT = os.environ.get("SIMT_VARIANT", "asan")
 SAN_FLAGS = {
     "asan": ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-sanitize=vptr"],
     "msan": ["-fsanitize=memory", "-fsanitize-memory-track-origins", "-fsanitize-recover=memory"],
-    "fast": ["-O2"],
+    "fast": ["-O2", "-fPIC"],
 }[VARIANT]
 OUT = HERE / "_build" / VARIANT
 SRC = HERE / "_build" / "src"
@@ -160,6 +160,9 @@ def build(drivers=()) -> Path:
     _run(["ar", "rcs", str(lib), *map(str, objs)])
     for d in drivers:
         _run([CXX, *FLAGS, str(HERE / f"{d}.cpp"), str(lib), "-lpthread", "-o", str(OUT / d)])
+    if VARIANT == "fast":
+        # A shared library with libqo's C ABI, for host programs that load it by path.
+        _run([CXX, *FLAGS, "-shared", "-o", str(OUT / "libqo_cpu.so"), *map(str, objs), "-lpthread"])
     return lib
 
 

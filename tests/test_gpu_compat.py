@@ -87,3 +87,90 @@ def test_compat_errors_do_not_exit(qo):
         qo.operators.filter_agg(np.arange(10, dtype=np.int32), np.arange(10, dtype=np.int32), 0, 5)
     assert ei.value.status == 6
     assert qo.operators.filter_agg(np.arange(10, dtype=np.int32), np.arange(10, dtype=np.int32), 0, 5)[0] == 6
+
+
+def _mem(qo, arr):
+    lib = qo._lib.load()
+    a = np.ascontiguousarray(arr)
+    m = lib.qo_mem_create(max(a.nbytes, 4))
+    if a.nbytes:
+        qo._lib.call("qo_mem_write", m, a.ctypes.data, a.nbytes)
+    return m
+
+
+def _read(qo, m, dtype, count):
+    out = np.empty(count, dtype)
+    if out.nbytes:
+        qo._lib.call("qo_mem_read", out.ctypes.data, m, out.nbytes)
+    return out
+
+
+@pytest.mark.parametrize("op", ["max", "min", "sum", "avg"])
+def test_device_buffer_agg_afterGroupBy(qo, orc, op):
+    """agg_{max,min,sum,avg}_afterGroupBy (AggAfterGroupBy.cpp:9-24) over cl_mem-style buffers."""
+    lib = qo._lib.load()
+    n = 100_003
+    r = gen_records(orc, n, 300, 11)
+    srt, sp = qo.operators.CL_GroupBy(r)
+    g = sp.shape[0]
+    d_r, d_sp = _mem(qo, srt), _mem(qo, sp.astype(np.int32))
+    d_out = lib.qo_mem_create(g * 4)
+    try:
+        qo._lib.load().qo_clear_error()
+        getattr(lib, f"agg_{op}_afterGroupBy")(d_r, n, d_sp, g, None, d_out, 256, None, None, None, None, None, 0)
+        assert qo._lib.last_error() == ""
+        got = _read(qo, d_out, np.int32, g)
+        code = {"sum": 0, "max": 1, "min": 2, "avg": 3}[op]
+        assert np.array_equal(got, orc.parallel_aggregate(srt, sp, code))
+    finally:
+        for m in (d_r, d_sp, d_out):
+            lib.qo_mem_free(m)
+
+
+def test_sort_only_variants(qo, orc):
+    """CL_RadixSortOnly (Sort.cpp:11) sorts in place; CL_BitonicSortOnly (Sort.cpp:20)
+    copies the input to d_Rout and sorts d_Rin in place; radixSortImpl (testSort.cpp:274)."""
+    lib = qo._lib.load()
+    n = 70_001
+    r = gen_records(orc, n, 1 << 20, 12)
+    exp = orc.sort_records(r)
+    a = _mem(qo, r)
+    lib.CL_RadixSortOnly(a, n, 256, 128, 0)
+    assert np.array_equal(_read(qo, a, np.uint32, 2 * n).reshape(-1, 2), exp)
+    b, out = _mem(qo, r), lib.qo_mem_create(r.nbytes)
+    lib.CL_BitonicSortOnly(b, n, out, 256, 128, 0)
+    assert np.array_equal(_read(qo, b, np.uint32, 2 * n).reshape(-1, 2), exp)
+    assert np.array_equal(_read(qo, out, np.uint32, 2 * n).reshape(-1, 2), r)
+    c = _mem(qo, r)
+    lib.radixSortImpl(c, n, 32, 256, 128, None, None, None, None, None, 0)
+    assert np.array_equal(_read(qo, c, np.uint32, 2 * n).reshape(-1, 2), exp)
+    for m in (a, b, out, c):
+        lib.qo_mem_free(m)
+
+
+@pytest.mark.parametrize("n,offset", [(1, 0), (1000, 7), ((1 << 20) + 3, 1 << 16)])
+def test_scan_offset_and_scanpara(qo, orc, n, offset):
+    """scanImpl_offset (the offset overload of testScan.h:16) scans [offset, offset + n);
+    initScan fills the reference's pass / gLength / blockSize (testScan.cpp:26-75)."""
+    import ctypes as C
+
+    lib = qo._lib.load()
+    x = orc.gen_uniform_u32(n + offset, 13, 0, 0, 1 << 20).astype(np.int32)
+    d_x, d_y = _mem(qo, x), lib.qo_mem_create(4 * n)
+    sp = qo._lib.ScanPara()
+    lib.initScan(n, C.byref(sp))
+    try:
+        assert sp.gLength == n and sp.blockSize == 256 and not sp.outputBuffer and not sp.tempBuffer
+        p, v = 0, float(n)
+        while v >= 256.0:
+            v /= 256.0
+            p += 1
+        if n > 1 and p > 0 and 256 ** p == n:
+            p -= 1
+        assert sp.pass_ == p
+        lib.scanImpl_offset(d_x, offset, n, d_y, None, None, None, None, None, C.byref(sp), 0)
+        assert np.array_equal(_read(qo, d_y, np.int32, n), orc.scan_excl_i32(x[offset:offset + n]))
+    finally:
+        lib.closeScan(C.byref(sp))
+        lib.qo_mem_free(d_x)
+        lib.qo_mem_free(d_y)
