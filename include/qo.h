@@ -248,7 +248,8 @@ int qo_batch_destroy(qo_batch *b);
  * groups per fine bucket, param = 0 auto / 1 always / 2 never), 7 key-range join
  * table (items = largest such table in MiB, param = 0 when the keys allow /
  * 2 never), 8 batched join (class of the batch's mean |R|; items = S tiles per
- * probe ticket, param = R rows per build tile).  Size class = floor(log2 n).
+ * probe ticket, param = R rows per build tile), 9 batched join table load %
+ * (class of the mean |R|).  Size class = floor(log2 n).
  * An operator's class with no entry takes the nearest class that has one, and
  * an operator with none its built-in default.  qo_init loads the tune file
  * tune/<arch>.tune beside libqo.so (or $QO_TUNE_FILE; "none" loads nothing). */
@@ -261,8 +262,8 @@ int qo_tune_reset(void);
 /* Searches operator `op`'s launch parameters at size class `size_class` on the
  * device: builds a synthetic workload of that size in HBM, times the built-in
  * default and every candidate (median of `reps` runs after a warm-up, HIP
- * events) and sets the fastest into the table.  Tunable operators: 0, 2, 3, 6
- * and 8.  Outputs the chosen parameters and both times (ms). */
+ * events) and sets the fastest into the table.  Tunable operators: 0, 3, 6, 8
+ * and 9.  Outputs the chosen parameters and both times (ms). */
 int qo_autotune(int op, int size_class, int reps, int *block, int *items, int *param, double *best_ms,
                 double *default_ms);
 

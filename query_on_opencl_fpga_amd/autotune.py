@@ -25,7 +25,7 @@ PLAN = [
     (3, "groupby direct (1024 groups)", [20, 24, 28]),
     (6, "groupby partitioned (2^20 groups)", [20, 24, 28]),
     (8, "batched join (tiles per probe ticket, build tile)", [12, 14, 16, 18, 20, 22]),
-    (2, "join table load %", [12, 16, 20, 22]),
+    (9, "batched join table load %", [12, 16, 20, 22]),
 ]
 
 

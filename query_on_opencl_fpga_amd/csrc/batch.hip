@@ -138,7 +138,7 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
     uint64_t rsum = 0;
     for (int i = 0; i < nq; ++i) rsum += queries[i].nr;
     const size_t rmean = std::max<size_t>(1, (size_t)(rsum / (uint64_t)nq));
-    const int load = std::max(10, std::min(90, tune_get(TUNE_PROBE, rmean).param));
+    const int load = std::max(10, std::min(90, tune_get(TUNE_BATCH_LOAD, rmean).param));
     const TuneParams tb = tune_get(TUNE_BATCH, rmean);
     p.build_tile = (uint32_t)std::max(256, std::min(1 << 16, tb.param));
     p.probe_group = (uint32_t)std::max(1, std::min(64, tb.items));

@@ -85,7 +85,8 @@ enum TuneOp {
     TUNE_JOIN_RANGE = 7,    // key-range join table: items = largest such table in MiB, param = PartMode
     TUNE_BATCH = 8,         // batched join (class of the mean |R|): items = S tiles per probe ticket,
                             // param = R rows per build tile
-    TUNE_NUM_OPS = 9
+    TUNE_BATCH_LOAD = 9,    // batched join (class of the mean |R|): param = hash table load %
+    TUNE_NUM_OPS = 10
 };
 // Partitioned-path modes: by size (the default), always, never.
 // PART_LDS (join only): partition, and probe hash-mixed tables through LDS slices.

@@ -16,13 +16,18 @@
 
 namespace qo {
 
+// The device control block: static device memory (one copy per device,
+// initialised when the code object loads), so set-up allocates nothing.
+__device__ DevCtl qo_dev_ctl = {0u, kDefaultSpinLimit, {}};
+
 namespace {
 thread_local std::string g_last_error;
 std::atomic<int> g_inject{0};
 std::mutex g_mu;
 bool g_init = false;
 DeviceProps g_props;
-DevCtl *g_ctl = nullptr;
+constexpr int kMaxDevices = 64;
+DevCtl *g_ctl_dev[kMaxDevices];  // each device's copy of qo_dev_ctl
 struct Ws {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -75,12 +80,9 @@ static int init_locked(int device) {
     if (p.name[0]) snprintf(g_props.name, sizeof(g_props.name), "%s", p.name);
     else snprintf(g_props.name, sizeof(g_props.name), "AMD %s (%d CUs)", p.gcnArchName, p.multiProcessorCount);
     load_tune_file(g_props.arch);
-    if (!g_ctl) {
-        QO_HIP(hipMalloc(&g_ctl, sizeof(DevCtl)));
-        DevCtl init{};
-        init.spin_limit = kDefaultSpinLimit;
-        QO_HIP(hipMemcpy(g_ctl, &init, sizeof(DevCtl), hipMemcpyHostToDevice));
-    }
+    QO_REQUIRE(device < kMaxDevices, "device %d beyond the %d the library tracks", device, kMaxDevices);
+    if (!g_ctl_dev[device])
+        QO_HIP(hipGetSymbolAddress(reinterpret_cast<void **>(&g_ctl_dev[device]), HIP_SYMBOL(qo_dev_ctl)));
     g_init = true;
     return QO_OK;
 }
@@ -99,9 +101,10 @@ int ensure_init() {
 
 const DeviceProps &props() { return g_props; }
 
-DevCtl *dev_ctl() { return g_ctl; }
+DevCtl *dev_ctl() { return g_ctl_dev[g_props.device]; }
 
 int check_device_error() {
+    DevCtl *g_ctl = dev_ctl();
     if (!g_ctl) return QO_OK;
     uint32_t err = 0;
     QO_HIP(hipMemcpy(&err, &g_ctl->error, sizeof(err), hipMemcpyDeviceToHost));
@@ -237,6 +240,7 @@ TuneParams tune_builtin(int op) {
         case TUNE_GROUPBY_PART: return {512, 1024, 0}; // block, groups per fine bucket, mode
         case TUNE_JOIN_RANGE: return {256, 256, 0};    // block, largest key-range table MiB, mode
         case TUNE_BATCH: return {256, (int)kProbeTicketTiles, (int)kBuildTile};  // block, tiles per ticket, build tile
+        case TUNE_BATCH_LOAD: return {256, 4, 50};     // block, -, batch hash table load %
         default: return {256, 4, 0};
     }
 }
@@ -411,7 +415,7 @@ int qo_device_sync(void) {
 int qo_debug_set_spin_limit(uint32_t sleeps) {
     QO_TRY(ensure_init());
     QO_HIP(hipDeviceSynchronize());
-    QO_HIP(hipMemcpy(&g_ctl->spin_limit, &sleeps, sizeof(sleeps), hipMemcpyHostToDevice));
+    QO_HIP(hipMemcpy(&dev_ctl()->spin_limit, &sleeps, sizeof(sleeps), hipMemcpyHostToDevice));
     return QO_OK;
 }
 

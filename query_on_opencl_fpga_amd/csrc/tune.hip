@@ -14,7 +14,7 @@
 //   TUNE_GROUPBY_PART groups per fine bucket {512, 1024, 2048} (2^20 groups, z = 0.5)
 //   TUNE_BATCH        S tiles per probe ticket {1, 2, 4, 8} x R rows per build
 //                     tile {2048, 4096, 8192} (joins of |R| = 2^class, |S| = 8|R|)
-//   TUNE_PROBE        join table load {33, 50, 66, 80} % (same batch workload)
+//   TUNE_BATCH_LOAD   batch hash table load {33, 50, 66, 80} % (same batch workload)
 //
 // The reference's analogue is the per-variant cost table its planner reads
 // (host/dynamic_programming.cpp:133-185), measured offline on the FPGA; here
@@ -103,7 +103,7 @@ std::vector<Cand> candidates(int op) {
             for (int tiles : {1, 2, 4, 8})
                 for (int bt : {2048, 4096, 8192}) c.push_back({256, tiles, bt});
             break;
-        case TUNE_PROBE:
+        case TUNE_BATCH_LOAD:
             for (int load : {33, 50, 66, 80}) c.push_back({256, 4, load});
             break;
         default: break;
@@ -168,7 +168,7 @@ int make_workload(int op, size_t n, hipStream_t st, Workload &w) {
         w.run = [=](hipStream_t s) { return qo_groupby_agg(k, v, n, G, ok, oc, os, omn, omx, num, s); };
         return QO_OK;
     }
-    if (op == TUNE_BATCH || op == TUNE_PROBE) {
+    if (op == TUNE_BATCH || op == TUNE_BATCH_LOAD) {
         // Queries of |R| = n, |S| = 8n, filter passing 25 %; at most 2^28 S rows in all.
         const size_t ns = 8 * n;
         const int nq = (int)std::max<size_t>(1, std::min<size_t>(64, (size_t(1) << 28) / ns));

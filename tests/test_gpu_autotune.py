@@ -1,5 +1,6 @@
 """GPU: the autotuner searches on the device and keeps a candidate in the table."""
 import ctypes as C
+from pathlib import Path
 
 import pytest
 
@@ -18,4 +19,9 @@ def test_autotune_sets_a_candidate(qo, op, cls, choices):
         lib.call("qo_tune_get", op, cls, C.byref(gb), C.byref(gi), C.byref(gp))
         assert (gb.value, gi.value, gp.value) == (b.value, it.value, pa.value)
     finally:
+        # Back to the table libqo loaded at init (the committed tune file).
         lib.call("qo_tune_reset")
+        arch = qo.device.device_info()["arch"].split(":")[0]
+        f = Path(lib.LIB_PATH).parent / "tune" / f"{arch}.tune"
+        if f.exists():
+            lib.call("qo_tune_load", str(f).encode())

@@ -46,4 +46,4 @@ def test_committed_tune_file_is_well_formed():
         for line in f.read_text().split("\n"):
             if line.strip():
                 op, cls, b, i, p = map(int, line.split())
-                assert 0 <= op < 9 and 0 <= cls < 64 and b > 0 and i > 0
+                assert 0 <= op < 10 and 0 <= cls < 64 and b > 0 and i > 0
