@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void k_l2_probe(const uint64_t *__restrict__ b
                                                   SoaEmit emit, Gate gate) {
     __shared__ uint32_t s_tot[tile_offsets_words(kPairJ, 4)];
     __shared__ uint64_t s_excl;
-    __shared__ JoinStage stage;
+    __shared__ JoinStageT<kPairJ * 1024> stage;
     if (gate.off()) return;
     const uint32_t x = blockIdx.x & 7u, q = blockIdx.x >> 3;
     const uint32_t per_bucket = kXcdRegions * tpb;

@@ -152,10 +152,13 @@ QO_DEV void probe_one(const uint64_t *slots, uint64_t cap, const JoinHash &hs, u
 
 // LDS staging of one tile's matches (unique-key path), so that the tile is
 // written out with consecutive threads on consecutive rows (coalesced stores).
-constexpr uint32_t kStageRows = 1024 * 4;
-struct JoinStage {
-    int32_t key[kStageRows], rpay[kStageRows], spay[kStageRows];
+template <uint32_t ROWS>
+struct JoinStageT {
+    static constexpr uint32_t kRows = ROWS;
+    int32_t key[ROWS], rpay[ROWS], spay[ROWS];
 };
+constexpr uint32_t kStageRows = 1024 * 4;
+using JoinStage = JoinStageT<kStageRows>;
 
 // Looks up the block's tuples (J groups of 4 per thread; bit k of pass[j]
 // marks element k of group j live), reserves the block's output range with one
@@ -166,12 +169,12 @@ struct JoinStage {
 // block; without it, and on the general path, each thread emits its own
 // matches.  Must be called by all 256 threads of the block;
 // s_tot holds tile_offsets_words(J, 4) words.
-template <int J, class Emit>
+template <int J, class Emit, class Stage = JoinStage>
 QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, const JoinHash &hsh, bool general,
                         uint32_t sentinel,
                         const int32_t (&key)[J][4], const int32_t (&sp)[J][4], const uint32_t (&pass)[J],
                         unsigned long long *counter, const Emit &emit, uint32_t *s_tot, uint64_t *s_excl,
-                        JoinStage *stage = nullptr) {
+                        Stage *stage = nullptr) {
     uint32_t c[J], ex[J];
     uint32_t hit[J];
     // On a hit of the unique-key path sv holds the matching slot (its payload
@@ -244,7 +247,7 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
     const uint32_t total = tile_offsets<J, 4>(c, ex, s_tot);
     if (threadIdx.x == 0) *s_excl = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
     if (stage && !general) {
-        static_assert(J * 1024 <= kStageRows, "tile larger than the stage");
+        static_assert(J * 1024 <= Stage::kRows, "tile larger than the stage");
 #pragma unroll
         for (int j = 0; j < J; ++j) {
             uint32_t pos = ex[j];
