@@ -125,6 +125,15 @@ int qo_filter_agg_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi
 /* Stable selection lo <= (int)R.y <= hi with compaction into rout (capacity n). */
 int qo_filter_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi,
                       qo_record *rout, uint64_t *d_count, qo_stream s);
+/* No counterpart in the reference: the same compaction for callers that
+ * cannot allocate (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Allocates nothing: its scratch (look-back status
+ * words and ticket) is a fixed static device array of 1 MiB + 256 bytes, which
+ * bounds n to 2^31 records (16-byte-aligned rin) or 2^30 (otherwise); larger n
+ * fails with QO_ERR_INVALID_ARG.  Every call on a device shares that device's
+ * array, so calls that overlap on one device race: order them (one stream). */
+int qo_filter_records_noalloc(const qo_record *rin, size_t n, int32_t lo, int32_t hi, qo_record *rout,
+                              uint64_t *d_count, qo_stream s);
 /* Exclusive prefix sum of int32 (wraps modulo 2^32). in == out is allowed. */
 int qo_scan_exclusive_i32(const int32_t *in, int32_t *out, size_t n, qo_stream s);
 

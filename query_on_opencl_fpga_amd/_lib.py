@@ -94,6 +94,7 @@ PROTOS = {
     "qo_filter_agg": (C.c_int, [vp, vp, c_size, c_i32, c_i32, vp, vp]),
     "qo_filter_agg_records": (C.c_int, [vp, c_size, c_i32, c_i32, vp, vp]),
     "qo_filter_records": (C.c_int, [vp, c_size, c_i32, c_i32, vp, vp, vp]),
+    "qo_filter_records_noalloc": (C.c_int, [vp, c_size, c_i32, c_i32, vp, vp, vp]),
     "qo_scan_exclusive_i32": (C.c_int, [vp, vp, c_size, vp]),
     # sort
     "qo_sort_records": (C.c_int, [vp, c_size, vp]),

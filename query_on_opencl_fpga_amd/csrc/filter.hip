@@ -277,19 +277,21 @@ __global__ __launch_bounds__(NT) void k_filter_records(const qo_record *__restri
     if (tile == num_tiles - 1 && threadIdx.x == 0) *d_count = base + total;
 }
 
-static size_t filter_ws_bytes(size_t tiles) { return tiles * 8 + 256; }
+// Tiles of a compaction of n records starting at `rin` (its alignment picks the
+// tile width).
+static size_t filter_tiles(const qo_record *rin, size_t n) {
+    const size_t per_tile = (size_t)kFiltJ * kFiltThreads * ((((uintptr_t)rin % 16) == 0) ? 2 : 1);
+    return (n + per_tile - 1) / per_tile;
+}
 
-static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi, qo_record *rout,
-                                 uint64_t *d_count, hipStream_t st) {
+size_t filter_records_ws_bytes(const qo_record *rin, size_t n) { return filter_tiles(rin, n) * 8 + 256; }
+
+int launch_filter_records_ws(const qo_record *rin, size_t n, int32_t lo, int32_t hi, qo_record *rout,
+                             uint64_t *d_count, void *ws, hipStream_t st) {
     constexpr int J = kFiltJ, NT = kFiltThreads;
     const bool vec = ((uintptr_t)rin % 16) == 0;
-    const int VEC = vec ? 2 : 1;
-    const size_t per_tile = (size_t)J * NT * VEC;
-    const size_t tiles = (n + per_tile - 1) / per_tile;
+    const size_t tiles = filter_tiles(rin, n);
     QO_REQUIRE(tiles < 0xFFFFFFFFull, "input too large");
-    int status = QO_OK;
-    void *ws = workspace(st, filter_ws_bytes(tiles), &status);
-    if (!ws) return status;
     Carver cv(ws);
     uint64_t *stat = cv.take<uint64_t>(tiles);
     uint32_t *ticket = cv.take<uint32_t>(1);
@@ -302,6 +304,14 @@ static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int
                                                                    (uint32_t)tiles, dev_ctl());
     QO_LAUNCHED();
     return QO_OK;
+}
+
+static int launch_filter_records(const qo_record *rin, size_t n, int32_t lo, int32_t hi, qo_record *rout,
+                                 uint64_t *d_count, hipStream_t st) {
+    int status = QO_OK;
+    void *ws = workspace(st, filter_records_ws_bytes(rin, n), &status);
+    if (!ws) return status;
+    return launch_filter_records_ws(rin, n, lo, hi, rout, d_count, ws, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -36,6 +36,10 @@ DevCtl *dev_ctl();
 int check_device_error();
 // hipStreamSynchronize(st) followed by check_device_error().
 int sync_and_check(hipStream_t st);
+// Set-up of qo_filter_records_noalloc on `device` (filter_noalloc.hip): takes
+// the device address of its static scratch array.  Called by the library's
+// init with that device current.
+int filter_noalloc_setup(int device);
 
 // The stream to launch on: a caller stream, or the library's per-thread stream.
 hipStream_t stream_of(qo_stream s);
@@ -111,6 +115,13 @@ inline unsigned grid_for(size_t work, unsigned per_block, unsigned cap) {
 // Internal launchers used by several operators (all asynchronous on `st`).
 int launch_fill_u32(void *dst, uint32_t value, size_t count, hipStream_t st);
 int launch_fill_u64(void *dst, uint64_t value, size_t count, hipStream_t st);
+
+// Stable filter compaction of Records with caller-provided workspace
+// (filter_records_ws_bytes bytes: the look-back status words and the ticket;
+// cleared here by a fill kernel).
+size_t filter_records_ws_bytes(const qo_record *rin, size_t n);
+int launch_filter_records_ws(const qo_record *rin, size_t n, int32_t lo, int32_t hi, qo_record *rout,
+                             uint64_t *d_count, void *ws, hipStream_t st);
 
 // Exclusive scan with caller-provided workspace (see scan_workspace_bytes).
 size_t scan_workspace_bytes(size_t n);

@@ -83,6 +83,7 @@ static int init_locked(int device) {
     QO_REQUIRE(device < kMaxDevices, "device %d beyond the %d the library tracks", device, kMaxDevices);
     if (!g_ctl_dev[device])
         QO_HIP(hipGetSymbolAddress(reinterpret_cast<void **>(&g_ctl_dev[device]), HIP_SYMBOL(qo_dev_ctl)));
+    QO_TRY(filter_noalloc_setup(device));
     g_init = true;
     return QO_OK;
 }
