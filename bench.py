@@ -16,7 +16,7 @@ launch against the plain per-query path) at the configured sizes and for
 small queries.  --headline-only skips them.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--config join|filter_agg|groupby|batched|served|copy|ops] [--headline-only]
+                    [--config join|filter_agg|groupby|batched|served|copy|ops|plan] [--headline-only]
 
 Prints ONE JSON line.  `value` is S tuples per second of the whole join step
 (build + probe); the roofline block covers the probe stage, timed with HIP
@@ -364,8 +364,9 @@ def _time_op(st, steps, warmup, fn):
 def run_ops(args, qo, lib) -> dict:
     """The remaining operators of the reference's path, each at a large size:
     filter compaction, exclusive scan, Record sort, the reference's sort-based
-    group-by pipeline (sort -> startPos -> four aggregates), and filter + join +
-    sort (hj_filter_sort) on the join workload.  Each row: ms per call (HIP
+    group-by pipeline (sort -> startPos -> four aggregates), split, gather,
+    scatter, projection, reduce, Records -> columns, and filter + join + sort
+    (hj_filter_sort) on the join workload.  Each row: ms per call (HIP
     events on the call's stream) and algorithmic GB/s against the HBM peak."""
     import ctypes as C
     from query_on_opencl_fpga_amd import data
@@ -387,6 +388,7 @@ def run_ops(args, qo, lib) -> dict:
     y = data.gen_uniform(n, seed, 1, 0, 1 << 31)
     rec = DeviceBuffer(n * 8)
     qo._lib.call("qo_pack_records", x.ptr, y.ptr, n, rec.ptr, None)
+    qo._lib.call("qo_device_sync")  # generated on the per-thread stream; timed on st
     x.free()
     out = DeviceBuffer(n * 8)
     cnt = DeviceBuffer(8)
@@ -410,6 +412,7 @@ def run_ops(args, qo, lib) -> dict:
     y = data.gen_uniform(ns, seed, 3, 0, G)
     src = DeviceBuffer(ns * 8)
     qo._lib.call("qo_pack_records", x.ptr, y.ptr, ns, src.ptr, None)
+    qo._lib.call("qo_device_sync")  # generated on the per-thread stream; timed on st
     x.free()
     y.free()
     work = DeviceBuffer(ns * 8)
@@ -443,6 +446,43 @@ def run_ops(args, qo, lib) -> dict:
         "8*N + 28*G")
     for b in aggs + [sp, num, work, src]:
         b.free()
+    # Split, gather, scatter, projection, reduce and Record -> columns over
+    # n_sort Records (.x = row id, .y uniform 32-bit).
+    x = data.gen_perm(ns, seed + 1)
+    y = data.gen_uniform(ns, seed, 4, 0, 0)
+    src = DeviceBuffer(ns * 8)
+    qo._lib.call("qo_pack_records", x.ptr, y.ptr, ns, src.ptr, None)
+    # The inputs are generated on the per-thread stream and the operators timed
+    # on st: wait for them here.  (The split reads its input twice, histogram
+    # then placement; an input still being written between the two reads
+    # gives placement offsets that do not match the rows placed.)
+    qo._lib.call("qo_device_sync")
+    dst = DeviceBuffer(ns * 8)
+    bound = DeviceBuffer(65536 * 8)
+    for parts, note in ((32, "one radix pass on the partition id"), (1000, "two radix passes on the partition id")):
+        ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+            "qo_split_records", src.ptr, ns, parts, dst.ptr, bound.ptr, st.handle))
+        row(f"split_records, {parts} partitions ({note})", ns, ms, 16 * ns, "8*N read + 8*N written")
+    loc = x  # a permutation of [0, N): every gather / scatter target is distinct
+    ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+        "qo_gather_records", src.ptr, ns, loc.ptr, ns, dst.ptr, st.handle))
+    row("gather_records (random permutation)", ns, ms, 20 * ns, "4*N loc + 8*N gathered + 8*N written")
+    ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+        "qo_scatter_records", src.ptr, ns, loc.ptr, dst.ptr, st.handle))
+    row("scatter_records (random permutation)", ns, ms, 20 * ns, "8*N read + 4*N loc + 8*N scattered")
+    ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+        "qo_project_records", src.ptr, ns, src.ptr, ns, dst.ptr, st.handle))
+    row("project_records (row ids a random permutation)", ns, ms, 20 * ns, "8*N proj + 4*N .y gathered + 8*N written")
+    res = DeviceBuffer(4)
+    ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+        "qo_reduce_records", src.ptr, ns, 0, res.ptr, st.handle))
+    row("reduce_records SUM (no grouping)", ns, ms, 8 * ns, "8*N read")
+    cx, cy = DeviceBuffer(ns * 4), DeviceBuffer(ns * 4)
+    ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+        "qo_unpack_records", src.ptr, ns, cx.ptr, cy.ptr, st.handle))
+    row("unpack_records (Records -> x, y columns)", ns, ms, 16 * ns, "8*N read + 8*N written")
+    for b in (x, y, src, dst, bound, res, cx, cy):
+        b.free()
     # Filter + join + sort (hj_filter_sort) on the join workload.
     nr, nsj = args.nr, args.ns
     t = data.JoinTables(nr, nsj, SEED_BASE + 2)
@@ -466,6 +506,75 @@ def run_ops(args, qo, lib) -> dict:
             "roofline": {"bound": "hbm", "kernel": worst["op"], "achieved": worst["GBps_alg"],
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": worst["frac_hbm"], "traffic": None,
                          "avg_launch_ms": worst["ms"]}}
+
+
+def run_plan(args, qo, lib) -> dict:
+    """The cost-model planner on filter -> group-by -> aggregate queries over
+    n_sort Records: for every cover of each query's steps, the planner's
+    estimate beside the measured time of qo_plan_execute (HIP events around the
+    call on its stream; the call synchronises between segments), and which
+    cover the planner picks.  With --plan-calibrate the primitive costs are
+    measured on this device first."""
+    import ctypes as C
+    from query_on_opencl_fpga_amd import data
+    from query_on_opencl_fpga_amd import planner as P
+    from query_on_opencl_fpga_amd.device import DeviceBuffer, Stream
+
+    n = args.n_sort
+    st = Stream()
+    if args.plan_calibrate:
+        log("[bench] plan: calibrating the primitive costs (2^24 rows)")
+        P.calibrate(24, 5)
+    costs = {name: {"fixed_us": f, "gbps": g} for name, (f, g) in zip(P.COST_NAMES, P.cost_table())}
+    # Implementation sets that force each cover (a query takes those that cover it).
+    masks = [(1 << P.IMPL_FILTER) | (1 << P.IMPL_SORT_GROUP_AGG), (1 << P.IMPL_FILTER) | (1 << P.IMPL_HASH_GROUP_AGG),
+             1 << P.IMPL_FILTER_HASH_GROUP_AGG]
+    queries = []
+    for G in (1 << 10, 1 << 20):
+        queries.append((f"filter (half) -> group by .y (G = {G}) -> SUM(.x)", G,
+                        P.Query(filter=True, lo=0, hi=G // 2 - 1, group=True, op=0, selectivity=0.5, groups=G)))
+    queries.append(("group by .y (G = 2^20) -> MAX(.x)", 1 << 20,
+                    P.Query(group=True, op=1, groups=1 << 20)))
+    rows = []
+    for name, G, q in queries:
+        x = data.gen_uniform(n, SEED_BASE + 9, 0, 0, 0)
+        y = data.gen_uniform(n, SEED_BASE + 9, 1, 0, G)
+        rec = DeviceBuffer(n * 8)
+        qo._lib.call("qo_pack_records", x.ptr, y.ptr, n, rec.ptr, None)
+        qo._lib.call("qo_device_sync")
+        x.free()
+        y.free()
+        keys, aggs, cnt = DeviceBuffer(G * 4), DeviceBuffer(G * 4), DeviceBuffer(8)
+        qc = q.c()
+        choice = P.plan(q, n)
+        covers = []
+        for mask in masks:
+            try:
+                pl = P.plan(q, n, mask)
+            except qo._lib.QoError:
+                continue
+            if any(c["impls"] == pl.impls for c in covers):
+                continue  # the same cover under a wider implementation set
+            label = " -> ".join(P.IMPL_NAMES[i] for i in pl.impls)
+            ms = _time_op(st, max(1, args.steps // 2), 1, lambda: qo._lib.call(
+                "qo_plan_execute", C.byref(pl.raw), C.byref(qc), rec.ptr, n, G, None, keys.ptr, aggs.ptr, cnt.ptr,
+                st.handle))
+            g = int(cnt.to_numpy(np.uint64, 1)[0])
+            covers.append({"cover": label, "impls": pl.impls, "est_ms": pl.est_ms, "measured_ms": ms, "groups": g,
+                           "chosen": pl.impls == choice.impls})
+            log(f"[bench] plan: {name}: {label}: est {pl.est_ms:.3f} ms, measured {ms:.3f} ms")
+        best = min(covers, key=lambda c: c["measured_ms"])
+        chosen = next(c for c in covers if c["chosen"])
+        rows.append({"query": name, "n": n, "covers": covers, "planner_choice": chosen["cover"],
+                     "fastest_measured": best["cover"],
+                     "choice_over_fastest": chosen["measured_ms"] / best["measured_ms"]})
+        for b in (rec, keys, aggs, cnt):
+            b.free()
+    worst = max(rows, key=lambda r: r["choice_over_fastest"])
+    return {"metric": METRIC, "value": 1.0 / worst["choice_over_fastest"],
+            "unit": "fastest measured cover time / planner-chosen cover time (worst query)", "ms_per_step": None,
+            "dtype": "int32", "config": {"workload": "cost-model planner, filter -> group-by -> aggregate",
+                                         "costs": costs, "queries": rows}}
 
 
 def run_copy(args, qo, lib) -> dict:
@@ -787,7 +896,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="join",
-                    choices=["join", "filter_agg", "groupby", "batched", "served", "copy", "ops"])
+                    choices=["join", "filter_agg", "groupby", "batched", "served", "copy", "ops", "plan"])
+    ap.add_argument("--plan-calibrate", action="store_true",
+                    help="plan config: measure the planner's primitive costs on this device first")
     ap.add_argument("--load", type=int, default=0, help="join table load factor in percent (default: tuned)")
     ap.add_argument("--join-mode", choices=["auto", "partitioned", "direct", "lds"], default="auto",
                     help="join build/probe strategy (default: partitioned once the table outgrows one L2)")
@@ -840,7 +951,7 @@ def main() -> int:
         for cls in range(40):
             qo._lib.call("qo_tune_set", 7, cls, 256, 256, 2)
     run = {"join": run_join, "filter_agg": run_filter_agg, "groupby": run_groupby, "copy": run_copy,
-           "batched": run_batched, "served": run_served, "ops": run_ops}
+           "batched": run_batched, "served": run_served, "ops": run_ops, "plan": run_plan}
     if args.config not in run:
         print(f"config {args.config} is not available", file=sys.stderr)
         return 2

@@ -154,6 +154,34 @@ int qo_group_startpos(const qo_record *sorted, size_t n, int32_t *startPos,
 int qo_aggregate_groups(const qo_record *sorted, size_t n, const int32_t *startPos,
                         size_t numGroups, int op, int32_t *out, qo_stream s);
 
+/* ---- split, scatter, gather, projection, reduce ------------------------------ */
+/* Split into num_part partitions (1..65536) by pid = R.y % num_part, stable
+ * within a partition (splitImpl, host/host/testSplit.cpp:156-209; validateSplit,
+ * Validate.cpp:510-527).  rin and rout are distinct n-Record arrays; d_bound
+ * (num_part Records, may be NULL) receives {start, end} of every partition
+ * (getBound_int, testSplit.cpp:121-140). */
+int qo_split_records(const qo_record *rin, size_t n, uint32_t num_part, qo_record *rout,
+                     qo_record *d_bound, qo_stream s);
+/* S[pos] = R[loc[pos]] for pos < s_len (gatherImpl, testScatter.cpp:105-130);
+ * a loc outside [0, r_len) leaves S[pos] unchanged. */
+int qo_gather_records(const qo_record *R, size_t r_len, const int32_t *loc, size_t s_len,
+                      qo_record *S, qo_stream s);
+/* S[loc[i]] = R[i] for i < r_len (scatterImpl, testScatter.cpp:61-81; S holds
+ * r_len Records); a loc outside [0, r_len) writes nothing.  When several i
+ * target one position, one of them lands there, in no specified order (as in
+ * the reference). */
+int qo_scatter_records(const qo_record *R, size_t r_len, const int32_t *loc, qo_record *S, qo_stream s);
+/* out[i] = {proj[i].x, R[proj[i].x].y} (validateProjection, Validate.cpp:39-71);
+ * a row id proj[i].x >= r_len keeps proj[i].y. */
+int qo_project_records(const qo_record *R, size_t r_len, const qo_record *proj, size_t p_len,
+                       qo_record *out, qo_stream s);
+/* Reduce R.y without grouping into *d_result (device memory), as validateReduce
+ * computes it (Validate.cpp:372-468): op 0 SUM (unsigned, wraps mod 2^32),
+ * 1 MAX and 2 MIN (unsigned), 3 AVERAGE (the wrapped sum / n); n == 0 gives 0. */
+int qo_reduce_records(const qo_record *rin, size_t n, int op, uint32_t *d_result, qo_stream s);
+/* Columns of Records: x[i] = R[i].x, y[i] = R[i].y (inverse of qo_pack_records). */
+int qo_unpack_records(const qo_record *R, size_t n, uint32_t *x, uint32_t *y, qo_stream s);
+
 /* ---- hash group-by aggregation -------------------------------------------- */
 /* Per distinct key: (key, COUNT, SUM int64, MIN, MAX) ascending by key (signed).
  * group_capacity bounds the number of distinct keys (output arrays hold it).
@@ -248,6 +276,76 @@ int qo_batch_launch(qo_batch *b, qo_stream s);
 /* After completion: per-query match counts (host array of nq). */
 int qo_batch_counts(qo_batch *b, uint64_t *h_counts);
 int qo_batch_destroy(qo_batch *b);
+
+/* ---- cost-model planner ------------------------------------------------------- */
+/* A filter -> group-by -> aggregate query over Records, the chains the
+ * reference's planner schedules (host/dynamic_programming.cpp:874-1023:
+ * selection, group + aggregation, filter + group + aggregation): an optional
+ * range filter lo <= (int)R.y <= hi, then a per-group aggregate of (int)R.x
+ * grouped by R.y (group = 1; op 0 SUM, 1 MAX, 2 MIN, 3 AVERAGE with the
+ * reference's group semantics: SUM wraps, MAX starts at 0, MIN at 2^30,
+ * AVERAGE truncates), or the no-group reduce of R.y (group = 0, op >= 0), or
+ * nothing more (group = 0, op = -1: the selection itself). */
+typedef struct {
+    int filter;          /* 1: apply lo <= (int)R.y <= hi first */
+    int32_t lo, hi;
+    int group;           /* 1: group by R.y */
+    int op;              /* 0..3, or -1 for a selection */
+    double selectivity;  /* expected fraction of rows passing the filter (cost model only) */
+    double groups;       /* expected number of groups (cost model only) */
+} qo_query;
+/* Implementations a plan segment can use. */
+enum {
+    QO_IMPL_FILTER = 0,                /* one-pass stable compaction (selected Records materialized) */
+    QO_IMPL_SORT_GROUP_AGG = 1,        /* radix sort, group start positions, per-group aggregate */
+    QO_IMPL_HASH_GROUP_AGG = 2,        /* Records -> columns, hash group-by, reference-semantics epilogue */
+    QO_IMPL_FILTER_HASH_GROUP_AGG = 3, /* the same with the filter inside the group-by kernels */
+    QO_IMPL_REDUCE = 4,                /* no-group reduce */
+    QO_IMPL_NUM = 5
+};
+/* Cost-model primitives: each costs fixed_us + bytes moved / gbps. */
+enum {
+    QO_COST_FILTER = 0,     /* filter compaction: 8 B read + 8 B per selected row */
+    QO_COST_SORT_PASS = 1,  /* one radix pass: 24 B per row */
+    QO_COST_STARTPOS = 2,   /* group start positions: 8 B per row */
+    QO_COST_AGG = 3,        /* one per-group aggregate: 8 B per row */
+    QO_COST_UNPACK = 4,     /* Records -> columns or a Record copy: 16 B per row */
+    QO_COST_HASH_SMALL = 5, /* hash group-by, <= 1024 groups: 8 B per row */
+    QO_COST_HASH_LARGE = 6, /* hash group-by, more groups: 8 B per row */
+    QO_COST_REDUCE = 7,     /* no-group reduce: 8 B per row */
+    QO_COST_NUM = 8
+};
+#define QO_PLAN_MAX_SEGS 4
+typedef struct {
+    int nseg;                          /* segments, in execution order */
+    int impl[QO_PLAN_MAX_SEGS];        /* QO_IMPL_* */
+    int first_step[QO_PLAN_MAX_SEGS];  /* steps covered (0-based positions in the chain filter?, group/reduce) */
+    int last_step[QO_PLAN_MAX_SEGS];
+    double seg_ms[QO_PLAN_MAX_SEGS];   /* estimated ms per segment */
+    double est_ms;                     /* estimated ms of the whole query */
+    int tune_op[QO_PLAN_MAX_SEGS];     /* tune-table operator setting the segment's launch (-1: none) */
+    int block[QO_PLAN_MAX_SEGS], items[QO_PLAN_MAX_SEGS], param[QO_PLAN_MAX_SEGS];
+} qo_plan;
+/* Cheapest cover of the query's steps by implementations in allowed_impls (a
+ * bit mask of QO_IMPL_*; 0 = all) for n input rows: a dynamic programme over
+ * the chain's prefixes (dp_for_selection, host/dynamic_programming.cpp:874-996)
+ * with each segment costed from the primitive table.  Host only: needs no device. */
+int qo_plan_query(const qo_query *q, size_t n, unsigned allowed_impls, qo_plan *plan);
+/* The primitive cost table (defaults measured on MI355X; qo_plan_calibrate
+ * re-measures it on the current device).  Host only. */
+int qo_plan_cost_set(int prim, double fixed_us, double gbps);
+int qo_plan_cost_get(int prim, double *fixed_us, double *gbps);
+/* Times every primitive on synthetic data of 2^size_class rows on the device
+ * (median of reps) and stores the measured costs. */
+int qo_plan_calibrate(int size_class, int reps);
+/* Runs a plan on n device Records; synchronizes stream s (intermediate sizes
+ * size the later segments).  Outputs: a selection writes d_sel (n Records) and
+ * *d_count = rows; a grouped query writes d_keys[g] (R.y, ascending unsigned),
+ * d_aggs[g] and *d_count = groups (more than group_capacity fails with
+ * QO_ERR_INVALID_ARG); a reduce writes its uint32 result to d_aggs[0] and
+ * *d_count = rows reduced. */
+int qo_plan_execute(const qo_plan *plan, const qo_query *q, const qo_record *rin, size_t n, size_t group_capacity,
+                    qo_record *d_sel, uint32_t *d_keys, int32_t *d_aggs, uint64_t *d_count, qo_stream s);
 
 /* ---- autotuner -------------------------------------------------------------- */
 /* Operators: 0 filter_agg, 1 filter, 2 probe (param = table load %), 3 groupby, 4 sort,

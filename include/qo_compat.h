@@ -80,6 +80,39 @@ int range_selection(qo_mem d_Rin, int rLen, int rangeSmallKey, int rangeLargeKey
 int CL_RangeSelection(Record *h_Rin, int rLen, int rangeSmallKey, int rangeLargeKey, Record **h_Rout,
                       int numThreadPB, int numBlock, int _CPU_GPU);
 
+/* Point and device-buffer selection (host/host/Selection.cpp:53-129, a block the
+ * reference compiles out with #if 0): point_selection is filterImpl with
+ * smallKey = largeKey = matchingKeyValue; the *Only forms take device buffers
+ * and allocate *d_Rout (rLen Records) for the caller to free. */
+int point_selection(qo_mem d_Rin, int rLen, int matchingKeyValue, qo_mem *d_Rout, int numThreadPB, int numBlock,
+                    int *index, void *eventList, void *Kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+int CL_PointSelection(Record *h_Rin, int rLen, int matchingKeyValue, Record **h_Rout, int numThreadPB,
+                      int numBlock, int _CPU_GPU);
+int CL_PointSelectionOnly(qo_mem d_Rin, int rLen, int matchingKeyValue, qo_mem *d_Rout, int numThreadPB,
+                          int numBlock, int _CPU_GPU);
+int CL_RangeSelectionOnly(qo_mem d_Rin, int rLen, int rangeSmallKey, int rangeLargeKey, qo_mem *d_Rout,
+                          int numThreadPB, int numBlock, int _CPU_GPU);
+
+/* Split (host/host/testSplit.h:6, testSplit.cpp:156-209): d_S receives d_R
+ * split into numPart partitions by .y % numPart (stable within a partition);
+ * d_bound, when not NULL, numPart Records {start, end}.  SPLIT_TYPE is accepted
+ * and ignored, as the reference's splitImpl maps with SPLIT whatever it is
+ * given (testSplit.cpp:159). */
+void splitImpl(qo_mem d_R, int rLen, int numPart, qo_mem d_S, qo_mem d_bound, int numThreadPB, int numBlock,
+               int SPLIT_TYPE, int *index, void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden,
+               int _CPU_GPU);
+/* Scatter d_S[d_loc[i]] = d_R[i] and gather d_S[i] = d_R[d_loc[i]]
+ * (host/host/testScatter.h:2-7, testScatter.cpp:61-130); locations outside
+ * [0, rLen) are skipped.  scatterImpl_forPart is the same scatter (numPart only
+ * set the reference's number of windows). */
+void scatterImpl(qo_mem d_R, int rLen, qo_mem d_loc, qo_mem d_S, int numThreadPB, int numBlock, int *index,
+                 void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+void scatterImpl_forPart(qo_mem d_R, int rLen, int numPart, qo_mem d_loc, qo_mem d_S, int numThreadPB,
+                         int numBlock, int *index, void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden,
+                         int _CPU_GPU);
+void gatherImpl(qo_mem d_R, int rLen, qo_mem d_loc, qo_mem d_S, int sLen, int numThreadPB, int numBlock,
+                int *index, void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);
+
 /* Sort ascending by .y (host/host/testSort.cpp:24, host/host/Sort.cpp:34).  Stable. */
 void radixSortImpl_sm(qo_mem d_R, int rLen, int meaningless, int numThreadPB, int numBlock, int *index,
                       void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU);

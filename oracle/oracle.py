@@ -40,6 +40,12 @@ _PROTOS = {
     "or_hash_join": (sz, [vp, vp, sz, vp, vp, vp, sz, i32, i32, vp, vp, vp, sz]),
     "or_sort_join_rows": (None, [vp, vp, vp, sz]),
     "or_probe_pairs": (sz, [vp, sz, vp, sz, vp, sz]),
+    "or_split_records": (None, [vp, sz, u32, vp, vp]),
+    "or_gather_records": (None, [vp, sz, vp, sz, vp]),
+    "or_scatter_records": (None, [vp, sz, vp, vp]),
+    "or_project_records": (None, [vp, sz, vp, sz, vp]),
+    "or_reduce_records": (u32, [vp, sz, C.c_int]),
+    "or_unpack_records": (None, [vp, sz, vp, vp]),
     # multi-threaded restatement (qo_cpu_par.c): the timed CPU baseline
     "pr_threads": (C.c_int, [C.c_int]),
     "pr_filter_agg": (None, [vp, vp, sz, i32, i32, vp, C.c_int]),
@@ -202,3 +208,50 @@ def probe_pairs(r_table, s_table) -> np.ndarray:
     out = np.empty(max(2 * m, 2), np.uint32)
     lib.or_probe_pairs(_p(r), r.shape[0] // 2, _p(s), s.shape[0] // 2, _p(out), m)
     return out[: 2 * m].reshape(m, 2).copy()
+
+
+# ---- split / scatter / gather / projection / reduce ---------------------------
+def split_records(rec, num_part: int):
+    """(out, bound): stable split by .y % num_part; bound[p] = (start, end)."""
+    r = _c(rec, np.uint32).reshape(-1, 2)
+    out = np.empty_like(r)
+    bound = np.empty((num_part, 2), np.uint32)
+    load().or_split_records(_p(r), r.shape[0], num_part, _p(out), _p(bound))
+    return out, bound
+
+
+def gather_records(rec, loc, s_init=None) -> np.ndarray:
+    r = _c(rec, np.uint32).reshape(-1, 2)
+    lc = _c(loc, np.int32)
+    s = np.zeros((lc.shape[0], 2), np.uint32) if s_init is None else _c(s_init, np.uint32).reshape(-1, 2).copy()
+    load().or_gather_records(_p(r), r.shape[0], _p(lc), lc.shape[0], _p(s))
+    return s
+
+
+def scatter_records(rec, loc, s_init=None) -> np.ndarray:
+    r = _c(rec, np.uint32).reshape(-1, 2)
+    lc = _c(loc, np.int32)
+    s = np.zeros_like(r) if s_init is None else _c(s_init, np.uint32).reshape(-1, 2).copy()
+    load().or_scatter_records(_p(r), r.shape[0], _p(lc), _p(s))
+    return s
+
+
+def project_records(rec, proj) -> np.ndarray:
+    r = _c(rec, np.uint32).reshape(-1, 2)
+    pj = _c(proj, np.uint32).reshape(-1, 2)
+    out = np.empty_like(pj)
+    load().or_project_records(_p(r), r.shape[0], _p(pj), pj.shape[0], _p(out))
+    return out
+
+
+def reduce_records(rec, op: int) -> int:
+    r = _c(rec, np.uint32).reshape(-1, 2)
+    return int(load().or_reduce_records(_p(r), r.shape[0], op))
+
+
+def unpack_records(rec):
+    r = _c(rec, np.uint32).reshape(-1, 2)
+    x = np.empty(r.shape[0], np.uint32)
+    y = np.empty(r.shape[0], np.uint32)
+    load().or_unpack_records(_p(r), r.shape[0], _p(x), _p(y))
+    return x, y

@@ -27,6 +27,15 @@
  *   or_hash_join                  : closed form for permutation keys (one match per
  *                                   selected S row), brute-force nested loop.
  *   or_sort_join_rows             : invariants + golden vectors.
+ *   or_split_records              : validateSplit's condition, brute force (pure-Python
+ *                                   stable bucketing), bounds = closed form from counts.
+ *   or_gather_records /
+ *   or_scatter_records            : brute force; gather o scatter = identity on a
+ *                                   permutation; numpy fancy indexing.
+ *   or_project_records            : validateProjection's rule restated in numpy.
+ *   or_reduce_records             : closed forms (sum of 0..n-1 mod 2^32, max/min of
+ *                                   a permutation), hand-computed unsigned quirks.
+ *   or_unpack_records             : numpy column view of the interleaved array.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -451,4 +460,91 @@ size_t or_probe_pairs(const uint32_t *rTable, size_t nr, const uint32_t *sTable,
     }
     free(rk); free(rv); free(sk); free(sv); free(ok); free(orr); free(os);
     return m;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Split, scatter, gather, projection, reduce (the operators around the hot   */
+/* path; their reference kernels are absent, so each follows the host driver */
+/* and the validator that fixes its result).                                  */
+/* ------------------------------------------------------------------------- */
+
+/* Split into numPart partitions by pid = R.y % numPart (Record is uint2,
+ * host/host/common.h:26, so the remainder is unsigned): splitImpl
+ * (host/host/testSplit.cpp:156-209: map pid, per-partition counts, scan,
+ * write, bounds, scatter), checked by validateSplit (host/host/Validate.cpp:
+ * 510-527: pid non-decreasing along the output).  Records keep input order
+ * within a partition (a counting sort).  bound[p] = {start, end} of partition
+ * p in the output (getBound_int, testSplit.cpp:121-140: psSum at the
+ * partition's first slot, the next partition's, or rLen for the last). */
+void or_split_records(const or_record *in, size_t n, uint32_t numPart, or_record *out, or_record *bound)
+{
+    size_t *start = (size_t *)calloc((size_t)numPart + 1, sizeof(size_t));
+    for (size_t i = 0; i < n; ++i) start[in[i].y % numPart + 1]++;
+    for (uint32_t p = 0; p < numPart; ++p) start[p + 1] += start[p];
+    if (bound)
+        for (uint32_t p = 0; p < numPart; ++p) {
+            bound[p].x = (uint32_t)start[p];
+            bound[p].y = (uint32_t)start[p + 1];
+        }
+    for (size_t i = 0; i < n; ++i) out[start[in[i].y % numPart]++] = in[i];
+    free(start);
+}
+
+/* S[pos] = R[loc[pos]] for pos < sLen (optGather: the windows [from, to) of
+ * gatherImpl, host/host/testScatter.cpp:105-130, cover [0, rLen) between
+ * them).  A loc outside [0, rLen) leaves S[pos] as it was. */
+void or_gather_records(const or_record *R, size_t rLen, const int32_t *loc, size_t sLen, or_record *S)
+{
+    for (size_t pos = 0; pos < sLen; ++pos)
+        if (loc[pos] >= 0 && (size_t)loc[pos] < rLen) S[pos] = R[loc[pos]];
+}
+
+/* S[loc[i]] = R[i] for i < rLen (optScatter over the windows of scatterImpl,
+ * host/host/testScatter.cpp:61-81; S holds rLen records).  A loc outside
+ * [0, rLen) writes nothing.  With repeated targets the reference's order is
+ * unspecified; this restatement lets the last i win. */
+void or_scatter_records(const or_record *R, size_t rLen, const int32_t *loc, or_record *S)
+{
+    for (size_t i = 0; i < rLen; ++i)
+        if (loc[i] >= 0 && (size_t)loc[i] < rLen) S[loc[i]] = R[i];
+}
+
+/* Projection checked by validateProjection (host/host/Validate.cpp:39-71):
+ * out[i] = {proj[i].x, R[proj[i].x].y}.  A row id beyond rLen keeps the
+ * projection record's own .y. */
+void or_project_records(const or_record *R, size_t rLen, const or_record *proj, size_t pLen, or_record *out)
+{
+    for (size_t i = 0; i < pLen; ++i) {
+        out[i].x = proj[i].x;
+        out[i].y = proj[i].x < rLen ? R[proj[i].x].y : proj[i].y;
+    }
+}
+
+/* Reduce without grouping over R.y, as validateReduce computes it
+ * (host/host/Validate.cpp:372-468; operator codes common.h:89-92):
+ * SUM (0): unsigned sum mod 2^32; MAX (1) / MIN (2): from element 0, compared
+ * as unsigned (the int accumulator meets the uint .y in an unsigned
+ * comparison); AVERAGE (3): the unsigned sum mod 2^32 divided by rLen.
+ * n == 0 gives 0. */
+uint32_t or_reduce_records(const or_record *R, size_t n, int op)
+{
+    if (n == 0) return 0;
+    uint32_t acc = op == 1 || op == 2 ? R[0].y : 0u;
+    for (size_t i = op == 1 || op == 2 ? 1 : 0; i < n; ++i) {
+        const uint32_t v = R[i].y;
+        if (op == 1) acc = v > acc ? v : acc;
+        else if (op == 2) acc = v < acc ? v : acc;
+        else acc += v;
+    }
+    return op == 3 ? (uint32_t)(acc / (uint64_t)n) : acc;
+}
+
+/* Columns of Records: x[i] = R[i].x, y[i] = R[i].y (the AoS -> SoA step that
+ * feeds the column operators; Record layout host/host/common.h:26). */
+void or_unpack_records(const or_record *R, size_t n, uint32_t *x, uint32_t *y)
+{
+    for (size_t i = 0; i < n; ++i) {
+        x[i] = R[i].x;
+        y[i] = R[i].y;
+    }
 }

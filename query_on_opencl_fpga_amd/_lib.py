@@ -53,6 +53,22 @@ class ScanPara(C.Structure):
                 ("gLength", C.c_uint), ("blockSize", C.c_uint), ("ws", vp)]
 
 
+class QoQuery(C.Structure):
+    _fields_ = [("filter", C.c_int), ("lo", c_i32), ("hi", c_i32), ("group", C.c_int), ("op", C.c_int),
+                ("selectivity", C.c_double), ("groups", C.c_double)]
+
+
+PLAN_MAX_SEGS = 4
+
+
+class QoPlan(C.Structure):
+    _fields_ = [("nseg", C.c_int), ("impl", C.c_int * PLAN_MAX_SEGS), ("first_step", C.c_int * PLAN_MAX_SEGS),
+                ("last_step", C.c_int * PLAN_MAX_SEGS), ("seg_ms", C.c_double * PLAN_MAX_SEGS),
+                ("est_ms", C.c_double), ("tune_op", C.c_int * PLAN_MAX_SEGS),
+                ("block", C.c_int * PLAN_MAX_SEGS), ("items", C.c_int * PLAN_MAX_SEGS),
+                ("param", C.c_int * PLAN_MAX_SEGS)]
+
+
 # name -> (restype, argtypes).  restype C.c_int means a qo_status.
 PROTOS = {
     # runtime
@@ -90,6 +106,17 @@ PROTOS = {
     "qo_zipf_table": (C.c_int, [vp, c_u32, C.c_double]),
     "qo_gen_zipf_u32": (C.c_int, [vp, c_size, vp, c_u32, c_u64, c_u64, c_u64, vp]),
     "qo_pack_records": (C.c_int, [vp, vp, c_size, vp, vp]),
+    "qo_unpack_records": (C.c_int, [vp, c_size, vp, vp, vp]),
+    "qo_split_records": (C.c_int, [vp, c_size, c_u32, vp, vp, vp]),
+    "qo_gather_records": (C.c_int, [vp, c_size, vp, c_size, vp, vp]),
+    "qo_scatter_records": (C.c_int, [vp, c_size, vp, vp, vp]),
+    "qo_project_records": (C.c_int, [vp, c_size, vp, c_size, vp, vp]),
+    "qo_reduce_records": (C.c_int, [vp, c_size, C.c_int, vp, vp]),
+    "qo_plan_query": (C.c_int, [P(QoQuery), c_size, C.c_uint, P(QoPlan)]),
+    "qo_plan_cost_set": (C.c_int, [C.c_int, C.c_double, C.c_double]),
+    "qo_plan_cost_get": (C.c_int, [C.c_int, P(C.c_double), P(C.c_double)]),
+    "qo_plan_calibrate": (C.c_int, [C.c_int, C.c_int]),
+    "qo_plan_execute": (C.c_int, [P(QoPlan), P(QoQuery), vp, c_size, c_size, vp, vp, vp, vp, vp]),
     # filter
     "qo_filter_agg": (C.c_int, [vp, vp, c_size, c_i32, c_i32, vp, vp]),
     "qo_filter_agg_records": (C.c_int, [vp, c_size, c_i32, c_i32, vp, vp]),
@@ -147,6 +174,14 @@ PROTOS = {
     "range_selection": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(vp), C.c_int, C.c_int,
                                   vp, vp, vp, vp, vp, C.c_int]),
     "CL_RangeSelection": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(vp), C.c_int, C.c_int, C.c_int]),
+    "point_selection": (C.c_int, [vp, C.c_int, C.c_int, P(vp), C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
+    "CL_PointSelection": (C.c_int, [vp, C.c_int, C.c_int, P(vp), C.c_int, C.c_int, C.c_int]),
+    "CL_PointSelectionOnly": (C.c_int, [vp, C.c_int, C.c_int, P(vp), C.c_int, C.c_int, C.c_int]),
+    "CL_RangeSelectionOnly": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(vp), C.c_int, C.c_int, C.c_int]),
+    "splitImpl": (None, [vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
+    "scatterImpl": (None, [vp, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
+    "scatterImpl_forPart": (None, [vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
+    "gatherImpl": (None, [vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
     "radixSortImpl_sm": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),
     "CL_RadixSort": (None, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int]),
     "radixSortImpl": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int]),

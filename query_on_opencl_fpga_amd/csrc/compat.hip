@@ -215,6 +215,86 @@ int CL_RangeSelection(Record *h_Rin, int rLen, int rangeSmallKey, int rangeLarge
     return outSize;
 }
 
+int point_selection(qo_mem d_Rin, int rLen, int matchingKeyValue, qo_mem *d_Rout, int numThreadPB, int numBlock,
+                    int *index, void *eventList, void *Kernel, int *Flag_CPU_GPU, double *burden, int _CPU_GPU) {
+    return range_selection(d_Rin, rLen, matchingKeyValue, matchingKeyValue, d_Rout, numThreadPB, numBlock, index,
+                           eventList, Kernel, Flag_CPU_GPU, burden, _CPU_GPU);
+}
+
+int CL_PointSelection(Record *h_Rin, int rLen, int matchingKeyValue, Record **h_Rout, int numThreadPB, int numBlock,
+                      int _CPU_GPU) {
+    return CL_RangeSelection(h_Rin, rLen, matchingKeyValue, matchingKeyValue, h_Rout, numThreadPB, numBlock,
+                             _CPU_GPU);
+}
+
+int CL_PointSelectionOnly(qo_mem d_Rin, int rLen, int matchingKeyValue, qo_mem *d_Rout, int numThreadPB,
+                          int numBlock, int _CPU_GPU) {
+    return range_selection(d_Rin, rLen, matchingKeyValue, matchingKeyValue, d_Rout, numThreadPB, numBlock, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, _CPU_GPU);
+}
+
+int CL_RangeSelectionOnly(qo_mem d_Rin, int rLen, int rangeSmallKey, int rangeLargeKey, qo_mem *d_Rout,
+                          int numThreadPB, int numBlock, int _CPU_GPU) {
+    return range_selection(d_Rin, rLen, rangeSmallKey, rangeLargeKey, d_Rout, numThreadPB, numBlock, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, _CPU_GPU);
+}
+
+// ---- split / scatter / gather ---------------------------------------------------------
+// True when buffer m holds at least `count` elements of `elem` bytes.
+static bool holds(qo_mem m, size_t count, size_t elem) { return m && m->ptr && m->bytes >= count * elem; }
+
+void splitImpl(qo_mem d_R, int rLen, int numPart, qo_mem d_S, qo_mem d_bound, int, int, int /*SPLIT_TYPE*/, int *,
+               void *, void *, int *, double *, int) {
+    if (rLen < 0 || numPart < 1 || (rLen > 0 && (!holds(d_R, rLen, 8) || !holds(d_S, rLen, 8))) ||
+        (d_bound && !holds(d_bound, numPart, 8))) {
+        set_error(QO_ERR_INVALID_ARG, "splitImpl: bad arguments");
+        compat_fail("splitImpl");
+        return;
+    }
+    int rc = qo_split_records(d_R ? static_cast<const qo_record *>(d_R->ptr) : nullptr, (size_t)rLen,
+                              (uint32_t)numPart, d_S ? static_cast<qo_record *>(d_S->ptr) : nullptr,
+                              d_bound ? static_cast<qo_record *>(d_bound->ptr) : nullptr, nullptr);
+    if (rc == QO_OK) rc = sync_and_check(stream_of(nullptr));
+    QO_COMPAT_CHECK(rc, "splitImpl", );
+}
+
+void scatterImpl(qo_mem d_R, int rLen, qo_mem d_loc, qo_mem d_S, int, int, int *, void *, void *, int *, double *,
+                 int) {
+    if (rLen < 0 || (rLen > 0 && (!holds(d_R, rLen, 8) || !holds(d_loc, rLen, 4) || !holds(d_S, rLen, 8)))) {
+        set_error(QO_ERR_INVALID_ARG, "scatterImpl: bad arguments");
+        compat_fail("scatterImpl");
+        return;
+    }
+    if (rLen == 0) return;
+    int rc = qo_scatter_records(static_cast<const qo_record *>(d_R->ptr), (size_t)rLen,
+                                static_cast<const int32_t *>(d_loc->ptr), static_cast<qo_record *>(d_S->ptr), nullptr);
+    if (rc == QO_OK) rc = sync_and_check(stream_of(nullptr));
+    QO_COMPAT_CHECK(rc, "scatterImpl", );
+}
+
+void scatterImpl_forPart(qo_mem d_R, int rLen, int /*numPart*/, qo_mem d_loc, qo_mem d_S, int numThreadPB,
+                         int numBlock, int *index, void *eventList, void *kernel, int *Flag_CPU_GPU, double *burden,
+                         int _CPU_GPU) {
+    scatterImpl(d_R, rLen, d_loc, d_S, numThreadPB, numBlock, index, eventList, kernel, Flag_CPU_GPU, burden,
+                _CPU_GPU);
+}
+
+void gatherImpl(qo_mem d_R, int rLen, qo_mem d_loc, qo_mem d_S, int sLen, int, int, int *, void *, void *, int *,
+                double *, int) {
+    if (rLen < 0 || sLen < 0 || (rLen > 0 && !holds(d_R, rLen, 8)) ||
+        (sLen > 0 && (!holds(d_loc, sLen, 4) || !holds(d_S, sLen, 8)))) {
+        set_error(QO_ERR_INVALID_ARG, "gatherImpl: bad arguments");
+        compat_fail("gatherImpl");
+        return;
+    }
+    if (sLen == 0) return;
+    int rc = qo_gather_records(d_R ? static_cast<const qo_record *>(d_R->ptr) : nullptr, (size_t)rLen,
+                               static_cast<const int32_t *>(d_loc->ptr), (size_t)sLen,
+                               static_cast<qo_record *>(d_S->ptr), nullptr);
+    if (rc == QO_OK) rc = sync_and_check(stream_of(nullptr));
+    QO_COMPAT_CHECK(rc, "gatherImpl", );
+}
+
 // ---- sort -------------------------------------------------------------------------
 void radixSortImpl_sm(qo_mem d_R, int rLen, int, int, int, int *, void *, void *, int *, double *, int) {
     if (!d_R || rLen < 0) {

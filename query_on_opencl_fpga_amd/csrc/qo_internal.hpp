@@ -56,6 +56,11 @@ StreamKey stream_key_of(hipStream_t s);
 // Per-stream grow-only workspace.  Operations on one stream are ordered by the
 // stream, so one workspace per stream is never used by two operations at once.
 void *workspace(hipStream_t s, size_t bytes, int *status);
+// A second grow-only workspace per stream (slot 1) for a compound operation
+// whose own temporaries must outlive the operators it calls, which take slot 0
+// (qo_plan_execute).
+constexpr int kWorkspaceSlots = 2;
+void *workspace_slot(int slot, hipStream_t s, size_t bytes, int *status);
 
 // Carves aligned sub-buffers out of a workspace; a null base only measures.
 struct Carver {
@@ -129,6 +134,12 @@ int launch_scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, void *ws, 
 
 // LSD radix sort of 64-bit keys with optional 32-bit values over bits [lo_bit, hi_bit).
 size_t radix_workspace_bytes(size_t n, bool with_vals);
+// Stable split of n Records (as little-endian uint64) into `parts` partitions
+// by pid = hi32 % parts (1 <= parts <= 65536), in -> out (distinct arrays);
+// bound (may be null) receives {start, end} per partition.
+size_t split_records_ws_bytes(size_t n, uint32_t parts);
+int launch_split_records(const uint64_t *in, size_t n, uint32_t parts, uint64_t *out, uint2 *bound, void *ws,
+                         hipStream_t st);
 int launch_radix_sort_u64(uint64_t *keys, uint32_t *vals, size_t n, int lo_bit, int hi_bit, void *ws,
                           hipStream_t st);
 // Same, reading the element count from device memory (grid sized for `capacity`).

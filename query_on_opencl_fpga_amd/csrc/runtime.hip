@@ -32,7 +32,7 @@ struct Ws {
     void *ptr = nullptr;
     size_t bytes = 0;
 };
-std::map<StreamKey, Ws> g_ws;
+std::map<std::pair<int, StreamKey>, Ws> g_ws;  // (slot, stream) -> workspace
 void load_tune_file(const char *arch);
 }  // namespace
 
@@ -128,11 +128,13 @@ hipStream_t stream_of(qo_stream s) {
     return hipStreamPerThread;
 }
 
-void *workspace(hipStream_t s, size_t bytes, int *status) {
+void *workspace(hipStream_t s, size_t bytes, int *status) { return workspace_slot(0, s, bytes, status); }
+
+void *workspace_slot(int slot, hipStream_t s, size_t bytes, int *status) {
     *status = QO_OK;
     if (bytes == 0) bytes = 256;
     std::lock_guard<std::mutex> lk(g_mu);
-    Ws &w = g_ws[stream_key_of(s)];
+    Ws &w = g_ws[{slot, stream_key_of(s)}];
     if (w.bytes >= bytes) return w.ptr;
     if (w.ptr) {
         // The old workspace may still be read by queued work on this stream.
@@ -391,9 +393,10 @@ int qo_stream_destroy(qo_stream s) {
     hipStream_t st = static_cast<hipStream_t>(s);
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        auto it = g_ws.find(stream_key_of(st));
-        if (it != g_ws.end()) {
-            (void)hipStreamSynchronize(st);
+        (void)hipStreamSynchronize(st);
+        for (int slot = 0; slot < kWorkspaceSlots; ++slot) {
+            auto it = g_ws.find({slot, stream_key_of(st)});
+            if (it == g_ws.end()) continue;
             if (it->second.ptr) (void)hipFree(it->second.ptr);
             g_ws.erase(it);
         }

@@ -37,6 +37,15 @@ QO_DEV uint64_t match_digit(uint32_t d, bool valid) {
     return m;
 }
 
+// Digit of key k in the pass at `shift`: key bits [shift, shift + 8), or, with
+// MOD (a split into `mod` partitions), bits [shift, shift + 8) of the
+// partition id hi32(k) % mod, so that the passes order keys by partition id.
+template <bool MOD>
+QO_DEV uint32_t key_digit(uint64_t k, int shift, uint32_t mod) {
+    if constexpr (MOD) return (((uint32_t)(k >> 32) % mod) >> shift) & 255u;
+    else return (uint32_t)(k >> shift) & 255u;
+}
+
 // Index of the segment that owns global tile `tile` (0 for a single segment).
 QO_DEV int seg_index_of(const RadixSeg *segs, int nseg, uint32_t tile) {
     if (nseg <= 1) return 0;
@@ -64,7 +73,9 @@ constexpr uint32_t kRowsOneBlock = 256;   // up to this many tiles, one block sc
 
 // Digit counts of one tile (rem keys at src) into 256 LDS counters; `vec`:
 // src is 16-byte aligned, so two keys are read per load.
-QO_DEV void hist_tile(const uint64_t *__restrict__ src, uint32_t rem, int shift, uint32_t *s_row, bool vec) {
+template <bool MOD = false>
+QO_DEV void hist_tile(const uint64_t *__restrict__ src, uint32_t rem, int shift, uint32_t *s_row, bool vec,
+                      uint32_t mod = 0) {
     if (vec) {
         constexpr int PER = kSortTile / 512;
         uint4 q[PER];
@@ -79,23 +90,25 @@ QO_DEV void hist_tile(const uint64_t *__restrict__ src, uint32_t rem, int shift,
             if (2 * v + 2 <= rem) {
                 const uint64_t k0 = ((uint64_t)q[j].y << 32) | q[j].x;
                 const uint64_t k1 = ((uint64_t)q[j].w << 32) | q[j].z;
-                atomicAdd(&s_row[(k0 >> shift) & 255u], 1u);
-                atomicAdd(&s_row[(k1 >> shift) & 255u], 1u);
+                atomicAdd(&s_row[key_digit<MOD>(k0, shift, mod)], 1u);
+                atomicAdd(&s_row[key_digit<MOD>(k1, shift, mod)], 1u);
             } else if (2 * v < rem) {
-                atomicAdd(&s_row[(src[2 * v] >> shift) & 255u], 1u);
+                atomicAdd(&s_row[key_digit<MOD>(src[2 * v], shift, mod)], 1u);
             }
         }
     } else {
-        for (uint32_t i = threadIdx.x; i < rem; i += 256) atomicAdd(&s_row[(src[i] >> shift) & 255u], 1u);
+        for (uint32_t i = threadIdx.x; i < rem; i += 256) atomicAdd(&s_row[key_digit<MOD>(src[i], shift, mod)], 1u);
     }
 }
 
 // rows[t * 256 + d] = keys of global tile t whose digit is d (zero rows for
 // tiles past their segment's element count).  A single array passes its
 // segment as seg0 with segs = null; a batch passes the segment list.
+template <bool MOD = false>
 __global__ __launch_bounds__(256) void k_tile_hist(const uint64_t *__restrict__ keys, RadixSeg seg0,
                                                    const RadixSeg *segs, int nseg, uint32_t ntile, int shift,
-                                                   uint32_t *__restrict__ rows, const uint32_t *run_if = nullptr) {
+                                                   uint32_t *__restrict__ rows, const uint32_t *run_if = nullptr,
+                                                   uint32_t mod = 0) {
     if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     __shared__ uint32_t s_h[kHistTiles][256];
     for (int i = threadIdx.x; i < kHistTiles * 256; i += 256) (&s_h[0][0])[i] = 0;
@@ -110,8 +123,8 @@ __global__ __launch_bounds__(256) void k_tile_hist(const uint64_t *__restrict__ 
         const size_t tb = (size_t)(t - sg.tile0) * kSortTile;
         if (tb >= n) continue;
         const uint64_t *src = keys + sg.base + tb;
-        hist_tile(src, (uint32_t)std::min<size_t>(kSortTile, n - tb), shift, s_h[tt],
-                  (reinterpret_cast<uintptr_t>(src) & 15) == 0);
+        hist_tile<MOD>(src, (uint32_t)std::min<size_t>(kSortTile, n - tb), shift, s_h[tt],
+                       (reinterpret_cast<uintptr_t>(src) & 15) == 0, mod);
     }
     __syncthreads();
 #pragma unroll
@@ -300,14 +313,14 @@ __global__ __launch_bounds__(256) void k_rows_apply_seg(uint32_t *__restrict__ r
 // Places global tile xcd_tile(blockIdx.x): ranks its keys by digit (stable),
 // stages them in LDS in digit order and writes each digit's run at the offset
 // from its row (plus, for a batch, the digit's start in the segment's output).
-template <bool HAS_VAL, bool UNPACK = false>
+template <bool HAS_VAL, bool UNPACK = false, bool MOD = false>
 __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                      RadixSeg seg0, const RadixSeg *segs, int nseg, uint32_t ntile,
                                                      int shift, const uint32_t *__restrict__ offs,
                                                      const uint32_t *__restrict__ dstart, uint32_t sstride,
                                                      RowOut out0 = RowOut{}, const RowOut *outs = nullptr,
-                                                     const uint32_t *run_if = nullptr) {
+                                                     const uint32_t *run_if = nullptr, uint32_t mod = 0) {
     if (run_if && *run_if == 0) return;  // gated off: runs only when *run_if != 0
     __shared__ uint32_t s_cnt[4][256];
     __shared__ uint32_t s_dpre[256];
@@ -342,7 +355,7 @@ __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict_
     for (int j = 0; j < kSortItems; ++j) {
         const uint32_t idx = wb + j * kWave + lane;
         const bool valid = idx < rem;
-        const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+        const uint32_t d = key_digit<MOD>(k[j], shift, mod);
         const uint64_t m = match_digit(d, valid);
         const uint32_t pre = s_cnt[w][d];
         const uint32_t r = popc(m & lanemask_lt());
@@ -366,7 +379,7 @@ __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict_
     for (int j = 0; j < kSortItems; ++j) {
         const uint32_t idx = wb + j * kWave + lane;
         if (idx < rem) {
-            const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+            const uint32_t d = key_digit<MOD>(k[j], shift, mod);
             const uint32_t p = s_dpre[d] + s_cnt[w][d] + rank[j];
             s_k[p] = k[j];
             if (HAS_VAL) s_v[p] = v[j];
@@ -378,7 +391,7 @@ __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict_
         const RowOut o = outs ? outs[si] : out0;
         for (uint32_t i = threadIdx.x; i < rem; i += 256) {
             const uint64_t kk = s_k[i];
-            const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+            const uint32_t d = key_digit<MOD>(kk, shift, mod);
             const uint32_t gp = s_gofs[d] + (i - s_dpre[d]);
             o.key[gp] = join_sort_key_key(kk);
             o.spay[gp] = join_sort_key_spay(kk);
@@ -390,7 +403,7 @@ __global__ __launch_bounds__(256) void k_radix_place(const uint64_t *__restrict_
     uint32_t *vd = HAS_VAL ? vout + sg.base : nullptr;
     for (uint32_t i = threadIdx.x; i < rem; i += 256) {
         const uint64_t kk = s_k[i];
-        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        const uint32_t d = key_digit<MOD>(kk, shift, mod);
         const uint32_t gp = s_gofs[d] + (i - s_dpre[d]);
         kd[gp] = kk;
         if (HAS_VAL) vd[gp] = s_v[i];
@@ -610,6 +623,82 @@ int launch_radix_sort_u64(uint64_t *keys, uint32_t *vals, size_t n, int lo_bit, 
 int launch_radix_sort_u64_dev(uint64_t *keys, uint32_t *vals, const uint64_t *d_n, size_t capacity, int lo_bit,
                               int hi_bit, void *ws, hipStream_t st) {
     return radix_impl(keys, vals, capacity, d_n, lo_bit, hi_bit, ws, st);
+}
+
+// ---------------------------------------------------------------------------
+// Split of Records into `parts` partitions by pid = R.y % parts (stable within a
+// partition): the radix passes above with the partition id as the key, one pass
+// for parts <= 256, two (low, then high byte of the pid) up to 65536; then the
+// partition bounds by binary search over the split output.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSplitMaxParts = 65536;
+
+// bound[p] = {start, end} of partition p: the first output position whose pid
+// is >= p, and the same for p + 1 (n for the last partition).
+__global__ __launch_bounds__(256) void k_split_bounds(const uint64_t *__restrict__ out, size_t n, uint32_t parts,
+                                                      uint2 *__restrict__ bound) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= parts) return;
+    auto first_at_least = [&](uint32_t q) {
+        size_t lo = 0, hi = n;
+        while (lo < hi) {
+            const size_t mid = lo + (hi - lo) / 2;
+            if ((uint32_t)(out[mid] >> 32) % parts < q) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    bound[p] = uint2{(uint32_t)first_at_least(p), p + 1 < parts ? (uint32_t)first_at_least(p + 1) : (uint32_t)n};
+}
+
+size_t split_records_ws_bytes(size_t n, uint32_t parts) {
+    Carver cv(nullptr);
+    if (parts > 256) cv.take<uint64_t>(n ? n : 1);
+    rows_carve(radix_tiles(n) + 1, cv);
+    return cv.off + 256;
+}
+
+int launch_split_records(const uint64_t *in, size_t n, uint32_t parts, uint64_t *out, uint2 *bound, void *ws,
+                         hipStream_t st) {
+    QO_REQUIRE(parts >= 1 && parts <= kSplitMaxParts, "numPart %u outside [1, %u]", parts, kSplitMaxParts);
+    QO_REQUIRE(n < 0xFFFFFFFFull, "split input too large");
+    if (n > 0) {
+        Carver cv(ws);
+        uint64_t *alt = parts > 256 ? cv.take<uint64_t>(n) : nullptr;
+        const uint32_t tiles = radix_tiles(n);
+        const RowsBufs b = rows_carve(tiles + 1, cv);
+        const RadixSeg sg{0, n, nullptr, 0, tiles};
+        const RowsPlan rp = rows_plan(tiles);
+        const unsigned hblocks = (tiles + kHistTiles - 1) / kHistTiles;
+        const int npass = parts > 256 ? 2 : 1;
+        for (int pass = 0; pass < npass; ++pass) {
+            const uint64_t *src = pass == 0 ? in : alt;
+            uint64_t *dst = pass == npass - 1 ? out : alt;
+            const int sh = 8 * pass;
+            k_tile_hist<true><<<hblocks, 256, 0, st>>>(src, sg, nullptr, 1, tiles, sh, b.rows, nullptr, parts);
+            QO_LAUNCHED();
+            if (tiles <= kRowsOneBlock) {
+                k_rows_starts<<<1, 1024, 0, st>>>(b.rows, tiles);
+                QO_LAUNCHED();
+            } else {
+                k_rows_reduce<<<rp.nb, 256, 0, st>>>(b.rows, tiles, rp.span, b.bsum);
+                QO_LAUNCHED();
+                k_rows_starts<<<1, 1024, 0, st>>>(b.bsum, rp.nb);
+                QO_LAUNCHED();
+                k_rows_apply<<<rp.nb, 256, 0, st>>>(b.rows, tiles, rp.span, b.bsum);
+                QO_LAUNCHED();
+            }
+            k_radix_place<false, false, true><<<tiles, 256, 0, st>>>(src, nullptr, dst, nullptr, sg, nullptr, 1, tiles,
+                                                                      sh, b.rows, nullptr, 0, RowOut{}, nullptr,
+                                                                      nullptr, parts);
+            QO_LAUNCHED();
+        }
+    }
+    if (bound) {
+        k_split_bounds<<<(parts + 255) / 256, 256, 0, st>>>(out, n, parts, bound);
+        QO_LAUNCHED();
+    }
+    return QO_OK;
 }
 
 // (key, s_payload) as signed int32 -> order-preserving unsigned 64-bit key.

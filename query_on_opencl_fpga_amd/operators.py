@@ -57,6 +57,79 @@ def CL_RangeSelection(h_Rin, rangeSmallKey: int, rangeLargeKey: int, numThreadPB
     return res
 
 
+def CL_PointSelection(h_Rin, matchingKeyValue: int, numThreadPB: int = 256, numBlock: int = 128) -> np.ndarray:
+    """Records with (int)y == matchingKeyValue, in input order (Selection.cpp:65)."""
+    rin = _records(h_Rin)
+    lib = _lib.load()
+    out = C.c_void_p()
+    n = lib.CL_PointSelection(rin.ctypes.data, rin.shape[0], int(matchingKeyValue), C.byref(out), numThreadPB,
+                              numBlock, 0)
+    if n < 0:
+        raise _lib.QoError(n, "CL_PointSelection", _lib.last_error())
+    res = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint32)), shape=(max(n, 1) * 2,))[: 2 * n]
+    res = res.reshape(n, 2).copy()
+    _free_c(out.value)
+    return res
+
+
+def _mem_call(name: str, lib, *args) -> None:
+    _clear_error()
+    getattr(lib, name)(*args)
+    _raise_if_error(name)
+
+
+def splitImpl(h_R, numPart: int, numThreadPB: int = 128, numBlock: int = 128):
+    """(split records, bounds): records grouped by y % numPart, input order kept
+    within a partition; bounds[p] = (start, end) (testSplit.cpp:156-209)."""
+    r = _records(h_R)
+    n = r.shape[0]
+    lib = _lib.load()
+    with DeviceBuffer.from_numpy(r) as d_r, DeviceBuffer(max(n, 1) * 8) as d_s, \
+            DeviceBuffer(max(int(numPart), 1) * 8) as d_b:
+        m = [lib.qo_mem_wrap(b.ptr, b.nbytes) for b in (d_r, d_s, d_b)]
+        try:
+            _mem_call("splitImpl", lib, m[0], n, int(numPart), m[1], m[2], numThreadPB, numBlock, 4,
+                      None, None, None, None, None, 0)
+        finally:
+            for x in m:
+                lib.qo_mem_free(x)
+        return d_s.to_numpy(np.uint32, 2 * n).reshape(n, 2), d_b.to_numpy(np.uint32, 2 * int(numPart)).reshape(-1, 2)
+
+
+def scatterImpl(h_R, loc, h_S=None, numThreadPB: int = 256, numBlock: int = 128) -> np.ndarray:
+    """S[loc[i]] = R[i] (testScatter.cpp:61-81) over S (default zeros, len(R) records)."""
+    r = _records(h_R)
+    lc = _i32(loc)
+    s = np.zeros_like(r) if h_S is None else _records(h_S)
+    lib = _lib.load()
+    with DeviceBuffer.from_numpy(r) as d_r, DeviceBuffer.from_numpy(lc) as d_l, DeviceBuffer.from_numpy(s) as d_s:
+        m = [lib.qo_mem_wrap(b.ptr, b.nbytes) for b in (d_r, d_l, d_s)]
+        try:
+            _mem_call("scatterImpl", lib, m[0], r.shape[0], m[1], m[2], numThreadPB, numBlock,
+                      None, None, None, None, None, 0)
+        finally:
+            for x in m:
+                lib.qo_mem_free(x)
+        return d_s.to_numpy(np.uint32, s.size).reshape(-1, 2)
+
+
+def gatherImpl(h_R, loc, h_S=None, numThreadPB: int = 256, numBlock: int = 128) -> np.ndarray:
+    """S[i] = R[loc[i]] (testScatter.cpp:105-130) over S (default zeros, len(loc) records)."""
+    r = _records(h_R)
+    lc = _i32(loc)
+    s = np.zeros((lc.shape[0], 2), np.uint32) if h_S is None else _records(h_S)
+    lib = _lib.load()
+    with DeviceBuffer.from_numpy(r) as d_r, DeviceBuffer.from_numpy(lc) as d_l, DeviceBuffer.from_numpy(s) as d_s:
+        m = [lib.qo_mem_wrap(b.ptr, b.nbytes) for b in (d_r, d_l, d_s)]
+        try:
+            _mem_call("gatherImpl", lib, m[0], r.shape[0], m[1], m[2], lc.shape[0], numThreadPB, numBlock,
+                      None, None, None, None, None, 0)
+        finally:
+            for x in m:
+                lib.qo_mem_free(x)
+        return d_s.to_numpy(np.uint32, s.size).reshape(-1, 2)
+
+
 def CL_RadixSort(h_Rin, numThread: int = 256, numBlock: int = 128) -> np.ndarray:
     """Records sorted ascending by y (uint32); equal keys keep input order."""
     rin = _records(h_Rin)
@@ -309,6 +382,32 @@ def batch_hj_filter_sort(queries):
         for x in bufs:
             x.free()
     return out
+
+
+def projection(h_Rin, proj) -> np.ndarray:
+    """out[i] = (proj[i].x, Rin[proj[i].x].y) (validateProjection, Validate.cpp:39-71)."""
+    r, pj = _records(h_Rin), _records(proj)
+    with DeviceBuffer.from_numpy(r) as d_r, DeviceBuffer.from_numpy(pj) as d_p, \
+            DeviceBuffer(max(pj.nbytes, 8)) as d_o:
+        _lib.call("qo_project_records", d_r.ptr, r.shape[0], d_p.ptr, pj.shape[0], d_o.ptr, None)
+        return d_o.to_numpy(np.uint32, pj.size).reshape(-1, 2)
+
+
+def reduce(h_Rin, op: int) -> int:
+    """REDUCE_SUM / MAX / MIN / AVERAGE over y as validateReduce computes it (uint32)."""
+    r = _records(h_Rin)
+    with DeviceBuffer.from_numpy(r) as d_r, DeviceBuffer(4) as d_out:
+        _lib.call("qo_reduce_records", d_r.ptr, r.shape[0], int(op), d_out.ptr, None)
+        return int(d_out.to_numpy(np.uint32, 1)[0])
+
+
+def unpack_records(h_Rin):
+    """(x, y) columns of the records."""
+    r = _records(h_Rin)
+    n = r.shape[0]
+    with DeviceBuffer.from_numpy(r) as d_r, DeviceBuffer(max(n, 1) * 4) as dx, DeviceBuffer(max(n, 1) * 4) as dy:
+        _lib.call("qo_unpack_records", d_r.ptr, n, dx.ptr, dy.ptr, None)
+        return dx.to_numpy(np.uint32, n), dy.to_numpy(np.uint32, n)
 
 
 def sort_records(h_Rin) -> np.ndarray:

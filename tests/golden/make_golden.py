@@ -9,7 +9,10 @@ validators check (host/host/Validate.cpp: validateScan :4-37, validateFilter
 :317-370, validateSort :470-487, validateGroupBy :257-316) plus the
 reference's aggregate operator semantics (kernel/aggregation/1/
 group_by_cu_op_dp.cl:547-716) and the hash-join match definition
-(kernel/hj_filter_sort/1/hj_filter_sort_op.cl:62-109).
+(kernel/hj_filter_sort/1/hj_filter_sort_op.cl:62-109), and the results the
+split / scatter / gather / projection / reduce drivers and validators fix
+(host/host/testSplit.cpp:156-209, testScatter.cpp:61-130, Validate.cpp:39-71,
+372-468, 510-527).
 """
 from __future__ import annotations
 
@@ -86,6 +89,49 @@ def join(rk, rp, sk, sp, sf, flo, fhi):
     return rows
 
 
+def split(recs, parts):
+    """Bucket by y % parts, keeping input order inside each bucket (validateSplit,
+    Validate.cpp:510-527, checks the order of the bucket ids); bounds per bucket."""
+    buckets = [[] for _ in range(parts)]
+    for r in recs:
+        buckets[r[1] % parts].append(r)
+    out, bounds, pos = [], [], 0
+    for b in buckets:
+        bounds.append([pos, pos + len(b)])
+        out.extend(b)
+        pos += len(b)
+    return out, bounds
+
+
+def gather(recs, loc, s_init):
+    return [list(recs[l]) if 0 <= l < len(recs) else list(s) for l, s in zip(loc, s_init)]
+
+
+def scatter(recs, loc, s_init):
+    s = [list(x) for x in s_init]
+    for r, l in zip(recs, loc):
+        if 0 <= l < len(recs):
+            s[l] = list(r)
+    return s
+
+
+def project(recs, proj):
+    return [[p[0], recs[p[0]][1] if p[0] < len(recs) else p[1]] for p in proj]
+
+
+def reduce_(recs, op):
+    """validateReduce (Validate.cpp:372-468) on uint .y."""
+    ys = [r[1] for r in recs]
+    if not ys:
+        return 0
+    if op == 1:
+        return max(ys)
+    if op == 2:
+        return min(ys)
+    t = sum(ys) % M32
+    return t // len(ys) if op == 3 else t
+
+
 def main() -> None:
     rnd = random.Random(20240611)
     cases = {"scan": [], "filter": [], "sort": [], "groupby_sorted": [], "groupby_hash": [], "join": []}
@@ -120,6 +166,31 @@ def main() -> None:
         cases["join"].append({"rkey": rk, "rpay": rp, "skey": sk, "spay": sp, "sfilt": sf, "flo": 0, "fhi": 49,
                               "rows": rows,
                               "sorted": sorted(rows, key=lambda r: (r[0], r[2]))})
+    # Split / gather / scatter / projection / reduce (drawn after the cases
+    # above, so their inputs are unchanged).
+    cases.update({"split": [], "gather": [], "scatter": [], "project": [], "reduce": []})
+    for n, parts, kmax in ((0, 3, 10), (1, 1, 5), (37, 4, 1 << 32), (300, 32, 1 << 32), (500, 300, 1 << 32),
+                           (200, 7, 7)):
+        recs = [[i, rnd.randrange(kmax)] for i in range(n)]
+        out, bounds = split(recs, parts)
+        cases["split"].append({"in": recs, "parts": parts, "out": out, "bound": bounds})
+    for rl, sl in ((0, 4), (5, 0), (40, 60), (300, 300)):
+        recs = [[rnd.randrange(M32), rnd.randrange(M32)] for _ in range(rl)]
+        loc = [rnd.randrange(-3, rl + 3) for _ in range(sl)]
+        s_init = [[rnd.randrange(M32), rnd.randrange(M32)] for _ in range(sl)]
+        cases["gather"].append({"R": recs, "loc": loc, "S0": s_init, "S": gather(recs, loc, s_init)})
+    for rl in (0, 1, 50, 257):
+        recs = [[rnd.randrange(M32), rnd.randrange(M32)] for _ in range(rl)]
+        loc = [rnd.randrange(-2, rl + 2) for _ in range(rl)]
+        s_init = [[rnd.randrange(M32), rnd.randrange(M32)] for _ in range(rl)]
+        cases["scatter"].append({"R": recs, "loc": loc, "S0": s_init, "S": scatter(recs, loc, s_init)})
+    for rl, pl in ((0, 3), (10, 20), (300, 100)):
+        recs = [[rnd.randrange(M32), rnd.randrange(M32)] for _ in range(rl)]
+        proj = [[rnd.randrange(rl + 4), rnd.randrange(M32)] for _ in range(pl)]
+        cases["project"].append({"R": recs, "proj": proj, "out": project(recs, proj)})
+    for n, kmax in ((0, 5), (1, M32), (9, 100), (400, M32), (513, 1 << 31)):
+        recs = [[rnd.randrange(M32), rnd.randrange(kmax)] for _ in range(n)]
+        cases["reduce"].append({"in": recs, "out": [reduce_(recs, op) for op in range(4)]})
     OUT.write_text(json.dumps(cases))
     print(f"wrote {OUT} ({OUT.stat().st_size} bytes)")
 

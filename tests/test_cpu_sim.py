@@ -7,6 +7,9 @@ and kernel logic are exercised without a GPU.
 * serve_mix: the serving tests' four tables (scaled to 2%) and 24 mixed requests
   in batches of 8 through the serving loop, every result equal to the same
   request run alone on the plain path.
+* record_ops: split (one and two radix passes), gather, scatter, projection,
+  reduce and Records -> columns on the benchmark's input shape, each against a
+  plain loop.
 """
 import os
 import subprocess
@@ -21,8 +24,8 @@ HERE = Path(__file__).resolve().parent / "cpu_sim"
 @pytest.fixture(scope="module")
 def cpu_build():
     env = dict(os.environ, SIMT_VARIANT="fast")
-    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks"], check=True, env=env,
-                   timeout=900)
+    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks", "record_ops"],
+                   check=True, env=env, timeout=900)
     return HERE / "_build" / "fast"
 
 
@@ -40,3 +43,8 @@ def test_runtime_checks(cpu_build):
 def test_served_mix_equals_alone_cpu(cpu_build):
     out = _run(cpu_build / "serve_mix", 0.02, 24)
     assert out.count(" ok") == 24
+
+
+def test_record_ops_cpu(cpu_build):
+    out = _run(cpu_build / "record_ops", 16)
+    assert "split 32 partitions: 0 mismatches" in out and "split 1000 partitions: 0 mismatches" in out

@@ -178,3 +178,105 @@ def test_planted_mistakes_are_caught(orc, mutant):
     else:
         bad = [-v for v in good]
     assert bad != c["sum"]
+
+
+# ---- split / gather / scatter / projection / reduce ---------------------------------------
+def _pid_ok(out, parts):
+    """validateSplit (host/host/Validate.cpp:510-527): y % numPart never decreases."""
+    pid = out[:, 1].astype(np.uint64) % parts
+    return bool(np.all(np.diff(pid.astype(np.int64)) >= 0))
+
+
+@pytest.mark.parametrize("i", range(6))
+def test_split_brute_force(orc, i):
+    c = BF["split"][i]
+    out, bound = orc.split_records(rec(c["in"]), c["parts"])
+    assert out.tolist() == c["out"]
+    assert bound.tolist() == c["bound"]
+    assert _pid_ok(out, c["parts"])
+
+
+def test_split_invariants(orc):
+    """Permutation of the input, validateSplit's order, stability (row ids rise
+    inside a partition), and bounds = exclusive prefix of the partition counts."""
+    n, parts = 20_000, 37
+    r = np.stack([np.arange(n, dtype=np.uint32), orc.gen_uniform_u32(n, 5, 0, 0, 0)], 1)
+    out, bound = orc.split_records(r, parts)
+    assert _pid_ok(out, parts)
+    assert np.array_equal(np.sort(out[:, 0]), r[:, 0])
+    pid = out[:, 1] % parts
+    for p in (0, 5, 36):
+        ids = out[pid == p, 0]
+        assert np.all(np.diff(ids.astype(np.int64)) > 0)
+    counts = np.bincount(r[:, 1] % parts, minlength=parts)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    assert np.array_equal(bound[:, 0], starts) and np.array_equal(bound[:, 1], starts + counts)
+
+
+@pytest.mark.parametrize("i", range(4))
+def test_gather_brute_force(orc, i):
+    c = BF["gather"][i]
+    assert orc.gather_records(rec(c["R"]), c["loc"], rec(c["S0"])).tolist() == c["S"]
+
+
+@pytest.mark.parametrize("i", range(4))
+def test_scatter_brute_force(orc, i):
+    c = BF["scatter"][i]
+    assert orc.scatter_records(rec(c["R"]), c["loc"], rec(c["S0"])).tolist() == c["S"]
+
+
+def test_gather_scatter_closed_forms(orc):
+    """On a permutation p: gather(R, p) = R[p] (numpy fancy indexing), scatter
+    inverts it, and scatter(gather(R, p), p) = R."""
+    n = 5000
+    r = np.stack([orc.gen_uniform_u32(n, 1, 0, 0, 0), orc.gen_uniform_u32(n, 1, 1, 0, 0)], 1)
+    p = orc.gen_perm(n, 9).astype(np.int32)
+    g = orc.gather_records(r, p)
+    assert np.array_equal(g, r[p])
+    assert np.array_equal(orc.scatter_records(g, p), r)
+    s = np.zeros_like(r)
+    s[p] = r
+    assert np.array_equal(orc.scatter_records(r, p), s)
+
+
+@pytest.mark.parametrize("i", range(3))
+def test_project_brute_force(orc, i):
+    c = BF["project"][i]
+    assert orc.project_records(rec(c["R"]), rec(c["proj"])).tolist() == c["out"]
+
+
+@pytest.mark.parametrize("i", range(5))
+def test_reduce_brute_force(orc, i):
+    c = BF["reduce"][i]
+    assert [orc.reduce_records(rec(c["in"]), op) for op in range(4)] == c["out"]
+
+
+def test_reduce_closed_forms(orc):
+    """y = 0..n-1 (a permutation of it): SUM = n(n-1)/2 mod 2^32, MAX = n-1,
+    MIN = 0, AVERAGE = floor(SUM mod 2^32 / n); and unsigned comparison: a y of
+    2^31 is the maximum, not a negative minimum."""
+    n = 100_003
+    y = orc.gen_perm(n, 4)
+    r = np.stack([np.zeros(n, np.uint32), y], 1)
+    s = (n * (n - 1) // 2) % (1 << 32)
+    assert [orc.reduce_records(r, op) for op in range(4)] == [s, n - 1, 0, s // n]
+    q = rec([[0, 5], [0, 1 << 31], [0, 3]])
+    assert orc.reduce_records(q, 1) == 1 << 31 and orc.reduce_records(q, 2) == 3
+
+
+def test_unpack_records(orc):
+    r = np.stack([orc.gen_uniform_u32(1001, 2, 0, 0, 0), orc.gen_uniform_u32(1001, 2, 1, 0, 0)], 1)
+    x, y = orc.unpack_records(r)
+    assert np.array_equal(x, r[:, 0]) and np.array_equal(y, r[:, 1])
+
+
+def test_split_planted_mistake_is_caught(orc):
+    """A split that orders partitions by y // parts (a transposed operand) or
+    loses stability fails the fixtures the oracle passes."""
+    c = BF["split"][3]
+    recs = rec(c["in"])
+    wrong_key = recs[np.argsort(recs[:, 1] // c["parts"], kind="stable")]
+    assert wrong_key.tolist() != c["out"]
+    unstable = recs[np.argsort(recs[:, 1] % c["parts"], kind="stable")][::-1]
+    unstable = unstable[np.argsort(unstable[:, 1] % c["parts"], kind="stable")]
+    assert unstable.tolist() != c["out"]
