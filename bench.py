@@ -481,6 +481,9 @@ def run_ops(args, qo, lib) -> dict:
     ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
         "qo_unpack_records", src.ptr, ns, cx.ptr, cy.ptr, st.handle))
     row("unpack_records (Records -> x, y columns)", ns, ms, 16 * ns, "8*N read + 8*N written")
+    ms = _time_op(st, args.steps, args.warmup, lambda: qo._lib.call(
+        "qo_pack_records", cx.ptr, cy.ptr, ns, dst.ptr, st.handle))
+    row("pack_records (x, y columns -> Records)", ns, ms, 16 * ns, "8*N read + 8*N written")
     for b in (x, y, src, dst, bound, res, cx, cy):
         b.free()
     # Filter + join + sort (hj_filter_sort) on the join workload.

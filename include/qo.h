@@ -313,7 +313,9 @@ enum {
     QO_COST_HASH_SMALL = 5, /* hash group-by, <= 1024 groups: 8 B per row */
     QO_COST_HASH_LARGE = 6, /* hash group-by, more groups: 8 B per row */
     QO_COST_REDUCE = 7,     /* no-group reduce: 8 B per row */
-    QO_COST_NUM = 8
+    QO_COST_FHASH_SMALL = 8, /* hash group-by with the filter fused, half the rows selected, <= 1024 groups */
+    QO_COST_FHASH_LARGE = 9, /* the same, more groups */
+    QO_COST_NUM = 10
 };
 #define QO_PLAN_MAX_SEGS 4
 typedef struct {

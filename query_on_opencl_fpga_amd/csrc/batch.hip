@@ -81,18 +81,6 @@ __global__ __launch_bounds__(256) void k_b_build(const BQuery *q, int nq, uint32
     QO_TILE_LOOP_END
 }
 
-struct PackedEmit {
-    uint64_t *keys;
-    uint32_t *vals;
-    uint64_t cap;
-    QO_DEV void operator()(uint64_t pos, int32_t k, int32_t rp, int32_t sp) const {
-        if (pos < cap) {
-            keys[pos] = join_sort_key(k, sp);
-            vals[pos] = (uint32_t)rp;
-        }
-    }
-};
-
 // One ticket covers `group` consecutive S tiles (of any queries): fewer
 // same-address ticket atomics for a batch of ~10^6 tiles.  The launch bound
 // asks for 3 waves per SIMD (the 48 KB output stage allows no more): at 184
@@ -109,7 +97,9 @@ __global__ __launch_bounds__(256, 3) void k_b_probe(const BQuery *q, int nq, uin
     const int qi = query_of<1>(q, nq, t);
     const BQuery &Q = q[qi];
     const uint64_t *slots = reinterpret_cast<const uint64_t *>(Q.hdr + 1);
-    PackedEmit emit{mkeys + Q.m_base, mvals + Q.m_base, Q.out_cap};
+    // Sort-ready rows (join_sort_key, r_payload); every query's rows start at a
+    // multiple of 32, so the staged tiles are written 4 rows per thread.
+    const SoaEmit emit{nullptr, nullptr, nullptr, Q.out_cap, mkeys + Q.m_base, mvals + Q.m_base};
     if (Q.sfilt)
         probe_tile<true>(Q.hdr, slots, Q.skey, Q.spay, Q.sfilt, Q.ns, Q.flo, Q.fhi, Q.vec != 0, t - Q.stile0, Q.count,
                          emit, s_tot, &s_base, &stage);

@@ -54,10 +54,23 @@ __global__ void k_gen_zipf(uint32_t *out, size_t n, const uint64_t *__restrict__
     }
 }
 
-__global__ void k_pack_records(const uint32_t *x, const uint32_t *y, size_t n, qo_record *out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) out[i] = qo_record{x[i], y[i]};
+// out[i] = {x[i], y[i]}; with all three arrays 16-byte aligned, 4 Records per
+// thread: one 16-byte load per column, two 16-byte stores.
+__global__ void k_pack_records(const uint32_t *x, const uint32_t *y, size_t n, qo_record *out, bool vec) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t from = 0;
+    if (vec) {
+        const size_t n4 = n / 4;
+        for (size_t q = tid; q < n4; q += stride) {
+            const uint4 a = reinterpret_cast<const uint4 *>(x)[q], b = reinterpret_cast<const uint4 *>(y)[q];
+            uint4 *o = reinterpret_cast<uint4 *>(out) + 2 * q;
+            o[0] = uint4{a.x, b.x, a.y, b.y};
+            o[1] = uint4{a.z, b.z, a.w, b.w};
+        }
+        from = n4 * 4;
+    }
+    for (size_t i = from + tid; i < n; i += stride) out[i] = qo_record{x[i], y[i]};
 }
 
 static unsigned gen_grid(size_t n) { return grid_for(n, 256, (unsigned)props().cu_count * 16); }
@@ -139,7 +152,8 @@ int qo_pack_records(const uint32_t *x, const uint32_t *y, size_t n, qo_record *o
     QO_ENTER();
     if (n == 0) return QO_OK;
     QO_REQUIRE(x && y && out, "null pointer");
-    k_pack_records<<<gen_grid(n), 256, 0, stream_of(s)>>>(x, y, n, out);
+    const bool vec = ((uintptr_t)x % 16 == 0) && ((uintptr_t)y % 16 == 0) && ((uintptr_t)out % 16 == 0);
+    k_pack_records<<<gen_grid(n), 256, 0, stream_of(s)>>>(x, y, n, out, vec);
     QO_LAUNCHED();
     return QO_OK;
 }

@@ -48,6 +48,8 @@ static PrimCost g_cost[QO_COST_NUM] = {
     {15.0, 4300.0},  // QO_COST_HASH_SMALL: hash group-by, at most 1024 groups, 8 B per row
     {40.0, 1000.0},  // QO_COST_HASH_LARGE: hash group-by, more groups (partitioned), 8 B per row
     {8.0, 6000.0},   // QO_COST_REDUCE: no-group reduce, 8 B per row
+    {15.0, 5200.0},  // QO_COST_FHASH_SMALL: filtered hash group-by (half selected), <= 1024 groups
+    {40.0, 1200.0},  // QO_COST_FHASH_LARGE: filtered hash group-by (half selected), more groups
 };
 static std::mutex g_cost_mu;
 
@@ -109,8 +111,16 @@ static double impl_ms(int impl, const qo_query &q, double m, bool owns_input) {
         }
         case QO_IMPL_HASH_GROUP_AGG:
             return prim_ms(QO_COST_UNPACK, 16 * m) + prim_ms(hash, 8 * m + 64 * G);
-        case QO_IMPL_FILTER_HASH_GROUP_AGG:
-            return prim_ms(QO_COST_UNPACK, 16 * m) + prim_ms(hash, 8 * m + 64 * G);
+        case QO_IMPL_FILTER_HASH_GROUP_AGG: {
+            // The fused kernels read every row but update the tables for the
+            // selected ones only: time taken as linear in the selectivity s
+            // between the measured filtered group-by (s = 1/2) and the plain
+            // one (s = 1), and never below the filtered one's fixed cost.
+            const int fhash = G <= 1024.0 ? QO_COST_FHASH_SMALL : QO_COST_FHASH_LARGE;
+            const double t1 = prim_ms(hash, 8 * m + 64 * G), th = prim_ms(fhash, 8 * m + 64 * G);
+            const double t = std::max(g_cost[fhash].fixed_us * 1e-3, t1 + 2.0 * (1.0 - q.selectivity) * (th - t1));
+            return prim_ms(QO_COST_UNPACK, 16 * m) + t;
+        }
         case QO_IMPL_REDUCE: return prim_ms(QO_COST_REDUCE, 8 * m);
         default: return 1e30;
     }
@@ -353,10 +363,21 @@ static int calibrate_impl(int cls, int reps) {
     };
     QO_TRY(fit_cost(tm, reps, [&](hipStream_t st, size_t m) { return hash(st, m, g_small); },
                     8 * dn0 + 64.0 * g_small, 8 * dn + 64.0 * g_small, n0, n, &c[QO_COST_HASH_SMALL]));
+    // The same with the filter fused (.y below half the key range: half selected).
+    auto fhash = [&](hipStream_t st, size_t m, size_t cap, int32_t hi) {
+        return qo_groupby_agg_filtered(cy.as<int32_t>(), cx.as<int32_t>(), cy.as<int32_t>(), m, 0, hi, cap,
+                                       okey.as<int32_t>(), ocnt.as<int64_t>(), osum.as<int64_t>(),
+                                       omin.as<int32_t>(), omax.as<int32_t>(), cnt.as<uint64_t>(), st);
+    };
+    QO_TRY(fit_cost(tm, reps, [&](hipStream_t st, size_t m) { return fhash(st, m, g_small, g_small / 2 - 1); },
+                    8 * dn0 + 64.0 * g_small, 8 * dn + 64.0 * g_small, n0, n, &c[QO_COST_FHASH_SMALL]));
     QO_TRY(qo_unpack_records(rec2.as<qo_record>(), n, cx.as<uint32_t>(), cy.as<uint32_t>(), tm.st));
     QO_TRY(fit_cost(tm, reps, [&](hipStream_t st, size_t m) { return hash(st, m, g_large); },
                     8 * dn0 + 64.0 * std::min<double>(dn0, g_large), 8 * dn + 64.0 * g_large, n0, n,
                     &c[QO_COST_HASH_LARGE]));
+    QO_TRY(fit_cost(tm, reps, [&](hipStream_t st, size_t m) {
+        return fhash(st, m, g_large, (int32_t)(g_large / 2 - 1)); },
+        8 * dn0 + 64.0 * std::min<double>(dn0, g_large), 8 * dn + 64.0 * g_large, n0, n, &c[QO_COST_FHASH_LARGE]));
     // Reduce.
     QO_TRY(fit_cost(tm, reps, [&](hipStream_t st, size_t m) {
         return qo_reduce_records(R, m, 0, cnt.as<uint32_t>() + 8, st); }, 8 * dn0, 8 * dn, n0, n,

@@ -262,7 +262,12 @@ QO_DEV void lookup_emit(const uint64_t *__restrict__ slots, uint64_t cap_slots, 
         }
         __syncthreads();
         const uint64_t base = *s_excl;
-        for (uint32_t i = threadIdx.x; i < total; i += 256) emit(base + i, stage->key[i], stage->rpay[i], stage->spay[i]);
+        if constexpr (Emit::kFlush) {
+            emit.flush(base, total, stage->key, stage->rpay, stage->spay);
+        } else {
+            for (uint32_t i = threadIdx.x; i < total; i += 256)
+                emit(base + i, stage->key[i], stage->rpay[i], stage->spay[i]);
+        }
         return;
     }
     __syncthreads();
@@ -353,6 +358,7 @@ QO_DEV void probe_tile(const JoinHeader *__restrict__ hdr, const uint64_t *__res
 // int32 columns, or, when k64 is set, the sort-ready form of a join that is
 // sorted next: k64 = join_sort_key(key, s_payload), v32 = r_payload.
 struct SoaEmit {
+    static constexpr bool kFlush = true;  // writes a staged tile with flush()
     int32_t *okey, *orpay, *ospay;
     size_t cap;
     uint64_t *k64 = nullptr;
@@ -366,6 +372,47 @@ struct SoaEmit {
             okey[pos] = k;
             orpay[pos] = rp;
             ospay[pos] = sp;
+        }
+    }
+    // Rows [base, base + total) from LDS columns, written by all 256 threads of
+    // the block: the rows before the first 4-aligned output row and those past
+    // the last full group of 4 one per thread, the rest 4 rows per thread with
+    // 16-byte stores to each column (two to k64 in the sort-ready form), so a
+    // wave writes 1 KiB runs per column with a quarter of the store
+    // instructions.  Needs 16-byte-aligned columns; otherwise, and for rows
+    // past `cap`, rows go one per thread.
+    QO_DEV void flush(uint64_t base, uint32_t total, const int32_t *key, const int32_t *rp,
+                      const int32_t *sp) const {
+        const bool vec = base + total <= cap &&
+                         ((k64 ? (reinterpret_cast<uintptr_t>(k64) | reinterpret_cast<uintptr_t>(v32))
+                               : (reinterpret_cast<uintptr_t>(okey) | reinterpret_cast<uintptr_t>(orpay) |
+                                  reinterpret_cast<uintptr_t>(ospay))) &
+                          15u) == 0;
+        if (!vec) {
+            for (uint32_t i = threadIdx.x; i < total; i += 256) (*this)(base + i, key[i], rp[i], sp[i]);
+            return;
+        }
+        const uint32_t head = std::min<uint32_t>(total, (uint32_t)((4u - (base & 3u)) & 3u));
+        const uint32_t groups = (total - head) / 4u, tail0 = head + groups * 4u;
+        for (uint32_t i = threadIdx.x; i < head; i += 256) (*this)(base + i, key[i], rp[i], sp[i]);
+        for (uint32_t i = tail0 + threadIdx.x; i < total; i += 256) (*this)(base + i, key[i], rp[i], sp[i]);
+        for (uint32_t g = threadIdx.x; g < groups; g += 256) {
+            const uint32_t i = head + 4u * g;
+            const uint64_t pos = base + i;  // a multiple of 4
+            if (k64) {
+                uint64_t kk[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) kk[e] = join_sort_key(key[i + e], sp[i + e]);
+                uint4 *k4 = reinterpret_cast<uint4 *>(k64 + pos);
+                k4[0] = uint4{(uint32_t)kk[0], (uint32_t)(kk[0] >> 32), (uint32_t)kk[1], (uint32_t)(kk[1] >> 32)};
+                k4[1] = uint4{(uint32_t)kk[2], (uint32_t)(kk[2] >> 32), (uint32_t)kk[3], (uint32_t)(kk[3] >> 32)};
+                *reinterpret_cast<uint4 *>(v32 + pos) =
+                    uint4{(uint32_t)rp[i], (uint32_t)rp[i + 1], (uint32_t)rp[i + 2], (uint32_t)rp[i + 3]};
+            } else {
+                *reinterpret_cast<int4 *>(okey + pos) = int4{key[i], key[i + 1], key[i + 2], key[i + 3]};
+                *reinterpret_cast<int4 *>(orpay + pos) = int4{rp[i], rp[i + 1], rp[i + 2], rp[i + 3]};
+                *reinterpret_cast<int4 *>(ospay + pos) = int4{sp[i], sp[i + 1], sp[i + 2], sp[i + 3]};
+            }
         }
     }
 };

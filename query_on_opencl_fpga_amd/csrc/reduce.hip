@@ -9,14 +9,17 @@
 // commutative, so any combining order gives those bits exactly.
 //
 // One streaming pass: each block folds a grid-stride share of the Records
-// (16-byte loads of two Records), writes one partial, and the last block to
-// finish (a ticket counter) folds the partials and writes the result.
+// (16-byte loads of two Records, 8 in flight per thread, as many blocks per
+// CU as the filter-aggregate stream takes), writes one partial, and the last
+// block to finish (a ticket counter) folds the partials and writes the result.
+#include <algorithm>
+
 #include "qo_device.hpp"
 #include "qo_internal.hpp"
 
 namespace qo {
 
-constexpr unsigned kReduceBlocks = 2048;  // at most; 8 per CU on 256 CUs
+constexpr unsigned kReduceBlocks = 2048;  // partial slots in the workspace: at most 8 blocks per CU
 
 template <int OP>
 QO_DEV uint32_t fold(uint32_t a, uint32_t b) {
@@ -40,7 +43,9 @@ QO_DEV uint32_t block_fold(uint32_t v, uint32_t *s_w) {
     return v;
 }
 
-template <int OP>
+// UNROLL 16-byte loads (two Records each) in flight per thread before any is
+// folded; grid-stride over the 16-byte groups.
+template <int OP, int UNROLL>
 __global__ __launch_bounds__(256) void k_reduce_records(const uint2 *__restrict__ R, size_t n, bool vec,
                                                         uint32_t *__restrict__ partial, uint32_t *ticket,
                                                         uint32_t *__restrict__ result) {
@@ -52,11 +57,12 @@ __global__ __launch_bounds__(256) void k_reduce_records(const uint2 *__restrict_
         const uint4 *r4 = reinterpret_cast<const uint4 *>(R);
         const size_t n2 = n / 2;
         size_t q = tid;
-        for (; q + 3 * stride < n2; q += 4 * stride) {  // four 16-byte loads in flight
-            const uint4 a = ld_stream(r4 + q), b = ld_stream(r4 + q + stride);
-            const uint4 c = ld_stream(r4 + q + 2 * stride), d = ld_stream(r4 + q + 3 * stride);
-            acc = fold<OP>(acc, fold<OP>(fold<OP>(a.y, a.w), fold<OP>(b.y, b.w)));
-            acc = fold<OP>(acc, fold<OP>(fold<OP>(c.y, c.w), fold<OP>(d.y, d.w)));
+        for (; q + (UNROLL - 1) * stride < n2; q += UNROLL * stride) {
+            uint4 v[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) v[u] = ld_stream(r4 + q + u * stride);
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) acc = fold<OP>(acc, fold<OP>(v[u].y, v[u].w));
         }
         for (; q < n2; q += stride) {
             const uint4 a = ld_stream(r4 + q);
@@ -98,7 +104,12 @@ int qo_reduce_records(const qo_record *rin, size_t n, int op, uint32_t *d_result
     hipStream_t st = stream_of(s);
     if (n == 0) return launch_fill_u32(d_result, 0, 1, st);
     QO_REQUIRE(rin && (uintptr_t)rin % 8 == 0, "Records must be 8-byte aligned device memory");
-    const unsigned grid = grid_for((n + 7) / 8, 256, kReduceBlocks);
+    // The streaming shape of the filter-aggregate kernel (tune operator 0: blocks
+    // per CU), 8 loads in flight per thread.
+    const TuneParams tp = tune_get(TUNE_FILTER_AGG, n);
+    const unsigned per_cu = (unsigned)(tp.param > 0 ? tp.param : 1);
+    const unsigned cap = std::min<unsigned>(kReduceBlocks, (unsigned)props().cu_count * per_cu);
+    const unsigned grid = grid_for((n + 15) / 16, 256, cap);
     int status = QO_OK;
     void *ws = workspace(st, (size_t)kReduceBlocks * 4 + 256, &status);
     if (!ws) return status;
@@ -108,10 +119,10 @@ int qo_reduce_records(const qo_record *rin, size_t n, int op, uint32_t *d_result
     const uint2 *R = reinterpret_cast<const uint2 *>(rin);
     const bool vec = (uintptr_t)rin % 16 == 0;
     switch (op) {
-        case 0: k_reduce_records<0><<<grid, 256, 0, st>>>(R, n, vec, partial, ticket, d_result); break;
-        case 1: k_reduce_records<1><<<grid, 256, 0, st>>>(R, n, vec, partial, ticket, d_result); break;
-        case 2: k_reduce_records<2><<<grid, 256, 0, st>>>(R, n, vec, partial, ticket, d_result); break;
-        default: k_reduce_records<3><<<grid, 256, 0, st>>>(R, n, vec, partial, ticket, d_result); break;
+        case 0: k_reduce_records<0, 8><<<grid, 256, 0, st>>>(R, n, vec, partial, ticket, d_result); break;
+        case 1: k_reduce_records<1, 8><<<grid, 256, 0, st>>>(R, n, vec, partial, ticket, d_result); break;
+        case 2: k_reduce_records<2, 8><<<grid, 256, 0, st>>>(R, n, vec, partial, ticket, d_result); break;
+        default: k_reduce_records<3, 8><<<grid, 256, 0, st>>>(R, n, vec, partial, ticket, d_result); break;
     }
     QO_LAUNCHED();
     return QO_OK;

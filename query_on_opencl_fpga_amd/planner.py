@@ -26,7 +26,9 @@ IMPL_FILTER, IMPL_SORT_GROUP_AGG, IMPL_HASH_GROUP_AGG, IMPL_FILTER_HASH_GROUP_AG
 IMPL_NAMES = ["filter (one-pass compaction)", "sort + startPos + aggregate", "hash group-by",
               "filtered hash group-by (fused)", "reduce"]
 COST_NAMES = ["filter", "sort pass", "group start positions", "per-group aggregate", "Records -> columns",
-              "hash group-by (<= 1024 groups)", "hash group-by (partitioned)", "reduce"]
+              "hash group-by (<= 1024 groups)", "hash group-by (partitioned)", "reduce",
+              "filtered hash group-by, half selected (<= 1024 groups)",
+              "filtered hash group-by, half selected (partitioned)"]
 
 
 @dataclass

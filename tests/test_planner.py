@@ -62,14 +62,14 @@ def test_plan_is_the_cheapest_cover(qi, n):
 def test_plan_follows_the_cost_table():
     q = P.Query(filter=True, lo=0, hi=49, group=True, op=0, selectivity=0.5, groups=1 << 20)
     n = 1 << 26
-    # Hash group-by made very slow: the sort-based pipeline wins.
-    P.set_cost(5, 10.0, 1.0)
-    P.set_cost(6, 10.0, 1.0)
+    # Hash group-by (plain and filtered) made very slow: the sort-based pipeline wins.
+    for prim in (5, 6, 8, 9):
+        P.set_cost(prim, 10.0, 1.0)
     p = P.plan(q, n)
     assert p.impls == [P.IMPL_FILTER, P.IMPL_SORT_GROUP_AGG]
     # Radix passes made very slow instead: a hash implementation wins.
-    P.set_cost(5, 10.0, 4000.0)
-    P.set_cost(6, 10.0, 4000.0)
+    for prim in (5, 6, 8, 9):
+        P.set_cost(prim, 10.0, 4000.0)
     P.set_cost(1, 10.0, 1.0)
     assert P.IMPL_SORT_GROUP_AGG not in P.plan(q, n).impls
     # Filter compaction made very slow: the fused filter + hash group-by wins.
