@@ -526,8 +526,9 @@ def run_plan(args, qo, lib) -> dict:
     n = args.n_sort
     st = Stream()
     if args.plan_calibrate:
-        log("[bench] plan: calibrating the primitive costs (2^24 rows)")
-        P.calibrate(24, 5)
+        cls = min(26, max(16, n.bit_length() - 1))  # the workload's size class (at most 2^26 rows)
+        log(f"[bench] plan: calibrating the primitive costs (2^{cls} rows)")
+        P.calibrate(cls, 5)
     costs = {name: {"fixed_us": f, "gbps": g} for name, (f, g) in zip(P.COST_NAMES, P.cost_table())}
     # Implementation sets that force each cover (a query takes those that cover it).
     masks = [(1 << P.IMPL_FILTER) | (1 << P.IMPL_SORT_GROUP_AGG), (1 << P.IMPL_FILTER) | (1 << P.IMPL_HASH_GROUP_AGG),
