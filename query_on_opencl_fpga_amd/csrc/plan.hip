@@ -35,21 +35,22 @@
 namespace qo {
 
 // Primitive costs: fixed launch cost (us) and bandwidth over the bytes the
-// primitive moves (GB/s).  Defaults measured on one MI355X (2^26-2^28 inputs).
+// primitive moves (GB/s).  Defaults: qo_plan_calibrate at 2^26 rows on one
+// MI355X (the fixed cost is the time at 2^12 rows).
 struct PrimCost {
     double fixed_us, gbps;
 };
 static PrimCost g_cost[QO_COST_NUM] = {
-    {12.0, 4100.0},  // QO_COST_FILTER: filter compaction, 8 B read + 8 B per selected row written
-    {20.0, 4100.0},  // QO_COST_SORT_PASS: one radix pass, 24 B per row (histogram read, read, write)
-    {10.0, 3000.0},  // QO_COST_STARTPOS: group start positions, 8 B per row
-    {10.0, 4600.0},  // QO_COST_AGG: one per-group aggregate, 8 B per row
-    {8.0, 5000.0},   // QO_COST_UNPACK: Records -> columns (or a Record copy), 16 B per row
-    {15.0, 4300.0},  // QO_COST_HASH_SMALL: hash group-by, at most 1024 groups, 8 B per row
-    {40.0, 1000.0},  // QO_COST_HASH_LARGE: hash group-by, more groups (partitioned), 8 B per row
-    {8.0, 6000.0},   // QO_COST_REDUCE: no-group reduce, 8 B per row
-    {15.0, 5200.0},  // QO_COST_FHASH_SMALL: filtered hash group-by (half selected), <= 1024 groups
-    {40.0, 1200.0},  // QO_COST_FHASH_LARGE: filtered hash group-by (half selected), more groups
+    {13.0, 4300.0},  // QO_COST_FILTER: filter compaction, 8 B read + 8 B per selected row written
+    {15.0, 4000.0},  // QO_COST_SORT_PASS: one radix pass, 24 B per row (histogram read, read, write)
+    {12.0, 3300.0},  // QO_COST_STARTPOS: group start positions, 8 B per row
+    {28.0, 850.0},   // QO_COST_AGG: one per-group aggregate, 8 B per row (1024 groups)
+    {6.0, 5000.0},   // QO_COST_UNPACK: Records -> columns (or a Record copy), 16 B per row
+    {81.0, 3400.0},  // QO_COST_HASH_SMALL: hash group-by, at most 1024 groups, 8 B per row
+    {245.0, 1260.0}, // QO_COST_HASH_LARGE: hash group-by, more groups (partitioned), 8 B per row
+    {10.0, 6300.0},  // QO_COST_REDUCE: no-group reduce, 8 B per row
+    {75.0, 3960.0},  // QO_COST_FHASH_SMALL: filtered hash group-by (half selected), <= 1024 groups
+    {222.0, 1300.0}, // QO_COST_FHASH_LARGE: filtered hash group-by (half selected), more groups
 };
 static std::mutex g_cost_mu;
 
