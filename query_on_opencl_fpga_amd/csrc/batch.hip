@@ -146,12 +146,22 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
         // Rounded to 32 slots so every query's 256-byte header stays 256-byte aligned.
         q.table_cap = (std::max<uint64_t>(64, (uint64_t)((double)u.nr * 100.0 / load) + 1) + 31) / 32 * 32;
         QO_REQUIRE(q.table_cap <= 0xFFFFFFFFull, "query %d: build side too large", i);
+        // Tables too large for direct lookups take the partitioned probe (the
+        // batch's tables are hash-mixed: header of 32 slots, then the slots).
+        const size_t tbytes = (size_t)(32 + q.table_cap) * 8;
+        int part = 0, bits = 0, splits = 0;
+        join_part_strategy(tbytes, (size_t)u.ns, (int)kJoinHashMix, &part, &bits, &splits);
+        const bool partitioned = part != 0 && u.ns > 0;
+        if (partitioned) {
+            p.part_q.push_back(i);
+            p.part_ws_bytes = std::max(p.part_ws_bytes, join_part_ws_bytes(tbytes, (size_t)u.ns, false));
+        }
         q.vec = ((uintptr_t)u.skey % 16 == 0 && (uintptr_t)u.spay % 16 == 0 &&
                  (!u.sfilt || (uintptr_t)u.sfilt % 16 == 0)) ? 1u : 0u;
         q.rtile0 = (uint32_t)rt;
         q.nrtile = (uint32_t)((u.nr + p.build_tile - 1) / p.build_tile);
         q.stile0 = (uint32_t)stt;
-        q.nstile = (uint32_t)((u.ns + kProbeTile - 1) / kProbeTile);
+        q.nstile = partitioned ? 0u : (uint32_t)((u.ns + kProbeTile - 1) / kProbeTile);
         q.mtile0 = (uint32_t)mt;
         q.nmtile = (uint32_t)((u.out_capacity + kSortTile - 1) / kSortTile);
         q.m_base = p.total_match;
@@ -185,6 +195,7 @@ static void plan_carve(JoinBatchPlan &p, Carver &cv) {
     p.d_chunk0 = cv.take<uint32_t>(p.hchunk0.size());
     p.d_outs = cv.take<RowOut>(p.nq);
     p.sort_ws = cv.take<char>(radix_rows_seg_bytes(p.mt, p.nq, (uint32_t)p.hchunks.size()));
+    p.part_ws = p.part_ws_bytes ? cv.take<char>(p.part_ws_bytes) : nullptr;
 }
 
 size_t join_plan_bytes(JoinBatchPlan &p) {
@@ -234,6 +245,17 @@ int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
         k_b_probe<<<std::min<unsigned>(p.stt, persist), 256, 0, st>>>(p.d_q, p.nq, p.stt, tickets + 1, p.mkeys,
                                                                       p.mvals, p.probe_group);
         QO_LAUNCHED();
+    }
+    for (int i : p.part_q) {
+        const BQuery &q = p.hq[i];
+        JoinOut out;
+        out.cap = q.out_cap;
+        out.k64 = p.mkeys + q.m_base;
+        out.v32 = p.mvals + q.m_base;
+        bool hp = false, rp = false;
+        QO_TRY(launch_join_part_probe(q.hdr, (size_t)(32 + q.table_cap) * 8, q.skey, q.spay, q.sfilt, q.ns, q.flo,
+                                      q.fhi, out, reinterpret_cast<uint64_t *>(q.count), p.part_ws, st, &hp, &rp));
+        QO_REQUIRE(hp, "query %d: the partitioned probe did not take its hash-mixed table", i);
     }
     if (p.mt) {
         // Radix passes over the key bits only, then one pass that orders every

@@ -41,6 +41,11 @@ struct JoinBatchPlan {
     uint32_t build_tile = kBuildTile;        // R rows per build tile
     uint32_t probe_group = kProbeTicketTiles;  // S tiles per probe ticket
     size_t small_words = 0;
+    // Queries whose tables outgrow the direct probe: probed one after another by
+    // the plain path's partitioned probe (split of S by table slice, L2- or
+    // LDS-resident slices), sharing one workspace; the persistent probe skips them.
+    std::vector<int> part_q;
+    size_t part_ws_bytes = 0;
     // Bound by join_plan_bind:
     uint64_t *slots_all = nullptr;
     uint32_t *small = nullptr;  // tickets, then per-query match counts
@@ -52,6 +57,7 @@ struct JoinBatchPlan {
     uint32_t *d_chunk0 = nullptr;
     RowOut *d_outs = nullptr;
     void *sort_ws = nullptr;
+    void *part_ws = nullptr;
     unsigned long long *counts = nullptr;
 };
 

@@ -119,3 +119,30 @@ def test_batch_duplicate_keys_multiset(qo, orc):
         # sorted by (key, s_payload); rows equal in both may differ in order
         assert np.array_equal(g[:, [0, 2]], e[:, [0, 2]])
         assert np.array_equal(g[np.lexsort((g[:, 1], g[:, 2], g[:, 0]))], e[np.lexsort((e[:, 1], e[:, 2], e[:, 0]))])
+
+
+def test_batch_partitioned_probe_queries(qo, orc, queries):
+    """Queries whose tables outgrow direct lookups are probed by the plain
+    path's partitioned probe inside the batch graph.  With the partitioned join
+    forced for every size (tune operator 5, param 1 = always) every query of the
+    mix takes that path; results must still equal the oracle and the plain path."""
+    import ctypes as C
+
+    lib = qo._lib.load()
+    saved = []
+    for cls in range(0, 41):
+        b, it, pa = C.c_int(), C.c_int(), C.c_int()
+        qo._lib.call("qo_tune_get", 5, cls, C.byref(b), C.byref(it), C.byref(pa))
+        saved.append((cls, b.value, it.value, pa.value))
+    try:
+        for cls, b, it, _ in saved:
+            qo._lib.call("qo_tune_set", 5, cls, b, it, 1)
+        got = qo.operators.batch_hj_filter_sort(queries)
+        for q, g in zip(queries, got):
+            rk, rp, sk, sp, sf, flo, fhi = q
+            exp = orc.sort_join_rows(*orc.hash_join(rk, rp, sk, sp, sf, flo, fhi))
+            for a, e in zip(g, exp):
+                assert np.array_equal(a, e)
+    finally:
+        for cls, b, it, pa in saved:
+            lib.qo_tune_set(5, cls, b, it, pa)
