@@ -54,6 +54,9 @@ __global__ void k_b_init(const BQuery *q, int nq) {
     h->slots_alloc = q[i].table_cap;
     h->kmin = 0;
     h->shift = 0;
+    h->rmin = INT32_MAX;
+    h->rmax = INT32_MIN;
+    h->occupied = 0;
     *q[i].count = 0;
 }
 
@@ -122,7 +125,10 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
     p.hchunks.clear();
     p.hchunk0.assign(1, 0u);
     p.houts.assign(nq, RowOut{});
-    p.total_slots = p.total_match = 0;
+    p.total_slots = p.total_match = p.fill_slots = 0;
+    p.part_q.clear();
+    p.pbuild_q.clear();
+    p.part_ws_bytes = 0;
     uint64_t rt = 0, stt = 0, mt = 0;
     // Launch parameters by the size class of the batch's mean build side.
     uint64_t rsum = 0;
@@ -156,10 +162,15 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
             p.part_q.push_back(i);
             p.part_ws_bytes = std::max(p.part_ws_bytes, join_part_ws_bytes(tbytes, (size_t)u.ns, false));
         }
+        const size_t bws = u.nr ? join_build_ws_bytes(tbytes, (size_t)u.nr) : 0;
+        if (bws) {
+            p.pbuild_q.push_back(i);
+            p.part_ws_bytes = std::max(p.part_ws_bytes, bws);
+        }
         q.vec = ((uintptr_t)u.skey % 16 == 0 && (uintptr_t)u.spay % 16 == 0 &&
                  (!u.sfilt || (uintptr_t)u.sfilt % 16 == 0)) ? 1u : 0u;
         q.rtile0 = (uint32_t)rt;
-        q.nrtile = (uint32_t)((u.nr + p.build_tile - 1) / p.build_tile);
+        q.nrtile = bws ? 0u : (uint32_t)((u.nr + p.build_tile - 1) / p.build_tile);
         q.stile0 = (uint32_t)stt;
         q.nstile = partitioned ? 0u : (uint32_t)((u.ns + kProbeTile - 1) / kProbeTile);
         q.mtile0 = (uint32_t)mt;
@@ -172,6 +183,7 @@ int join_plan_init(const qo_join_query *queries, int nq, JoinBatchPlan &p) {
             p.hchunks.push_back(SweepChunk{(uint32_t)i, t0, std::min(q.nmtile, t0 + kSweepChunkTiles)});
         p.hchunk0.push_back((uint32_t)p.hchunks.size());
         p.total_slots += 32 + q.table_cap;  // 256-byte header = 32 slots
+        if (!bws) p.fill_slots += 32 + q.table_cap;
         p.total_match += (u.out_capacity + 31) / 32 * 32;
     }
     QO_REQUIRE(rt < 0xFFFFFFFFull && stt < 0xFFFFFFFFull && mt * 256 < 0xFFFFFFFFull, "batch too large");
@@ -207,12 +219,17 @@ size_t join_plan_bytes(JoinBatchPlan &p) {
 void join_plan_bind(JoinBatchPlan &p, void *arena) {
     Carver cv(arena);
     plan_carve(p, cv);
-    uint64_t slot_off = 0;
+    // Tables the persistent build fills come first, those of partitioned builds after.
+    uint64_t slot_off = 0, pslot_off = p.fill_slots;
+    size_t pb = 0;
     p.counts = reinterpret_cast<unsigned long long *>(p.small + 64);
     for (int i = 0; i < p.nq; ++i) {
         BQuery &q = p.hq[i];
-        q.hdr = reinterpret_cast<JoinHeader *>(p.slots_all + slot_off);
-        slot_off += 32 + q.table_cap;
+        const bool pbuild = pb < p.pbuild_q.size() && p.pbuild_q[pb] == i;
+        pb += pbuild;
+        uint64_t &off = pbuild ? pslot_off : slot_off;
+        q.hdr = reinterpret_cast<JoinHeader *>(p.slots_all + off);
+        off += 32 + q.table_cap;
         q.count = p.counts + i;
         p.hs[i] = RadixSeg{q.m_base, q.out_cap, reinterpret_cast<const uint64_t *>(q.count), q.mtile0, q.nmtile};
     }
@@ -232,7 +249,7 @@ int join_plan_upload(const JoinBatchPlan &p, hipStream_t st) {
 
 int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
     const unsigned persist = (unsigned)props().cu_count * 8;
-    QO_TRY(launch_fill_u64(p.slots_all, kEmptySlot, p.total_slots, st));
+    if (p.fill_slots) QO_TRY(launch_fill_u64(p.slots_all, kEmptySlot, p.fill_slots, st));
     QO_TRY(launch_fill_u32(p.small, 0, p.small_words, st));
     k_b_init<<<(p.nq + 255) / 256, 256, 0, st>>>(p.d_q, p.nq);
     QO_LAUNCHED();
@@ -240,6 +257,13 @@ int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
     if (p.rt) {
         k_b_build<<<std::min<unsigned>(p.rt, persist), 256, 0, st>>>(p.d_q, p.nq, p.rt, tickets + 0, p.build_tile);
         QO_LAUNCHED();
+    }
+    for (int i : p.pbuild_q) {
+        const BQuery &q = p.hq[i];
+        bool used = false;
+        QO_TRY(launch_join_part_build(q.hdr, (size_t)(32 + q.table_cap) * 8, q.rkey, q.rpay, q.nr, p.part_ws, st,
+                                      &used));
+        QO_REQUIRE(used, "query %d: the partitioned build did not take its table", i);
     }
     if (p.stt) {
         k_b_probe<<<std::min<unsigned>(p.stt, persist), 256, 0, st>>>(p.d_q, p.nq, p.stt, tickets + 1, p.mkeys,

@@ -45,6 +45,12 @@ struct JoinBatchPlan {
     // the plain path's partitioned probe (split of S by table slice, L2- or
     // LDS-resident slices), sharing one workspace; the persistent probe skips them.
     std::vector<int> part_q;
+    // Queries whose build side is large enough for the plain path's partitioned
+    // build (split of R by table slice, each slice built in LDS and written out
+    // whole); the persistent build skips them.  Their tables are placed after
+    // all others and need no fill, since every slot is written by the build.
+    std::vector<int> pbuild_q;
+    uint64_t fill_slots = 0;  // slots of the tables before them (filled empty)
     size_t part_ws_bytes = 0;
     // Bound by join_plan_bind:
     uint64_t *slots_all = nullptr;

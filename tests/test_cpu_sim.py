@@ -10,6 +10,10 @@ and kernel logic are exercised without a GPU.
 * record_ops: split (one and two radix passes), gather, scatter, projection,
   reduce and Records -> columns on the benchmark's input shape, each against a
   plain loop.
+* batch_parts: a batch of ten joins (one with an empty build side, one with a
+  single row) with the partitioned build and probe of large tables by their
+  rule, forced for every query and switched off, each query's rows equal to the
+  same query run alone through qo_hj_filter_sort.
 """
 import os
 import subprocess
@@ -24,7 +28,7 @@ HERE = Path(__file__).resolve().parent / "cpu_sim"
 @pytest.fixture(scope="module")
 def cpu_build():
     env = dict(os.environ, SIMT_VARIANT="fast")
-    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks", "record_ops"],
+    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks", "record_ops", "batch_parts"],
                    check=True, env=env, timeout=900)
     return HERE / "_build" / "fast"
 
@@ -48,3 +52,8 @@ def test_served_mix_equals_alone_cpu(cpu_build):
 def test_record_ops_cpu(cpu_build):
     out = _run(cpu_build / "record_ops", 16)
     assert "split 32 partitions: 0 mismatches" in out and "split 1000 partitions: 0 mismatches" in out
+
+
+def test_batch_parts_cpu(cpu_build):
+    out = _run(cpu_build / "batch_parts", 0.25)
+    assert out.count("0 of 10 queries differ") == 3
