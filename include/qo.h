@@ -163,7 +163,7 @@ int qo_aggregate_groups(const qo_record *sorted, size_t n, const int32_t *startP
 int qo_split_records(const qo_record *rin, size_t n, uint32_t num_part, qo_record *rout,
                      qo_record *d_bound, qo_stream s);
 /* S[pos] = R[loc[pos]] for pos < s_len (gatherImpl, testScatter.cpp:105-130);
- * a loc outside [0, r_len) leaves S[pos] unchanged. */
+ * a loc outside [0, r_len) leaves S[pos] unchanged.  Out of place: S != R. */
 int qo_gather_records(const qo_record *R, size_t r_len, const int32_t *loc, size_t s_len,
                       qo_record *S, qo_stream s);
 /* S[loc[i]] = R[i] for i < r_len (scatterImpl, testScatter.cpp:61-81; S holds
@@ -172,7 +172,7 @@ int qo_gather_records(const qo_record *R, size_t r_len, const int32_t *loc, size
  * the reference). */
 int qo_scatter_records(const qo_record *R, size_t r_len, const int32_t *loc, qo_record *S, qo_stream s);
 /* out[i] = {proj[i].x, R[proj[i].x].y} (validateProjection, Validate.cpp:39-71);
- * a row id proj[i].x >= r_len keeps proj[i].y. */
+ * a row id proj[i].x >= r_len keeps proj[i].y.  out may be proj, not R. */
 int qo_project_records(const qo_record *R, size_t r_len, const qo_record *proj, size_t p_len,
                        qo_record *out, qo_stream s);
 /* Reduce R.y without grouping into *d_result (device memory), as validateReduce

@@ -144,6 +144,7 @@ int qo_gather_records(const qo_record *R, size_t r_len, const int32_t *loc, size
     QO_ENTER();
     if (s_len == 0) return QO_OK;
     QO_REQUIRE(loc && S && (R || r_len == 0), "null pointer");
+    QO_REQUIRE(R != S || r_len == 0, "gather is out of place: R and S must differ");
     k_gather_by_loc<<<move_grid(s_len), 256, 0, stream_of(s)>>>(reinterpret_cast<const uint2 *>(R), r_len, loc,
                                                                   s_len, reinterpret_cast<uint2 *>(S));
     QO_LAUNCHED();
@@ -166,6 +167,7 @@ int qo_project_records(const qo_record *R, size_t r_len, const qo_record *proj, 
     QO_ENTER();
     if (p_len == 0) return QO_OK;
     QO_REQUIRE(proj && out && (R || r_len == 0), "null pointer");
+    QO_REQUIRE(out != R || r_len == 0, "projection writes out while reading R: out and R must differ");
     k_project_records<<<move_grid(p_len), 256, 0, stream_of(s)>>>(
         reinterpret_cast<const uint2 *>(R), r_len, reinterpret_cast<const uint2 *>(proj), p_len,
         reinterpret_cast<uint2 *>(out));

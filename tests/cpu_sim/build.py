@@ -102,10 +102,36 @@ def rewrite_launches(text: str) -> str:
 EXTERN_LDS = re.compile(r"extern\s+__shared__\s+(?:__attribute__\(\(aligned\(\d+\)\)\)\s+)?([\w:]+(?:\s+[\w:]+)*?)\s+(\w+)\[\];")
 
 
+CARVE_OLD = """        off += count * sizeof(T);
+        return p;"""
+CARVE_NEW = """        off += count * sizeof(T);
+        if (::simt::canary_on()) {
+            if (base) ::simt::canary_set(base + off, 256, count * sizeof(T), file, line);
+            off += 256;
+        }
+        return p;"""
+TAKE_OLD = "T *take(size_t count, size_t align = 256) {"
+TAKE_NEW = ("T *take(size_t count, size_t align = 256, const char *file = __builtin_FILE(), "
+            "int line = __builtin_LINE()) {")
+
+
+CARVE_FILL = re.compile(r"launch_fill_u32\((\w+), 0, cv\.off / 4, st\)")
+SCAN_WS_OLD = "    return tiles * 8 + 512;\n}"
+SCAN_WS_NEW = "    return tiles * 8 + 512 + (::simt::canary_on() ? 512 : 0);\n}"
+
+
 def rewrite(text: str) -> str:
     text = text.replace('#include "../../include/', '#include "')
     # The GPU build keeps the serving loop switched off; the CPU build runs it.
     text = re.sub(r'\n\s*if \(cfg\) return set_error\(QO_ERR_NOT_FOUND, "the serving loop is disabled[^\n]*', "", text)
+    # Carver: while canaries are enabled, 256 bytes of pattern after every
+    # buffer (in the sizing pass too, so sizes and offsets stay consistent).
+    text = text.replace(CARVE_OLD, CARVE_NEW).replace(TAKE_OLD, TAKE_NEW)
+    # A fill over a whole carve (several buffers zeroed at once) retires the
+    # canaries between them first, and the scan workspace leaves room for its
+    # two canaries.
+    text = CARVE_FILL.sub(r"launch_fill_u32(\1, 0, (::simt::canary_retire(\1, cv.off), cv.off / 4), st)", text)
+    text = text.replace(SCAN_WS_OLD, SCAN_WS_NEW)
     text = EXTERN_LDS.sub(lambda m: f"{m.group(1)} *{m.group(2)} = reinterpret_cast<{m.group(1)} *>(::simt::dyn_lds());",
                           text)
     return rewrite_launches(text)

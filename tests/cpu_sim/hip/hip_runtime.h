@@ -80,6 +80,11 @@ inline LaunchCfg cfg(dim3 g, dim3 b, size_t lds = 0, void *st = nullptr) { retur
 // Runs (or, under stream capture, records) a kernel: body() is called once per
 // work-item.
 void launch(const LaunchCfg &c, const char *name, std::function<void()> body);
+// Canaries after carved buffers (qo_sim_canary_enable / qo_sim_canary_check).
+bool canary_on();
+void canary_set(void *p, size_t n, size_t buf_bytes, const char *file = "", int line = 0);
+// Forgets the canaries inside [p, p + n) (memory about to be written whole).
+void canary_retire(const void *p, size_t n);
 
 }  // namespace simt
 

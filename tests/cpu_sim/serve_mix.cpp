@@ -1,11 +1,21 @@
 // CPU build driver: the served mixed batch of the serving tests (four tables
 // of 50K-3M rows, 24 mixed requests in batches of 8), each result compared
 // with the same request run alone on the plain path.
+//
+//     serve_mix [scale] [requests] [canary | canary-part]
+//
+// With "canary" every buffer carved from a batch's arena (and from the
+// workspaces carved inside it) is followed by 256 bytes of pattern, checked
+// after each batch: a kernel that writes past its buffer damages it.
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "qo.h"
+
+extern "C" void qo_sim_canary_enable(int on);
+extern "C" long qo_sim_canary_check(void);
 
 #define CK(x)                                                                         \
     do {                                                                              \
@@ -69,14 +79,35 @@ int main(int argc, char **argv) {
                             {1, 0, -1, 255}, {1, 3, -1, 397}, {0, 3, -1, 373}, {1, 2, -1, 215}, {1, 0, -1, 396},
                             {1, 3, -1, 411}, {2, 1, 2, 311}, {1, 3, -1, 388}, {0, 2, -1, 418}};
     const int nreq = argc > 2 ? atoi(argv[2]) : 24;
+    // "canary": one batch of 8 per run, with a canary after every buffer carved
+    // from the batch's arena, checked after each batch (the kernels of this
+    // loop run as they are launched, so a workspace carved again by a later
+    // step has retired the earlier canaries it covers).
+    // "canary-part": the same with the partitioned join build and probe forced
+    // for every table (tune operator 5, param 1).
+    const bool canary = argc > 3 && !strncmp(argv[3], "canary", 6);
+    if (argc > 3 && !strcmp(argv[3], "canary-part"))
+        for (int cls = 0; cls <= 40; ++cls) {
+            int b = 0, it = 0, pa = 0;
+            CK(qo_tune_get(5, cls, &b, &it, &pa));
+            CK(qo_tune_set(5, cls, b, it, 1));
+        }
     std::vector<qo_request> reqs;
+    long damaged = 0;
     for (int i = 0; i < nreq; ++i) {
         qo_request r{mix[i][0], ids[mix[i][1]], mix[i][2] < 0 ? -1 : ids[mix[i][2]], INT32_MIN, mix[i][3]};
         uint64_t id;
         CK(qo_server_submit(sv, &r, &id));
         reqs.push_back(r);
+        if (canary && (i % 8 == 7 || i + 1 == nreq)) {
+            qo_sim_canary_enable(1);
+            CK(qo_server_run(sv));
+            qo_sim_canary_enable(0);
+            damaged += qo_sim_canary_check();
+        }
     }
-    CK(qo_server_run(sv));
+    if (canary) printf("canaries: %ld damaged\n", damaged);
+    else CK(qo_server_run(sv));
     std::vector<qo_result> res(nreq);
     const int got = qo_server_poll(sv, res.data(), nreq, 0);
     fprintf(stderr, "[serve_mix] %d results\n", got);
@@ -93,6 +124,7 @@ int main(int argc, char **argv) {
         bad += !ok;
     }
     CK(qo_server_destroy(sv));
+    bad += damaged != 0;
     printf("%s\n", bad ? "FAIL" : "PASS");
     return bad ? 1 : 0;
 }

@@ -4,9 +4,12 @@
 
 #include <sys/mman.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -338,7 +341,109 @@ void launch(const LaunchCfg &c, const char *name, std::function<void()> body) {
     g_kernel = "";
 }
 
+// Canaries after carved buffers (see Carver::take in the CPU build): while
+// enabled, every buffer a Carver hands out is followed by 256 bytes of
+// pattern; a kernel that writes past its buffer into the next one damages it.
+namespace {
+constexpr unsigned char kCanary = 0xCB;
+std::mutex g_canary_mu;
+std::vector<std::pair<unsigned char *, size_t>> g_canaries;
+std::vector<size_t> g_canary_buf;  // bytes of the buffer each canary follows
+std::vector<std::string> g_canary_at;  // file:line of the take() that carved that buffer
+std::map<char *, size_t> g_allocs;  // device allocation -> bytes
+bool g_canary_on = false;
+}  // namespace
+bool canary_on() { return g_canary_on; }
+void canary_set(void *p, size_t n, size_t buf_bytes, const char *file, int line) {
+    std::lock_guard<std::mutex> lk(g_canary_mu);
+    // A later carve of the same memory (a workspace reused by the next step)
+    // retires the canaries its buffer and canary cover.
+    const unsigned char *lo = static_cast<unsigned char *>(p) - buf_bytes, *hi = static_cast<unsigned char *>(p) + n;
+    size_t k = 0;
+    for (size_t i = 0; i < g_canaries.size(); ++i) {
+        const unsigned char *c = g_canaries[i].first;
+        if (c < hi && c + g_canaries[i].second > lo) continue;
+        g_canaries[k] = g_canaries[i];
+        g_canary_buf[k] = g_canary_buf[i];
+        g_canary_at[k] = g_canary_at[i];
+        ++k;
+    }
+    g_canaries.resize(k);
+    g_canary_buf.resize(k);
+    g_canary_at.resize(k);
+    memset(p, kCanary, n);
+    g_canaries.emplace_back(static_cast<unsigned char *>(p), n);
+    g_canary_buf.push_back(buf_bytes);
+    const char *slash = strrchr(file, '/');
+    g_canary_at.push_back(std::string(slash ? slash + 1 : file) + ":" + std::to_string(line));
+}
+void canary_retire(const void *p, size_t n) {
+    std::lock_guard<std::mutex> lk(g_canary_mu);
+    const unsigned char *lo = static_cast<const unsigned char *>(p), *hi = lo + n;
+    size_t k = 0;
+    for (size_t i = 0; i < g_canaries.size(); ++i) {
+        const unsigned char *c = g_canaries[i].first;
+        if (c < hi && c + g_canaries[i].second > lo) continue;
+        g_canaries[k] = g_canaries[i];
+        g_canary_buf[k] = g_canary_buf[i];
+        g_canary_at[k] = g_canary_at[i];
+        ++k;
+    }
+    g_canaries.resize(k);
+    g_canary_buf.resize(k);
+    g_canary_at.resize(k);
+}
+// Forgets the canaries inside an allocation that is being freed.
+static void canary_forget(char *p) {
+    std::lock_guard<std::mutex> lk(g_canary_mu);
+    auto a = g_allocs.find(p);
+    if (a == g_allocs.end()) return;
+    const size_t n = a->second;
+    g_allocs.erase(a);
+    size_t k = 0;
+    for (size_t i = 0; i < g_canaries.size(); ++i) {
+        const char *c = (const char *)g_canaries[i].first;
+        if (c >= p && c < p + n) continue;
+        g_canaries[k] = g_canaries[i];
+        g_canary_buf[k] = g_canary_buf[i];
+        g_canary_at[k] = g_canary_at[i];
+        ++k;
+    }
+    g_canaries.resize(k);
+    g_canary_buf.resize(k);
+    g_canary_at.resize(k);
+}
 }  // namespace simt
+
+extern "C" void qo_sim_canary_enable(int on) { simt::g_canary_on = on != 0; }
+// Number of damaged canaries since the last check (each reported on stderr with
+// its allocation and offset); clears the list.
+extern "C" long qo_sim_canary_check(void) {
+    std::lock_guard<std::mutex> lk(simt::g_canary_mu);
+    long bad = 0;
+    for (size_t ci = 0; ci < simt::g_canaries.size(); ++ci) {
+        const auto &c = simt::g_canaries[ci];
+        for (size_t i = 0; i < c.second; ++i)
+            if (c.first[i] != simt::kCanary) {
+                char *base = nullptr;
+                for (const auto &a : simt::g_allocs)
+                    if ((char *)c.first >= a.first && (char *)c.first < a.first + a.second) base = a.first;
+                if (bad < 16)
+                    fprintf(stderr,
+                            "[canary] damaged at allocation %p + %zu (byte %zu of the canary after a buffer of %zu "
+                            "bytes carved at %s; canary %zu of %zu)\n",
+                            (void *)base, base ? (size_t)((char *)c.first - base) : (size_t)0, i,
+                            simt::g_canary_buf[ci], simt::g_canary_at[ci].c_str(), ci, simt::g_canaries.size());
+                ++bad;
+                break;
+            }
+    }
+    fprintf(stderr, "[canary] %zu checked, %ld damaged\n", simt::g_canaries.size(), bad);
+    simt::g_canaries.clear();
+    simt::g_canary_buf.clear();
+    simt::g_canary_at.clear();
+    return bad;
+}
 
 using namespace simt;
 
@@ -385,9 +490,14 @@ hipError_t hipMalloc(void **p, size_t bytes) {
     static const int fill = getenv("SIMT_FILL") ? atoi(getenv("SIMT_FILL")) : 0xA5;
     memset(*p, fill, n ? n : 256);
 #endif
+    {
+        std::lock_guard<std::mutex> lk(simt::g_canary_mu);
+        simt::g_allocs[static_cast<char *>(*p)] = n ? n : 256;
+    }
     return hipSuccess;
 }
 hipError_t hipFree(void *p) {
+    simt::canary_forget(static_cast<char *>(p));
     free(p);
     return hipSuccess;
 }

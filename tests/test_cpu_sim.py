@@ -6,7 +6,8 @@ and kernel logic are exercised without a GPU.
   look-back wait raising QO_ERR_DEVICE_WAIT when its predecessor never publishes.
 * serve_mix: the serving tests' four tables (scaled to 2%) and 24 mixed requests
   in batches of 8 through the serving loop, every result equal to the same
-  request run alone on the plain path.
+  request run alone on the plain path; with canaries after every buffer carved
+  from a batch's arena, no kernel writes past its buffer.
 * record_ops: split (one and two radix passes), gather, scatter, projection,
   reduce and Records -> columns on the benchmark's input shape, each against a
   plain loop.
@@ -47,6 +48,17 @@ def test_runtime_checks(cpu_build):
 def test_served_mix_equals_alone_cpu(cpu_build):
     out = _run(cpu_build / "serve_mix", 0.02, 24)
     assert out.count(" ok") == 24
+
+
+def test_served_mix_writes_stay_in_their_buffers_cpu(cpu_build):
+    """The same mix one batch at a time, with a canary after every buffer carved
+    from the batch's arena (result slots, filter descriptors, group-by outputs
+    and workspaces, join outputs, the join plan and its sort and partition
+    workspaces): no kernel writes past its buffer.  canary-part forces the
+    partitioned join build and probe for every table."""
+    for mode in ("canary", "canary-part"):
+        out = _run(cpu_build / "serve_mix", 0.05, 24, mode)
+        assert "canaries: 0 damaged" in out and out.count(" ok") == 24
 
 
 def test_record_ops_cpu(cpu_build):

@@ -178,6 +178,24 @@ def test_projection_matches_oracle(qo, orc):
     assert np.array_equal(got[inr, 1], r[proj[inr, 0], 1])  # validateProjection's rule
 
 
+def test_projection_in_place_and_aliasing_errors(qo, orc):
+    """out may overwrite proj (each row reads only its own proj row); gather into
+    R itself and a projection writing R race, and are rejected."""
+    n, m = 50_000, 70_001
+    r = gen_records(orc, n, 32)
+    proj = np.stack([orc.gen_uniform_u32(m, 33, 0, 0, n + 100), orc.gen_uniform_u32(m, 33, 1, 0, 0)], 1)
+    loc = orc.gen_uniform_u32(n, 34, 0, 0, n).astype(np.int32)
+    with DeviceBuffer.from_numpy(r) as d_r, DeviceBuffer.from_numpy(proj) as d_p, \
+            DeviceBuffer.from_numpy(loc) as d_l:
+        qo._lib.call("qo_project_records", d_r.ptr, n, d_p.ptr, m, d_p.ptr, None)
+        assert np.array_equal(d_p.to_numpy(np.uint32, 2 * m).reshape(-1, 2), orc.project_records(r, proj))
+        with pytest.raises(qo._lib.QoError):
+            qo._lib.call("qo_project_records", d_r.ptr, n, d_p.ptr, m, d_r.ptr, None)
+        with pytest.raises(qo._lib.QoError):
+            qo._lib.call("qo_gather_records", d_r.ptr, n, d_l.ptr, n, d_r.ptr, None)
+        assert np.array_equal(d_r.to_numpy(np.uint32, 2 * n).reshape(-1, 2), r)  # nothing ran
+
+
 # ---- reduce ---------------------------------------------------------------------------------
 @pytest.mark.parametrize("i", range(len(BF["reduce"])))
 def test_reduce_fixtures(qo, i):
