@@ -214,6 +214,12 @@ typedef enum hipStreamCaptureMode {
     hipStreamCaptureModeThreadLocal = 1,
     hipStreamCaptureModeRelaxed = 2
 } hipStreamCaptureMode;
+typedef enum hipStreamCaptureStatus {
+    hipStreamCaptureStatusNone = 0,
+    hipStreamCaptureStatusActive = 1,
+    hipStreamCaptureStatusInvalidated = 2
+} hipStreamCaptureStatus;
+#define hipEventDisableTiming 2
 struct ihipStream_t;
 typedef ihipStream_t *hipStream_t;
 struct ihipEvent_t;
@@ -268,6 +274,9 @@ hipError_t hipGetLastError();
 hipError_t hipPeekAtLastError();
 const char *hipGetErrorString(hipError_t e);
 hipError_t hipEventCreate(hipEvent_t *e);
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned flags);
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags = 0);
+hipError_t hipStreamIsCapturing(hipStream_t s, hipStreamCaptureStatus *status);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);

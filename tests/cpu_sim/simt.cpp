@@ -560,6 +560,14 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
     e->t = std::chrono::steady_clock::now();
     return hipSuccess;
 }
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
+// Work runs when it is launched, so a wait between streams has nothing to wait for.
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+hipError_t hipStreamIsCapturing(hipStream_t s, hipStreamCaptureStatus *status) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    *status = g_capturing.count(s) ? hipStreamCaptureStatusActive : hipStreamCaptureStatusNone;
+    return hipSuccess;
+}
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) {
