@@ -203,7 +203,7 @@ struct RecVec<1> {
     using T = uint2;
 };
 
-// Tiles of J * NT * VEC records (8192 with the launch below): one ticket per
+// Tiles of J * NT * VEC records (16384 with the launch below): one ticket per
 // tile keeps the same-address ticket atomics few.  The selected records are
 // staged in LDS in input order before the look-back and written out row by row
 // across the block.  In-tile offsets are 32-bit from the tile's base pointer:
