@@ -211,12 +211,18 @@ int launch_join_build(void *table, size_t table_bytes, const int32_t *rkey, cons
         QO_LAUNCHED();
         k_range_decide<<<1, 64, 0, st>>>(hdr, (uint64_t)std::max(1, tr.items) << 20);
         QO_LAUNCHED();
-        k_fill_slots<<<fill_grid, 256, 0, st>>>(hdr, kJoinKeyRange);
-        QO_LAUNCHED();
-        k_range_scatter<<<grid_for(nr / 4 + 1, 256, cus * 8), 256, 0, st>>>(rkey, rpay, nr, hdr);
-        QO_LAUNCHED();
-        k_range_count<<<cus * 2, 256, 0, st>>>(hdr);
-        QO_LAUNCHED();
+        // Large tables: slices assembled in LDS and written whole.  Else every
+        // tuple stored at its slot, which leaves partial lines to merge.
+        bool sliced = false;
+        QO_TRY(launch_range_part_build(table, table_bytes, rkey, rpay, nr, ws, st, &sliced));
+        if (!sliced) {
+            k_fill_slots<<<fill_grid, 256, 0, st>>>(hdr, kJoinKeyRange);
+            QO_LAUNCHED();
+            k_range_scatter<<<grid_for(nr / 4 + 1, 256, cus * 8), 256, 0, st>>>(rkey, rpay, nr, hdr);
+            QO_LAUNCHED();
+            k_range_count<<<cus * 2, 256, 0, st>>>(hdr);
+            QO_LAUNCHED();
+        }
         k_range_commit<<<1, 64, 0, st>>>(hdr);
         QO_LAUNCHED();
     }

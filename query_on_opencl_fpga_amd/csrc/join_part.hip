@@ -366,6 +366,85 @@ __global__ void k_spill_insert(const uint64_t *spill, const unsigned long long *
     }
 }
 
+// Key-range table, partitioned build: slice f of the table is assembled in LDS
+// from bucket f of R (a split by the table's own hash, whose top bits are the
+// top bits of key - kmin) with a plain store at slot key - kmin, then written
+// out whole, empty slots included, so the table needs no fill and every line is
+// written once.  A duplicate key overwrites its slot, and the occupied count
+// (added to the header) then falls short of the tuples, as the key-range
+// commit expects; a tuple whose slot lies outside the slice (never, unless the
+// table has fewer slots than buckets) goes to the spill list.
+__global__ __launch_bounds__(256) void k_range_slice_build(const uint64_t *__restrict__ bucket, uint64_t bcap,
+                                                           const uint32_t *ctr, int L, JoinHeader *hdr,
+                                                           uint64_t *spill, unsigned long long *spill_n) {
+    extern __shared__ __attribute__((aligned(16))) uint64_t s_slots[];
+    __shared__ uint32_t s_occ, s_sentinel;
+    if (hdr->mode != kJoinKeyRange) return;
+    const uint32_t f = blockIdx.x;
+    uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
+    const uint32_t kmin = (uint32_t)hdr->kmin;
+    uint64_t lo, hi;
+    slice_bounds(f, L, hdr->cap, lo, hi);
+    const uint32_t len = (uint32_t)(hi - lo);
+    for (uint32_t i = threadIdx.x; i < len; i += 256) s_slots[i] = kEmptySlot;
+    if (threadIdx.x == 0) s_occ = s_sentinel = 0;
+    __syncthreads();
+    uint64_t n = ctr[f * kCtrStride];
+    if (n > bcap) n = bcap;
+    const uint64_t *src = bucket + (size_t)f * bcap;
+    for (uint64_t i = threadIdx.x; i < n; i += 256) {
+        const uint64_t v = ld_stream(src + i);
+        if (v == kEmptySlot) {  // the (-1, -1) build tuple is counted, not stored
+            atomicAdd(&s_sentinel, 1u);
+            continue;
+        }
+        const uint64_t slot = (uint32_t)v - kmin;
+        if (slot >= lo && slot < hi) s_slots[slot - lo] = v;
+        else spill[atomicAdd(spill_n, 1ull)] = v;
+    }
+    __syncthreads();
+    uint32_t occ = 0;
+    for (uint32_t i = threadIdx.x; i < len; i += 256) {
+        const uint64_t e = s_slots[i];
+        occ += e != kEmptySlot;
+        st_stream(slots + lo + i, e);
+    }
+    occ = wave_sum(occ);
+    if (lane_id() == 0 && occ) atomicAdd(&s_occ, occ);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_occ) atomicAdd(&hdr->occupied, (unsigned long long)s_occ);
+        if (s_sentinel) atomicAdd(&hdr->sentinel_count, s_sentinel);
+    }
+}
+
+// Key-range tuples set aside by the split or the slice build, stored at their
+// slot in the finished table; a slot already taken (a duplicate key) is not
+// counted as occupied, so the commit falls back to the hash-mixed table.
+__global__ void k_range_spill(const uint64_t *spill, const unsigned long long *spill_n, JoinHeader *hdr) {
+    if (hdr->mode != kJoinKeyRange) return;
+    const uint64_t n = *spill_n;
+    uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
+    const uint32_t kmin = (uint32_t)hdr->kmin;
+    uint32_t occ = 0, sentinel = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = spill[i];
+        if (v == kEmptySlot) {
+            ++sentinel;
+            continue;
+        }
+        const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long *>(slots + ((uint32_t)v - kmin)),
+                                        (unsigned long long)kEmptySlot, (unsigned long long)v);
+        occ += prev == kEmptySlot;
+    }
+    occ = wave_sum(occ);
+    sentinel = wave_sum(sentinel);
+    if (lane_id() == 0) {
+        if (occ) atomicAdd(&hdr->occupied, (unsigned long long)occ);
+        if (sentinel) atomicAdd(&hdr->sentinel_count, sentinel);
+    }
+}
+
 // L2 probe from the S buckets (either table mode), each bucket kXcdRegions regions
 // (XcdLayout).  Block i works on XCD x = i % 8: with q = i / 8 it takes bucket
 // b = (q / (kXcdRegions * tpb)) * 8 + x, region (q / tpb) % kXcdRegions and its tile
@@ -420,6 +499,34 @@ int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey,
     k_slice_build<<<1u << p.L, 256, (size_t)p.slice_max * 8, st>>>(fb, fcap, fctr, p.L, hdr, b.spill, b.spill_n);
     QO_LAUNCHED();
     k_spill_insert<<<(unsigned)props().cu_count * 4, 256, 0, st>>>(b.spill, b.spill_n, hdr);
+    QO_LAUNCHED();
+    *used = true;
+    return QO_OK;
+}
+
+int launch_range_part_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
+                            void *ws, hipStream_t st, bool *used) {
+    *used = false;
+    const uint64_t cap = join_table_slots(table_bytes);
+    const PartPlan p = part_plan(cap, nr);
+    if (!p.on || !ws) return QO_OK;
+    Carver cv(ws);
+    const PartBufs b = part_carve(p, nr, true, cv);
+    JoinHeader *hdr = static_cast<JoinHeader *>(table);
+    QO_TRY(launch_fill_u32(b.sb.ctr, 0, b.sb.ctr_words, st));
+    QO_TRY(launch_fill_u64(b.spill_n, 0, 1, st));
+    // Split by the table's own hash: for a key-range table its top bits are
+    // those of key - kmin, so bucket f holds the keys of table slice f.
+    QO_TRY(launch_splits(p.s, b.sb, rkey, rpay, nullptr, nr, 0, 0, AppendOverflow{b.spill, b.spill_n},
+                         Gate{&hdr->mode, kJoinKeyRange}, HashSrc{hdr}, st));
+    const uint64_t *fb;
+    uint64_t fcap;
+    const uint32_t *fctr;
+    split_fine(p.s, b.sb, fb, fcap, fctr);
+    k_range_slice_build<<<1u << p.L, 256, (size_t)p.slice_max * 8, st>>>(fb, fcap, fctr, p.L, hdr, b.spill,
+                                                                         b.spill_n);
+    QO_LAUNCHED();
+    k_range_spill<<<(unsigned)props().cu_count * 4, 256, 0, st>>>(b.spill, b.spill_n, hdr);
     QO_LAUNCHED();
     *used = true;
     return QO_OK;

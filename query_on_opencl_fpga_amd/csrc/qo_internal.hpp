@@ -174,6 +174,12 @@ size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build);
 void join_part_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits);
 int launch_join_part_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
                            void *ws, hipStream_t st, bool *used);
+// Key-range table (decided on the device): the same split of R, by the table's
+// own hash, then every slice built in LDS with plain stores and written out
+// whole; the kernels return at once for a hash-mixed table.  *used: launched
+// (the plan's size rule; the workspace is the partitioned build's).
+int launch_range_part_build(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay, size_t nr,
+                            void *ws, hipStream_t st, bool *used);
 // Sorts m sort-ready join rows (k64 = join_sort_key(key, s_payload), v32 =
 // r_payload) with workspace rws (radix_workspace_bytes(m, true)) and writes
 // them out as three int32 columns.  Only bits [0, hi_bit) of k64 are sorted:

@@ -11,6 +11,9 @@ and kernel logic are exercised without a GPU.
 * record_ops: split (one and two radix passes), gather, scatter, projection,
   reduce and Records -> columns on the benchmark's input shape, each against a
   plain loop.
+* range_build: the key-range join table built slice by slice in LDS equals the
+  table built by storing every tuple at its slot (permutation keys, a negative
+  base with (-1, -1) tuples, keys with gaps, a duplicate key's hash fallback).
 * batch_parts: a batch of ten joins (one with an empty build side, one with a
   single row) with the partitioned build and probe of large tables by their
   rule, forced for every query and switched off, each query's rows equal to the
@@ -29,7 +32,7 @@ HERE = Path(__file__).resolve().parent / "cpu_sim"
 @pytest.fixture(scope="module")
 def cpu_build():
     env = dict(os.environ, SIMT_VARIANT="fast")
-    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks", "record_ops", "batch_parts"],
+    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks", "record_ops", "batch_parts", "range_build"],
                    check=True, env=env, timeout=900)
     return HERE / "_build" / "fast"
 
@@ -69,3 +72,8 @@ def test_record_ops_cpu(cpu_build):
 def test_batch_parts_cpu(cpu_build):
     out = _run(cpu_build / "batch_parts", 0.25)
     assert out.count("0 of 10 queries differ") == 3
+
+
+def test_range_build_sliced_equals_direct_cpu(cpu_build):
+    out = _run(cpu_build / "range_build", 20)
+    assert out.count("same table") == 4
