@@ -40,6 +40,7 @@ namespace qo {
 
 constexpr uint32_t kSliceMargin = 256;              // slots past a slice end also kept in LDS by the probe
 constexpr uint32_t kMaxSliceSlots = 8192;           // LDS slots of one work-group (64 KiB)
+constexpr uint32_t kRangeSliceSlots = 16384;        // key-range build: LDS slots of one slice (128 KiB)
 constexpr int kPairJ = 4;                           // pair quads per thread per probe step
 constexpr uint32_t kPairStep = kPairJ * 1024;       // pairs per probe step
 
@@ -50,17 +51,22 @@ struct PartPlan {
     uint32_t slice_max = 0;  // slots of the largest slice
 };
 
-static PartPlan part_plan(uint64_t cap, size_t n) {
+// slice_slots: the LDS slots of one slice for a build that keeps no probe
+// margin (the key-range build); 0: the tuned slice of the hash-mixed paths.
+static PartPlan part_plan(uint64_t cap, size_t n, uint64_t slice_slots = 0) {
     PartPlan p;
     const TuneParams tp = tune_get(TUNE_JOIN_PART, n);
     if (tp.param == PART_NEVER || n == 0 || n >= 0xFFFFFFFFull || cap < 2) return p;
     if (tp.param != PART_ALWAYS && tp.param != PART_LDS && (cap * 8 <= (uint64_t(4) << 20) || n < (size_t(1) << 20)))
         return p;
     // Slice target in slots (tune items = KiB per slice).
-    const uint64_t target = std::min<uint64_t>(kMaxSliceSlots - kSliceMargin - 2, (uint64_t)std::max(1, tp.items) * 128);
+    const uint64_t target = slice_slots ? slice_slots - 2
+                                        : std::min<uint64_t>(kMaxSliceSlots - kSliceMargin - 2,
+                                                             (uint64_t)std::max(1, tp.items) * 128);
+    const uint64_t room = slice_slots ? slice_slots : kMaxSliceSlots - kSliceMargin;
     int L = 1;
     while (L < 2 * kMaxSplitBits && (cap >> L) > target) ++L;
-    if ((cap >> L) + 2 + kSliceMargin > kMaxSliceSlots) return p;  // needs more than two splits
+    if ((cap >> L) + 2 > room) return p;  // needs more than two splits
     p.on = true;
     p.L = L;
     p.s = split_plan(L, n);
@@ -153,6 +159,14 @@ size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build) {
         Carver cv(nullptr);
         part_carve(p, n, build, cv);
         bytes = cv.off + 256;
+    }
+    if (build) {  // the key-range build's plan shares the workspace
+        const PartPlan pr = part_plan(slots, n, kRangeSliceSlots);
+        if (pr.on) {
+            Carver cv(nullptr);
+            part_carve(pr, n, true, cv);
+            bytes = std::max(bytes, cv.off + 256);
+        }
     }
     if (r.on) {
         // The probes share the workspace: only one of them runs.
@@ -391,23 +405,39 @@ __global__ __launch_bounds__(256) void k_range_slice_build(const uint64_t *__res
     __syncthreads();
     uint64_t n = ctr[f * kCtrStride];
     if (n > bcap) n = bcap;
-    const uint64_t *src = bucket + (size_t)f * bcap;
-    for (uint64_t i = threadIdx.x; i < n; i += 256) {
-        const uint64_t v = ld_stream(src + i);
+    const uint64_t *src = bucket + (size_t)f * bcap;  // 16-byte aligned: bcap is a multiple of 32
+    auto put = [&](uint64_t v) {
         if (v == kEmptySlot) {  // the (-1, -1) build tuple is counted, not stored
             atomicAdd(&s_sentinel, 1u);
-            continue;
+            return;
         }
         const uint64_t slot = (uint32_t)v - kmin;
         if (slot >= lo && slot < hi) s_slots[slot - lo] = v;
         else spill[atomicAdd(spill_n, 1ull)] = v;
+    };
+    // Two tuples per 16-byte load, then the odd one.
+    for (uint64_t i = threadIdx.x; 2 * i + 1 < n; i += 256) {
+        const uint4 q = ld_stream(reinterpret_cast<const uint4 *>(src) + i);
+        put(((uint64_t)q.y << 32) | q.x);
+        put(((uint64_t)q.w << 32) | q.z);
     }
+    if ((n & 1) && threadIdx.x == 0) put(ld_stream(src + n - 1));
     __syncthreads();
     uint32_t occ = 0;
-    for (uint32_t i = threadIdx.x; i < len; i += 256) {
-        const uint64_t e = s_slots[i];
-        occ += e != kEmptySlot;
-        st_stream(slots + lo + i, e);
+    if (((lo | len) & 1) == 0) {
+        // Two slots per 16-byte store (the slot array is 16-byte aligned).
+        uint4 *dst = reinterpret_cast<uint4 *>(slots + lo);
+        for (uint32_t i = threadIdx.x; 2 * i < len; i += 256) {
+            const uint64_t a = s_slots[2 * i], b = s_slots[2 * i + 1];
+            occ += (a != kEmptySlot) + (b != kEmptySlot);
+            dst[i] = uint4{(uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32)};
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < len; i += 256) {
+            const uint64_t e = s_slots[i];
+            occ += e != kEmptySlot;
+            st_stream(slots + lo + i, e);
+        }
     }
     occ = wave_sum(occ);
     if (lane_id() == 0 && occ) atomicAdd(&s_occ, occ);
@@ -508,8 +538,8 @@ int launch_range_part_build(void *table, size_t table_bytes, const int32_t *rkey
                             void *ws, hipStream_t st, bool *used) {
     *used = false;
     const uint64_t cap = join_table_slots(table_bytes);
-    const PartPlan p = part_plan(cap, nr);
-    if (!p.on || !ws) return QO_OK;
+    const PartPlan p = part_plan(cap, nr, kRangeSliceSlots);
+    if (!p.on || !ws || (size_t)p.slice_max * 8 > (size_t)props().lds_per_block) return QO_OK;
     Carver cv(ws);
     const PartBufs b = part_carve(p, nr, true, cv);
     JoinHeader *hdr = static_cast<JoinHeader *>(table);
