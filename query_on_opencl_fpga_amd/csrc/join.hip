@@ -67,9 +67,23 @@ __global__ __launch_bounds__(256) void k_key_range(const int32_t *__restrict__ r
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t from = 0;
     if (((uintptr_t)rkey & 15u) == 0) {
+        // Four 16-byte loads in flight per thread: one block per CU (its min/max
+        // are two same-address atomics, which serialise across blocks).
         const size_t nv = nr / 4;
-        for (size_t i = t; i < nv; i += stride) {
-            const int4 k = ld_stream(reinterpret_cast<const int4 *>(rkey) + i);
+        const int4 *k4 = reinterpret_cast<const int4 *>(rkey);
+        size_t i = t;
+        for (; i + 3 * stride < nv; i += 4 * stride) {
+            int4 k[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) k[u] = ld_stream(k4 + i + u * stride);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                mn = min(mn, min(min(k[u].x, k[u].y), min(k[u].z, k[u].w)));
+                mx = max(mx, max(max(k[u].x, k[u].y), max(k[u].z, k[u].w)));
+            }
+        }
+        for (; i < nv; i += stride) {
+            const int4 k = ld_stream(k4 + i);
             mn = min(mn, min(min(k.x, k.y), min(k.z, k.w)));
             mx = max(mx, max(max(k.x, k.y), max(k.z, k.w)));
         }
