@@ -23,6 +23,7 @@ from pathlib import Path
 PLAN = [
     (0, "filter_agg", [20, 24, 28]),
     (3, "groupby direct (1024 groups)", [20, 24, 28]),
+    (5, "partitioned hash join build (LDS slice KiB)", [20, 22, 24]),
     (6, "groupby partitioned (2^20 groups)", [20, 24, 28]),
     (8, "batched join (tiles per probe ticket, build tile)", [12, 14, 16, 18, 20, 22]),
     (9, "batched join table load %", [12, 16, 20, 22]),

@@ -15,6 +15,8 @@
 //   TUNE_BATCH        S tiles per probe ticket {1, 2, 4, 8} x R rows per build
 //                     tile {2048, 4096, 8192} (joins of |R| = 2^class, |S| = 8|R|)
 //   TUNE_BATCH_LOAD   batch hash table load {33, 50, 66, 80} % (same batch workload)
+//   TUNE_JOIN_PART    LDS table slice of the partitioned hash-mixed build {16, 24, 32, 48, 60} KiB
+//                     (build and probe of |R| = 2^class sparse 31-bit keys, |S| = 4|R|, half selected)
 //
 // The reference's analogue is the per-variant cost table its planner reads
 // (host/dynamic_programming.cpp:133-185), measured offline on the FPGA; here
@@ -54,6 +56,9 @@ std::vector<Cand> candidates(int op) {
             break;
         case TUNE_BATCH_LOAD:
             for (int load : {33, 50, 66, 80}) c.push_back({256, 4, load});
+            break;
+        case TUNE_JOIN_PART:
+            for (int kib : {16, 24, 32, 48, 60}) c.push_back({256, kib, PART_AUTO});
             break;
         default: break;
     }
@@ -115,6 +120,31 @@ int make_workload(int op, size_t n, hipStream_t st, Workload &w) {
         int64_t *oc = w.bufs[3].as<int64_t>(), *os = w.bufs[4].as<int64_t>();
         uint64_t *num = w.bufs[7].as<uint64_t>();
         w.run = [=](hipStream_t s) { return qo_groupby_agg(k, v, n, G, ok, oc, os, omn, omx, num, s); };
+        return QO_OK;
+    }
+    if (op == TUNE_JOIN_PART) {
+        // Hash-mixed table (sparse keys), built and probed once per step.
+        const size_t ns = std::min<size_t>(4 * n, size_t(1) << 28);
+        const size_t tb = qo_join_table_bytes(n);
+        w.bufs = std::vector<DevBuf>(10);
+        for (int i = 0; i < 2; ++i) QO_TRY(w.bufs[i].alloc(n * 4));
+        for (int i = 2; i < 8; ++i) QO_TRY(w.bufs[i].alloc(ns * 4));
+        QO_TRY(w.bufs[8].alloc(tb));
+        QO_TRY(w.bufs[9].alloc(64));
+        QO_TRY(qo_gen_bij_u32(w.bufs[0].as<uint32_t>(), n, 31, seed, st));
+        QO_TRY(qo_gen_uniform_u32(w.bufs[1].as<uint32_t>(), n, seed, 1, 0, 0, st));
+        QO_TRY(qo_gen_fk_u32(w.bufs[2].as<uint32_t>(), ns, seed, 2, n, 31, seed, st));
+        QO_TRY(qo_gen_uniform_u32(w.bufs[3].as<uint32_t>(), ns, seed, 3, 0, 0, st));
+        QO_TRY(qo_gen_uniform_u32(w.bufs[4].as<uint32_t>(), ns, seed, 4, 0, 100, st));
+        QO_HIP(hipStreamSynchronize(st));
+        int32_t *rk = w.col(0), *rp = w.col(1), *sk = w.col(2), *sp = w.col(3), *sf = w.col(4);
+        int32_t *ok = w.col(5), *orp = w.col(6), *osp = w.col(7);
+        void *table = w.bufs[8].p;
+        uint64_t *cnt = w.bufs[9].as<uint64_t>();
+        w.run = [=](hipStream_t s) {
+            QO_TRY(qo_hash_build(table, tb, rk, rp, n, s));
+            return qo_hash_probe(table, tb, sk, sp, sf, ns, INT32_MIN, 49, ok, orp, osp, ns, cnt, s);
+        };
         return QO_OK;
     }
     if (op == TUNE_BATCH || op == TUNE_BATCH_LOAD) {

@@ -7,7 +7,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("op,cls,choices", [(0, 20, {(2,), (4,), (8,)}), (8, 12, {(1,), (2,), (4,), (8,)})])
+@pytest.mark.parametrize("op,cls,choices", [(0, 20, {(2,), (4,), (8,)}), (8, 12, {(1,), (2,), (4,), (8,)}),
+                                            (5, 20, {(16,), (24,), (32,), (48,), (60,)})])
 def test_autotune_sets_a_candidate(qo, op, cls, choices):
     lib = qo._lib
     b, it, pa = C.c_int(), C.c_int(), C.c_int()
