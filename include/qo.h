@@ -235,7 +235,8 @@ int qo_hash_probe(const void *table, size_t table_bytes, const int32_t *skey, co
  * several build tuples (the probe then walks whole chains). */
 int qo_join_table_info(const void *table, int *mode, uint64_t *slots_used, int *has_dups, qo_stream s);
 /* Filter S, build on R, probe, sort output by (key, s_payload).  Synchronous:
- * *h_count receives the match count. */
+ * *h_count receives the match count.  Rows equal in (key, s_payload), which
+ * only duplicate R keys produce, come in no specified order. */
 int qo_hj_filter_sort(void *table, size_t table_bytes, const int32_t *rkey, const int32_t *rpay,
                       size_t nr, const int32_t *skey, const int32_t *spay, const int32_t *sfilt,
                       size_t ns, int32_t flo, int32_t fhi, int32_t *okey, int32_t *orpay,

@@ -354,18 +354,21 @@ class JoinBatch:
             self.handle = None
 
 
-def batch_hj_filter_sort(queries):
+def batch_hj_filter_sort(queries, capacities=None):
     """Host-array convenience wrapper: queries are tuples (rkey, rpay, skey, spay,
-    sfilt, flo, fhi); returns a list of (key, r_payload, s_payload)."""
+    sfilt, flo, fhi); returns a list of (key, r_payload, s_payload).  Each query's
+    output holds |S| rows unless `capacities` gives its row capacity (duplicate
+    R keys can give more matches than S has tuples)."""
     dev, bufs = [], []
-    for rk, rp, sk, sp, sf, flo, fhi in queries:
+    for i, (rk, rp, sk, sp, sf, flo, fhi) in enumerate(queries):
         rk, rp, sk, sp, sf = (_i32(a) for a in (rk, rp, sk, sp, sf))
+        cap = sk.shape[0] if capacities is None else int(capacities[i])
         d = {"rkey": DeviceBuffer.from_numpy(rk), "rpay": DeviceBuffer.from_numpy(rp),
              "skey": DeviceBuffer.from_numpy(sk), "spay": DeviceBuffer.from_numpy(sp),
              "sfilt": DeviceBuffer.from_numpy(sf), "nr": rk.shape[0], "ns": sk.shape[0],
-             "flo": flo, "fhi": fhi, "cap": sk.shape[0]}
+             "flo": flo, "fhi": fhi, "cap": cap}
         for c in ("okey", "orpay", "ospay"):
-            d[c] = DeviceBuffer(max(sk.shape[0], 1) * 4)
+            d[c] = DeviceBuffer(max(cap, 1) * 4)
         dev.append(d)
         bufs += [v for v in d.values() if isinstance(v, DeviceBuffer)]
     b = JoinBatch(dev)
