@@ -187,3 +187,73 @@ def test_random_batch(qo, orc, seed):
         for a, b in zip(g[:1] + g[2:], e[:1] + e[2:]):
             assert np.array_equal(a, b)
         assert np.array_equal(canon(np.stack(g, 1)), canon(np.stack(e, 1)))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_groupby_filtered(qo, orc, seed):
+    """The fused filter -> group-by, with the filter on its own column or on the
+    group key itself, against the oracle's filter then group-by."""
+    from test_gpu_groupby_filtered import run_filtered
+
+    rng = np.random.default_rng(11_000 + seed)
+    n = size(rng)
+    k = keys(rng, n, KINDS[seed % len(KINDS)])
+    v = keys(rng, n, "uniform")
+    f = k if seed % 2 else keys(rng, n, "dense")
+    lo, hi = bounds(rng)
+    got = run_filtered(qo, k, v, f, lo, hi, n)
+    sel = (f >= lo) & (f <= hi)
+    exp = orc.groupby_agg(k[sel], v[sel])
+    for a, b in zip(got, exp):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_gather_scatter_projection(qo, orc, seed):
+    """Locations and row ids anywhere in int32, most of them in range."""
+    rng = np.random.default_rng(12_000 + seed)
+    r = records(rng, size(rng), KINDS[seed % len(KINDS)])
+    n = r.shape[0]
+    m = size(rng)
+    loc = np.where(rng.random(m) < 0.9, rng.integers(0, n, m), keys(rng, m, "uniform")).astype(np.int32)
+    assert np.array_equal(qo.operators.gatherImpl(r, loc), orc.gather_records(r, loc))
+    perm = rng.permutation(n).astype(np.int32)  # distinct targets: the scatter's result is defined
+    assert np.array_equal(qo.operators.scatterImpl(r, perm), orc.scatter_records(r, perm))
+    proj = np.stack([loc.view(np.uint32), keys(rng, m, "uniform").view(np.uint32)], 1)
+    assert np.array_equal(qo.operators.projection(r, proj), orc.project_records(r, proj))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_planner_covers(qo, orc, seed):
+    """A random filter -> group-by -> aggregate (or reduce, or selection) query
+    through a random implementation cover, against the oracle's chain."""
+    from query_on_opencl_fpga_amd import planner as P
+
+    rng = np.random.default_rng(13_000 + seed)
+    n = size(rng)
+    G = int(rng.choice([1, 7, 1000, 5000]))
+    r = np.stack([keys(rng, n, "uniform").view(np.uint32), rng.integers(0, G, n).astype(np.uint32)], 1)
+    filt = bool(rng.random() < 0.7)
+    lo, hi = sorted(int(x) for x in rng.integers(-10, G + 10, 2))
+    group = bool(rng.random() < 0.6)
+    op = int(rng.integers(0, 4)) if (group or rng.random() < 0.7) else -1
+    if not filt and not group and op < 0:
+        filt = True
+    q = P.Query(filter=filt, lo=lo, hi=hi, group=group, op=op, selectivity=0.5, groups=G)
+    mask = int(rng.integers(0, 1 << 5))
+    try:
+        p = P.plan(q, n, mask)
+    except qo._lib.QoError:
+        p = P.plan(q, n)  # the mask allows no cover of this chain
+    got = P.execute(p, q, r, group_capacity=G + 1)
+    sel = orc.filter_records(r, lo, hi) if filt else r
+    if group:
+        srt = orc.sort_records(sel)
+        sp = orc.group_startpos(srt)
+        ek = srt[sp, 1] if len(sp) else np.zeros(0, np.uint32)
+        assert np.array_equal(got[0], ek), p.describe()
+        assert np.array_equal(got[1], orc.parallel_aggregate(srt, sp, op)), p.describe()
+    elif op >= 0:
+        assert got == orc.reduce_records(sel, op), p.describe()
+    else:
+        assert np.array_equal(got, sel), p.describe()
