@@ -257,3 +257,23 @@ def test_random_planner_covers(qo, orc, seed):
         assert got == orc.reduce_records(sel, op), p.describe()
     else:
         assert np.array_equal(got, sel), p.describe()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_compat_chain(qo, orc, seed):
+    """The reference-named host entry points on random Records: range selection,
+    radix sort, group-by (sorted Records + start positions) and the four
+    after-group-by aggregates of .x per group."""
+    rng = np.random.default_rng(14_000 + seed)
+    n = size(rng)
+    r = np.stack([keys(rng, n, "uniform").view(np.uint32), keys(rng, n, KINDS[seed % len(KINDS)]).view(np.uint32)],
+                 1)
+    lo, hi = bounds(rng)
+    assert np.array_equal(qo.operators.CL_RangeSelection(r, lo, hi), orc.filter_records(r, lo, hi))
+    srt = orc.sort_records(r)
+    assert np.array_equal(qo.operators.CL_RadixSort(r), srt)
+    got_srt, sp = qo.operators.CL_GroupBy(r)
+    assert np.array_equal(got_srt, srt) and np.array_equal(sp, orc.group_startpos(srt))
+    for name, code in (("sum", 0), ("max", 1), ("min", 2), ("avg", 3)):
+        got = getattr(qo.operators, f"CL_agg_{name}_afterGroupBy")(srt, sp)
+        assert np.array_equal(got, orc.parallel_aggregate(srt, sp, code)), name
