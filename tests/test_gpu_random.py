@@ -277,3 +277,37 @@ def test_random_compat_chain(qo, orc, seed):
     for name, code in (("sum", 0), ("max", 1), ("min", 2), ("avg", 3)):
         got = getattr(qo.operators, f"CL_agg_{name}_afterGroupBy")(srt, sp)
         assert np.array_equal(got, orc.parallel_aggregate(srt, sp, code)), name
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_large_join_and_groupby(qo, orc, seed):
+    """Sizes that take the partitioned paths: the join's sliced key-range or
+    partitioned hash build and its L2-slice probe (|R| >= 2^20), the group-by's
+    one-pass partition (groups beyond one LDS table), a 2-pass split."""
+    rng = np.random.default_rng(15_000 + seed)
+    nr = (1 << 21) + int(rng.integers(0, 1 << 20))
+    ns = (1 << 22) + int(rng.integers(0, 1 << 21))
+    if seed % 2 == 0:
+        rk = (int(rng.choice([0, -nr, INT_MAX - nr + 1])) + rng.permutation(nr)).astype(np.int64).astype(np.int32)
+    else:
+        rk = rng.choice(np.arange(INT_MIN, INT_MAX, 997, dtype=np.int64), nr, replace=False).astype(np.int32)
+    rp = keys(rng, nr, "uniform")
+    sk = np.where(rng.random(ns) < 0.9, rk[rng.integers(0, nr, ns)], keys(rng, ns, "uniform")).astype(np.int32)
+    sp = keys(rng, ns, "uniform")
+    sf = rng.integers(0, 100, ns).astype(np.int32)
+    got = qo.operators.hash_join(rk, rp, sk, sp, sf, INT_MIN, 59)
+    exp = orc.hash_join(rk, rp, sk, sp, sf, INT_MIN, 59)
+    assert got[3] == len(exp[0])
+    assert np.array_equal(canon(np.stack(got[:3], 1)), canon(np.stack(exp, 1)))
+    n = ns
+    G = int(rng.choice([1 << 16, 1 << 20]))
+    k = (rng.integers(0, G, n) * 2654435761 % (1 << 32)).astype(np.uint32).view(np.int32)
+    v = keys(rng, n, "uniform")
+    gg = qo.operators.groupby_agg(k, v, G)
+    for a, b in zip(gg, orc.groupby_agg(k, v)):
+        assert np.array_equal(a, b)
+    r = np.stack([np.arange(n, dtype=np.uint32), rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)], 1)
+    parts = int(rng.choice([300, 4097, 65536]))
+    out, bound = qo.operators.splitImpl(r, parts)
+    eo, eb = orc.split_records(r, parts)
+    assert np.array_equal(out, eo) and np.array_equal(bound, eb)
