@@ -235,6 +235,8 @@ def test_random_planner_covers(qo, orc, seed):
     r = np.stack([keys(rng, n, "uniform").view(np.uint32), rng.integers(0, G, n).astype(np.uint32)], 1)
     filt = bool(rng.random() < 0.7)
     lo, hi = sorted(int(x) for x in rng.integers(-10, G + 10, 2))
+    if seed % 4 == 3:
+        lo, hi = hi + 1, lo  # an empty range
     group = bool(rng.random() < 0.6)
     op = int(rng.integers(0, 4)) if (group or rng.random() < 0.7) else -1
     if not filt and not group and op < 0:
