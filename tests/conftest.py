@@ -28,3 +28,19 @@ def orc():
 
     oracle.load()
     return oracle
+
+
+@pytest.fixture(autouse=True)
+def _tune_table_restored(request, tmp_path_factory):
+    """Every GPU test leaves libqo's launch-parameter table as it found it
+    (entries and absent entries alike: saved to a file, then reset and
+    reloaded), so parameters a test sets cannot leak into later tests."""
+    if request.node.get_closest_marker("gpu") is None:
+        yield
+        return
+    pkg = request.getfixturevalue("qo")
+    snap = tmp_path_factory.mktemp("tune") / "table.tune"
+    pkg._lib.call("qo_tune_save", str(snap).encode())
+    yield
+    pkg._lib.call("qo_tune_reset")
+    pkg._lib.call("qo_tune_load", str(snap).encode())

@@ -98,10 +98,11 @@ def gb_mode(request, qo):
     lib = qo._lib.load()
     mode, target = (2, 1024) if request.param == "direct" else (1, 16)
     saved = []
-    for cls in range(40):
+    for cls in range(40):  # all read before any is set (a class without an entry reads its nearest)
         b, i, p = C.c_int(), C.c_int(), C.c_int()
         qo._lib.call("qo_tune_get", TUNE_GROUPBY_PART, cls, C.byref(b), C.byref(i), C.byref(p))
         saved.append((cls, b.value, i.value, p.value))
+    for cls in range(40):
         qo._lib.call("qo_tune_set", TUNE_GROUPBY_PART, cls, 512, target, mode)
     yield request.param
     for cls, b, i, p in saved:
