@@ -250,7 +250,13 @@ def run_join(args, qo, lib) -> dict:
     qo._lib.call("qo_join_strategy", tb, ns, mode.value, C.byref(part), C.byref(bits), C.byref(splits))
     table_kind = (f"key-range table ({used.value} slots, home slot = key - min key)" if mode.value == 1
                   else f"hash-mixed table ({used.value} slots)")
-    if part.value and splits.value == 1:
+    # The key-range LDS route is taken on the device by tables of 2^21 slots and
+    # more (the bench's keys have no (-1, -1) build tuple); smaller ones take L2.
+    if part.value == 2 and mode.value == 1 and not dups.value and used.value >= (1 << 21):
+        kernel = "probe stage (one qo_hash_probe call): k_range_lds_split + k_range_lds_probe"
+        strategy = (f"{table_kind}; probe: one split by key into {used.value >> 15} buckets of 8 per-XCD "
+                    "regions, lookups in LDS slices of 2^15 slots")
+    elif part.value and splits.value == 1:
         kernel = "probe stage (one qo_hash_probe call): k_split_cols + k_l2_probe"
         strategy = f"{table_kind}; probe: one split into {1 << bits.value} buckets of 8 per-XCD regions, XCD-local L2 lookups"
     elif part.value:

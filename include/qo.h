@@ -210,7 +210,12 @@ size_t qo_join_table_bytes(size_t build_rows);
  * *partitioned = 1 for a partitioned probe, *bits = log2 of the partition
  * count, *splits = 1 for one split and L2-resident table slices (either mode),
  * 2 for two splits and LDS-resident slices (hash-mixed tables too large for
- * the first); *partitioned = 0 for the direct probe.  Host-only. */
+ * the first); *partitioned = 2 for a key-range table split by key into slices
+ * of 2^15 slots whose payloads are looked up in LDS, *bits = log2 of the
+ * partition count for a table of the largest capacity these slots allow
+ * (capacity / 2^15 partitions for a smaller one; tables under 2^21 slots or
+ * with (-1, -1) build tuples take the L2-resident slices); *partitioned = 0
+ * for the direct probe.  Host-only. */
 int qo_join_strategy(size_t table_bytes, size_t n, int mode, int *partitioned, int *bits, int *splits);
 /* Builds the table in `table` (256-byte aligned device memory of table_bytes,
  * e.g. qo_join_table_bytes(nr)).  When the build keys span a range that fits a
@@ -354,7 +359,8 @@ int qo_plan_execute(const qo_plan *plan, const qo_query *q, const qo_record *rin
 /* Operators: 0 filter_agg, 1 filter, 2 probe (param = table load %), 3 groupby, 4 sort,
  * 5 partitioned join (items = LDS table slice KiB per partition, param = 0 when
  * the table outgrows one L2 / 1 always / 2 never / 3 always, hash-mixed probes
- * through LDS slices), 6 partitioned group-by (items =
+ * through LDS slices / 4 when the table outgrows one L2, key-range probes through
+ * L2 only), 6 partitioned group-by (items =
  * groups per fine bucket, param = 0 auto / 1 always / 2 never), 7 key-range join
  * table (items = largest such table in MiB, param = 0 when the keys allow /
  * 2 never), 8 batched join (class of the batch's mean |R|; items = S tiles per

@@ -278,7 +278,8 @@ int join_plan_launch(const JoinBatchPlan &p, hipStream_t st) {
         out.v32 = p.mvals + q.m_base;
         bool hp = false, rp = false;
         QO_TRY(launch_join_part_probe(q.hdr, (size_t)(32 + q.table_cap) * 8, q.skey, q.spay, q.sfilt, q.ns, q.flo,
-                                      q.fhi, out, reinterpret_cast<uint64_t *>(q.count), p.part_ws, st, &hp, &rp));
+                                      q.fhi, out, reinterpret_cast<uint64_t *>(q.count), p.part_ws, st, &hp, &rp,
+                                      true));
         QO_REQUIRE(hp, "query %d: the partitioned probe did not take its hash-mixed table", i);
     }
     if (p.mt) {

@@ -108,6 +108,99 @@ static L2Plan l2_plan(uint64_t slots, size_t n) {
     return r;
 }
 
+// Key-range LDS route: the table slice of a bucket is held in LDS as 4-byte
+// payloads plus presence bits (a key-range slot names its key), 2^15 slots per
+// slice, so each lookup is an LDS read instead of an L1 -> L2 request.
+constexpr int kRangeLdsSliceBits = 15;
+constexpr uint32_t kRangeLdsSlice = 1u << kRangeLdsSliceBits;
+constexpr int kRangeLdsMinCapBits = 21;  // smaller key-range tables keep the L2 probe
+constexpr int kRangeLdsMaxBits = 9;      // at most 512 buckets: one split pass
+
+// Which probe serves a table, decided on the device from its header: a
+// key-range table of 2^21 .. 2^(15 + lds_bits) slots without (-1, -1) build
+// tuples (one slot per key) takes the LDS route, other key-range tables the L2 route, hash-mixed
+// tables the hash route.
+enum JoinRoute : uint32_t { kRouteHash = 0, kRouteRangeL2 = 1, kRouteRangeLds = 2 };
+struct RouteGate {
+    const JoinHeader *hdr = nullptr;  // null: always runs
+    uint32_t mask = 0;                // bit r set: runs for route r
+    uint32_t lds_bits = 0;            // 0: no table takes the LDS route
+    QO_DEV static uint32_t route(const JoinHeader *h, uint32_t lds_bits) {
+        if (h->mode != kJoinKeyRange) return kRouteHash;
+        const uint32_t capbits = 32u - h->shift;
+        const bool lds = lds_bits != 0 && (h->sentinel_count | h->has_dups) == 0 &&
+                         capbits >= (uint32_t)kRangeLdsMinCapBits &&
+                         capbits <= (uint32_t)kRangeLdsSliceBits + lds_bits;
+        return lds ? kRouteRangeLds : kRouteRangeL2;
+    }
+    QO_DEV bool off() const { return hdr && !((mask >> route(hdr, lds_bits)) & 1u); }
+};
+
+// LDS of the key-range slice probe: the slice's payloads and presence bits,
+// and one step's matches staged as (slot in the slice, s_payload).
+// The probe's work-groups are 16 waves, one pair quad per thread per step of
+// kPairStep pairs: the slice takes most of the LDS, so a CU runs one
+// work-group, and a wave waiting for its stores or loads leaves 15 others.
+constexpr int kLdsProbeThreads = 1024;
+constexpr int kLdsProbeJ = kPairStep / 4 / kLdsProbeThreads;
+static_assert(kLdsProbeJ * 4 * kLdsProbeThreads == (int)kPairStep, "probe step");
+struct RangeLdsProbeLds {
+    uint32_t pay[kRangeLdsSlice];
+    uint32_t bits[kRangeLdsSlice / 32];
+    uint16_t loc[kPairStep];
+    int32_t spay[kPairStep];
+    uint32_t tot[tile_offsets_words(kLdsProbeJ, kLdsProbeThreads / kWave)];
+    uint64_t excl;
+};
+
+// Host plan of the LDS route.  The table's capacity is known only on the
+// device (a power of two >= the key range, <= the slots allocated), so the
+// regions are sized for the largest capacity that takes the route, 2^(15 +
+// bits) slots: 2^bits buckets of 8 regions (one per XCD) of bcap pairs; a
+// table of 2^(15 + L) slots, L < bits, has 2^L buckets whose regions are
+// 2^(bits - L) times larger, in the same memory.
+struct RangeLdsPlan {
+    bool on = false;
+    uint32_t bits = 0;
+    uint64_t bcap = 0;
+    uint32_t gpb = 1;  // probe work-groups per bucket at 2^bits buckets (1, 2, 4 or 8)
+};
+
+static RangeLdsPlan range_lds_plan(uint64_t slots, size_t n) {
+    RangeLdsPlan r;
+    const TuneParams tp = tune_get(TUNE_JOIN_PART, n);
+    if (tp.param == PART_NEVER || tp.param == PART_LDS || tp.param == PART_L2 || n == 0 || n >= 0xFFFFFFFFull)
+        return r;
+    if (tp.param != PART_ALWAYS && (slots * 8 <= (uint64_t(4) << 20) || n < (size_t(1) << 20))) return r;
+    if (sizeof(RangeLdsProbeLds) > (size_t)props().lds_per_block) return r;
+    int capbits = 0;  // the largest capacity a key-range table can take: 2^capbits <= slots
+    while (capbits < 31 && (uint64_t(2) << capbits) <= slots) ++capbits;
+    if (capbits < kRangeLdsMinCapBits) return r;
+    r.on = true;
+    r.bits = (uint32_t)std::min(capbits - kRangeLdsSliceBits, kRangeLdsMaxBits);
+    const uint64_t regions = (uint64_t(1) << r.bits) * kXcdRegions;
+    const uint64_t per = (n + regions - 1) / regions;
+    r.bcap = (per + per / 16 + 1024 + 31) / 32 * 32;
+    const uint32_t want = ((uint32_t)props().cu_count * 2) >> r.bits;
+    while (r.gpb < kXcdRegions && r.gpb < want) r.gpb *= 2;
+    return r;
+}
+
+struct RangeLdsBufs {
+    uint32_t *ctr = nullptr;  // one cursor per region, kCtrStride apart
+    uint64_t *bucket = nullptr;
+    size_t ctr_words = 0;
+};
+
+static RangeLdsBufs range_lds_carve(const RangeLdsPlan &r, Carver &cv) {
+    RangeLdsBufs b;
+    const size_t regions = (size_t(1) << r.bits) * kXcdRegions;
+    b.ctr_words = regions * kCtrStride;
+    b.ctr = cv.take<uint32_t>(b.ctr_words);
+    b.bucket = cv.take<uint64_t>(regions * r.bcap);
+    return b;
+}
+
 // The probe's plans: the L2 plan, and the LDS-slice plan only for a
 // hash-mixed table the L2 plan does not serve.
 static void probe_plans(uint64_t slots, size_t n, L2Plan &r, PartPlan &p) {
@@ -137,7 +230,12 @@ void join_part_strategy(size_t table_bytes, size_t n, int mode, int *partitioned
     PartPlan p;
     probe_plans(join_table_slots(table_bytes), n, r, p);
     *partitioned = *bits = *splits = 0;
-    if (r.on && (mode == (int)kJoinKeyRange || r.hash_fits)) {
+    const RangeLdsPlan rl = range_lds_plan(join_table_slots(table_bytes), n);
+    if (rl.on && mode == (int)kJoinKeyRange) {  // bits: for the largest key-range table of these slots
+        *partitioned = 2;
+        *bits = (int)rl.bits;
+        *splits = 1;
+    } else if (r.on && (mode == (int)kJoinKeyRange || r.hash_fits)) {
         *partitioned = 1;
         *bits = r.s.L;
         *splits = 1;
@@ -172,6 +270,12 @@ size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build) {
         // The probes share the workspace: only one of them runs.
         Carver cv(nullptr);
         split_carve(r.s, cv);
+        bytes = std::max(bytes, cv.off + 256);
+    }
+    const RangeLdsPlan rl = build ? RangeLdsPlan{} : range_lds_plan(slots, n);
+    if (rl.on) {
+        Carver cv(nullptr);
+        range_lds_carve(rl, cv);
         bytes = std::max(bytes, cv.off + 256);
     }
     return bytes;
@@ -483,7 +587,7 @@ __global__ void k_range_spill(const uint64_t *spill, const unsigned long long *s
 __global__ __launch_bounds__(256) void k_l2_probe(const uint64_t *__restrict__ bucket, uint64_t bcap,
                                                   const uint32_t *ctr, uint32_t nb, uint32_t tpb,
                                                   const JoinHeader *__restrict__ hdr, unsigned long long *counter,
-                                                  SoaEmit emit, Gate gate) {
+                                                  SoaEmit emit, RouteGate gate) {
     __shared__ uint32_t s_tot[tile_offsets_words(kPairJ, 4)];
     __shared__ uint64_t s_excl;
     __shared__ JoinStageT<kPairJ * 1024> stage;
@@ -504,6 +608,192 @@ __global__ __launch_bounds__(256) void k_l2_probe(const uint64_t *__restrict__ b
     const uint64_t *slots = reinterpret_cast<const uint64_t *>(hdr + 1);
     lookup_emit<kPairJ>(slots, hdr->cap, JoinHash::of(hdr), (hdr->has_dups | hdr->sentinel_count) != 0,
                         hdr->sentinel_count, key, sp, pass, counter, emit, s_tot, &s_excl, &stage);
+}
+
+// Split of the S tuples (filter fused) for the LDS route: 2^L buckets by the
+// top L bits of key - kmin, L = log2(cap) - 15 read from the table, so bucket f
+// holds the keys of slice f; 8 regions per bucket, one per XCD (XcdLayout), of
+// bcap << (lds_bits - L) pairs each.  Keys outside the table's range cannot
+// match and are dropped here.
+template <bool HAS_FILTER>
+__global__ __launch_bounds__(kSplitThreads) void k_range_lds_split(const int32_t *__restrict__ kcol,
+                                                                   const int32_t *__restrict__ pcol,
+                                                                   const int32_t *__restrict__ fcol, size_t n,
+                                                                   int32_t flo, int32_t fhi, bool vec, uint64_t bcap,
+                                                                   uint64_t *__restrict__ bucket, uint32_t *ctr,
+                                                                   ProbeOverflow overflow, RouteGate gate) {
+    __shared__ SplitLdsT<(1u << kRangeLdsMaxBits)> sh;
+    if (gate.off()) return;
+    const JoinHash hs = JoinHash::of(gate.hdr);
+    const int L = (int)(32u - hs.shift) - kRangeLdsSliceBits;  // in [6, lds_bits] on this route
+    int32_t key[kSplitJ][4], pay[kSplitJ][4];
+    uint32_t pass[kSplitJ];
+    load_s_tile<kSplitJ, HAS_FILTER, kSplitThreads>(kcol, pcol, fcol, n, flo, fhi, vec, blockIdx.x, key, pay, pass);
+#pragma unroll
+    for (int j = 0; j < kSplitJ; ++j)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t h;
+            if (!hs(key[j][k], h)) pass[j] &= ~(1u << k);
+        }
+    split_block<kSplitJ>(key, pay, pass, hs, 32 - L, 1u << L, 0, bucket, XcdLayout{bcap << (gate.lds_bits - L)}, ctr,
+                         sh, overflow);
+}
+
+// Bit j of the low 16 bits of x to bit 2j.
+QO_DEV uint32_t spread_bits16(uint32_t x) {
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// Probe of the LDS route: work-group (f, g) copies slice f of the table (slots
+// [f << 15, (f + 1) << 15)) to LDS as payloads plus presence bits and looks up
+// the pairs of its share of bucket f's 8 regions there.  A hit is staged as its
+// slot in the slice and its s_payload; the rows are written out with the key
+// and r_payload taken from the slot, one step later, while the next step's
+// output range is being reserved.  The next two steps of 4096 pairs are loaded
+// before the current one is looked up.  Dynamic LDS: sizeof(RangeLdsProbeLds).
+__global__ __launch_bounds__(kLdsProbeThreads) void k_range_lds_probe(const uint64_t *__restrict__ bucket, uint64_t bcap,
+                                                         const uint32_t *ctr, RouteGate gate,
+                                                         unsigned long long *counter, SoaEmit emit) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
+    RangeLdsProbeLds &s = *reinterpret_cast<RangeLdsProbeLds *>(s_raw);
+    if (gate.off()) return;
+    const JoinHeader *hdr = gate.hdr;
+    const int L = (int)(32u - hdr->shift) - kRangeLdsSliceBits;
+    const uint64_t rcap = bcap << (gate.lds_bits - L);  // pairs per region at this table's 2^L buckets
+    // The grid is (1, 2, 4 or 8) << lds_bits work-groups: 2^L buckets of gpb each.
+    const uint32_t gpb = std::min<uint32_t>(kXcdRegions, gridDim.x >> L);
+    const uint32_t f = blockIdx.x / gpb, g = blockIdx.x % gpb;
+    if (f >= (1u << L)) return;
+    const uint32_t rpb = kXcdRegions / gpb, r0 = f * kXcdRegions + g * rpb;
+    auto region_n = [&](uint32_t i) -> uint32_t {
+        const uint64_t c = ctr[(size_t)(r0 + i) * kCtrStride];
+        return (uint32_t)(c < rcap ? c : rcap);
+    };
+    uint32_t ri = 0, e = 0, rn = region_n(0);
+    auto seek = [&](uint32_t &i, uint32_t &pos, uint32_t &cnt) {
+        while (i < rpb && pos >= cnt) {
+            ++i;
+            pos = 0;
+            cnt = i < rpb ? region_n(i) : 0u;
+        }
+    };
+    seek(ri, e, rn);
+    if (ri >= rpb) return;  // no pairs (whole block): no barrier has been reached
+
+    // The slice: 16-byte loads of two slots, 16 in flight per thread.  Each
+    // wave's 64 loads cover 128 consecutive slots, whose presence bits are 4
+    // words built from two ballots (even and odd slots).
+    const uint4 *src4 = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint64_t *>(hdr + 1) +
+                                                        ((uint64_t)f << kRangeLdsSliceBits));
+    const unsigned lane = lane_id();
+    constexpr uint32_t kQuadRounds = kRangeLdsSlice / 2 / kLdsProbeThreads;
+    constexpr uint32_t kBatch = kQuadRounds < 16 ? kQuadRounds : 16;
+    for (uint32_t u0 = 0; u0 < kQuadRounds; u0 += kBatch) {
+        uint4 q[kBatch];
+#pragma unroll
+        for (uint32_t v = 0; v < kBatch; ++v) q[v] = ld_stream(src4 + (u0 + v) * kLdsProbeThreads + threadIdx.x);
+#pragma unroll
+        for (uint32_t v = 0; v < kBatch; ++v) {
+            const uint32_t i = (u0 + v) * kLdsProbeThreads + threadIdx.x;  // slots 2i, 2i + 1
+            reinterpret_cast<uint2 *>(s.pay)[i] = uint2{q[v].y, q[v].w};
+            const uint64_t m0 = ballot(!(q[v].x == ~0u && q[v].y == ~0u));
+            const uint64_t m1 = ballot(!(q[v].z == ~0u && q[v].w == ~0u));
+            if (lane < 4)
+                s.bits[(i - lane) / 16 + lane] = spread_bits16((uint32_t)(m0 >> (16 * lane))) |
+                                                 (spread_bits16((uint32_t)(m1 >> (16 * lane))) << 1);
+        }
+    }
+    __syncthreads();
+
+    const uint32_t kbase = (uint32_t)hdr->kmin + (f << kRangeLdsSliceBits);
+    auto load = [&](uint32_t i, uint32_t pos, uint32_t cnt, int32_t (&k)[kLdsProbeJ][4], int32_t (&p)[kLdsProbeJ][4],
+                    uint32_t (&ps)[kLdsProbeJ]) {
+        load_pairs<kLdsProbeJ, kLdsProbeThreads>(bucket + (size_t)(r0 + i) * rcap + pos, cnt - pos < kPairStep ? cnt - pos : kPairStep, k,
+                           p, ps);
+    };
+    auto row = [&](uint32_t i, int32_t &k, int32_t &r, int32_t &q) {
+        const uint32_t l = s.loc[i];
+        k = (int32_t)(kbase + l);
+        r = (int32_t)s.pay[l];
+        q = s.spay[i];
+    };
+    uint32_t staged = 0;  // rows of the previous step in the stage, written out at s.excl
+    // Steps t (key), t + 1 (key2) and t + 2 (key3) in registers: the loads of
+    // two steps are in flight while one is looked up.
+    int32_t key[kLdsProbeJ][4], sp[kLdsProbeJ][4], key2[kLdsProbeJ][4], sp2[kLdsProbeJ][4];
+    uint32_t pass[kLdsProbeJ], pass2[kLdsProbeJ];
+    load(ri, e, rn, key, sp, pass);
+    uint32_t ni = ri, ne = e + kPairStep, nn = rn;  // step t + 1
+    seek(ni, ne, nn);
+    if (ni < rpb) load(ni, ne, nn, key2, sp2, pass2);
+    for (;;) {
+        const bool more = ni < rpb;
+        uint32_t mi = ni, me = ne + kPairStep, mn = nn;  // step t + 2
+        if (more) seek(mi, me, mn);
+        const bool more2 = more && mi < rpb;
+        int32_t key3[kLdsProbeJ][4], sp3[kLdsProbeJ][4];
+        uint32_t pass3[kLdsProbeJ];
+        if (more2) load(mi, me, mn, key3, sp3, pass3);
+        uint32_t hit[kLdsProbeJ], c[kLdsProbeJ], ex[kLdsProbeJ], loc[kLdsProbeJ][4];
+#pragma unroll
+        for (int j = 0; j < kLdsProbeJ; ++j) {
+            hit[j] = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t l = (uint32_t)key[j][k] - kbase;
+                loc[j][k] = l;
+                if (((pass[j] >> k) & 1u) && l < kRangeLdsSlice && ((s.bits[l >> 5] >> (l & 31u)) & 1u))
+                    hit[j] |= 1u << k;
+            }
+            c[j] = __popc(hit[j]);
+        }
+        const uint32_t total = tile_offsets<kLdsProbeJ, kLdsProbeThreads / kWave>(c, ex, s.tot);
+        // The step's output range is reserved now and used after the previous
+        // step's rows are written out, which hides the reservation's wait
+        // behind those stores (one atomic counter serves every work-group).
+        unsigned long long mine = 0;
+        if (threadIdx.x == 0 && total) mine = atomicAdd(counter, (unsigned long long)total);
+        if (staged) emit.flush_rows<kLdsProbeThreads>(s.excl, staged, row);
+        __syncthreads();  // the stage and s.excl are free
+        if (threadIdx.x == 0) s.excl = mine;
+#pragma unroll
+        for (int j = 0; j < kLdsProbeJ; ++j) {
+            uint32_t pos = ex[j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!((hit[j] >> k) & 1u)) continue;
+                s.loc[pos] = (uint16_t)loc[j][k];
+                s.spay[pos] = sp[j][k];
+                ++pos;
+            }
+        }
+        __syncthreads();
+        staged = total;
+        if (!more) break;
+#pragma unroll
+        for (int j = 0; j < kLdsProbeJ; ++j) {
+            pass[j] = pass2[j];
+            pass2[j] = pass3[j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                key[j][k] = key2[j][k];
+                sp[j][k] = sp2[j][k];
+                key2[j][k] = key3[j][k];
+                sp2[j][k] = sp3[j][k];
+            }
+        }
+        ni = mi;
+        ne = me;
+        nn = mn;
+        if (!more2) ni = rpb;
+    }
+    emit.flush_rows<kLdsProbeThreads>(s.excl, staged, row);
 }
 
 // ---------------------------------------------------------------------------
@@ -564,7 +854,8 @@ int launch_range_part_build(void *table, size_t table_bytes, const int32_t *rkey
 
 int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
                            const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, const JoinOut &out,
-                           uint64_t *d_count, void *ws, hipStream_t st, bool *hash_part, bool *range_part) {
+                           uint64_t *d_count, void *ws, hipStream_t st, bool *hash_part, bool *range_part,
+                           bool hash_only) {
     *hash_part = *range_part = false;
     if (!ws) return QO_OK;
     const uint64_t slots = join_table_slots(table_bytes);
@@ -589,8 +880,33 @@ int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t 
         QO_LAUNCHED();
         *hash_part = true;
     }
-    if (r.on) {  // one split by the table's own hash, L2-resident slices
-        const Gate gate = r.hash_fits ? Gate{} : Gate{&hdr->mode, kJoinKeyRange};
+    const RangeLdsPlan rl = hash_only ? RangeLdsPlan{} : range_lds_plan(slots, ns);
+    if (rl.on) {  // key-range tables whose slices fit LDS: split by key, look up in LDS
+        Carver cv(ws);
+        const RangeLdsBufs b = range_lds_carve(rl, cv);
+        const RouteGate gate{hdr, 1u << kRouteRangeLds, rl.bits};
+        QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));
+        const bool vec = ((uintptr_t)skey % 16 == 0) && ((uintptr_t)spay % 16 == 0) &&
+                         (!sfilt || (uintptr_t)sfilt % 16 == 0);
+        const size_t tiles = (ns + kSplitTile - 1) / kSplitTile;
+        QO_REQUIRE(tiles < 0x7FFFFFFFull, "probe input too large");
+        const ProbeOverflow ovf{hdr, counter, emit};
+        if (sfilt)
+            k_range_lds_split<true><<<(unsigned)tiles, kSplitThreads, 0, st>>>(skey, spay, sfilt, ns, flo, fhi, vec,
+                                                                              rl.bcap, b.bucket, b.ctr, ovf, gate);
+        else
+            k_range_lds_split<false><<<(unsigned)tiles, kSplitThreads, 0, st>>>(skey, spay, sfilt, ns, flo, fhi, vec,
+                                                                               rl.bcap, b.bucket, b.ctr, ovf, gate);
+        QO_LAUNCHED();
+        k_range_lds_probe<<<rl.gpb << rl.bits, kLdsProbeThreads, sizeof(RangeLdsProbeLds), st>>>(b.bucket, rl.bcap, b.ctr, gate,
+                                                                                   counter, emit);
+        QO_LAUNCHED();
+        *range_part = true;
+    }
+    if (r.on && (r.hash_fits || !hash_only)) {  // one split by the table's own hash, L2-resident slices
+        // Key-range tables the LDS route does not take, and hash-mixed ones
+        // whose slices stay small enough.
+        const RouteGate gate{hdr, (1u << kRouteRangeL2) | (r.hash_fits ? 1u << kRouteHash : 0u), rl.on ? rl.bits : 0u};
         Carver cv(ws);
         const SplitBufs b = split_carve(r.s, cv);
         QO_TRY(launch_fill_u32(b.ctr, 0, b.ctr_words, st));

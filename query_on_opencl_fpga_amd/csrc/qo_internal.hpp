@@ -99,7 +99,9 @@ enum TuneOp {
 };
 // Partitioned-path modes: by size (the default), always, never.
 // PART_LDS (join only): partition, and probe hash-mixed tables through LDS slices.
-enum PartMode { PART_AUTO = 0, PART_ALWAYS = 1, PART_NEVER = 2, PART_LDS = 3 };
+// PART_L2 (join only): by size, with every key-range table probed through L2
+// (no key-range LDS slices).
+enum PartMode { PART_AUTO = 0, PART_ALWAYS = 1, PART_NEVER = 2, PART_LDS = 3, PART_L2 = 4 };
 TuneParams tune_get(int op, size_t n);
 // The built-in default of an operator (used where no class is tuned).
 TuneParams tune_builtin(int op);
@@ -196,9 +198,12 @@ struct JoinOut {
     uint64_t *k64 = nullptr;
     uint32_t *v32 = nullptr;
 };
+// hash_only: the table is known to be hash-mixed (the key-range routes are not
+// launched).
 int launch_join_part_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
                            const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, const JoinOut &out,
-                           uint64_t *d_count, void *ws, hipStream_t st, bool *hash_part, bool *range_part);
+                           uint64_t *d_count, void *ws, hipStream_t st, bool *hash_part, bool *range_part,
+                           bool hash_only = false);
 int launch_join_probe(const void *table, size_t table_bytes, const int32_t *skey, const int32_t *spay,
                       const int32_t *sfilt, size_t ns, int32_t flo, int32_t fhi, const JoinOut &out,
                       uint64_t *d_count, void *ws, hipStream_t st);

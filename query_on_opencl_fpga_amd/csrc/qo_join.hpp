@@ -383,35 +383,53 @@ struct SoaEmit {
     // past `cap`, rows go one per thread.
     QO_DEV void flush(uint64_t base, uint32_t total, const int32_t *key, const int32_t *rp,
                       const int32_t *sp) const {
+        flush_rows(base, total, [&](uint32_t i, int32_t &k, int32_t &r, int32_t &s) {
+            k = key[i];
+            r = rp[i];
+            s = sp[i];
+        });
+    }
+    // The same for rows that `row(i, key, r_payload, s_payload)` reads, by
+    // all NT threads of the block.
+    template <int NT = 256, class Row>
+    QO_DEV void flush_rows(uint64_t base, uint32_t total, const Row &row) const {
         const bool vec = base + total <= cap &&
                          ((k64 ? (reinterpret_cast<uintptr_t>(k64) | reinterpret_cast<uintptr_t>(v32))
                                : (reinterpret_cast<uintptr_t>(okey) | reinterpret_cast<uintptr_t>(orpay) |
                                   reinterpret_cast<uintptr_t>(ospay))) &
                           15u) == 0;
+        auto one = [&](uint32_t i) {
+            int32_t k, r, s;
+            row(i, k, r, s);
+            (*this)(base + i, k, r, s);
+        };
         if (!vec) {
-            for (uint32_t i = threadIdx.x; i < total; i += 256) (*this)(base + i, key[i], rp[i], sp[i]);
+            for (uint32_t i = threadIdx.x; i < total; i += NT) one(i);
             return;
         }
         const uint32_t head = std::min<uint32_t>(total, (uint32_t)((4u - (base & 3u)) & 3u));
         const uint32_t groups = (total - head) / 4u, tail0 = head + groups * 4u;
-        for (uint32_t i = threadIdx.x; i < head; i += 256) (*this)(base + i, key[i], rp[i], sp[i]);
-        for (uint32_t i = tail0 + threadIdx.x; i < total; i += 256) (*this)(base + i, key[i], rp[i], sp[i]);
-        for (uint32_t g = threadIdx.x; g < groups; g += 256) {
+        for (uint32_t i = threadIdx.x; i < head; i += NT) one(i);
+        for (uint32_t i = tail0 + threadIdx.x; i < total; i += NT) one(i);
+        for (uint32_t g = threadIdx.x; g < groups; g += NT) {
             const uint32_t i = head + 4u * g;
             const uint64_t pos = base + i;  // a multiple of 4
+            int32_t key[4], rp[4], sp[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) row(i + e, key[e], rp[e], sp[e]);
             if (k64) {
                 uint64_t kk[4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) kk[e] = join_sort_key(key[i + e], sp[i + e]);
+                for (int e = 0; e < 4; ++e) kk[e] = join_sort_key(key[e], sp[e]);
                 uint4 *k4 = reinterpret_cast<uint4 *>(k64 + pos);
                 k4[0] = uint4{(uint32_t)kk[0], (uint32_t)(kk[0] >> 32), (uint32_t)kk[1], (uint32_t)(kk[1] >> 32)};
                 k4[1] = uint4{(uint32_t)kk[2], (uint32_t)(kk[2] >> 32), (uint32_t)kk[3], (uint32_t)(kk[3] >> 32)};
                 *reinterpret_cast<uint4 *>(v32 + pos) =
-                    uint4{(uint32_t)rp[i], (uint32_t)rp[i + 1], (uint32_t)rp[i + 2], (uint32_t)rp[i + 3]};
+                    uint4{(uint32_t)rp[0], (uint32_t)rp[1], (uint32_t)rp[2], (uint32_t)rp[3]};
             } else {
-                *reinterpret_cast<int4 *>(okey + pos) = int4{key[i], key[i + 1], key[i + 2], key[i + 3]};
-                *reinterpret_cast<int4 *>(orpay + pos) = int4{rp[i], rp[i + 1], rp[i + 2], rp[i + 3]};
-                *reinterpret_cast<int4 *>(ospay + pos) = int4{sp[i], sp[i + 1], sp[i + 2], sp[i + 3]};
+                *reinterpret_cast<int4 *>(okey + pos) = int4{key[0], key[1], key[2], key[3]};
+                *reinterpret_cast<int4 *>(orpay + pos) = int4{rp[0], rp[1], rp[2], rp[3]};
+                *reinterpret_cast<int4 *>(ospay + pos) = int4{sp[0], sp[1], sp[2], sp[3]};
             }
         }
     }

@@ -142,21 +142,25 @@ struct HashSrc {
     }
 };
 
-struct SplitLds {
+// LDS of a split block into at most MAXP (<= kSplitThreads) buckets.
+template <uint32_t MAXP>
+struct SplitLdsT {
+    static_assert(MAXP <= (uint32_t)kSplitThreads, "one bucket per thread at most");
     uint64_t stage[kSplitTile];
-    uint64_t gpos[kMaxSplit];  // where the part's run starts in `bucket`
-    uint32_t cnt[kMaxSplit], loff[kMaxSplit], room[kMaxSplit];
+    uint64_t gpos[MAXP];  // where the part's run starts in `bucket`
+    uint32_t cnt[MAXP], loff[MAXP], room[MAXP];
     uint32_t w[kSplitThreads / kWave];
 };
+using SplitLds = SplitLdsT<kMaxSplit>;
 
 // Splits the block's live tuples (J groups of 4 per thread, bit k of pass[j]
 // marks element k of group j) into `parts` buckets by
 // part = (hs(key) >> shift) & (parts - 1); bucket index = bucket0 + part,
 // placed by `out` in `bucket`; ctr holds one fill cursor per bucket.
-template <int J, class Layout, class Overflow>
+template <int J, class Layout, class Overflow, class Lds>
 QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], const uint32_t (&pass)[J],
                         const JoinHash &hs, int shift, uint32_t parts, uint32_t bucket0,
-                        uint64_t *__restrict__ bucket, const Layout &out, uint32_t *ctr, SplitLds &sh,
+                        uint64_t *__restrict__ bucket, const Layout &out, uint32_t *ctr, Lds &sh,
                         const Overflow &overflow) {
     const uint32_t mask = parts - 1;
     for (uint32_t i = threadIdx.x; i < parts; i += kSplitThreads) sh.cnt[i] = 0;
@@ -211,12 +215,12 @@ QO_DEV void split_block(const int32_t (&key)[J][4], const int32_t (&pay)[J][4], 
 }
 
 // Split A: columns (key, payload[, filter]) -> 2^L1 buckets.
-template <bool HAS_FILTER, class Layout, class Overflow>
+template <bool HAS_FILTER, class Layout, class Overflow, class G = Gate>
 __global__ __launch_bounds__(kSplitThreads) void k_split_cols(const int32_t *__restrict__ kcol, const int32_t *__restrict__ pcol,
                                                     const int32_t *__restrict__ fcol, size_t n, int32_t flo,
                                                     int32_t fhi, bool vec, int shift, uint32_t parts,
                                                     uint64_t *__restrict__ bucket, Layout out, uint32_t *ctr,
-                                                    Overflow overflow, Gate gate, HashSrc hsrc) {
+                                                    Overflow overflow, G gate, HashSrc hsrc) {
     __shared__ SplitLds sh;
     if (gate.off()) return;
     int32_t key[kSplitJ][4], pay[kSplitJ][4];
@@ -301,11 +305,11 @@ QO_DEV void split_pairs_tile(const uint64_t *__restrict__ input, const InLayout 
 }
 
 // Split B with fixed-capacity input buckets: block = (bucket, tile).
-template <class Layout, class Overflow>
+template <class Layout, class Overflow, class G = Gate>
 __global__ __launch_bounds__(kSplitThreads) void k_split_pairs(const uint64_t *__restrict__ input, FixedLayout in,
                                                      const uint32_t *in_ctr, uint32_t tiles_per_bucket, int shift,
                                                      uint32_t parts, uint64_t *__restrict__ bucket, Layout out,
-                                                     uint32_t *ctr, Overflow overflow, Gate gate, HashSrc hsrc) {
+                                                     uint32_t *ctr, Overflow overflow, G gate, HashSrc hsrc) {
     __shared__ SplitLds sh;
     if (gate.off()) return;
     split_pairs_tile(input, in, in_ctr, blockIdx.x / tiles_per_bucket, blockIdx.x % tiles_per_bucket, hsrc.get(),
@@ -363,10 +367,10 @@ __global__ __launch_bounds__(kSplitThreads) void k_split_pairs_exact(const uint6
 // when not null, with the range [flo, fhi]) -> fine buckets.  The counters must
 // have been zeroed.  The kernels run only where `gate` lets them and hash as
 // `hsrc` says.
-template <class Overflow>
+template <class Overflow, class G>
 inline int launch_splits(const SplitPlan &p, const SplitBufs &b, const int32_t *kcol, const int32_t *pcol,
                          const int32_t *fcol, size_t n, int32_t flo, int32_t fhi, const Overflow &overflow,
-                         Gate gate, HashSrc hsrc, hipStream_t st) {
+                         G gate, HashSrc hsrc, hipStream_t st) {
     const bool vec = ((uintptr_t)kcol % 16 == 0) && ((uintptr_t)pcol % 16 == 0) && (!fcol || (uintptr_t)fcol % 16 == 0);
     const uint32_t partsA = 1u << p.L1;
     const size_t tiles = (n + kSplitTile - 1) / kSplitTile;

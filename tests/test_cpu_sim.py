@@ -14,6 +14,11 @@ and kernel logic are exercised without a GPU.
 * range_build: the key-range join table built slice by slice in LDS equals the
   table built by storing every tuple at its slot (permutation keys, a negative
   base with (-1, -1) tuples, keys with gaps, a duplicate key's hash fallback).
+* range_probe: joins whose key-range table is probed through LDS slices (the
+  default plan from 2^21 slots), through L2 and unpartitioned, each equal to a
+  host hash join (a permutation, sparse keys in a larger table, a (-1, -1) build
+  tuple, a table below the LDS route's minimum), and qo_hj_filter_sort's rows in
+  order.
 * batch_parts: a batch of ten joins (one with an empty build side, one with a
   single row) with the partitioned build and probe of large tables by their
   rule, forced for every query and switched off, each query's rows equal to the
@@ -32,7 +37,7 @@ HERE = Path(__file__).resolve().parent / "cpu_sim"
 @pytest.fixture(scope="module")
 def cpu_build():
     env = dict(os.environ, SIMT_VARIANT="fast")
-    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks", "record_ops", "batch_parts", "range_build"],
+    subprocess.run([sys.executable, str(HERE / "build.py"), "serve_mix", "runtime_checks", "record_ops", "batch_parts", "range_build", "range_probe"],
                    check=True, env=env, timeout=900)
     return HERE / "_build" / "fast"
 
@@ -77,3 +82,9 @@ def test_batch_parts_cpu(cpu_build):
 def test_range_build_sliced_equals_direct_cpu(cpu_build):
     out = _run(cpu_build / "range_build", 20)
     assert out.count("same table") == 4
+
+
+def test_range_probe_routes_equal_host_join_cpu(cpu_build):
+    out = _run(cpu_build / "range_probe", 20)
+    assert out.count("default plan same rows, L2 same rows, direct same rows") == 4
+    assert "hj_filter_sort same rows in order" in out

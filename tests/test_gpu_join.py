@@ -191,12 +191,16 @@ TUNE_JOIN_RANGE = 7
 
 
 def _set_tune(qo, op, block, items, param):
+    """Sets every size class of `op`; returns what each class read before.
+    All classes are read first: a class without an entry reads its nearest
+    class's entry, which must not be one set here."""
     import ctypes as C
     saved = []
     for cls in range(40):
         b, i, p = C.c_int(), C.c_int(), C.c_int()
         qo._lib.call("qo_tune_get", op, cls, C.byref(b), C.byref(i), C.byref(p))
         saved.append((op, cls, b.value, i.value, p.value))
+    for cls in range(40):
         qo._lib.call("qo_tune_set", op, cls, block, items, param)
     return saved
 
@@ -382,6 +386,63 @@ def test_hj_filter_sort_modes(qo, orc, join_mode):
     sf = orc.gen_uniform_u32(ns, 32, 3, 0, 1000).astype(np.int32)
     got = qo.operators.hj_filter_sort(rk, rp, sk, sp, sf, 100, 599)
     exp = orc.sort_join_rows(*orc.hash_join(rk, rp, sk, sp, sf, 100, 599))
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+PART_L2 = 4  # tune operator 5: key-range tables probed through L2 only
+
+
+def _lds_route_case(orc, nr, ns, seed, skew=False, sentinel=False):
+    rk = (orc.gen_perm(nr, seed).astype(np.int64) - 777).astype(np.int32)
+    rp = orc.gen_uniform_u32(nr, seed + 1, 0, 0, 0).astype(np.int32)
+    # probe keys over the key range widened by 1000 at both ends (misses)
+    sk = (orc.gen_uniform_u32(ns, seed + 1, 1, 0, nr + 2000).astype(np.int64) - 1777).astype(np.int32)
+    if skew:  # 3 of 4 probe tuples carry one key: its bucket's regions overflow
+        sk[np.arange(ns) % 4 != 0] = rk[5]
+    if sentinel:  # a (-1, -1) build tuple: the table keeps the key-range mode, the probe takes the L2 route
+        rp[np.flatnonzero(rk == -1)[0]] = -1  # key -1 is in the permutation's range
+        sk[::101] = -1
+    sp = orc.gen_uniform_u32(ns, seed + 1, 2, 0, 0).astype(np.int32)
+    sf = orc.gen_uniform_u32(ns, seed + 1, 3, 0, 100).astype(np.int32)
+    return rk, rp, sk, sp, sf
+
+
+@pytest.mark.parametrize("nr,ns,skew,sentinel", [
+    (1 << 21, 1 << 22, False, False),  # capacity 2^21 in slots for 2^22: buckets take two planned regions
+    (3 << 20, 1 << 22, False, False),  # capacity 2^22, as planned
+    (3 << 20, 1 << 22, True, False),
+    (3 << 20, 1 << 22, False, True),
+    (1 << 24, 1 << 24, False, False),  # capacity 2^24: 512 buckets
+])
+def test_key_range_lds_route_matches_oracle(qo, orc, nr, ns, skew, sentinel):
+    """Key-range tables of 2^21 .. 2^24 slots are probed through LDS slices of
+    2^15 slots by default (a one-pass split by key into up to 512 buckets):
+    rows equal to the oracle's, and to the L2 route's (tune operator 5,
+    param 4).  Skewed probe keys overflow their bucket's regions into direct
+    probes; a (-1, -1) build tuple sends the table to the L2 route."""
+    rk, rp, sk, sp, sf = _lds_route_case(orc, nr, ns, 61, skew, sentinel)
+    assert build_info(qo, rk, rp)[0] == 1  # key-range table
+    exp = canon(rows(orc.hash_join(rk, rp, sk, sp, sf, 0, 49)))
+    got = qo.operators.hash_join(rk, rp, sk, sp, sf, 0, 49, capacity=ns + 16)
+    assert got[3] == len(exp)
+    assert np.array_equal(canon(rows(got)), exp)
+    saved = _set_tune(qo, TUNE_JOIN_PART, 256, 32, PART_L2)
+    try:
+        got = qo.operators.hash_join(rk, rp, sk, sp, sf, 0, 49, capacity=ns + 16)
+    finally:
+        _restore_tune(qo, saved)
+    assert got[3] == len(exp)
+    assert np.array_equal(canon(rows(got)), exp)
+
+
+def test_hj_filter_sort_lds_route(qo, orc):
+    """hj_filter_sort on a key-range table of 2^22 slots (the probe's
+    sort-ready output through the LDS route): rows in (key, s_payload) order,
+    equal to the oracle's."""
+    rk, rp, sk, sp, sf = _lds_route_case(orc, 3 << 20, 1 << 23, 71)
+    got = qo.operators.hj_filter_sort(rk, rp, sk, sp, sf, 0, 49)
+    exp = orc.sort_join_rows(*orc.hash_join(rk, rp, sk, sp, sf, 0, 49))
     for g, e in zip(got, exp):
         assert np.array_equal(g, e)
 
