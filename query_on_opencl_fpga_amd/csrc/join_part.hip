@@ -120,6 +120,13 @@ constexpr int kRangeLdsMaxBits = 9;      // at most 512 buckets: one split pass
 // key-range table of 2^21 .. 2^(15 + lds_bits) slots without (-1, -1) build
 // tuples (one slot per key) takes the LDS route, other key-range tables the L2 route, hash-mixed
 // tables the hash route.
+// A key-range table whose capacity the LDS route serves (build and probe).
+QO_DEV bool range_lds_sized(const JoinHeader *h, uint32_t lds_bits) {
+    const uint32_t capbits = 32u - h->shift;
+    return lds_bits != 0 && capbits >= (uint32_t)kRangeLdsMinCapBits &&
+           capbits <= (uint32_t)kRangeLdsSliceBits + lds_bits;
+}
+
 enum JoinRoute : uint32_t { kRouteHash = 0, kRouteRangeL2 = 1, kRouteRangeLds = 2 };
 struct RouteGate {
     const JoinHeader *hdr = nullptr;  // null: always runs
@@ -127,13 +134,19 @@ struct RouteGate {
     uint32_t lds_bits = 0;            // 0: no table takes the LDS route
     QO_DEV static uint32_t route(const JoinHeader *h, uint32_t lds_bits) {
         if (h->mode != kJoinKeyRange) return kRouteHash;
-        const uint32_t capbits = 32u - h->shift;
-        const bool lds = lds_bits != 0 && (h->sentinel_count | h->has_dups) == 0 &&
-                         capbits >= (uint32_t)kRangeLdsMinCapBits &&
-                         capbits <= (uint32_t)kRangeLdsSliceBits + lds_bits;
+        const bool lds = (h->sentinel_count | h->has_dups) == 0 && range_lds_sized(h, lds_bits);
         return lds ? kRouteRangeLds : kRouteRangeL2;
     }
     QO_DEV bool off() const { return hdr && !((mask >> route(hdr, lds_bits)) & 1u); }
+};
+
+// Key-range builds: `lds` set, the LDS-route build of the tables it serves by
+// capacity; clear, the sliced build of the other key-range tables.
+struct RangeBuildGate {
+    JoinHeader *hdr;
+    uint32_t lds_bits;
+    bool lds;
+    QO_DEV bool off() const { return hdr->mode != kJoinKeyRange || range_lds_sized(hdr, lds_bits) != lds; }
 };
 
 // LDS of the key-range slice probe: the slice's payloads and presence bits,
@@ -151,6 +164,12 @@ struct RangeLdsProbeLds {
     int32_t spay[kPairStep];
     uint32_t tot[tile_offsets_words(kLdsProbeJ, kLdsProbeThreads / kWave)];
     uint64_t excl;
+};
+// LDS of the key-range slice build: the slice's payloads and presence bits.
+struct RangeLdsBuildLds {
+    uint32_t pay[kRangeLdsSlice];
+    uint32_t bits[kRangeLdsSlice / 32];
+    uint32_t occ, sentinel;
 };
 
 // Host plan of the LDS route.  The table's capacity is known only on the
@@ -258,12 +277,20 @@ size_t join_part_ws_bytes(size_t table_bytes, size_t n, bool build) {
         part_carve(p, n, build, cv);
         bytes = cv.off + 256;
     }
-    if (build) {  // the key-range build's plan shares the workspace
+    if (build) {  // the key-range builds' plans share the workspace
         const PartPlan pr = part_plan(slots, n, kRangeSliceSlots);
         if (pr.on) {
             Carver cv(nullptr);
             part_carve(pr, n, true, cv);
             bytes = std::max(bytes, cv.off + 256);
+            const RangeLdsPlan rl = range_lds_plan(slots, n);
+            if (rl.on) {
+                Carver cl(nullptr);
+                range_lds_carve(rl, cl);
+                cl.take<unsigned long long>(1);
+                cl.take<uint64_t>(n);
+                bytes = std::max(bytes, cl.off + 256);
+            }
         }
     }
     if (r.on) {
@@ -494,10 +521,11 @@ __global__ void k_spill_insert(const uint64_t *spill, const unsigned long long *
 // table has fewer slots than buckets) goes to the spill list.
 __global__ __launch_bounds__(256) void k_range_slice_build(const uint64_t *__restrict__ bucket, uint64_t bcap,
                                                            const uint32_t *ctr, int L, JoinHeader *hdr,
-                                                           uint64_t *spill, unsigned long long *spill_n) {
+                                                           uint64_t *spill, unsigned long long *spill_n,
+                                                           uint32_t lds_bits) {
     extern __shared__ __attribute__((aligned(16))) uint64_t s_slots[];
     __shared__ uint32_t s_occ, s_sentinel;
-    if (hdr->mode != kJoinKeyRange) return;
+    if (RangeBuildGate{hdr, lds_bits, false}.off()) return;  // the LDS-route build takes this table
     const uint32_t f = blockIdx.x;
     uint64_t *slots = reinterpret_cast<uint64_t *>(hdr + 1);
     const uint32_t kmin = (uint32_t)hdr->kmin;
@@ -610,18 +638,18 @@ __global__ __launch_bounds__(256) void k_l2_probe(const uint64_t *__restrict__ b
                         hdr->sentinel_count, key, sp, pass, counter, emit, s_tot, &s_excl, &stage);
 }
 
-// Split of the S tuples (filter fused) for the LDS route: 2^L buckets by the
-// top L bits of key - kmin, L = log2(cap) - 15 read from the table, so bucket f
+// Split of the S tuples (filter fused) or the R tuples for the LDS route: 2^L
+// buckets by the top L bits of key - kmin, L = log2(cap) - 15 read from the table, so bucket f
 // holds the keys of slice f; 8 regions per bucket, one per XCD (XcdLayout), of
 // bcap << (lds_bits - L) pairs each.  Keys outside the table's range cannot
 // match and are dropped here.
-template <bool HAS_FILTER>
+template <bool HAS_FILTER, class Overflow, class G>
 __global__ __launch_bounds__(kSplitThreads) void k_range_lds_split(const int32_t *__restrict__ kcol,
                                                                    const int32_t *__restrict__ pcol,
                                                                    const int32_t *__restrict__ fcol, size_t n,
                                                                    int32_t flo, int32_t fhi, bool vec, uint64_t bcap,
                                                                    uint64_t *__restrict__ bucket, uint32_t *ctr,
-                                                                   ProbeOverflow overflow, RouteGate gate) {
+                                                                   Overflow overflow, G gate) {
     __shared__ SplitLdsT<(1u << kRangeLdsMaxBits)> sh;
     if (gate.off()) return;
     const JoinHash hs = JoinHash::of(gate.hdr);
@@ -638,6 +666,82 @@ __global__ __launch_bounds__(kSplitThreads) void k_range_lds_split(const int32_t
         }
     split_block<kSplitJ>(key, pay, pass, hs, 32 - L, 1u << L, 0, bucket, XcdLayout{bcap << (gate.lds_bits - L)}, ctr,
                          sh, overflow);
+}
+
+// Build of slice f = blockIdx.x of the LDS route: every tuple of bucket f (its
+// 8 regions) sets its presence bit and payload in an LDS copy of the slice
+// (an LDS atomic-or finds a second tuple of one key, which is not counted as
+// occupied, so the commit falls back to the hash-mixed table), and the slice
+// is written out whole with 16-byte stores, key = kmin + slot.  (-1, -1)
+// tuples are counted, not stored.  Dynamic LDS: sizeof(RangeLdsBuildLds).
+__global__ __launch_bounds__(kLdsProbeThreads) void k_range_lds_build(const uint64_t *__restrict__ bucket,
+                                                                      uint64_t bcap, const uint32_t *ctr,
+                                                                      RangeBuildGate gate, uint64_t *spill,
+                                                                      unsigned long long *spill_n) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
+    RangeLdsBuildLds &s = *reinterpret_cast<RangeLdsBuildLds *>(s_raw);
+    if (gate.off()) return;
+    JoinHeader *hdr = gate.hdr;
+    const int L = (int)(32u - hdr->shift) - kRangeLdsSliceBits;
+    const uint64_t rcap = bcap << (gate.lds_bits - L);
+    const uint32_t f = blockIdx.x;
+    if (f >= (1u << L)) return;
+    for (uint32_t i = threadIdx.x; i < kRangeLdsSlice / 32; i += kLdsProbeThreads) s.bits[i] = 0;
+    if (threadIdx.x == 0) s.occ = s.sentinel = 0;
+    __syncthreads();
+    const uint32_t kbase = (uint32_t)hdr->kmin + (f << kRangeLdsSliceBits);
+    uint32_t occ = 0, sentinel = 0;
+    auto put = [&](uint32_t key, uint32_t pay) {
+        const uint64_t v = ((uint64_t)pay << 32) | key;
+        if (v == kEmptySlot) {
+            ++sentinel;
+            return;
+        }
+        const uint32_t l = key - kbase;
+        if (l >= kRangeLdsSlice) {  // not this slice's key (a split never sends one): stored by k_range_spill
+            spill[atomicAdd(spill_n, 1ull)] = v;
+            return;
+        }
+        const uint32_t bit = 1u << (l & 31u);
+        if (!(atomicOr(&s.bits[l >> 5], bit) & bit)) {
+            s.pay[l] = pay;
+            ++occ;
+        }
+    };
+    for (uint32_t r = 0; r < kXcdRegions; ++r) {
+        const size_t reg = (size_t)f * kXcdRegions + r;
+        uint64_t n = ctr[reg * kCtrStride];
+        if (n > rcap) n = rcap;
+        const uint64_t *src = bucket + reg * rcap;  // 16-byte aligned: bcap is a multiple of 32
+        for (uint64_t i = threadIdx.x; 2 * i + 1 < n; i += kLdsProbeThreads) {
+            const uint4 q = ld_stream(reinterpret_cast<const uint4 *>(src) + i);
+            put(q.x, q.y);
+            put(q.z, q.w);
+        }
+        if ((n & 1) && threadIdx.x == 0) {
+            const uint64_t v = ld_stream(src + n - 1);
+            put((uint32_t)v, (uint32_t)(v >> 32));
+        }
+    }
+    __syncthreads();
+    uint4 *dst = reinterpret_cast<uint4 *>(reinterpret_cast<uint64_t *>(hdr + 1) + ((uint64_t)f << kRangeLdsSliceBits));
+    for (uint32_t i = threadIdx.x; i < kRangeLdsSlice / 2; i += kLdsProbeThreads) {
+        const uint32_t w = s.bits[i >> 4], sh = (2u * i) & 31u;
+        const bool b0 = (w >> sh) & 1u, b1 = (w >> (sh + 1)) & 1u;
+        dst[i] = uint4{b0 ? kbase + 2 * i : ~0u, b0 ? s.pay[2 * i] : ~0u, b1 ? kbase + 2 * i + 1 : ~0u,
+                       b1 ? s.pay[2 * i + 1] : ~0u};
+    }
+    occ = wave_sum(occ);
+    sentinel = wave_sum(sentinel);
+    if (lane_id() == 0) {
+        if (occ) atomicAdd(&s.occ, occ);
+        if (sentinel) atomicAdd(&s.sentinel, sentinel);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s.occ) atomicAdd(&hdr->occupied, (unsigned long long)s.occ);
+        if (s.sentinel) atomicAdd(&hdr->sentinel_count, s.sentinel);
+    }
 }
 
 // Bit j of the low 16 bits of x to bit 2j.
@@ -830,21 +934,45 @@ int launch_range_part_build(void *table, size_t table_bytes, const int32_t *rkey
     const uint64_t cap = join_table_slots(table_bytes);
     const PartPlan p = part_plan(cap, nr, kRangeSliceSlots);
     if (!p.on || !ws || (size_t)p.slice_max * 8 > (size_t)props().lds_per_block) return QO_OK;
+    JoinHeader *hdr = static_cast<JoinHeader *>(table);
+    const RangeLdsPlan rl = range_lds_plan(cap, nr);
+    const uint32_t lds_bits = rl.on ? rl.bits : 0u;
+    if (rl.on) {  // tables of 2^21 slots and more: one split by key, 2^15-slot slices built in LDS
+        Carver cv(ws);
+        const RangeLdsBufs lb = range_lds_carve(rl, cv);
+        unsigned long long *spill_n = cv.take<unsigned long long>(1);
+        uint64_t *spill = cv.take<uint64_t>(nr);
+        const RangeBuildGate gate{hdr, lds_bits, true};
+        QO_TRY(launch_fill_u32(lb.ctr, 0, lb.ctr_words, st));
+        QO_TRY(launch_fill_u64(spill_n, 0, 1, st));
+        const bool vec = ((uintptr_t)rkey % 16 == 0) && ((uintptr_t)rpay % 16 == 0);
+        const size_t tiles = (nr + kSplitTile - 1) / kSplitTile;
+        QO_REQUIRE(tiles < 0x7FFFFFFFull, "build input too large");
+        k_range_lds_split<false><<<(unsigned)tiles, kSplitThreads, 0, st>>>(
+            rkey, rpay, nullptr, nr, 0, 0, vec, rl.bcap, lb.bucket, lb.ctr, AppendOverflow{spill, spill_n}, gate);
+        QO_LAUNCHED();
+        k_range_lds_build<<<1u << rl.bits, kLdsProbeThreads, sizeof(RangeLdsBuildLds), st>>>(lb.bucket, rl.bcap,
+                                                                                           lb.ctr, gate, spill,
+                                                                                           spill_n);
+        QO_LAUNCHED();
+        k_range_spill<<<(unsigned)props().cu_count * 4, 256, 0, st>>>(spill, spill_n, hdr);
+        QO_LAUNCHED();
+    }
+    // Other key-range tables: split by the table's own hash (for a key-range
+    // table its top bits are those of key - kmin, so bucket f holds the keys
+    // of table slice f), 16384-slot slices of full slots in LDS.
     Carver cv(ws);
     const PartBufs b = part_carve(p, nr, true, cv);
-    JoinHeader *hdr = static_cast<JoinHeader *>(table);
     QO_TRY(launch_fill_u32(b.sb.ctr, 0, b.sb.ctr_words, st));
     QO_TRY(launch_fill_u64(b.spill_n, 0, 1, st));
-    // Split by the table's own hash: for a key-range table its top bits are
-    // those of key - kmin, so bucket f holds the keys of table slice f.
     QO_TRY(launch_splits(p.s, b.sb, rkey, rpay, nullptr, nr, 0, 0, AppendOverflow{b.spill, b.spill_n},
-                         Gate{&hdr->mode, kJoinKeyRange}, HashSrc{hdr}, st));
+                         RangeBuildGate{hdr, lds_bits, false}, HashSrc{hdr}, st));
     const uint64_t *fb;
     uint64_t fcap;
     const uint32_t *fctr;
     split_fine(p.s, b.sb, fb, fcap, fctr);
     k_range_slice_build<<<1u << p.L, 256, (size_t)p.slice_max * 8, st>>>(fb, fcap, fctr, p.L, hdr, b.spill,
-                                                                         b.spill_n);
+                                                                         b.spill_n, lds_bits);
     QO_LAUNCHED();
     k_range_spill<<<(unsigned)props().cu_count * 4, 256, 0, st>>>(b.spill, b.spill_n, hdr);
     QO_LAUNCHED();

@@ -11,8 +11,10 @@ and kernel logic are exercised without a GPU.
 * record_ops: split (one and two radix passes), gather, scatter, projection,
   reduce and Records -> columns on the benchmark's input shape, each against a
   plain loop.
-* range_build: the key-range join table built slice by slice in LDS equals the
-  table built by storing every tuple at its slot (permutation keys, a negative
+* range_build: the key-range join table built slice by slice in LDS (16384 full
+  slots per slice, or, from 2^21 slots, 2^15 payloads and presence bits per
+  slice after one split by key) equals the table built by storing every tuple
+  at its slot (permutation keys, a negative
   base with (-1, -1) tuples, keys with gaps, a duplicate key's hash fallback).
 * range_probe: joins whose key-range table is probed through LDS slices (the
   default plan from 2^21 slots), through L2 and unpartitioned, each equal to a
@@ -80,7 +82,9 @@ def test_batch_parts_cpu(cpu_build):
 
 
 def test_range_build_sliced_equals_direct_cpu(cpu_build):
-    out = _run(cpu_build / "range_build", 20)
+    out = _run(cpu_build / "range_build", 20)  # 2^20 slots: 16384-slot slices of full slots
+    assert out.count("same table") == 4
+    out = _run(cpu_build / "range_build", 21)  # 2^21 slots and more: 2^15-slot slices built in LDS
     assert out.count("same table") == 4
 
 
