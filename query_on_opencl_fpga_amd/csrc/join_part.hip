@@ -958,6 +958,16 @@ int launch_range_part_build(void *table, size_t table_bytes, const int32_t *rkey
         k_range_spill<<<(unsigned)props().cu_count * 4, 256, 0, st>>>(spill, spill_n, hdr);
         QO_LAUNCHED();
     }
+    // With more than 2^20 build rows and at most 2^(15 + bits) slots, every
+    // key-range table the LDS build does not take has a repeated key (its
+    // key range is under 2^21): it falls back to the hash-mixed table, built or
+    // not, so the sliced build is not launched.
+    int capbits = 0;
+    while (capbits < 31 && (uint64_t(2) << capbits) <= cap) ++capbits;
+    if (rl.on && nr > (size_t(1) << 20) && capbits <= kRangeLdsSliceBits + kRangeLdsMaxBits) {
+        *used = true;
+        return QO_OK;
+    }
     // Other key-range tables: split by the table's own hash (for a key-range
     // table its top bits are those of key - kmin, so bucket f holds the keys
     // of table slice f), 16384-slot slices of full slots in LDS.
