@@ -436,6 +436,18 @@ def test_key_range_lds_route_matches_oracle(qo, orc, nr, ns, skew, sentinel):
     assert np.array_equal(canon(rows(got)), exp)
 
 
+@pytest.mark.parametrize("flo,fhi", [(100, 200), (0, 99), (50, 49)])
+def test_key_range_lds_route_filter_edges(qo, orc, flo, fhi):
+    """The LDS route with every probe tuple filtered out (each bucket empty:
+    the probe's work-groups return before any barrier), with none filtered out
+    (the fullest buckets), and with an empty filter range."""
+    rk, rp, sk, sp, sf = _lds_route_case(orc, 3 << 20, 1 << 21, 81)
+    exp = canon(rows(orc.hash_join(rk, rp, sk, sp, sf, flo, fhi)))
+    got = qo.operators.hash_join(rk, rp, sk, sp, sf, flo, fhi, capacity=(1 << 21) + 16)
+    assert got[3] == len(exp)
+    assert np.array_equal(canon(rows(got)), exp)
+
+
 def test_hj_filter_sort_lds_route(qo, orc):
     """hj_filter_sort on a key-range table of 2^22 slots (the probe's
     sort-ready output through the LDS route): rows in (key, s_payload) order,
